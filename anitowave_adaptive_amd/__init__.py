@@ -1,0 +1,446 @@
+"""MI355X-native MSBG FLIP step: ctypes host over the C ABI in include/aw.h.
+
+The classes mirror the reference's host surface (MSBGManager, P2GTransfer,
+G2PManager, AdaptiveSmoother, MultigridTransfer, VCycleSolver; reference
+include/*.cuh) on top of the HIP library libanitowave.so built in-tree by
+anitowave_adaptive_amd.build.  There is no CPU fallback: if the library is
+missing, loading fails with an error.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libanitowave.so")
+INCLUDE_DIR = os.path.join(os.path.dirname(_HERE), "include")
+
+AW_OK, AW_ERR_INVALID, AW_ERR_OOM, AW_ERR_DEVICE, AW_ERR_NOT_CONVERGED, AW_ERR_NAN, AW_ERR_UNSUPPORTED = range(7)
+MAX_LEVELS = 4
+
+_V = ctypes.c_void_p
+_P = ctypes.POINTER
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_int * 3), ("base_res", ctypes.c_int), ("num_levels", ctypes.c_int),
+                ("h", ctypes.c_float), ("domain_min", ctypes.c_float * 3), ("band", ctypes.c_int * MAX_LEVELS)]
+
+
+class Layout(ctypes.Structure):
+    _fields_ = [("num_blocks", ctypes.c_uint32), ("total_cells", ctypes.c_uint32),
+                ("total_faces", ctypes.c_uint32 * 3), ("num_mg_levels", ctypes.c_int),
+                ("d_level", _V), ("d_cell_off", _V), ("d_face_off", _V * 3), ("d_particle_count", _V),
+                ("d_particle_off", _V), ("d_pressure", _V), ("d_rhs", _V), ("d_liquid", _V), ("d_beta", _V * 3),
+                ("d_u_old", _V * 3), ("d_u_star", _V * 3), ("d_u_new", _V * 3), ("d_mass", _V * 3), ("d_fine", _V)]
+
+
+class SolverParams(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int), ("pre_sweeps", ctypes.c_int), ("post_sweeps", ctypes.c_int),
+                ("coarse_sweeps", ctypes.c_int), ("local_sweeps", ctypes.c_int), ("omega", ctypes.c_float),
+                ("prolong_scale", ctypes.c_float), ("precision", ctypes.c_int)]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("outer_iterations", ctypes.c_int), ("rel_residual", ctypes.c_double),
+                ("b_norm", ctypes.c_double)]
+
+
+class StepParams(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_float), ("gravity", ctypes.c_float * 3), ("flip_alpha", ctypes.c_float),
+                ("rel_tol", ctypes.c_double), ("max_iters", ctypes.c_int), ("rebuild_topology", ctypes.c_int)]
+
+
+class StepStats(ctypes.Structure):
+    _fields_ = [("topology_changed", ctypes.c_int), ("solve", SolveStats),
+                ("num_blocks_by_level", ctypes.c_uint32 * MAX_LEVELS), ("total_cells", ctypes.c_uint32),
+                ("liquid_cells", ctypes.c_uint32)]
+
+
+class LevelInfo(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_int * 3), ("num_blocks", ctypes.c_uint32), ("total_cells", ctypes.c_uint32),
+                ("total_faces", ctypes.c_uint32 * 3), ("aggregated", ctypes.c_int), ("d_e", _V),
+                ("d_cell_off", _V), ("d_x", _V), ("d_b", _V), ("d_r", _V), ("d_liquid", _V)]
+
+
+_I = ctypes.c_int
+_I64 = ctypes.c_int64
+_F = ctypes.c_float
+_D = ctypes.c_double
+
+# name -> (restype, argtypes); every entry is declared in include/aw.h
+SIGNATURES = {
+    "aw_status_str": (ctypes.c_char_p, [_I]),
+    "aw_last_error": (ctypes.c_char_p, []),
+    "aw_version": (_I, []),
+    "aw_ctx_create": (_I, [_I, _V, _P(_V)]),
+    "aw_ctx_destroy": (_I, [_V]),
+    "aw_ctx_synchronize": (_I, [_V]),
+    "aw_malloc": (_I, [_V, ctypes.c_uint64, _P(_V)]),
+    "aw_free": (_I, [_V, _V]),
+    "aw_copy": (_I, [_V, _V, _V, ctypes.c_uint64]),
+    "aw_grid_create": (_I, [_V, _P(GridDesc), _P(_V)]),
+    "aw_grid_destroy": (_I, [_V]),
+    "aw_grid_build_topology": (_I, [_V, _V, _I64, _P(_I)]),
+    "aw_grid_set_levels": (_I, [_V, _V]),
+    "aw_grid_layout": (_I, [_V, _P(Layout)]),
+    "aw_bin_particles": (_I, [_V, _V, _V, _V, _I64]),
+    "aw_p2g": (_I, [_V, _V, _V, _I64, _F, _P(_F)]),
+    "aw_divergence": (_I, [_V]),
+    "aw_project_velocity": (_I, [_V]),
+    "aw_g2p_advect": (_I, [_V, _V, _V, _I64, _F, _F]),
+    "aw_solver_create": (_I, [_V, _P(SolverParams), _P(_V)]),
+    "aw_solver_destroy": (_I, [_V]),
+    "aw_solver_setup": (_I, [_V]),
+    "aw_solver_level_info": (_I, [_V, _I, _P(LevelInfo)]),
+    "aw_solver_smooth": (_I, [_V, _I, _I, _I]),
+    "aw_solver_residual": (_I, [_V, _I]),
+    "aw_solver_apply": (_I, [_V, _I, _V, _V]),
+    "aw_solver_restrict": (_I, [_V, _I]),
+    "aw_solver_prolong": (_I, [_V, _I]),
+    "aw_solver_vcycle": (_I, [_V, _I]),
+    "aw_solver_solve": (_I, [_V, _D, _I, _P(SolveStats)]),
+    "aw_step": (_I, [_V, _V, _V, _V, _V, _I64, _P(StepParams), _P(StepStats)]),
+}
+
+_lib = None
+
+
+class AWError(RuntimeError):
+    def __init__(self, status, where, detail):
+        super().__init__(f"{where}: {detail} (status {status})")
+        self.status = status
+
+
+def lib():
+    """Load libanitowave.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m anitowave_adaptive_amd.build` first")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status, where):
+    if status != AW_OK:
+        raise AWError(status, where, lib().aw_last_error().decode())
+    return status
+
+
+class Context:
+    """One device and stream (aw_ctx)."""
+
+    def __init__(self, device=0, stream=None):
+        self._h = _V()
+        check(lib().aw_ctx_create(device, stream, ctypes.byref(self._h)), "aw_ctx_create")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def synchronize(self):
+        check(lib().aw_ctx_synchronize(self._h), "aw_ctx_synchronize")
+
+    def close(self):
+        if self._h:
+            lib().aw_ctx_destroy(self._h)
+            self._h = _V()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """Device buffer owned by the library's allocator (aw_malloc)."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        self.ptr = _V()
+        check(lib().aw_malloc(ctx.handle, self.nbytes, ctypes.byref(self.ptr)), "aw_malloc")
+
+    @classmethod
+    def from_numpy(cls, ctx, arr):
+        arr = np.ascontiguousarray(arr)
+        d = cls(ctx, arr.nbytes)
+        d.upload(arr)
+        return d
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        check(lib().aw_copy(self.ctx.handle, self.ptr, arr.ctypes.data_as(_V), arr.nbytes), "aw_copy")
+
+    def download(self, dtype, count=None):
+        return download(self.ctx, self.ptr, dtype, count if count is not None else self.nbytes // np.dtype(dtype).itemsize)
+
+    def free(self):
+        if self.ptr:
+            lib().aw_free(self.ctx.handle, self.ptr)
+            self.ptr = _V()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def download(ctx, ptr, dtype, count):
+    out = np.empty(int(count), dtype)
+    if count:
+        check(lib().aw_copy(ctx.handle, out.ctypes.data_as(_V), ptr, out.nbytes), "aw_copy")
+    return out
+
+
+def upload(ctx, ptr, arr):
+    arr = np.ascontiguousarray(arr)
+    if arr.nbytes:
+        check(lib().aw_copy(ctx.handle, ptr, arr.ctypes.data_as(_V), arr.nbytes), "aw_copy")
+
+
+class MSBGManager:
+    """The block grid (MSBGManager, reference include/MSBG.cuh:502-677).
+
+    dims: blocks per axis (8^3 cells at native level 0); h: finest cell size;
+    num_levels native levels; band[l]: level-l band width in blocks."""
+
+    def __init__(self, ctx, dims, h, num_levels=1, band=(0, 0, 0, 0)):
+        self.ctx = ctx
+        self.dims = tuple(int(d) for d in dims)
+        d = GridDesc()
+        d.dims[:] = list(self.dims)
+        d.base_res = 8
+        d.num_levels = int(num_levels)
+        d.h = float(h)
+        d.domain_min[:] = [0.0, 0.0, 0.0]
+        d.band[:] = [int(x) for x in (list(band) + [0, 0, 0, 0])[:4]]
+        self.desc = d
+        self._h = _V()
+        check(lib().aw_grid_create(ctx.handle, ctypes.byref(d), ctypes.byref(self._h)), "aw_grid_create")
+        self.num_blocks = self.dims[0] * self.dims[1] * self.dims[2]
+
+    @property
+    def handle(self):
+        return self._h
+
+    def build_topology(self, d_pos, n):
+        ch = ctypes.c_int(0)
+        check(lib().aw_grid_build_topology(self._h, d_pos, int(n), ctypes.byref(ch)), "aw_grid_build_topology")
+        return bool(ch.value)
+
+    def set_levels(self, levels):
+        lv = np.ascontiguousarray(levels, np.uint8)
+        check(lib().aw_grid_set_levels(self._h, lv.ctypes.data_as(_V)), "aw_grid_set_levels")
+
+    def layout(self):
+        L = Layout()
+        check(lib().aw_grid_layout(self._h, ctypes.byref(L)), "aw_grid_layout")
+        return L
+
+    # host copies of grid arrays (for tests and tools)
+    def levels(self):
+        L = self.layout()
+        return download(self.ctx, L.d_level, np.uint8, L.num_blocks)
+
+    def offsets(self):
+        L = self.layout()
+        c = download(self.ctx, L.d_cell_off, np.uint32, L.num_blocks + 1)
+        f = download(self.ctx, L.d_face_off[0], np.uint32, L.num_blocks + 1)
+        return c, f
+
+    def particle_counts(self):
+        L = self.layout()
+        return download(self.ctx, L.d_particle_count, np.uint32, L.num_blocks)
+
+    def cell(self, name):
+        L = self.layout()
+        ptr = {"p": L.d_pressure, "rhs": L.d_rhs}.get(name)
+        if name == "liquid":
+            return download(self.ctx, L.d_liquid, np.uint8, L.total_cells)
+        return download(self.ctx, ptr, np.float32, L.total_cells)
+
+    def set_cell(self, name, vals):
+        L = self.layout()
+        ptr = {"p": L.d_pressure, "rhs": L.d_rhs, "liquid": L.d_liquid}[name]
+        dt = np.uint8 if name == "liquid" else np.float32
+        upload(self.ctx, ptr, np.ascontiguousarray(vals, dt))
+
+    def face(self, name, axis):
+        L = self.layout()
+        ptr = {"u_old": L.d_u_old, "u_star": L.d_u_star, "u_new": L.d_u_new, "mass": L.d_mass,
+               "beta": L.d_beta}[name][axis]
+        return download(self.ctx, ptr, np.float32, L.total_faces[axis])
+
+    def set_face(self, name, axis, vals):
+        L = self.layout()
+        ptr = {"u_old": L.d_u_old, "u_star": L.d_u_star, "u_new": L.d_u_new, "beta": L.d_beta}[name][axis]
+        upload(self.ctx, ptr, np.ascontiguousarray(vals, np.float32))
+
+    def fine(self):
+        """Virtual fine lattice [block][field][axis][512] (field 0 = u_new, 1 = u_new - u_old)."""
+        L = self.layout()
+        return download(self.ctx, L.d_fine, np.float32, L.num_blocks * 5400).reshape(L.num_blocks, 5400)
+
+    def close(self):
+        if self._h:
+            lib().aw_grid_destroy(self._h)
+            self._h = _V()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class P2GTransfer:
+    """Particle-to-grid (P2GTransfer::run, reference include/P2G.cuh:173-196)."""
+
+    @staticmethod
+    def bin(grid, d_pos, d_vel, d_ids, n):
+        check(lib().aw_bin_particles(grid.handle, d_pos, d_vel, d_ids, int(n)), "aw_bin_particles")
+
+    @staticmethod
+    def run(grid, d_pos, d_vel, n, dt=0.0, gravity=(0.0, 0.0, 0.0)):
+        g = (ctypes.c_float * 3)(*gravity)
+        check(lib().aw_p2g(grid.handle, d_pos, d_vel, int(n), float(dt), g), "aw_p2g")
+
+
+class G2PManager:
+    """FLIP/PIC update and RK3 advection (G2PManager, reference include/G2P.cuh:175-228)."""
+
+    @staticmethod
+    def divergence(grid):
+        check(lib().aw_divergence(grid.handle), "aw_divergence")
+
+    @staticmethod
+    def project(grid):
+        check(lib().aw_project_velocity(grid.handle), "aw_project_velocity")
+
+    @staticmethod
+    def g2p_advect(grid, d_pos, d_vel, n, alpha=0.95, dt=1.0 / 240.0):
+        check(lib().aw_g2p_advect(grid.handle, d_pos, d_vel, int(n), float(alpha), float(dt)), "aw_g2p_advect")
+
+
+def solver_params(method=0, pre=2, post=2, coarse=16, local=1, omega=0.85714, prolong_scale=1.0, precision=0):
+    p = SolverParams()
+    p.method, p.pre_sweeps, p.post_sweeps, p.coarse_sweeps, p.local_sweeps = method, pre, post, coarse, local
+    p.omega, p.prolong_scale, p.precision = omega, prolong_scale, precision
+    return p
+
+
+class VCycleSolver:
+    """Multigrid pressure solver (VCycleSolver + AdaptiveSmoother +
+    MultigridTransfer, reference include/VCycle.cuh, Smoother.cuh,
+    MultigridTransfer.cuh)."""
+
+    def __init__(self, grid, **kw):
+        self.grid = grid
+        self.params = solver_params(**kw)
+        self._h = _V()
+        check(lib().aw_solver_create(grid.handle, ctypes.byref(self.params), ctypes.byref(self._h)),
+              "aw_solver_create")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def setup(self):
+        check(lib().aw_solver_setup(self._h), "aw_solver_setup")
+
+    def level_info(self, k):
+        L = LevelInfo()
+        check(lib().aw_solver_level_info(self._h, int(k), ctypes.byref(L)), "aw_solver_level_info")
+        return L
+
+    def num_levels(self):
+        k = 0
+        while True:
+            L = LevelInfo()
+            if lib().aw_solver_level_info(self._h, k, ctypes.byref(L)) != AW_OK:
+                return k
+            k += 1
+
+    def level_get(self, k, name):
+        L = self.level_info(k)
+        if name == "liquid":
+            return download(self.grid.ctx, L.d_liquid, np.uint8, L.total_cells)
+        if name == "e":
+            return download(self.grid.ctx, L.d_e, np.uint8, L.num_blocks)
+        if name == "coff":
+            return download(self.grid.ctx, L.d_cell_off, np.uint32, L.num_blocks + 1)
+        return download(self.grid.ctx, {"x": L.d_x, "b": L.d_b, "r": L.d_r}[name], np.float32, L.total_cells)
+
+    def level_set(self, k, name, vals):
+        L = self.level_info(k)
+        upload(self.grid.ctx, {"x": L.d_x, "b": L.d_b, "r": L.d_r}[name], np.ascontiguousarray(vals, np.float32))
+
+    def smooth(self, k, iters=1, reverse=False):
+        check(lib().aw_solver_smooth(self._h, int(k), int(iters), int(bool(reverse))), "aw_solver_smooth")
+
+    def residual(self, k):
+        check(lib().aw_solver_residual(self._h, int(k)), "aw_solver_residual")
+
+    def apply(self, k, d_x, d_y):
+        check(lib().aw_solver_apply(self._h, int(k), d_x, d_y), "aw_solver_apply")
+
+    def restrict(self, k):
+        check(lib().aw_solver_restrict(self._h, int(k)), "aw_solver_restrict")
+
+    def prolong(self, k):
+        check(lib().aw_solver_prolong(self._h, int(k)), "aw_solver_prolong")
+
+    def run_cycle(self, k=0):
+        check(lib().aw_solver_vcycle(self._h, int(k)), "aw_solver_vcycle")
+
+    def solve(self, tol=1e-6, max_iters=200, allow_not_converged=False):
+        st = SolveStats()
+        s = lib().aw_solver_solve(self._h, float(tol), int(max_iters), ctypes.byref(st))
+        if not (allow_not_converged and s == AW_ERR_NOT_CONVERGED):
+            check(s, "aw_solver_solve")
+        return st
+
+    def close(self):
+        if self._h:
+            lib().aw_solver_destroy(self._h)
+            self._h = _V()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class FlipStep:
+    """The whole step (aw_step) on device particle buffers."""
+
+    def __init__(self, grid, solver, dt=1.0 / 240.0, gravity=(0.0, -9.81, 0.0), alpha=0.95, tol=1e-6,
+                 max_iters=200, rebuild_topology=True):
+        self.grid, self.solver = grid, solver
+        p = StepParams()
+        p.dt = dt
+        p.gravity[:] = list(gravity)
+        p.flip_alpha = alpha
+        p.rel_tol = tol
+        p.max_iters = max_iters
+        p.rebuild_topology = int(rebuild_topology)
+        self.params = p
+
+    def __call__(self, d_pos, d_vel, d_ids, n, allow_not_converged=False):
+        st = StepStats()
+        s = lib().aw_step(self.grid.handle, self.solver.handle, d_pos, d_vel, d_ids, int(n), ctypes.byref(self.params),
+                          ctypes.byref(st))
+        if not (allow_not_converged and s == AW_ERR_NOT_CONVERGED):
+            check(s, "aw_step")
+        return st

@@ -1,0 +1,105 @@
+// Internal declarations shared by the HIP kernels (aw_kernels.hip) and the
+// host runtime (aw_runtime.cpp).  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace aw {
+
+constexpr int BASE = 8;        // cells per block edge at native level 0
+constexpr int LOGB = 3;
+constexpr int PAD = 900;       // padded fine nodes per axis per block (9 x 10 x 10)
+constexpr int SLAB = 2 * 3 * PAD;  // floats per block in the P2G slab / fine lattice
+constexpr int FINE = 512;      // own fine nodes per axis per block
+
+// One multigrid level as the kernels see it.  Level 0 aliases the grid.
+struct LevelDev {
+  int D[3];
+  int nb;
+  int bs;                      // block edge in fine cells (8, or 8 << k after k aggregations)
+  const uint8_t* e;            // log2 of the cell edge in fine cells, per block
+  const uint32_t* coff;        // cell offsets, nb+1
+  const uint32_t* foff[3];     // face-slot offsets per axis, nb+1
+  const int32_t* nbr;          // 6 face neighbours per block (-x,+x,-y,+y,-z,+z), -1 = tank wall
+  const uint8_t* liquid;       // per cell
+  const float* beta[3];        // per face slot
+};
+
+// The native grid as the transfer kernels see it.
+struct GridDev {
+  int D[3];
+  int N[3];                    // fine cells per axis
+  int nb;
+  float h, inv_h;
+  const uint8_t* level;
+  const uint32_t* coff;
+  const uint32_t* foff[3];
+  const int32_t* nbr;
+};
+
+// Work-group table entry: `count` blocks of resolution 1 << lr starting at
+// blist[start].  A group holds at most 512 cells.
+struct WG {
+  int start, count, lr, pad;
+};
+
+enum Scalar { SC_RZ = 0, SC_RZOLD = 1, SC_PQ = 2, SC_RR = 3, SC_BB = 4, SC_N = 8 };
+
+// ---- launchers (aw_kernels.hip) ----
+void k_fill_f32(hipStream_t s, float* p, size_t n, float v);
+void k_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v);
+void k_fill_f64(hipStream_t s, double* p, size_t n, double v);
+
+// topology
+void k_bin(hipStream_t s, const GridDev& g, const float* pos, int64_t n, int32_t* key, uint32_t* count);
+void k_levels(hipStream_t s, const int* D, int L, const int* band, const uint32_t* count, uint8_t* surf,
+              uint8_t* level, uint8_t* tmp);
+void k_scan_u32(hipStream_t s, const uint32_t* in, uint32_t* out, int n);  // exclusive, out[n] = total
+void k_layout_offsets(hipStream_t s, const uint8_t* level, int nb, uint32_t* coff, uint32_t* fx, uint32_t* fy,
+                      uint32_t* fz);
+void k_scatter(hipStream_t s, const float* pos, const float* vel, const int32_t* ids, const int32_t* key,
+               uint32_t* cursor, int64_t n, float* spos, float* svel, int32_t* sids);
+void k_init_beta(hipStream_t s, const LevelDev& L, float* bx, float* by, float* bz);
+
+// transfers
+void k_p2g_deposit(hipStream_t s, const GridDev& g, const float* spos, const float* svel, const uint32_t* pcount,
+                   const uint32_t* poff, float* slab, uint8_t* liquid);
+void k_p2g_gather(hipStream_t s, const GridDev& g, const uint32_t* pcount, const float* slab, float* const* Macc,
+                  float* const* macc);
+void k_fold(hipStream_t s, const GridDev& g, float* const* Macc, float* const* macc, float* const* uold,
+            float* const* ustar, float* const* mass, float dt, const float* grav);
+void k_divergence(hipStream_t s, const GridDev& g, const uint8_t* liquid, float* const* ustar, float* rhs);
+void k_project(hipStream_t s, const GridDev& g, const float* p, float* const* ustar, float* const* beta,
+               float* const* unew);
+void k_to_fine(hipStream_t s, const GridDev& g, float* const* unew, float* const* uold, float* fine);
+void k_g2p_advect(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const float* svel,
+                  int64_t n, float alpha, float dt, float* pos_out, float* vel_out);
+
+// solver
+void k_smooth(hipStream_t s, const LevelDev& L, float* x, const float* b, const WG* wg, int nwg,
+              const uint32_t* blist, float omega, int local_sweeps, int reverse);
+void k_residual(hipStream_t s, const LevelDev& L, const float* x, const float* b, float* r, const WG* wg, int nwg,
+                const uint32_t* blist);
+void k_apply_dot(hipStream_t s, const LevelDev& L, const float* x, float* y, const WG* wg, int nwg,
+                 const uint32_t* blist, double* acc);
+void k_restrict(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, const float* rf, float* bc,
+                const WG* wgc, int nwgc, const uint32_t* blistc);
+void k_prolong(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, float* xf, const float* xc,
+               float scale, const WG* wgf, int nwgf, const uint32_t* blistf);
+void k_coarsen_liquid(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, uint8_t* liquid_c,
+                      const WG* wgc, int nwgc, const uint32_t* blistc);
+void k_coarse_solve(hipStream_t s, const LevelDev& L, float* x, const float* b, int ncells, int sweeps,
+                    float omega, int local_sweeps);
+
+// PCG vector work (scalars live in device memory, see Scalar)
+void k_dot(hipStream_t s, const float* a, const float* b, const uint8_t* liquid, int n, double* acc);
+void k_scalar_shift(hipStream_t s, double* sc);
+void k_pcg_p(hipStream_t s, const float* z, float* p, int n, const double* sc, int first);
+void k_pcg_xr(hipStream_t s, float* x, float* r, const float* p, const float* q, const uint8_t* liquid, int n,
+              double* sc);
+void k_mask(hipStream_t s, float* x, const uint8_t* liquid, int n);
+void k_add(hipStream_t s, float* x, const float* z, const uint8_t* liquid, int n);
+
+}  // namespace aw

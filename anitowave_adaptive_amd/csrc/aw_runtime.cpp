@@ -1,0 +1,1002 @@
+// Host runtime of the MI355X MSBG FLIP step: contexts, grids, the topology
+// pipeline, the multigrid hierarchy and the solver loops behind the C ABI in
+// include/aw.h.  Device work is enqueued on the context's stream; host
+// synchronisation happens only where the ABI says so.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/aw.h"
+#include "aw_internal.h"
+
+using namespace aw;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define AW_HIP(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess) return fail(AW_ERR_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+#define AW_TRY(expr)         \
+  do {                       \
+    int s_ = (expr);         \
+    if (s_ != AW_OK) return s_; \
+  } while (0)
+
+// grow-only device allocation
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int ensure(size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    if (bytes <= cap) return AW_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(AW_ERR_OOM, "hipMalloc of %zu bytes failed", bytes);
+    }
+    cap = bytes;
+    return AW_OK;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+int launch_status(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(AW_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+  return AW_OK;
+}
+
+void neighbour_table(const int* D, std::vector<int32_t>& nbr) {
+  const int nb = D[0] * D[1] * D[2];
+  nbr.assign((size_t)nb * 6, -1);
+  for (int b = 0; b < nb; ++b) {
+    int c[3] = {b % D[0], (b / D[0]) % D[1], b / (D[0] * D[1])};
+    for (int a = 0; a < 3; ++a)
+      for (int side = 0; side < 2; ++side) {
+        int q[3] = {c[0], c[1], c[2]};
+        q[a] += side ? 1 : -1;
+        if (q[a] < 0 || q[a] >= D[a]) continue;
+        nbr[(size_t)b * 6 + 2 * a + side] = q[0] + D[0] * (q[1] + D[1] * q[2]);
+      }
+  }
+}
+
+}  // namespace
+
+struct aw_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+};
+
+struct aw_grid {
+  aw_ctx* ctx = nullptr;
+  aw_grid_desc desc{};
+  int D[3]{}, N[3]{};
+  int nb = 0;
+  float h = 0.f, inv_h = 0.f;
+  DevBuf level, surf, ltmp, count, poff, cursor, coff, foff, nbr, key;
+  DevBuf spos, svel, sids;
+  DevBuf liquid, p, rhs, slab;
+  DevBuf beta[3], Macc[3], macc[3], uold[3], ustar[3], unew[3], mass[3];
+  std::vector<uint8_t> h_level;
+  std::vector<uint32_t> h_coff;
+  bool have_topology = false;
+  int topo_version = 0;
+  uint32_t ncells = 0, nfaces = 0;
+  int64_t n_sorted = -1;
+
+  GridDev view() const {
+    GridDev g;
+    for (int a = 0; a < 3; ++a) {
+      g.D[a] = D[a];
+      g.N[a] = N[a];
+      g.foff[a] = foff.as<uint32_t>();
+    }
+    g.nb = nb;
+    g.h = h;
+    g.inv_h = inv_h;
+    g.level = level.as<uint8_t>();
+    g.coff = coff.as<uint32_t>();
+    g.nbr = nbr.as<int32_t>();
+    return g;
+  }
+  float* face(DevBuf* f, int a) const { return f[a].as<float>(); }
+};
+
+namespace {
+
+struct MGLevel {
+  int D[3]{};
+  int nb = 0, bs = BASE, aggregated = 0;
+  uint32_t ncells = 0, nfaces = 0;
+  std::vector<uint8_t> h_e;
+  std::vector<uint32_t> h_coff;
+  DevBuf e, coff, foff, nbr, liquid, beta[3], x, b, r, blist, wg;
+  int nwg = 0, nwg0 = 0;  // work-group table: colour 0 = [0, nwg0), colour 1 = [nwg0, nwg)
+  // device views (level 0 aliases the grid)
+  const uint8_t* d_e = nullptr;
+  const uint32_t* d_coff = nullptr;
+  const uint32_t* d_foff = nullptr;
+  const int32_t* d_nbr = nullptr;
+  uint8_t* d_liq = nullptr;
+  float* d_beta[3] = {nullptr, nullptr, nullptr};
+  float* d_x = nullptr;
+  float* d_b = nullptr;
+  float* d_r = nullptr;
+
+  LevelDev view() const {
+    LevelDev L;
+    for (int a = 0; a < 3; ++a) {
+      L.D[a] = D[a];
+      L.foff[a] = d_foff;
+      L.beta[a] = d_beta[a];
+    }
+    L.nb = nb;
+    L.bs = bs;
+    L.e = d_e;
+    L.coff = d_coff;
+    L.nbr = d_nbr;
+    L.liquid = d_liq;
+    return L;
+  }
+  const WG* wgp() const { return wg.as<WG>(); }
+  const uint32_t* bl() const { return blist.as<uint32_t>(); }
+};
+
+}  // namespace
+
+struct aw_solver {
+  aw_grid* g = nullptr;
+  aw_solver_params prm{};
+  std::vector<std::unique_ptr<MGLevel>> lv;
+  int built_version = -1;
+  DevBuf r0, z, pv, q, sc;
+  double* h_sc = nullptr;
+};
+
+namespace {
+
+int apply_levels(aw_grid* g) {
+  hipStream_t s = g->ctx->stream;
+  const int nb = g->nb;
+  uint64_t nc = 0, nf = 0;
+  g->h_coff.assign(nb + 1, 0);
+  for (int b = 0; b < nb; ++b) {
+    const int r = BASE >> g->h_level[b];
+    g->h_coff[b + 1] = g->h_coff[b] + r * r * r;
+    nc += (uint64_t)r * r * r;
+    nf += (uint64_t)(r + 1) * r * r;
+  }
+  if (nf > 0xFFFFFFFFull) return fail(AW_ERR_INVALID, "grid too large for 32-bit offsets");
+  g->ncells = (uint32_t)nc;
+  g->nfaces = (uint32_t)nf;
+  AW_TRY(g->coff.ensure((nb + 1) * sizeof(uint32_t)));
+  AW_TRY(g->foff.ensure((nb + 1) * sizeof(uint32_t)));
+  // countBlockElements + exclusive_scan on the device (MSBG.cuh:353-376, main.cu:270-273)
+  k_layout_offsets(s, g->level.as<uint8_t>(), nb, g->coff.as<uint32_t>(), g->foff.as<uint32_t>(),
+                   g->foff.as<uint32_t>(), g->foff.as<uint32_t>());
+  AW_TRY(g->liquid.ensure(nc));
+  AW_TRY(g->p.ensure(nc * sizeof(float)));
+  AW_TRY(g->rhs.ensure(nc * sizeof(float)));
+  for (int a = 0; a < 3; ++a)
+    for (DevBuf* f : {g->beta, g->Macc, g->macc, g->uold, g->ustar, g->unew, g->mass})
+      AW_TRY(f[a].ensure(nf * sizeof(float)));
+  AW_TRY(g->slab.ensure((size_t)nb * SLAB * sizeof(float)));
+  LevelDev L;
+  for (int a = 0; a < 3; ++a) {
+    L.D[a] = g->D[a];
+    L.foff[a] = g->foff.as<uint32_t>();
+    L.beta[a] = g->beta[a].as<float>();
+  }
+  L.nb = nb;
+  L.bs = BASE;
+  L.e = g->level.as<uint8_t>();
+  L.coff = g->coff.as<uint32_t>();
+  L.nbr = g->nbr.as<int32_t>();
+  L.liquid = g->liquid.as<uint8_t>();
+  k_init_beta(s, L, g->beta[0].as<float>(), g->beta[1].as<float>(), g->beta[2].as<float>());
+  k_fill_f32(s, g->p.as<float>(), nc, 0.f);
+  k_fill_f32(s, g->rhs.as<float>(), nc, 0.f);
+  hipMemsetAsync(g->liquid.p, 0, nc, s);
+  g->have_topology = true;
+  g->topo_version++;
+  g->n_sorted = -1;
+  return launch_status("apply_levels");
+}
+
+int bin_and_levels(aw_grid* g, const float* d_pos, int64_t n, bool mark_levels, int* changed) {
+  hipStream_t s = g->ctx->stream;
+  AW_TRY(g->key.ensure((size_t)std::max<int64_t>(n, 1) * sizeof(int32_t)));
+  k_fill_u32(s, g->count.as<uint32_t>(), g->nb, 0u);
+  GridDev gv = g->view();
+  k_bin(s, gv, d_pos, n, g->key.as<int32_t>(), g->count.as<uint32_t>());
+  if (changed) *changed = 0;
+  if (!mark_levels) return launch_status("bin");
+  k_levels(s, g->D, g->desc.num_levels, g->desc.band, g->count.as<uint32_t>(), g->surf.as<uint8_t>(),
+           g->level.as<uint8_t>(), g->ltmp.as<uint8_t>());
+  std::vector<uint8_t> nl(g->nb);
+  AW_HIP(hipMemcpyAsync(nl.data(), g->level.p, g->nb, hipMemcpyDeviceToHost, s));
+  AW_HIP(hipStreamSynchronize(s));
+  AW_TRY(launch_status("levels"));
+  bool ch = !g->have_topology || nl != g->h_level;
+  if (ch) {
+    g->h_level.swap(nl);
+    AW_TRY(apply_levels(g));
+  } else {
+    // the device map was rewritten with identical values; nothing else changes
+  }
+  if (changed) *changed = ch ? 1 : 0;
+  return AW_OK;
+}
+
+int sort_particles(aw_grid* g, const float* d_pos, const float* d_vel, const int32_t* d_ids, int64_t n) {
+  hipStream_t s = g->ctx->stream;
+  AW_TRY(g->spos.ensure((size_t)std::max<int64_t>(n, 1) * 3 * sizeof(float)));
+  AW_TRY(g->svel.ensure((size_t)std::max<int64_t>(n, 1) * 3 * sizeof(float)));
+  if (d_ids) AW_TRY(g->sids.ensure((size_t)std::max<int64_t>(n, 1) * sizeof(int32_t)));
+  k_scan_u32(s, g->count.as<uint32_t>(), g->poff.as<uint32_t>(), g->nb);
+  AW_HIP(hipMemcpyAsync(g->cursor.p, g->poff.p, (size_t)g->nb * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+  k_scatter(s, d_pos, d_vel, d_ids, g->key.as<int32_t>(), g->cursor.as<uint32_t>(), n, g->spos.as<float>(),
+            g->svel.as<float>(), d_ids ? g->sids.as<int32_t>() : nullptr);
+  g->n_sorted = n;
+  return launch_status("sort_particles");
+}
+
+int p2g_sorted(aw_grid* g, const float* spos, const float* svel, float dt, const float* grav) {
+  hipStream_t s = g->ctx->stream;
+  GridDev gv = g->view();
+  k_p2g_deposit(s, gv, spos, svel, g->count.as<uint32_t>(), g->poff.as<uint32_t>(), g->slab.as<float>(),
+                g->liquid.as<uint8_t>());
+  float* M[3] = {g->Macc[0].as<float>(), g->Macc[1].as<float>(), g->Macc[2].as<float>()};
+  float* m[3] = {g->macc[0].as<float>(), g->macc[1].as<float>(), g->macc[2].as<float>()};
+  float* uo[3] = {g->uold[0].as<float>(), g->uold[1].as<float>(), g->uold[2].as<float>()};
+  float* us[3] = {g->ustar[0].as<float>(), g->ustar[1].as<float>(), g->ustar[2].as<float>()};
+  float* ms[3] = {g->mass[0].as<float>(), g->mass[1].as<float>(), g->mass[2].as<float>()};
+  k_p2g_gather(s, gv, g->count.as<uint32_t>(), g->slab.as<float>(), M, m);
+  k_fold(s, gv, M, m, uo, us, ms, dt, grav);
+  return launch_status("p2g");
+}
+
+int divergence(aw_grid* g) {
+  float* us[3] = {g->ustar[0].as<float>(), g->ustar[1].as<float>(), g->ustar[2].as<float>()};
+  k_divergence(g->ctx->stream, g->view(), g->liquid.as<uint8_t>(), us, g->rhs.as<float>());
+  return launch_status("divergence");
+}
+
+int project(aw_grid* g) {
+  hipStream_t s = g->ctx->stream;
+  GridDev gv = g->view();
+  float* us[3] = {g->ustar[0].as<float>(), g->ustar[1].as<float>(), g->ustar[2].as<float>()};
+  float* bt[3] = {g->beta[0].as<float>(), g->beta[1].as<float>(), g->beta[2].as<float>()};
+  float* un[3] = {g->unew[0].as<float>(), g->unew[1].as<float>(), g->unew[2].as<float>()};
+  float* uo[3] = {g->uold[0].as<float>(), g->uold[1].as<float>(), g->uold[2].as<float>()};
+  k_project(s, gv, g->p.as<float>(), us, bt, un);
+  k_to_fine(s, gv, un, uo, g->slab.as<float>());
+  return launch_status("project");
+}
+
+// ---------------------------------------------------------------- multigrid
+void build_wg(MGLevel& L, std::vector<uint32_t>& blist, std::vector<WG>& wg) {
+  blist.clear();
+  wg.clear();
+  for (int col = 0; col < 2; ++col) {
+    for (int lr = 3; lr >= 0; --lr) {
+      const int per = 512 >> (3 * lr);
+      int start = (int)blist.size(), cnt = 0;
+      for (int b = 0; b < L.nb; ++b) {
+        int c[3] = {b % L.D[0], (b / L.D[0]) % L.D[1], b / (L.D[0] * L.D[1])};
+        if (((c[0] + c[1] + c[2]) & 1) != col) continue;
+        const int r = L.bs >> L.h_e[b];
+        int l2 = 0;
+        while ((1 << l2) < r) ++l2;
+        if (l2 != lr) continue;
+        blist.push_back((uint32_t)b);
+        if (++cnt == per) {
+          wg.push_back(WG{start, cnt, lr, 0});
+          start = (int)blist.size();
+          cnt = 0;
+        }
+      }
+      if (cnt) wg.push_back(WG{start, cnt, lr, 0});
+    }
+    if (col == 0) L.nwg0 = (int)wg.size();
+  }
+  L.nwg = (int)wg.size();
+}
+
+int upload_level(hipStream_t s, MGLevel& L, bool own_topology) {
+  L.nb = L.D[0] * L.D[1] * L.D[2];
+  L.h_coff.assign(L.nb + 1, 0);
+  std::vector<uint32_t> foff(L.nb + 1, 0);
+  for (int b = 0; b < L.nb; ++b) {
+    const int r = L.bs >> L.h_e[b];
+    L.h_coff[b + 1] = L.h_coff[b] + r * r * r;
+    foff[b + 1] = foff[b] + (r + 1) * r * r;
+  }
+  L.ncells = L.h_coff[L.nb];
+  L.nfaces = foff[L.nb];
+  std::vector<uint32_t> blist;
+  std::vector<WG> wg;
+  build_wg(L, blist, wg);
+  AW_TRY(L.blist.ensure(blist.size() * sizeof(uint32_t)));
+  AW_TRY(L.wg.ensure(wg.size() * sizeof(WG)));
+  AW_HIP(hipMemcpyAsync(L.blist.p, blist.data(), blist.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  AW_HIP(hipMemcpyAsync(L.wg.p, wg.data(), wg.size() * sizeof(WG), hipMemcpyHostToDevice, s));
+  AW_TRY(L.r.ensure((size_t)L.ncells * sizeof(float)));
+  L.d_r = L.r.as<float>();
+  if (!own_topology) return AW_OK;
+  std::vector<int32_t> nbr;
+  neighbour_table(L.D, nbr);
+  AW_TRY(L.e.ensure(L.nb));
+  AW_TRY(L.coff.ensure((L.nb + 1) * sizeof(uint32_t)));
+  AW_TRY(L.foff.ensure((L.nb + 1) * sizeof(uint32_t)));
+  AW_TRY(L.nbr.ensure(nbr.size() * sizeof(int32_t)));
+  AW_TRY(L.liquid.ensure(L.ncells));
+  AW_TRY(L.x.ensure((size_t)L.ncells * sizeof(float)));
+  AW_TRY(L.b.ensure((size_t)L.ncells * sizeof(float)));
+  for (int a = 0; a < 3; ++a) AW_TRY(L.beta[a].ensure((size_t)L.nfaces * sizeof(float)));
+  AW_HIP(hipMemcpyAsync(L.e.p, L.h_e.data(), L.nb, hipMemcpyHostToDevice, s));
+  AW_HIP(hipMemcpyAsync(L.coff.p, L.h_coff.data(), (L.nb + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  AW_HIP(hipMemcpyAsync(L.foff.p, foff.data(), (L.nb + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  AW_HIP(hipMemcpyAsync(L.nbr.p, nbr.data(), nbr.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  L.d_e = L.e.as<uint8_t>();
+  L.d_coff = L.coff.as<uint32_t>();
+  L.d_foff = L.foff.as<uint32_t>();
+  L.d_nbr = L.nbr.as<int32_t>();
+  L.d_liq = L.liquid.as<uint8_t>();
+  for (int a = 0; a < 3; ++a) L.d_beta[a] = L.beta[a].as<float>();
+  L.d_x = L.x.as<float>();
+  L.d_b = L.b.as<float>();
+  k_init_beta(s, L.view(), L.d_beta[0], L.d_beta[1], L.d_beta[2]);
+  k_fill_f32(s, L.d_x, L.ncells, 0.f);
+  // hierarchy arrays for the host, used by block synchronisation below
+  return AW_OK;
+}
+
+// Hierarchy (replaces VCycleSolver::init, VCycle.cuh:177-249): level 0 is the
+// native grid; block level k has cell edge e_b = min(max(l_b, l_min + k), 3)
+// (the finest blocks coarsen first, no block vanishes); once every block is
+// 1^3, 2x2x2 aggregation halves the dims while every dim is even and >= 4.
+int build_hierarchy(aw_solver* S) {
+  aw_grid* g = S->g;
+  hipStream_t s = g->ctx->stream;
+  std::vector<std::unique_ptr<MGLevel>> lv;
+  {
+    auto L0 = std::make_unique<MGLevel>();
+    for (int a = 0; a < 3; ++a) L0->D[a] = g->D[a];
+    L0->bs = BASE;
+    L0->h_e = g->h_level;
+    AW_TRY(upload_level(s, *L0, false));
+    L0->d_e = g->level.as<uint8_t>();
+    L0->d_coff = g->coff.as<uint32_t>();
+    L0->d_foff = g->foff.as<uint32_t>();
+    L0->d_nbr = g->nbr.as<int32_t>();
+    L0->d_liq = g->liquid.as<uint8_t>();
+    for (int a = 0; a < 3; ++a) L0->d_beta[a] = g->beta[a].as<float>();
+    L0->d_x = g->p.as<float>();
+    L0->d_b = g->rhs.as<float>();
+    lv.push_back(std::move(L0));
+  }
+  int lmin = 3;
+  for (uint8_t l : g->h_level) lmin = std::min(lmin, (int)l);
+  bool all1 = true;
+  for (uint8_t l : g->h_level) all1 = all1 && l == LOGB;
+  for (int k = 1; !all1; ++k) {
+    auto L = std::make_unique<MGLevel>();
+    for (int a = 0; a < 3; ++a) L->D[a] = g->D[a];
+    L->bs = BASE;
+    L->h_e.resize(g->nb);
+    all1 = true;
+    for (int b = 0; b < g->nb; ++b) {
+      L->h_e[b] = (uint8_t)std::min(std::max((int)g->h_level[b], lmin + k), LOGB);
+      all1 = all1 && L->h_e[b] == LOGB;
+    }
+    AW_TRY(upload_level(s, *L, true));
+    lv.push_back(std::move(L));
+  }
+  for (;;) {
+    const MGLevel& P = *lv.back();
+    const int mn = std::min(P.D[0], std::min(P.D[1], P.D[2]));
+    if (mn < 4 || (P.D[0] & 1) || (P.D[1] & 1) || (P.D[2] & 1)) break;
+    if ((int)lv.size() >= AW_MAX_MG_LEVELS) break;
+    auto L = std::make_unique<MGLevel>();
+    for (int a = 0; a < 3; ++a) L->D[a] = P.D[a] / 2;
+    L->bs = P.bs * 2;
+    L->aggregated = 1;
+    L->h_e.assign((size_t)L->D[0] * L->D[1] * L->D[2], (uint8_t)(P.h_e[0] + 1));
+    AW_TRY(upload_level(s, *L, true));
+    lv.push_back(std::move(L));
+  }
+  S->lv.swap(lv);
+  S->built_version = g->topo_version;
+  const uint32_t n0 = S->lv[0]->ncells;
+  AW_TRY(S->r0.ensure((size_t)n0 * sizeof(float)));
+  AW_TRY(S->z.ensure((size_t)n0 * sizeof(float)));
+  AW_TRY(S->pv.ensure((size_t)n0 * sizeof(float)));
+  AW_TRY(S->q.ensure((size_t)n0 * sizeof(float)));
+  return launch_status("build_hierarchy");
+}
+
+void coarsen_masks(aw_solver* S) {
+  hipStream_t s = S->g->ctx->stream;
+  for (size_t k = 0; k + 1 < S->lv.size(); ++k) {
+    const MGLevel& F = *S->lv[k];
+    MGLevel& C = *S->lv[k + 1];
+    k_coarsen_liquid(s, F.view(), C.view(), C.aggregated, C.d_liq, C.wgp(), C.nwg, C.bl());
+  }
+}
+
+int ensure_setup(aw_solver* S) {
+  if (!S->g->have_topology) return fail(AW_ERR_INVALID, "grid has no topology");
+  if (S->built_version != S->g->topo_version) AW_TRY(build_hierarchy(S));
+  coarsen_masks(S);
+  return launch_status("solver setup");
+}
+
+void smooth_level(aw_solver* S, int k, float* x, const float* b, int iters, bool reverse) {
+  const MGLevel& L = *S->lv[k];
+  hipStream_t s = S->g->ctx->stream;
+  const LevelDev V = L.view();
+  for (int it = 0; it < iters; ++it)
+    for (int ph = 0; ph < 2; ++ph) {
+      const int col = reverse ? 1 - ph : ph;
+      const int off = col ? L.nwg0 : 0, cnt = col ? L.nwg - L.nwg0 : L.nwg0;
+      k_smooth(s, V, x, b, L.wgp() + off, cnt, L.bl(), S->prm.omega, S->prm.local_sweeps, reverse ? 1 : 0);
+    }
+}
+
+void coarse_level(aw_solver* S, int k, float* x, const float* b) {
+  const MGLevel& L = *S->lv[k];
+  if (L.ncells <= 512) {
+    k_coarse_solve(S->g->ctx->stream, L.view(), x, b, (int)L.ncells, S->prm.coarse_sweeps, S->prm.omega,
+                   S->prm.local_sweeps);
+  } else {
+    for (int i = 0; i < S->prm.coarse_sweeps; ++i) {
+      smooth_level(S, k, x, b, 1, false);
+      smooth_level(S, k, x, b, 1, true);
+    }
+  }
+}
+
+void residual_level(aw_solver* S, int k, const float* x, const float* b, float* r) {
+  const MGLevel& L = *S->lv[k];
+  k_residual(S->g->ctx->stream, L.view(), x, b, r, L.wgp(), L.nwg, L.bl());
+}
+
+void restrict_level(aw_solver* S, int k, const float* r, float* bc) {
+  const MGLevel& F = *S->lv[k];
+  const MGLevel& C = *S->lv[k + 1];
+  k_restrict(S->g->ctx->stream, F.view(), C.view(), C.aggregated, r, bc, C.wgp(), C.nwg, C.bl());
+}
+
+void prolong_level(aw_solver* S, int k, float* x, const float* xc) {
+  const MGLevel& F = *S->lv[k];
+  const MGLevel& C = *S->lv[k + 1];
+  k_prolong(S->g->ctx->stream, F.view(), C.view(), C.aggregated, x, xc, S->prm.prolong_scale, F.wgp(), F.nwg,
+            F.bl());
+}
+
+// recursive V-cycle (VCycleSolver::runCycle, VCycle.cuh:251-304)
+void vcycle(aw_solver* S, int k, float* x, const float* b) {
+  hipStream_t s = S->g->ctx->stream;
+  if (k + 1 == (int)S->lv.size()) {
+    coarse_level(S, k, x, b);
+    return;
+  }
+  MGLevel& L = *S->lv[k];
+  MGLevel& C = *S->lv[k + 1];
+  smooth_level(S, k, x, b, S->prm.pre_sweeps, false);
+  residual_level(S, k, x, b, L.d_r);
+  restrict_level(S, k, L.d_r, C.d_b);
+  k_fill_f32(s, C.d_x, C.ncells, 0.f);
+  vcycle(S, k + 1, C.d_x, C.d_b);
+  prolong_level(S, k, x, C.d_x);
+  smooth_level(S, k, x, b, S->prm.post_sweeps, true);
+}
+
+int read_scalars(aw_solver* S) {
+  hipStream_t s = S->g->ctx->stream;
+  AW_HIP(hipMemcpyAsync(S->h_sc, S->sc.p, SC_N * sizeof(double), hipMemcpyDeviceToHost, s));
+  AW_HIP(hipStreamSynchronize(s));
+  return launch_status("solver");
+}
+
+int solve(aw_solver* S, double tol, int max_iters, aw_solve_stats* st) {
+  aw_grid* g = S->g;
+  hipStream_t s = g->ctx->stream;
+  MGLevel& L0 = *S->lv[0];
+  const int n = (int)L0.ncells;
+  float* x = g->p.as<float>();
+  const float* b = g->rhs.as<float>();
+  const uint8_t* liq = g->liquid.as<uint8_t>();
+  float* r = S->r0.as<float>();
+  float* z = S->z.as<float>();
+  float* pv = S->pv.as<float>();
+  float* q = S->q.as<float>();
+  double* sc = S->sc.as<double>();
+  const LevelDev V0 = L0.view();
+  k_mask(s, x, liq, n);
+  k_fill_f64(s, sc, SC_N, 0.0);
+  residual_level(S, 0, x, b, r);
+  k_dot(s, b, b, liq, n, sc + SC_BB);
+  k_dot(s, r, r, liq, n, sc + SC_RR);
+  AW_TRY(read_scalars(S));
+  const double bn = std::sqrt(S->h_sc[SC_BB]);
+  double rn = std::sqrt(S->h_sc[SC_RR]);
+  int it = 0;
+  if (st) {
+    st->b_norm = bn;
+    st->outer_iterations = 1;
+  }
+  if (!(bn > 0.0)) {
+    k_fill_f32(s, x, n, 0.f);
+    if (st) {
+      st->iterations = 0;
+      st->rel_residual = 0.0;
+    }
+    return launch_status("solve");
+  }
+  if (!std::isfinite(bn) || !std::isfinite(rn)) return fail(AW_ERR_NAN, "non-finite right-hand side");
+  if (S->prm.method == 1) {
+    while (rn > tol * bn && it < max_iters) {
+      k_fill_f32(s, z, n, 0.f);
+      vcycle(S, 0, z, r);
+      k_add(s, x, z, liq, n);
+      residual_level(S, 0, x, b, r);
+      k_scalar_shift(s, sc);
+      k_dot(s, r, r, liq, n, sc + SC_RR);
+      AW_TRY(read_scalars(S));
+      rn = std::sqrt(S->h_sc[SC_RR]);
+      ++it;
+      if (!std::isfinite(rn)) return fail(AW_ERR_NAN, "V-cycle iteration diverged to non-finite values");
+    }
+  } else {
+    k_scalar_shift(s, sc);
+    k_fill_f32(s, z, n, 0.f);
+    vcycle(S, 0, z, r);
+    k_dot(s, r, z, liq, n, sc + SC_RZ);
+    k_pcg_p(s, z, pv, n, sc, 1);
+    while (rn > tol * bn && it < max_iters) {
+      k_apply_dot(s, V0, pv, q, L0.wgp(), L0.nwg, L0.bl(), sc + SC_PQ);
+      k_pcg_xr(s, x, r, pv, q, liq, n, sc);
+      AW_TRY(read_scalars(S));
+      rn = std::sqrt(S->h_sc[SC_RR]);
+      ++it;
+      if (!std::isfinite(rn)) return fail(AW_ERR_NAN, "PCG produced non-finite residual");
+      if (rn <= tol * bn) break;
+      k_scalar_shift(s, sc);
+      k_fill_f32(s, z, n, 0.f);
+      vcycle(S, 0, z, r);
+      k_dot(s, r, z, liq, n, sc + SC_RZ);
+      k_pcg_p(s, z, pv, n, sc, 0);
+    }
+  }
+  if (st) {
+    st->iterations = it;
+    st->rel_residual = rn / bn;
+  }
+  AW_TRY(launch_status("solve"));
+  return rn <= tol * bn ? AW_OK : fail(AW_ERR_NOT_CONVERGED, "relative residual %.3e after %d iterations", rn / bn, it);
+}
+
+bool valid_params(const aw_solver_params* p) {
+  return p && (p->method == 0 || p->method == 1) && p->pre_sweeps >= 0 && p->post_sweeps >= 0 &&
+         p->coarse_sweeps >= 0 && p->local_sweeps >= 1 && p->omega > 0.f && p->omega < 2.f &&
+         (p->precision == 0 || p->precision == 1);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* aw_status_str(int status) {
+  switch (status) {
+    case AW_OK: return "ok";
+    case AW_ERR_INVALID: return "invalid argument";
+    case AW_ERR_OOM: return "out of device memory";
+    case AW_ERR_DEVICE: return "device error";
+    case AW_ERR_NOT_CONVERGED: return "not converged";
+    case AW_ERR_NAN: return "non-finite values";
+    case AW_ERR_UNSUPPORTED: return "unsupported";
+  }
+  return "unknown status";
+}
+
+const char* aw_last_error(void) { return g_err.c_str(); }
+
+int aw_version(void) { return 1; }
+
+int aw_ctx_create(int device, void* stream, aw_ctx** out) {
+  if (!out) return fail(AW_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    return fail(AW_ERR_DEVICE, "no HIP device available");
+  }
+  if (device < 0 || device >= ndev) return fail(AW_ERR_INVALID, "device %d out of range [0,%d)", device, ndev);
+  AW_HIP(hipSetDevice(device));
+  auto* c = new aw_ctx();
+  c->device = device;
+  if (stream) {
+    c->stream = static_cast<hipStream_t>(stream);
+  } else {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete c;
+      return fail(AW_ERR_DEVICE, "hipStreamCreate failed");
+    }
+    c->own_stream = true;
+  }
+  *out = c;
+  return AW_OK;
+}
+
+int aw_ctx_destroy(aw_ctx* c) {
+  if (!c) return fail(AW_ERR_INVALID, "ctx is NULL");
+  (void)hipStreamSynchronize(c->stream);
+  if (c->own_stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return AW_OK;
+}
+
+int aw_ctx_synchronize(aw_ctx* c) {
+  if (!c) return fail(AW_ERR_INVALID, "ctx is NULL");
+  AW_HIP(hipStreamSynchronize(c->stream));
+  return launch_status("synchronize");
+}
+
+int aw_malloc(aw_ctx* c, uint64_t bytes, void** out) {
+  if (!c || !out) return fail(AW_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  AW_HIP(hipSetDevice(c->device));
+  if (hipMalloc(out, bytes ? bytes : 1) != hipSuccess) {
+    (void)hipGetLastError();
+    *out = nullptr;
+    return fail(AW_ERR_OOM, "hipMalloc of %llu bytes failed", (unsigned long long)bytes);
+  }
+  return AW_OK;
+}
+
+int aw_free(aw_ctx* c, void* p) {
+  if (!c) return fail(AW_ERR_INVALID, "ctx is NULL");
+  if (p) {
+    AW_HIP(hipStreamSynchronize(c->stream));
+    AW_HIP(hipFree(p));
+  }
+  return AW_OK;
+}
+
+int aw_copy(aw_ctx* c, void* dst, const void* src, uint64_t bytes) {
+  if (!c || (bytes && (!dst || !src))) return fail(AW_ERR_INVALID, "NULL argument");
+  if (!bytes) return AW_OK;
+  AW_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, c->stream));
+  AW_HIP(hipStreamSynchronize(c->stream));
+  return launch_status("copy");
+}
+
+int aw_grid_create(aw_ctx* ctx, const aw_grid_desc* d, aw_grid** out) {
+  if (!ctx || !d || !out) return fail(AW_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  if (d->base_res != BASE) return fail(AW_ERR_UNSUPPORTED, "base_res %d (only 8 is supported)", d->base_res);
+  if (d->num_levels < 1 || d->num_levels > AW_MAX_LEVELS)
+    return fail(AW_ERR_INVALID, "num_levels %d outside [1,%d]", d->num_levels, AW_MAX_LEVELS);
+  for (int a = 0; a < 3; ++a) {
+    if (d->dims[a] < 1 || d->dims[a] > 1024) return fail(AW_ERR_INVALID, "dims[%d] = %d", a, d->dims[a]);
+    if (d->domain_min[a] != 0.f) return fail(AW_ERR_UNSUPPORTED, "domain_min must be the origin");
+  }
+  int ex = 0;
+  const float fr = std::frexp(d->h, &ex);
+  if (!(d->h > 0.f) || fr != 0.5f) return fail(AW_ERR_INVALID, "h must be a positive power of two");
+  for (int l = 0; l + 1 < d->num_levels; ++l)
+    if (d->band[l] < 0) return fail(AW_ERR_INVALID, "negative band");
+  const int64_t nb = (int64_t)d->dims[0] * d->dims[1] * d->dims[2];
+  if (nb > (1 << 24)) return fail(AW_ERR_INVALID, "too many blocks");
+  AW_HIP(hipSetDevice(ctx->device));
+  auto g = std::make_unique<aw_grid>();
+  g->ctx = ctx;
+  g->desc = *d;
+  g->nb = (int)nb;
+  for (int a = 0; a < 3; ++a) {
+    g->D[a] = d->dims[a];
+    g->N[a] = d->dims[a] * BASE;
+  }
+  g->h = d->h;
+  g->inv_h = 1.0f / d->h;
+  hipStream_t s = ctx->stream;
+  AW_TRY(g->level.ensure(nb));
+  AW_TRY(g->surf.ensure(nb));
+  AW_TRY(g->ltmp.ensure(nb));
+  AW_TRY(g->count.ensure(nb * sizeof(uint32_t)));
+  AW_TRY(g->poff.ensure((nb + 1) * sizeof(uint32_t)));
+  AW_TRY(g->cursor.ensure(nb * sizeof(uint32_t)));
+  std::vector<int32_t> nbr;
+  neighbour_table(g->D, nbr);
+  AW_TRY(g->nbr.ensure(nbr.size() * sizeof(int32_t)));
+  AW_HIP(hipMemcpyAsync(g->nbr.p, nbr.data(), nbr.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  AW_HIP(hipStreamSynchronize(s));
+  *out = g.release();
+  return AW_OK;
+}
+
+int aw_grid_destroy(aw_grid* g) {
+  if (!g) return fail(AW_ERR_INVALID, "grid is NULL");
+  (void)hipStreamSynchronize(g->ctx->stream);
+  delete g;
+  return AW_OK;
+}
+
+int aw_grid_build_topology(aw_grid* g, const float* d_pos, int64_t n, int* changed) {
+  if (!g || (n > 0 && !d_pos) || n < 0) return fail(AW_ERR_INVALID, "bad arguments");
+  if (n > 0x7FFFFFFF) return fail(AW_ERR_INVALID, "too many particles");
+  AW_TRY(bin_and_levels(g, d_pos, n, true, changed));
+  AW_HIP(hipStreamSynchronize(g->ctx->stream));
+  return launch_status("build_topology");
+}
+
+int aw_grid_set_levels(aw_grid* g, const uint8_t* h_levels) {
+  if (!g || !h_levels) return fail(AW_ERR_INVALID, "NULL argument");
+  for (int b = 0; b < g->nb; ++b)
+    if (h_levels[b] >= g->desc.num_levels) return fail(AW_ERR_INVALID, "level %d at block %d", h_levels[b], b);
+  // 2:1 grading over the 26-neighbourhood is a precondition of the stencil
+  for (int b = 0; b < g->nb; ++b) {
+    int c[3] = {b % g->D[0], (b / g->D[0]) % g->D[1], b / (g->D[0] * g->D[1])};
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          int q[3] = {c[0] + dx, c[1] + dy, c[2] + dz};
+          if (q[0] < 0 || q[1] < 0 || q[2] < 0 || q[0] >= g->D[0] || q[1] >= g->D[1] || q[2] >= g->D[2]) continue;
+          int nb = q[0] + g->D[0] * (q[1] + g->D[1] * q[2]);
+          if (std::abs((int)h_levels[b] - (int)h_levels[nb]) > 1)
+            return fail(AW_ERR_INVALID, "levels violate 2:1 grading at block %d", b);
+        }
+  }
+  hipStream_t s = g->ctx->stream;
+  g->h_level.assign(h_levels, h_levels + g->nb);
+  AW_HIP(hipMemcpyAsync(g->level.p, h_levels, g->nb, hipMemcpyHostToDevice, s));
+  AW_TRY(apply_levels(g));
+  AW_HIP(hipStreamSynchronize(s));
+  return launch_status("set_levels");
+}
+
+int aw_grid_layout(aw_grid* g, aw_layout* o) {
+  if (!g || !o) return fail(AW_ERR_INVALID, "NULL argument");
+  std::memset(o, 0, sizeof *o);
+  o->num_blocks = (uint32_t)g->nb;
+  o->total_cells = g->ncells;
+  for (int a = 0; a < 3; ++a) {
+    o->total_faces[a] = g->nfaces;
+    o->d_face_off[a] = g->foff.as<uint32_t>();
+    o->d_beta[a] = g->beta[a].as<float>();
+    o->d_u_old[a] = g->uold[a].as<float>();
+    o->d_u_star[a] = g->ustar[a].as<float>();
+    o->d_u_new[a] = g->unew[a].as<float>();
+    o->d_mass[a] = g->mass[a].as<float>();
+  }
+  o->d_level = g->level.as<uint8_t>();
+  o->d_cell_off = g->coff.as<uint32_t>();
+  o->d_particle_count = g->count.as<uint32_t>();
+  o->d_particle_off = g->poff.as<uint32_t>();
+  o->d_pressure = g->p.as<float>();
+  o->d_rhs = g->rhs.as<float>();
+  o->d_liquid = g->liquid.as<uint8_t>();
+  o->d_fine = g->slab.as<float>();
+  return AW_OK;
+}
+
+int aw_bin_particles(aw_grid* g, float* d_pos, float* d_vel, int32_t* d_ids, int64_t n) {
+  if (!g || n < 0 || (n > 0 && (!d_pos || !d_vel))) return fail(AW_ERR_INVALID, "bad arguments");
+  if (!g->have_topology) return fail(AW_ERR_INVALID, "grid has no topology");
+  hipStream_t s = g->ctx->stream;
+  AW_TRY(bin_and_levels(g, d_pos, n, false, nullptr));
+  AW_TRY(sort_particles(g, d_pos, d_vel, d_ids, n));
+  if (n > 0) {
+    AW_HIP(hipMemcpyAsync(d_pos, g->spos.p, n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s));
+    AW_HIP(hipMemcpyAsync(d_vel, g->svel.p, n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s));
+    if (d_ids) AW_HIP(hipMemcpyAsync(d_ids, g->sids.p, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+  }
+  return launch_status("bin_particles");
+}
+
+int aw_p2g(aw_grid* g, const float* d_pos, const float* d_vel, int64_t n, float dt, const float gravity[3]) {
+  if (!g || n < 0 || (n > 0 && (!d_pos || !d_vel))) return fail(AW_ERR_INVALID, "bad arguments");
+  if (!g->have_topology || g->n_sorted != n) return fail(AW_ERR_INVALID, "particles not binned for this topology");
+  const float zero[3] = {0.f, 0.f, 0.f};
+  return p2g_sorted(g, d_pos, d_vel, dt, gravity ? gravity : zero);
+}
+
+int aw_divergence(aw_grid* g) {
+  if (!g || !g->have_topology) return fail(AW_ERR_INVALID, "grid has no topology");
+  return divergence(g);
+}
+
+int aw_project_velocity(aw_grid* g) {
+  if (!g || !g->have_topology) return fail(AW_ERR_INVALID, "grid has no topology");
+  return project(g);
+}
+
+int aw_g2p_advect(aw_grid* g, float* d_pos, float* d_vel, int64_t n, float alpha, float dt) {
+  if (!g || n < 0 || (n > 0 && (!d_pos || !d_vel))) return fail(AW_ERR_INVALID, "bad arguments");
+  if (!g->have_topology) return fail(AW_ERR_INVALID, "grid has no topology");
+  k_g2p_advect(g->ctx->stream, g->view(), g->slab.as<float>(), d_pos, d_vel, n, alpha, dt, d_pos, d_vel);
+  return launch_status("g2p_advect");
+}
+
+int aw_solver_create(aw_grid* g, const aw_solver_params* p, aw_solver** out) {
+  if (!g || !out) return fail(AW_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  if (!valid_params(p)) return fail(AW_ERR_INVALID, "invalid solver parameters");
+  if (p->precision != 0) return fail(AW_ERR_UNSUPPORTED, "mixed precision is not available in this build");
+  auto S = std::make_unique<aw_solver>();
+  S->g = g;
+  S->prm = *p;
+  AW_TRY(S->sc.ensure(SC_N * sizeof(double)));
+  AW_HIP(hipHostMalloc((void**)&S->h_sc, SC_N * sizeof(double), hipHostMallocDefault));
+  *out = S.release();
+  return AW_OK;
+}
+
+int aw_solver_destroy(aw_solver* S) {
+  if (!S) return fail(AW_ERR_INVALID, "solver is NULL");
+  (void)hipStreamSynchronize(S->g->ctx->stream);
+  if (S->h_sc) (void)hipHostFree(S->h_sc);
+  delete S;
+  return AW_OK;
+}
+
+int aw_solver_setup(aw_solver* S) {
+  if (!S) return fail(AW_ERR_INVALID, "solver is NULL");
+  return ensure_setup(S);
+}
+
+int aw_solver_level_info(aw_solver* S, int k, aw_level_info* o) {
+  if (!S || !o) return fail(AW_ERR_INVALID, "NULL argument");
+  if (k < 0 || k >= (int)S->lv.size()) return fail(AW_ERR_INVALID, "level %d not built", k);
+  const MGLevel& L = *S->lv[k];
+  std::memset(o, 0, sizeof *o);
+  for (int a = 0; a < 3; ++a) {
+    o->dims[a] = L.D[a];
+    o->total_faces[a] = L.nfaces;
+  }
+  o->num_blocks = (uint32_t)L.nb;
+  o->total_cells = L.ncells;
+  o->aggregated = L.aggregated;
+  o->d_e = L.d_e;
+  o->d_cell_off = L.d_coff;
+  o->d_x = L.d_x;
+  o->d_b = L.d_b;
+  o->d_r = L.d_r;
+  o->d_liquid = L.d_liq;
+  return AW_OK;
+}
+
+static int check_level(aw_solver* S, int k, bool need_next) {
+  if (!S) return fail(AW_ERR_INVALID, "solver is NULL");
+  if (S->built_version != S->g->topo_version) return fail(AW_ERR_INVALID, "solver not set up for this topology");
+  if (k < 0 || k >= (int)S->lv.size() || (need_next && k + 1 >= (int)S->lv.size()))
+    return fail(AW_ERR_INVALID, "level %d out of range", k);
+  return AW_OK;
+}
+
+int aw_solver_smooth(aw_solver* S, int k, int iters, int reverse) {
+  AW_TRY(check_level(S, k, false));
+  if (iters < 0) return fail(AW_ERR_INVALID, "iterations < 0");
+  MGLevel& L = *S->lv[k];
+  smooth_level(S, k, L.d_x, L.d_b, iters, reverse != 0);
+  return launch_status("smooth");
+}
+
+int aw_solver_residual(aw_solver* S, int k) {
+  AW_TRY(check_level(S, k, false));
+  MGLevel& L = *S->lv[k];
+  residual_level(S, k, L.d_x, L.d_b, L.d_r);
+  return launch_status("residual");
+}
+
+int aw_solver_apply(aw_solver* S, int k, const float* d_x, float* d_y) {
+  AW_TRY(check_level(S, k, false));
+  if (!d_x || !d_y) return fail(AW_ERR_INVALID, "NULL vector");
+  MGLevel& L = *S->lv[k];
+  k_apply_dot(S->g->ctx->stream, L.view(), d_x, d_y, L.wgp(), L.nwg, L.bl(), nullptr);
+  return launch_status("apply");
+}
+
+int aw_solver_restrict(aw_solver* S, int k) {
+  AW_TRY(check_level(S, k, true));
+  restrict_level(S, k, S->lv[k]->d_r, S->lv[k + 1]->d_b);
+  return launch_status("restrict");
+}
+
+int aw_solver_prolong(aw_solver* S, int k) {
+  AW_TRY(check_level(S, k, true));
+  prolong_level(S, k, S->lv[k]->d_x, S->lv[k + 1]->d_x);
+  return launch_status("prolong");
+}
+
+int aw_solver_vcycle(aw_solver* S, int k) {
+  AW_TRY(check_level(S, k, false));
+  MGLevel& L = *S->lv[k];
+  vcycle(S, k, L.d_x, L.d_b);
+  return launch_status("vcycle");
+}
+
+int aw_solver_solve(aw_solver* S, double tol, int max_iters, aw_solve_stats* st) {
+  if (!S) return fail(AW_ERR_INVALID, "solver is NULL");
+  if (!(tol > 0.0) || max_iters < 0) return fail(AW_ERR_INVALID, "bad tolerance or iteration cap");
+  AW_TRY(ensure_setup(S));
+  return solve(S, tol, max_iters, st);
+}
+
+int aw_step(aw_grid* g, aw_solver* S, float* d_pos, float* d_vel, int32_t* d_ids, int64_t n,
+            const aw_step_params* prm, aw_step_stats* stats) {
+  if (!g || !S || !prm || S->g != g || n < 0 || (n > 0 && (!d_pos || !d_vel)))
+    return fail(AW_ERR_INVALID, "bad arguments");
+  if (n > 0x7FFFFFFF) return fail(AW_ERR_INVALID, "too many particles");
+  hipStream_t s = g->ctx->stream;
+  int changed = 0;
+  if (prm->rebuild_topology || !g->have_topology) {
+    AW_TRY(bin_and_levels(g, d_pos, n, true, &changed));
+  } else {
+    AW_TRY(bin_and_levels(g, d_pos, n, false, nullptr));
+  }
+  AW_TRY(sort_particles(g, d_pos, d_vel, d_ids, n));
+  AW_TRY(p2g_sorted(g, g->spos.as<float>(), g->svel.as<float>(), prm->dt, prm->gravity));
+  AW_TRY(divergence(g));
+  AW_TRY(ensure_setup(S));
+  k_fill_f32(s, g->p.as<float>(), g->ncells, 0.f);
+  aw_solve_stats sst{};
+  int st = solve(S, prm->rel_tol, prm->max_iters, &sst);
+  if (st != AW_OK && st != AW_ERR_NOT_CONVERGED) return st;
+  std::string keep = g_err;
+  AW_TRY(project(g));
+  k_g2p_advect(s, g->view(), g->slab.as<float>(), g->spos.as<float>(), g->svel.as<float>(), n, prm->flip_alpha,
+               prm->dt, d_pos, d_vel);
+  if (d_ids && n > 0) AW_HIP(hipMemcpyAsync(d_ids, g->sids.p, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+  AW_HIP(hipStreamSynchronize(s));
+  AW_TRY(launch_status("step"));
+  if (stats) {
+    std::memset(stats, 0, sizeof *stats);
+    stats->topology_changed = changed;
+    stats->solve = sst;
+    for (uint8_t l : g->h_level) stats->num_blocks_by_level[l]++;
+    stats->total_cells = g->ncells;
+  }
+  if (st != AW_OK) g_err = keep;
+  return st;
+}
+
+}  // extern "C"

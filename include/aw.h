@@ -1,0 +1,197 @@
+/*
+ * aw.h — C ABI of the MI355X MSBG FLIP step (anitowave_adaptive_amd).
+ *
+ * Plain pointers and sizes only.  Every pointer argument named d_* or described
+ * as "device" is HIP device memory; work is enqueued on the context's stream
+ * and is asynchronous unless the function says it synchronises.  Handles are
+ * not thread-safe: one host thread per context, as the reference's classes.
+ * The caller owns particle buffers; the library owns grid and solver memory.
+ *
+ * Reference interfaces each entry point replaces (paths relative to the
+ * reference checkout):
+ *   aw_grid_create / aw_grid_destroy     MSBGManager ctor / cleanup          include/MSBG.cuh:506-597
+ *   aw_grid_build_topology               initRefinementMap, markActiveBlocks, enforceGrading,
+ *                                        countBlockElements, exclusive_scan, allocateChannels
+ *                                                                            src/main.cu:156-302, include/MSBG.cuh:249-376,599-659
+ *   aw_grid_layout                       MSBG POD fields                     include/MSBG.cuh:71-106
+ *   aw_p2g                               P2GTransfer::run                    include/P2G.cuh:173-196
+ *   aw_divergence, aw_project_velocity   (no counterpart: projection is missing upstream)
+ *   aw_g2p_advect                        G2PManager::performG2P + advect     include/G2P.cuh:198-227
+ *   aw_solver_create / _destroy          VCycleSolver ctor / init / dtor     include/VCycle.cuh:161-249
+ *   aw_solver_vcycle                     VCycleSolver::runCycle              include/VCycle.cuh:251-304
+ *   aw_solver_smooth                     AdaptiveSmoother::solveLevel        include/Smoother.cuh:249-291
+ *   aw_solver_residual                   computeExplicitResidual             include/VCycle.cuh:81-131
+ *   aw_solver_restrict / _prolong        MultigridTransfer::restrictResiduals / prolongateAndCorrect
+ *                                                                            include/MultigridTransfer.cuh:107-149
+ *   aw_solver_solve                      main.cu's fixed V-cycle loop        src/main.cu:314-317
+ *   aw_step                              the main.cu step sequence, completed with the projection
+ */
+#ifndef ANITOWAVE_AW_H
+#define ANITOWAVE_AW_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  AW_OK = 0,
+  AW_ERR_INVALID = 1,
+  AW_ERR_OOM = 2,
+  AW_ERR_DEVICE = 3,
+  AW_ERR_NOT_CONVERGED = 4,
+  AW_ERR_NAN = 5,
+  AW_ERR_UNSUPPORTED = 6
+} aw_status;
+
+#define AW_MAX_LEVELS 4     /* native levels: cells per block edge 8, 4, 2, 1 */
+#define AW_MAX_MG_LEVELS 12
+
+typedef struct aw_ctx aw_ctx;
+typedef struct aw_grid aw_grid;
+typedef struct aw_solver aw_solver;
+
+/* Grid: dims[] blocks of base_res^3 cells at native level 0 (world size
+ * base_res*h), num_levels native levels, distance bands band[l] in blocks. */
+typedef struct {
+  int dims[3];
+  int base_res;        /* must be 8 */
+  int num_levels;      /* 1..AW_MAX_LEVELS */
+  float h;             /* finest cell size; a power of two */
+  float domain_min[3]; /* must be 0,0,0 */
+  int band[AW_MAX_LEVELS];
+} aw_grid_desc;
+
+typedef struct {
+  uint32_t num_blocks;
+  uint32_t total_cells;
+  uint32_t total_faces[3];
+  int num_mg_levels;
+  const uint8_t* d_level;       /* native level per block */
+  const uint32_t* d_cell_off;   /* num_blocks+1 */
+  const uint32_t* d_face_off[3];
+  const uint32_t* d_particle_count; /* particles per block from the last binning */
+  const uint32_t* d_particle_off;   /* num_blocks+1 */
+  float* d_pressure;            /* solver units: p * dt / (rho h) */
+  float* d_rhs;                 /* -divergence of u* (h^2 velocity) */
+  uint8_t* d_liquid;            /* 1 = liquid cell */
+  float* d_beta[3];
+  float* d_u_old[3];            /* P2G velocity */
+  float* d_u_star[3];           /* + body force */
+  float* d_u_new[3];            /* after projection */
+  float* d_mass[3];             /* folded particle weight per face */
+  float* d_fine;                /* virtual fine lattice: [block][field 0 new, 1 delta][axis][512] */
+} aw_layout;
+
+typedef struct {
+  int method;          /* 0 = conjugate gradients preconditioned by one V-cycle; 1 = stationary V-cycles */
+  int pre_sweeps;      /* V(pre, post) */
+  int post_sweeps;
+  int coarse_sweeps;   /* symmetric sweeps on the coarsest level */
+  int local_sweeps;    /* in-block red-black sweeps per block phase */
+  float omega;         /* Gauss-Seidel weight (reference 0.85714) */
+  float prolong_scale; /* coarse-correction scale (1 for PCG, 0.75 recommended for stationary) */
+  int precision;       /* 0 = FP32; 1 = FP32 V-cycles inside FP64 iterative refinement */
+} aw_solver_params;
+
+typedef struct {
+  int iterations;      /* PCG iterations or V-cycles (outer+inner for mixed precision) */
+  int outer_iterations;
+  double rel_residual; /* ||b - A p||_2 / ||b||_2 as tracked by the solver */
+  double b_norm;
+} aw_solve_stats;
+
+typedef struct {
+  float dt;
+  float gravity[3];
+  float flip_alpha;
+  double rel_tol;
+  int max_iters;
+  int rebuild_topology; /* 1 = mark/grade from the particles every step */
+} aw_step_params;
+
+typedef struct {
+  int topology_changed;
+  aw_solve_stats solve;
+  uint32_t num_blocks_by_level[AW_MAX_LEVELS];
+  uint32_t total_cells;
+  uint32_t liquid_cells;
+} aw_step_stats;
+
+typedef struct {
+  int dims[3];
+  uint32_t num_blocks;
+  uint32_t total_cells;
+  uint32_t total_faces[3];
+  int aggregated;          /* 1 if built from the previous level by 2x2x2 block aggregation */
+  const uint8_t* d_e;      /* log2 cell edge in fine cells, per block */
+  const uint32_t* d_cell_off;
+  float* d_x;
+  float* d_b;
+  float* d_r;
+  uint8_t* d_liquid;
+} aw_level_info;
+
+const char* aw_status_str(int status);
+const char* aw_last_error(void);     /* thread-local detail of the last failure */
+int aw_version(void);
+
+int aw_ctx_create(int device, void* stream, aw_ctx** out);
+int aw_ctx_destroy(aw_ctx* ctx);
+int aw_ctx_synchronize(aw_ctx* ctx);
+/* Plumbing for hosts without a device allocator of their own (no reference
+ * counterpart): device allocation and a synchronous copy in any direction. */
+int aw_malloc(aw_ctx* ctx, uint64_t bytes, void** out);
+int aw_free(aw_ctx* ctx, void* p);
+int aw_copy(aw_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+
+int aw_grid_create(aw_ctx* ctx, const aw_grid_desc* desc, aw_grid** out);
+int aw_grid_destroy(aw_grid* grid);
+/* Bins d_pos (AoS float3, n) to blocks, marks levels (distance bands + 2:1
+ * grading), scans offsets and (re)allocates the channels.  Synchronises.
+ * *changed (may be NULL) is set to 1 when the level map differs from the
+ * previous one. */
+int aw_grid_build_topology(aw_grid* grid, const float* d_pos, int64_t n, int* changed);
+/* Explicit native levels (host array, num_blocks) instead of the band rule. */
+int aw_grid_set_levels(aw_grid* grid, const uint8_t* h_levels);
+int aw_grid_layout(aw_grid* grid, aw_layout* out);
+/* Counting sort of particles by block into the caller's buffers (positions,
+ * velocities and optional ids permuted together).  d_scratch_* are library
+ * buffers; the sorted order is written back to d_pos/d_vel/d_ids. */
+int aw_bin_particles(aw_grid* grid, float* d_pos, float* d_vel, int32_t* d_ids, int64_t n);
+
+/* P2G of block-sorted particles: virtual-fine deposit, Pi^T, fold, normalise
+ * (u_old), body force (u_star), liquid mask.  Particles must be sorted by
+ * aw_bin_particles for the current topology. */
+int aw_p2g(aw_grid* grid, const float* d_pos, const float* d_vel, int64_t n, float dt, const float gravity[3]);
+int aw_divergence(aw_grid* grid);                 /* rhs = -D u_star on liquid cells */
+int aw_project_velocity(aw_grid* grid);           /* u_new = u_star - beta grad p; fine lattice */
+int aw_g2p_advect(aw_grid* grid, float* d_pos, float* d_vel, int64_t n, float alpha, float dt);
+
+int aw_solver_create(aw_grid* grid, const aw_solver_params* params, aw_solver** out);
+int aw_solver_destroy(aw_solver* s);
+/* Builds the multigrid hierarchy for the grid's current topology and liquid
+ * mask (coarse masks, offsets, neighbour tables, work lists). */
+int aw_solver_setup(aw_solver* s);
+int aw_solver_level_info(aw_solver* s, int level, aw_level_info* out);
+int aw_solver_smooth(aw_solver* s, int level, int iterations, int reverse);
+int aw_solver_residual(aw_solver* s, int level);  /* r = b - A x on the level */
+int aw_solver_apply(aw_solver* s, int level, const float* d_x, float* d_y);
+int aw_solver_restrict(aw_solver* s, int level);  /* b[level+1] = R r[level] */
+int aw_solver_prolong(aw_solver* s, int level);   /* x[level] += scale * P x[level+1] */
+int aw_solver_vcycle(aw_solver* s, int level);    /* one V-cycle rooted at level on its x, b */
+/* Solves A p = rhs on level 0 into the grid's pressure (initial guess: its
+ * current content).  Synchronises. */
+int aw_solver_solve(aw_solver* s, double rel_tol, int max_iters, aw_solve_stats* stats);
+
+/* One FLIP step on caller-owned AoS float3 particles (positions, velocities,
+ * optional int32 ids permuted with them).  Synchronises. */
+int aw_step(aw_grid* grid, aw_solver* s, float* d_pos, float* d_vel, int32_t* d_ids, int64_t n,
+            const aw_step_params* params, aw_step_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ANITOWAVE_AW_H */
