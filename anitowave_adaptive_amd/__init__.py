@@ -79,6 +79,8 @@ SIGNATURES = {
     "aw_malloc": (_I, [_V, ctypes.c_uint64, _P(_V)]),
     "aw_free": (_I, [_V, _V]),
     "aw_copy": (_I, [_V, _V, _V, ctypes.c_uint64]),
+    "aw_profile_enable": (_I, [_V, _I]),
+    "aw_profile_report": (_I, [_V, ctypes.c_char_p, ctypes.c_uint64]),
     "aw_grid_create": (_I, [_V, _P(GridDesc), _P(_V)]),
     "aw_grid_destroy": (_I, [_V]),
     "aw_grid_build_topology": (_I, [_V, _V, _I64, _P(_I)]),
@@ -146,6 +148,15 @@ class Context:
 
     def synchronize(self):
         check(lib().aw_ctx_synchronize(self._h), "aw_ctx_synchronize")
+
+    def profile(self, on=True):
+        check(lib().aw_profile_enable(self._h, int(bool(on))), "aw_profile_enable")
+
+    def profile_report(self):
+        import json
+        buf = ctypes.create_string_buffer(1 << 16)
+        check(lib().aw_profile_report(self._h, buf, len(buf)), "aw_profile_report")
+        return json.loads(buf.value.decode())
 
     def close(self):
         if self._h:

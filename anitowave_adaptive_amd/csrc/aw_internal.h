@@ -45,6 +45,58 @@ struct WG {
   int start, count, lr, pad;
 };
 
+// Coarse correction fused into a smoothing phase: x += scale * xc[parent]
+// on the phase's own cells (own) and on values read from neighbour blocks
+// that have not been corrected yet (nbr).
+struct Pro {
+  const float* xc;
+  LevelDev C;
+  int aggregated;
+  float scale;
+  int own, nbr;
+};
+
+// Restriction fused into the residual pass (coarse level with the same block
+// dims): b_c = sum of children, x_c = 0.
+struct Restr {
+  LevelDev C;
+  float* bc;
+  float* xc;
+  int on;
+};
+
+// Trailing multigrid levels run by one work-group with x, b, r in LDS.
+constexpr int TAIL_T = 1024;
+constexpr int TAIL_MAX_LEVELS = 8;
+constexpr size_t TAIL_LDS_BYTES = 64 * 1024;
+// one work-group evaluates every cell of a tail level per phase: beyond about
+// a thousand cells separate launches are faster
+constexpr uint32_t TAIL_CELLS_MAX = 1024;
+
+struct TailLevel {
+  LevelDev L;
+  const uint32_t* cblk;  // block of each cell
+  int ncells;
+  int agg;               // built from the previous tail level by 2x2x2 aggregation
+  int off;               // LDS offset (floats) of x; b at off + ncells, r at off + 2 ncells
+  int loff;              // byte offset of the liquid mask after the float arrays
+  int ulr;               // log2 block resolution when every block has it (arithmetic stencil), else -1
+  int ue;                // log2 cell edge of such a uniform level
+};
+
+struct TailArgs {
+  TailLevel lv[TAIL_MAX_LEVELS];
+  int n;
+  int fl;                // floats before the liquid masks
+  float omega, scale;
+  int pre, post, coarse_sweeps, local_sweeps;
+  float* x0;
+  const float* b0;
+  const double* cinv;    // FP64 inverse of the coarsest level, or null (sweeps)
+};
+
+constexpr int DIRECT_MAX = 64;  // coarsest levels up to this many cells are solved exactly
+
 enum Scalar { SC_RZ = 0, SC_RZOLD = 1, SC_PQ = 2, SC_RR = 3, SC_BB = 4, SC_N = 8 };
 
 // ---- launchers (aw_kernels.hip) ----
@@ -79,19 +131,19 @@ void k_g2p_advect(hipStream_t s, const GridDev& g, const float* fine, const floa
 
 // solver
 void k_smooth(hipStream_t s, const LevelDev& L, float* x, const float* b, const WG* wg, int nwg,
-              const uint32_t* blist, float omega, int local_sweeps, int reverse);
+              const uint32_t* blist, float omega, int local_sweeps, int reverse, const Pro& pro);
 void k_residual(hipStream_t s, const LevelDev& L, const float* x, const float* b, float* r, const WG* wg, int nwg,
-                const uint32_t* blist);
+                const uint32_t* blist, const Restr& rs);
 void k_apply_dot(hipStream_t s, const LevelDev& L, const float* x, float* y, const WG* wg, int nwg,
                  const uint32_t* blist, double* acc);
 void k_restrict(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, const float* rf, float* bc,
-                const WG* wgc, int nwgc, const uint32_t* blistc);
+                float* xc, const WG* wgc, int nwgc, const uint32_t* blistc);
 void k_prolong(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, float* xf, const float* xc,
                float scale, const WG* wgf, int nwgf, const uint32_t* blistf);
 void k_coarsen_liquid(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, uint8_t* liquid_c,
                       const WG* wgc, int nwgc, const uint32_t* blistc);
-void k_coarse_solve(hipStream_t s, const LevelDev& L, float* x, const float* b, int ncells, int sweeps,
-                    float omega, int local_sweeps);
+void k_tail(hipStream_t s, const TailArgs& args, size_t lds_bytes);
+void k_coarse_factor(hipStream_t s, const LevelDev& L, int n, double* inv);
 
 // PCG vector work (scalars live in device memory, see Scalar)
 void k_dot(hipStream_t s, const float* a, const float* b, const uint8_t* liquid, int n, double* acc);

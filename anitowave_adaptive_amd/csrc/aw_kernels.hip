@@ -115,21 +115,48 @@ inline int grid_for(size_t n) {
 }
 
 // ------------------------------------------------------------- topology
+// Runs of equal keys in consecutive lanes of a wave share one atomic: the
+// run's first lane adds the run length, every lane gets base + rank.  Input
+// that is already grouped by key (particles sorted by block in the previous
+// step) then issues about one atomic per wave instead of 64 on one address.
+// All lanes of the wave must call it; inactive lanes pass active = false.
+__device__ __forceinline__ uint32_t run_atomic_add(uint32_t* counters, int key, bool active, bool want_slot) {
+  const int lane = threadIdx.x & 63;
+  const int k = active ? key : -1;
+  const int prev = __shfl_up(k, 1, 64);
+  const bool head = active && (lane == 0 || prev != k);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long act = __ballot(active);
+  const unsigned long long le = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const unsigned long long my_heads = heads & le;
+  const int start = my_heads ? 63 - __clzll(my_heads) : lane;
+  const unsigned long long stop = (heads | ~act) & ~le;
+  const int end = stop ? __ffsll(stop) - 1 : 64;
+  uint32_t base = 0;
+  if (head) base = atomicAdd(&counters[k], (uint32_t)(end - start));
+  if (!want_slot) return 0;
+  base = __shfl(base, start, 64);
+  return base + (uint32_t)(lane - start);
+}
+
 // Block of each particle from its exact fine cell (worldToBlockCoord,
 // MSBG.cuh:151-159) and the per-block particle count.
-__global__ void bin_kernel(GridDev g, const float* __restrict__ pos, int64_t n, int32_t* __restrict__ key,
-                           uint32_t* __restrict__ count) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int c[3];
+__global__ __launch_bounds__(TPB) void bin_kernel(GridDev g, const float* __restrict__ pos, int64_t n,
+                                                  int32_t* __restrict__ key, uint32_t* __restrict__ count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = i < n;
+  int b = 0;
+  if (active) {
+    int c[3];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    float u = fminf(fmaxf(pos[3 * i + a] * g.inv_h, 0.f), (float)g.N[a] - 0.0009765625f);
-    c[a] = ((int)u) >> LOGB;
+    for (int a = 0; a < 3; ++a) {
+      float u = fminf(fmaxf(pos[3 * i + a] * g.inv_h, 0.f), (float)g.N[a] - 0.0009765625f);
+      c[a] = ((int)u) >> LOGB;
+    }
+    b = block_id(g.D, c);
+    key[i] = b;
   }
-  int b = block_id(g.D, c);
-  key[i] = b;
-  atomicAdd(&count[b], 1u);
+  run_atomic_add(count, b, active, false);
 }
 
 // occupied block with an unoccupied in-domain 26-neighbour
@@ -247,13 +274,16 @@ __global__ __launch_bounds__(SCAN_T) void scan_kernel(const uint32_t* __restrict
   if (t == 0) out[n] = carry;
 }
 
-__global__ void scatter_kernel(const float* __restrict__ pos, const float* __restrict__ vel,
-                               const int32_t* __restrict__ ids, const int32_t* __restrict__ key,
-                               uint32_t* __restrict__ cursor, int64_t n, float* __restrict__ spos,
-                               float* __restrict__ svel, int32_t* __restrict__ sids) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint32_t s = atomicAdd(&cursor[key[i]], 1u);
+__global__ __launch_bounds__(TPB) void scatter_kernel(const float* __restrict__ pos, const float* __restrict__ vel,
+                                                      const int32_t* __restrict__ ids,
+                                                      const int32_t* __restrict__ key, uint32_t* __restrict__ cursor,
+                                                      int64_t n, float* __restrict__ spos, float* __restrict__ svel,
+                                                      int32_t* __restrict__ sids) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = i < n;
+  const int k = active ? key[i] : 0;
+  const uint32_t s = run_atomic_add(cursor, k, active, true);
+  if (!active) return;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     spos[3 * (int64_t)s + a] = pos[3 * i + a];
@@ -280,84 +310,205 @@ __global__ void init_beta_kernel(LevelDev L, float* __restrict__ bx, float* __re
 }
 
 // ------------------------------------------------------------- P2G
-// One work-group per block: deposit the block's particles on the padded
-// virtual fine lattice in LDS (trilinear MAC weights; the reference's
-// per-particle global atomics, P2G.cuh:16-135, become LDS atomics), count
-// particles per native cell, then store the padded lattice to the block's
-// slab with plain stores.  No global atomics, no cross-block races.
-__global__ __launch_bounds__(TPB) void p2g_deposit_kernel(GridDev g, const float* __restrict__ spos,
+// One work-group per block, without float atomics: particles of the block are
+// counting-sorted by half-cell (16^3 bins of h/2) in LDS, chunk by chunk, and
+// every padded fine node of the block gathers the particles of the 4x4x4
+// half-cells that cover its trilinear support (P2G.cuh:16-135 scatters with
+// global float atomics instead).  The node weights are the same hat products
+// the scatter form uses, so the result is identical up to summation order.
+constexpr int DEP_CH = 1024;          // particles per chunk
+constexpr int DEP_T = 512;            // threads per work-group
+constexpr int DEP_NODES = 3 * PAD;    // padded nodes of all three axes
+
+__device__ __forceinline__ float hat(float d) { return fmaxf(0.f, 1.f - fabsf(d)); }
+
+__global__ __launch_bounds__(DEP_T) void p2g_deposit_kernel(GridDev g, const float* __restrict__ spos,
                                                           const float* __restrict__ svel,
                                                           const uint32_t* __restrict__ pcount,
                                                           const uint32_t* __restrict__ poff, float* __restrict__ slab,
                                                           uint8_t* __restrict__ liquid) {
-  __shared__ float acc[SLAB];
+  __shared__ uint32_t off[4097];
+  __shared__ float su[3][DEP_CH];
+  __shared__ float sv[3][DEP_CH];
   __shared__ uint32_t ccnt[512];
+  __shared__ uint32_t wsum[DEP_T / 64];
+  __shared__ float acc[2 * DEP_NODES];   // node sums (M, m), each node owned by one thread
+  __shared__ int zrange[2];
   const int B = blockIdx.x, tid = threadIdx.x;
   const uint32_t cnt = pcount[B];
   const int e = g.level[B], r = BASE >> e, nc = r * r * r;
   const uint32_t co = g.coff[B];
   if (cnt == 0) {
-    for (int k = tid; k < nc; k += TPB) liquid[co + k] = 0;
+    for (int k = tid; k < nc; k += DEP_T) liquid[co + k] = 0;
     return;
   }
-  for (int k = tid; k < SLAB; k += TPB) acc[k] = 0.f;
-  for (int k = tid; k < 512; k += TPB) ccnt[k] = 0;
-  __syncthreads();
   int Bc[3];
   block_coords(g.D, B, Bc);
+  for (int k = tid; k < 2 * DEP_NODES; k += DEP_T) acc[k] = 0.f;
+  for (int k = tid; k < 512; k += DEP_T) ccnt[k] = 0;
   const uint32_t p0 = poff[B];
-  for (uint32_t p = p0 + tid; p < p0 + cnt; p += TPB) {
-    float u[3], v[3];
-    int gc[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      u[a] = fminf(fmaxf(spos[3 * (size_t)p + a] * g.inv_h, 0.f), (float)g.N[a] - 0.0009765625f);
-      v[a] = svel[3 * (size_t)p + a];
-      gc[a] = (int)u[a];
+  for (uint32_t c0 = 0; c0 < cnt; c0 += DEP_CH) {
+    const int m = (int)min((uint32_t)DEP_CH, cnt - c0);
+    for (int k = tid; k < 4097; k += DEP_T) off[k] = 0;
+    if (tid == 0) {
+      zrange[0] = 16;
+      zrange[1] = -1;
     }
-    // a particle outside this block (unsorted input) is skipped: it would
-    // index outside the block's padded lattice
-    if ((gc[0] >> LOGB) != Bc[0] || (gc[1] >> LOGB) != Bc[1] || (gc[2] >> LOGB) != Bc[2]) continue;
-    int lc[3] = {(gc[0] - BASE * Bc[0]) >> e, (gc[1] - BASE * Bc[1]) >> e, (gc[2] - BASE * Bc[2]) >> e};
-    atomicAdd(&ccnt[cell_idx(lc, r)], 1u);
+    __syncthreads();
+    // 1. bin the chunk's particles by half-cell; the returned count is the rank
+    constexpr int PPT = DEP_CH / DEP_T;
+    float u[PPT][3], v[PPT][3];
+    int hb[PPT];
+    uint32_t rk[PPT];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      int i0[3];
-      float t[3];
+    for (int j = 0; j < PPT; ++j) {
+      const int q = tid + j * DEP_T;
+      hb[j] = -1;
+      if (q >= m) continue;
+      const size_t p = p0 + c0 + q;
+      int h[3];
+      bool inside = true;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const float ua = fminf(fmaxf(spos[3 * p + a] * g.inv_h, 0.f), (float)g.N[a] - 0.0009765625f);
+        u[j][a] = ua - (float)(BASE * Bc[a]);
+        v[j][a] = svel[3 * p + a];
+        h[a] = (int)(2.f * u[j][a]);
+        inside = inside && ((int)ua >> LOGB) == Bc[a];
+      }
+      // a particle outside this block (unsorted input) is skipped
+      if (!inside) continue;
+      hb[j] = h[0] + 16 * (h[1] + 16 * h[2]);
+      rk[j] = atomicAdd(&off[hb[j]], 1u);
+      atomicMin(&zrange[0], h[2]);
+      atomicMax(&zrange[1], h[2]);
+    }
+    __syncthreads();
+    const int zlo = zrange[0], zhi = zrange[1];
+    // 2. fine-cell particle counts (for the liquid mask), then an exclusive
+    //    scan of the 4096 bin counts in place
+    for (int c = tid; c < 512; c += DEP_T) {
+      const int x = c & 7, y = (c >> 3) & 7, z = c >> 6;
+      uint32_t s = 0;
+      for (int d = 0; d < 8; ++d) s += off[(2 * x + (d & 1)) + 16 * ((2 * y + ((d >> 1) & 1)) + 16 * (2 * z + (d >> 2)))];
+      ccnt[c] += s;
+    }
+    {
+      constexpr int BPT = 4096 / DEP_T;   // bins per thread
+      uint32_t loc[BPT], t = 0;
+#pragma unroll
+      for (int k = 0; k < BPT; ++k) {
+        loc[k] = off[tid * BPT + k];
+        t += loc[k];
+      }
+      // wave-level inclusive scan of the per-thread totals
+      uint32_t incl = t;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(incl, d, 64);
+        if ((tid & 63) >= d) incl += y;
+      }
+      if ((tid & 63) == 63) wsum[tid >> 6] = incl;
+      __syncthreads();
+      uint32_t base = incl - t;
+      for (int w = 0; w < (tid >> 6); ++w) base += wsum[w];
+#pragma unroll
+      for (int k = 0; k < BPT; ++k) {
+        off[tid * BPT + k] = base;
+        base += loc[k];
+      }
+      if (tid == DEP_T - 1) off[4096] = base;
+    }
+    __syncthreads();
+    // 3. place the particles in half-cell order
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      if (hb[j] < 0) continue;
+      const uint32_t sl = off[hb[j]] + rk[j];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        su[a][sl] = u[j][a];
+        sv[a][sl] = v[j][a];
+      }
+    }
+    __syncthreads();
+    // 4. node gather.  The half-cell (hx, hy, hz) of a particle fixes its
+    //    stencil base on every axis, so for a node the weight factor of each
+    //    axis is the same over whole half-cell ranges: t or 1 - t of the
+    //    particle's fractional coordinate (t = frac(u) on the face-normal
+    //    axis, frac(u - 1/2) on the others), or 1 for the outermost half-cell
+    //    of a node clamped at a tank wall.  A node's support is 4 half-cells
+    //    per axis: the lower two weigh t, the upper two 1 - t.
+    for (int k = tid; k < DEP_NODES; k += DEP_T) {
+      const int a = k / PAD, pi = k - a * PAD;
+      const int n0 = a == 0 ? 9 : 10, n1 = a == 1 ? 9 : 10;
+      int l[3] = {pi % n0, (pi / n0) % n1, pi / (n0 * n1)};
+      int h0[3];            // first half-cell of the support (may be < 0)
+      float ca[3][2], cb[3][2];  // weight = ca + cb * t for the lower / upper pair
+      bool dead = false;
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        const float c = d == a ? u[d] : u[d] - 0.5f;
-        const float fl = floorf(c);
-        i0[d] = (int)fl;
-        t[d] = c - fl;
-      }
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const int dd[3] = {k & 1, (k >> 1) & 1, k >> 2};
-        int l[3];
-        bool skip = false;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          int gn = i0[d] + dd[d];
-          if (d == a) {
-            if (gn >= g.N[d]) skip = true;
-          } else {
-            gn = iclamp(gn, 0, g.N[d] - 1);
-          }
-          l[d] = gn - BASE * Bc[d];
+        if (d != a) l[d] -= 1;
+        const int gn = BASE * Bc[d] + l[d];
+        const bool normal = d == a;
+        if (normal ? gn >= g.N[d] : (gn < 0 || gn >= g.N[d])) dead = true;
+        h0[d] = normal ? 2 * l[d] - 2 : 2 * l[d] - 1;
+        ca[d][0] = 0.f;  cb[d][0] = 1.f;   // lower pair: t
+        ca[d][1] = 1.f;  cb[d][1] = -1.f;  // upper pair: 1 - t
+        if (!normal && gn == 0) {          // clamped node below: outermost half-cell weighs 1
+          ca[d][0] = 1.f;  cb[d][0] = 0.f;
         }
-        if (skip) continue;
-        const float w = (dd[0] ? t[0] : 1.f - t[0]) * (dd[1] ? t[1] : 1.f - t[1]) * (dd[2] ? t[2] : 1.f - t[2]);
-        const int pi = a * PAD + pad_idx(a, l);
-        atomicAdd(&acc[pi], w * v[a]);
-        atomicAdd(&acc[3 * PAD + pi], w);
+        if (!normal && gn == g.N[d] - 1) {
+          ca[d][1] = 1.f;  cb[d][1] = 0.f;
+        }
       }
+      if (dead) continue;
+      // the chunk's particles cover half-cell z in [zlo, zhi]
+      if (h0[2] + 3 < zlo || h0[2] > zhi) continue;
+      float sM = 0.f, sm = 0.f;
+      const float sh0 = a == 0 ? 0.f : 0.5f, sh1 = a == 1 ? 0.f : 0.5f, sh2 = a == 2 ? 0.f : 0.5f;
+      for (int dz = 0; dz < 4; ++dz) {
+        const int hz = h0[2] + dz;
+        if (hz < 0 || hz > 15) continue;
+        const float za = ca[2][dz >> 1], zb = cb[2][dz >> 1];
+        for (int dy = 0; dy < 4; ++dy) {
+          const int hy = h0[1] + dy;
+          if (hy < 0 || hy > 15) continue;
+          const float ya = ca[1][dy >> 1], yb = cb[1][dy >> 1];
+          const int rb = 16 * (hy + 16 * hz);
+#pragma unroll
+          for (int px = 0; px < 2; ++px) {
+            const int hx_lo = max(h0[0] + 2 * px, 0), hx_hi = min(h0[0] + 2 * px + 2, 16);
+            if (hx_lo >= hx_hi) continue;
+            const float xa = ca[0][px], xb = cb[0][px];
+            const uint32_t pa = off[rb + hx_lo], pb = off[rb + hx_hi];
+            for (uint32_t q = pa; q < pb; ++q) {
+              const float ux = su[0][q] - sh0, uy = su[1][q] - sh1, uz = su[2][q] - sh2;
+              const float tx = ux - floorf(ux), ty = uy - floorf(uy), tz = uz - floorf(uz);
+              const float w = (xa + xb * tx) * (ya + yb * ty) * (za + zb * tz);
+              sM += w * sv[a][q];
+              sm += w;
+            }
+          }
+        }
+      }
+      acc[k] += sM;
+      acc[DEP_NODES + k] += sm;
     }
+    __syncthreads();
   }
-  __syncthreads();
   float* dst = slab + (size_t)B * SLAB;
-  for (int k = tid; k < SLAB; k += TPB) dst[k] = acc[k];
-  for (int k = tid; k < nc; k += TPB) liquid[co + k] = ccnt[k] > 0;
+  for (int k = tid; k < 2 * DEP_NODES; k += DEP_T) dst[k] = acc[k];
+  // native-cell liquid mask from the fine-cell counts
+  for (int k = tid; k < nc; k += DEP_T) {
+    const int x = k % r, y = (k / r) % r, z = k / (r * r);
+    const int s = 1 << e;
+    uint32_t t = 0;
+    for (int dz = 0; dz < s; ++dz)
+      for (int dy = 0; dy < s; ++dy)
+        for (int dx = 0; dx < s; ++dx) t += ccnt[(x * s + dx) + 8 * ((y * s + dy) + 8 * (z * s + dz))];
+    liquid[co + k] = t > 0;
+  }
 }
 
 // One work-group per block: collect the block's own fine nodes from its slab
@@ -692,51 +843,65 @@ __device__ __forceinline__ bool decode(const LevelDev& L, const WG& w, const uin
   return true;
 }
 
-// 7-point adaptive Poisson stencil of one cell: diag = sum of face
-// coefficients, off = sum of coefficient * neighbour.  Coefficient beta * s
-// between equal cells (s = cell edge in fine units), beta * s_f / 1.5 across a
-// coarse-fine interface (one term per fine sub-face); tank walls carry beta = 0.
-// inb(local index) reads the own block, outb(cell) any other block.
-template <class In, class Out>
-__device__ __forceinline__ void stencil(const LevelDev& L, const CellRef& cr, In inb, Out outb, float& diag,
-                                        float& off) {
+__device__ __forceinline__ void cell_ref(const LevelDev& L, int B, int li, CellRef& o) {
+  o.B = B;
+  o.r = L.bs >> L.e[B];
+  o.lr = 31 - __clz(o.r);
+  o.q = 0;
+  o.li = li;
+  o.l[0] = li & (o.r - 1);
+  o.l[1] = (li >> o.lr) & (o.r - 1);
+  o.l[2] = li >> (2 * o.lr);
+  o.c = L.coff[B] + li;
+}
+
+// 7-point adaptive Poisson stencil of one cell, split into its parts:
+// diag = sum of all face coefficients; ext = sum of coefficient * value over
+// neighbours in other blocks, read through outb(N, local coords, res, cell);
+// w[2a+side] = coefficient of the in-block neighbour on that side (0: none).
+// Coefficients: beta * s between equal cells (s = cell edge in fine cells),
+// beta * s_f / 1.5 across a coarse-fine interface, one term per fine
+// sub-face; tank walls carry beta = 0.
+template <class Out>
+__device__ __forceinline__ void stencil_split(const LevelDev& L, const CellRef& cr, Out outb, float& diag, float& ext,
+                                              float* w) {
   const int B = cr.B, r = cr.r;
   const int e = L.e[B];
   const float s = (float)(1 << e);
   diag = 0.f;
-  off = 0.f;
+  ext = 0.f;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
     const uint32_t fo = L.foff[a][B];
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
+      const int d = 2 * a + side;
+      w[d] = 0.f;
       const int nl = cr.l[a] + (side ? 1 : -1);
       int sl[3] = {cr.l[0], cr.l[1], cr.l[2]};
       sl[a] += side;
       if (nl >= 0 && nl < r) {
-        const float w = L.beta[a][fo + face_idx(a, sl, r)] * s;
-        int nc[3] = {cr.l[0], cr.l[1], cr.l[2]};
-        nc[a] = nl;
-        diag += w;
-        off += w * inb(cell_idx(nc, r));
+        const float c = L.beta[a][fo + face_idx(a, sl, r)] * s;
+        w[d] = c;
+        diag += c;
         continue;
       }
-      const int N = L.nbr[B * 6 + 2 * a + side];
+      const int N = L.nbr[B * 6 + d];
       if (N < 0) continue;
       const int en = L.e[N], rn = L.bs >> en;
       int nc[3] = {cr.l[0], cr.l[1], cr.l[2]};
       nc[a] = side ? 0 : rn - 1;
       if (en == e) {
-        const float w = L.beta[a][fo + face_idx(a, sl, r)] * s;
-        diag += w;
-        off += w * outb(L.coff[N] + cell_idx(nc, rn));
+        const float c = L.beta[a][fo + face_idx(a, sl, r)] * s;
+        diag += c;
+        ext += c * outb(N, nc, rn, L.coff[N] + cell_idx(nc, rn));
       } else if (en > e) {
-        const float w = L.beta[a][fo + face_idx(a, sl, r)] * s / 1.5f;
+        const float c = L.beta[a][fo + face_idx(a, sl, r)] * s / 1.5f;
         nc[t1] = cr.l[t1] >> 1;
         nc[t2] = cr.l[t2] >> 1;
-        diag += w;
-        off += w * outb(L.coff[N] + cell_idx(nc, rn));
+        diag += c;
+        ext += c * outb(N, nc, rn, L.coff[N] + cell_idx(nc, rn));
       } else {
         int ps[3];
         ps[a] = side ? 0 : rn;
@@ -748,39 +913,77 @@ __device__ __forceinline__ void stencil(const LevelDev& L, const CellRef& cr, In
           for (int d2 = 0; d2 < 2; ++d2) {
             nc[t1] = ps[t1] = 2 * cr.l[t1] + d1;
             nc[t2] = ps[t2] = 2 * cr.l[t2] + d2;
-            const float w = L.beta[a][fon + face_idx(a, ps, rn)] * sf / 1.5f;
-            diag += w;
-            off += w * outb(con + cell_idx(nc, rn));
+            const float c = L.beta[a][fon + face_idx(a, ps, rn)] * sf / 1.5f;
+            diag += c;
+            ext += c * outb(N, nc, rn, con + cell_idx(nc, rn));
           }
       }
     }
   }
 }
 
+// in-block part: sum of w[d] * value of the in-block neighbour d
+template <class In>
+__device__ __forceinline__ float inblock(const float* w, int li, int r, In inb) {
+  float s = 0.f;
+  const int st[3] = {1, r, r * r};
+#pragma unroll
+  for (int d = 0; d < 6; ++d)
+    if (w[d] != 0.f) s += w[d] * inb(li + ((d & 1) ? st[d >> 1] : -st[d >> 1]));
+  return s;
+}
+
+// parent cell on the next coarser level: same block halving (local / 2),
+// same block keeping its resolution, or the aggregated block (coords / 2)
+__device__ __forceinline__ uint32_t parent_cell(const LevelDev& F, const LevelDev& C, int aggregated, int B,
+                                                const int* l, int r, int li) {
+  if (aggregated) {
+    int Bc[3];
+    block_coords(F.D, B, Bc);
+    const int q[3] = {Bc[0] >> 1, Bc[1] >> 1, Bc[2] >> 1};
+    return C.coff[block_id(C.D, q)];
+  }
+  if (C.e[B] == F.e[B]) return C.coff[B] + li;
+  const int q[3] = {l[0] >> 1, l[1] >> 1, l[2] >> 1};
+  return C.coff[B] + cell_idx(q, r >> 1);
+}
+
+
 // Block red-black x cell red-black weighted Gauss-Seidel on one block colour
-// (smoothActiveBlocks, Smoother.cuh:105-202): the work-group's blocks sit in
-// LDS, neighbours of the other block colour are frozen in HBM.  `reverse`
-// runs the cell colours in the opposite order (post-smoother).
+// (smoothActiveBlocks, Smoother.cuh:105-202).  Load phase: own x into LDS,
+// b, diag and the contribution of the frozen other-colour neighbour blocks;
+// then `local_sweeps` x (cell colour 0, 1) sweeps entirely in LDS; store.
+// `reverse` runs the cell colours in the opposite order (post-smoother).
 __global__ __launch_bounds__(TPB) void smooth_kernel(LevelDev L, float* __restrict__ x, const float* __restrict__ b,
                                                      const WG* __restrict__ wgt, const uint32_t* __restrict__ blist,
-                                                     float omega, int local_sweeps, int reverse) {
+                                                     float omega, int local_sweeps, int reverse, Pro pro) {
   __shared__ float sx[512];
   const WG w = wgt[blockIdx.x];
-  const int sh = 3 * w.lr;
   CellRef cr[2];
-  bool ok[2];
-  float bb[2];
-  bool liq[2];
+  bool act[2];
+  float bb[2], dg[2], ex[2], cw[2][6];
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const int i = threadIdx.x + k * TPB;
-    ok[k] = decode(L, w, blist, i, cr[k]);
-    liq[k] = false;
-    if (ok[k]) {
-      sx[i] = x[cr[k].c];
-      bb[k] = b[cr[k].c];
-      liq[k] = L.liquid[cr[k].c] != 0;
-    }
+    act[k] = false;
+    if (!decode(L, w, blist, i, cr[k])) continue;
+    const uint32_t c = cr[k].c;
+    float xv = x[c];
+    const bool liq = L.liquid[c] != 0;
+    if (liq && pro.own) xv += pro.scale * pro.xc[parent_cell(L, pro.C, pro.aggregated, cr[k].B, cr[k].l, cr[k].r, cr[k].li)];
+    sx[i] = xv;
+    if (!liq) continue;
+    act[k] = true;
+    bb[k] = b[c];
+    stencil_split(
+        L, cr[k],
+        [&](int N, const int* nc, int rn, uint32_t cc) {
+          float v = x[cc];
+          if (pro.nbr && L.liquid[cc])
+            v += pro.scale * pro.xc[parent_cell(L, pro.C, pro.aggregated, N, nc, rn, cell_idx(nc, rn))];
+          return v;
+        },
+        dg[k], ex[k], cw[k]);
   }
   __syncthreads();
   for (int ls = 0; ls < local_sweeps; ++ls)
@@ -788,43 +991,83 @@ __global__ __launch_bounds__(TPB) void smooth_kernel(LevelDev L, float* __restri
       const int col = reverse ? 1 - ph : ph;
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
-        if (!ok[k] || !liq[k]) continue;
+        if (!act[k]) continue;
         const CellRef& c = cr[k];
         if (((c.l[0] + c.l[1] + c.l[2]) & 1) != col) continue;
-        const int base = c.q << sh;
-        float diag, off;
-        stencil(
-            L, c, [&](int li) { return sx[base + li]; }, [&](uint32_t cc) { return x[cc]; }, diag, off);
         const int i = threadIdx.x + k * TPB;
-        const float res = bb[k] - (diag * sx[i] - off);
-        if (diag > 0.f) sx[i] += omega * res / diag;
+        const int base = i - c.li;
+        const float off = ex[k] + inblock(cw[k], c.li, c.r, [&](int li) { return sx[base + li]; });
+        const float res = bb[k] - (dg[k] * sx[i] - off);
+        if (dg[k] > 0.f) sx[i] += omega * res / dg[k];
       }
       __syncthreads();
     }
 #pragma unroll
   for (int k = 0; k < 2; ++k)
-    if (ok[k] && liq[k]) x[cr[k].c] = sx[threadIdx.x + k * TPB];
+    if (act[k]) x[cr[k].c] = sx[threadIdx.x + k * TPB];
 }
 
-// r = b - A x (computeExplicitResidual, VCycle.cuh:81-131); 0 on air cells
+
+// r = b - A x (computeExplicitResidual, VCycle.cuh:81-131), 0 on air cells;
+// optionally fused with the restriction b_c = sum of children
+// (restrictResidualsKernel, MultigridTransfer.cuh:9-59) and x_c = 0.
 __global__ __launch_bounds__(TPB) void residual_kernel(LevelDev L, const float* __restrict__ x,
                                                        const float* __restrict__ b, float* __restrict__ r,
                                                        const WG* __restrict__ wgt,
-                                                       const uint32_t* __restrict__ blist) {
+                                                       const uint32_t* __restrict__ blist, Restr rs) {
+  __shared__ float sx[512];
+  __shared__ float sr[512];
   const WG w = wgt[blockIdx.x];
+  CellRef cr[2];
+  bool ok[2];
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
-    CellRef c;
-    if (!decode(L, w, blist, threadIdx.x + k * TPB, c)) continue;
-    if (!L.liquid[c.c]) {
-      r[c.c] = 0.f;
-      continue;
+    const int i = threadIdx.x + k * TPB;
+    ok[k] = decode(L, w, blist, i, cr[k]);
+    if (ok[k]) sx[i] = x[cr[k].c];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    if (!ok[k]) continue;
+    const int i = threadIdx.x + k * TPB;
+    const uint32_t c = cr[k].c;
+    float rv = 0.f;
+    if (L.liquid[c]) {
+      float dg, ex, cw[6];
+      stencil_split(
+          L, cr[k], [&](int, const int*, int, uint32_t cc) { return x[cc]; }, dg, ex, cw);
+      const int base = i - cr[k].li;
+      const float off = ex + inblock(cw, cr[k].li, cr[k].r, [&](int li) { return sx[base + li]; });
+      rv = b[c] - (dg * sx[i] - off);
     }
-    const uint32_t co = L.coff[c.B];
-    float diag, off;
-    stencil(
-        L, c, [&](int li) { return x[co + li]; }, [&](uint32_t cc) { return x[cc]; }, diag, off);
-    r[c.c] = b[c.c] - (diag * x[c.c] - off);
+    sr[i] = rv;
+    if (r) r[c] = rv;
+  }
+  if (!rs.on) return;
+  __syncthreads();
+  // coarse cells of this group's blocks (all of one resolution)
+  const int rF = 1 << w.lr;
+  const int B0 = blist[w.start];
+  const bool halves = rs.C.e[B0] != L.e[B0];
+  const int rC = halves ? rF >> 1 : rF;
+  const int cpbC = rC * rC * rC, cpbF = rF * rF * rF;
+  for (int j = threadIdx.x; j < w.count * cpbC; j += TPB) {
+    const int q = j / cpbC, lj = j - q * cpbC;
+    const int B = blist[w.start + q];
+    const int l[3] = {lj % rC, (lj / rC) % rC, lj / (rC * rC)};
+    float s = 0.f;
+    if (halves) {
+      for (int d = 0; d < 8; ++d) {
+        const int f[3] = {2 * l[0] + (d & 1), 2 * l[1] + ((d >> 1) & 1), 2 * l[2] + (d >> 2)};
+        s += sr[q * cpbF + cell_idx(f, rF)];
+      }
+    } else {
+      s = sr[q * cpbF + lj];
+    }
+    const uint32_t cc = rs.C.coff[B] + lj;
+    rs.bc[cc] = s;
+    rs.xc[cc] = 0.f;
   }
 }
 
@@ -832,23 +1075,35 @@ __global__ __launch_bounds__(TPB) void residual_kernel(LevelDev L, const float* 
 __global__ __launch_bounds__(TPB) void apply_dot_kernel(LevelDev L, const float* __restrict__ x, float* __restrict__ y,
                                                         const WG* __restrict__ wgt,
                                                         const uint32_t* __restrict__ blist, double* acc) {
+  __shared__ float sx[512];
   const WG w = wgt[blockIdx.x];
+  CellRef cr[2];
+  bool ok[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int i = threadIdx.x + k * TPB;
+    ok[k] = decode(L, w, blist, i, cr[k]);
+    if (ok[k]) sx[i] = x[cr[k].c];
+  }
+  __syncthreads();
   double part = 0.0;
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
-    CellRef c;
-    if (!decode(L, w, blist, threadIdx.x + k * TPB, c)) continue;
-    if (!L.liquid[c.c]) {
-      y[c.c] = 0.f;
+    if (!ok[k]) continue;
+    const int i = threadIdx.x + k * TPB;
+    const uint32_t c = cr[k].c;
+    if (!L.liquid[c]) {
+      y[c] = 0.f;
       continue;
     }
-    const uint32_t co = L.coff[c.B];
-    float diag, off;
-    stencil(
-        L, c, [&](int li) { return x[co + li]; }, [&](uint32_t cc) { return x[cc]; }, diag, off);
-    const float v = diag * x[c.c] - off;
-    y[c.c] = v;
-    part += (double)x[c.c] * (double)v;
+    float dg, ex, cw[6];
+    stencil_split(
+        L, cr[k], [&](int, const int*, int, uint32_t cc) { return x[cc]; }, dg, ex, cw);
+    const int base = i - cr[k].li;
+    const float off = ex + inblock(cw, cr[k].li, cr[k].r, [&](int li) { return sx[base + li]; });
+    const float v = dg * sx[i] - off;
+    y[c] = v;
+    part += (double)sx[i] * (double)v;
   }
   if (acc) {
     const double s = block_reduce_sum(part);
@@ -856,37 +1111,44 @@ __global__ __launch_bounds__(TPB) void apply_dot_kernel(LevelDev L, const float*
   }
 }
 
-// b_c = sum of the children's residuals (restrictResidualsKernel,
-// MultigridTransfer.cuh:9-59): 8 in-block children where the block halves, the
-// same cell where it keeps its resolution, 8 child blocks after aggregation.
+// children sum of one coarse cell (halving / keeping block, or aggregation)
+template <class Rd>
+__device__ __forceinline__ float children_sum(const LevelDev& F, const LevelDev& C, int aggregated, const CellRef& c,
+                                              Rd rd) {
+  float s = 0.f;
+  if (aggregated) {
+    int Bc[3];
+    block_coords(C.D, c.B, Bc);
+    for (int d = 0; d < 8; ++d) {
+      const int q[3] = {2 * Bc[0] + (d & 1), 2 * Bc[1] + ((d >> 1) & 1), 2 * Bc[2] + (d >> 2)};
+      s += rd(F.coff[block_id(F.D, q)]);
+    }
+  } else if (C.e[c.B] == F.e[c.B]) {
+    s = rd(F.coff[c.B] + c.li);
+  } else {
+    const int rF = 2 * c.r;
+    const uint32_t fo = F.coff[c.B];
+    for (int d = 0; d < 8; ++d) {
+      const int q[3] = {2 * c.l[0] + (d & 1), 2 * c.l[1] + ((d >> 1) & 1), 2 * c.l[2] + (d >> 2)};
+      s += rd(fo + cell_idx(q, rF));
+    }
+  }
+  return s;
+}
+
+// standalone restriction (used across aggregation levels): b_c = sum of
+// children, x_c = 0
 __global__ __launch_bounds__(TPB) void restrict_kernel(LevelDev F, LevelDev C, int aggregated,
                                                        const float* __restrict__ rf, float* __restrict__ bc,
-                                                       const WG* __restrict__ wgt,
+                                                       float* __restrict__ xc, const WG* __restrict__ wgt,
                                                        const uint32_t* __restrict__ blist) {
   const WG w = wgt[blockIdx.x];
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     CellRef c;
     if (!decode(C, w, blist, threadIdx.x + k * TPB, c)) continue;
-    float s = 0.f;
-    if (aggregated) {
-      int Bc[3];
-      block_coords(C.D, c.B, Bc);
-      for (int d = 0; d < 8; ++d) {
-        const int q[3] = {2 * Bc[0] + (d & 1), 2 * Bc[1] + ((d >> 1) & 1), 2 * Bc[2] + (d >> 2)};
-        s += rf[F.coff[block_id(F.D, q)]];
-      }
-    } else if (C.e[c.B] == F.e[c.B]) {
-      s = rf[F.coff[c.B] + c.li];
-    } else {
-      const int rF = 2 * c.r;
-      const uint32_t fo = F.coff[c.B];
-      for (int d = 0; d < 8; ++d) {
-        const int q[3] = {2 * c.l[0] + (d & 1), 2 * c.l[1] + ((d >> 1) & 1), 2 * c.l[2] + (d >> 2)};
-        s += rf[fo + cell_idx(q, rF)];
-      }
-    }
-    bc[c.c] = s;
+    bc[c.c] = children_sum(F, C, aggregated, c, [&](uint32_t i) { return rf[i]; });
+    if (xc) xc[c.c] = 0.f;
   }
 }
 
@@ -899,25 +1161,8 @@ __global__ __launch_bounds__(TPB) void coarsen_liquid_kernel(LevelDev F, LevelDe
   for (int k = 0; k < 2; ++k) {
     CellRef c;
     if (!decode(C, w, blist, threadIdx.x + k * TPB, c)) continue;
-    uint8_t all = 1;
-    if (aggregated) {
-      int Bc[3];
-      block_coords(C.D, c.B, Bc);
-      for (int d = 0; d < 8; ++d) {
-        const int q[3] = {2 * Bc[0] + (d & 1), 2 * Bc[1] + ((d >> 1) & 1), 2 * Bc[2] + (d >> 2)};
-        all &= F.liquid[F.coff[block_id(F.D, q)]];
-      }
-    } else if (C.e[c.B] == F.e[c.B]) {
-      all = F.liquid[F.coff[c.B] + c.li];
-    } else {
-      const int rF = 2 * c.r;
-      const uint32_t fo = F.coff[c.B];
-      for (int d = 0; d < 8; ++d) {
-        const int q[3] = {2 * c.l[0] + (d & 1), 2 * c.l[1] + ((d >> 1) & 1), 2 * c.l[2] + (d >> 2)};
-        all &= F.liquid[fo + cell_idx(q, rF)];
-      }
-    }
-    liq[c.c] = all ? 1 : 0;
+    const float all = children_sum(F, C, aggregated, c, [&](uint32_t i) { return F.liquid[i] ? 0.f : 1.f; });
+    liq[c.c] = all == 0.f ? 1 : 0;
   }
 }
 
@@ -933,86 +1178,286 @@ __global__ __launch_bounds__(TPB) void prolong_kernel(LevelDev F, LevelDev C, in
     CellRef c;
     if (!decode(F, w, blist, threadIdx.x + k * TPB, c)) continue;
     if (!F.liquid[c.c]) continue;
-    uint32_t pc;
-    if (aggregated) {
-      int Bc[3];
-      block_coords(F.D, c.B, Bc);
-      const int q[3] = {Bc[0] >> 1, Bc[1] >> 1, Bc[2] >> 1};
-      pc = C.coff[block_id(C.D, q)];
-    } else if (C.e[c.B] == F.e[c.B]) {
-      pc = C.coff[c.B] + c.li;
-    } else {
-      const int q[3] = {c.l[0] >> 1, c.l[1] >> 1, c.l[2] >> 1};
-      pc = C.coff[c.B] + cell_idx(q, c.r >> 1);
-    }
-    xf[c.c] += scale * xc[pc];
+    xf[c.c] += scale * xc[parent_cell(F, C, aggregated, c.B, c.l, c.r, c.li)];
   }
 }
 
-// Whole coarsest level in one work-group's LDS: `sweeps` symmetric sweeps
-// (forward block/cell colour order, then reverse).
-__global__ __launch_bounds__(TPB) void coarse_solve_kernel(LevelDev L, float* __restrict__ x,
-                                                           const float* __restrict__ b, int ncells, int sweeps,
-                                                           float omega, int local_sweeps) {
-  __shared__ float sx[512];
-  CellRef cr[2];
-  bool ok[2], liq[2];
-  float bb[2];
-  int bcol[2];
+// ---- single-work-group V-cycle over the small trailing levels, in LDS ----
+// Levels with one cell size everywhere (every level after the first
+// aggregation, and block levels of uniform resolution) evaluate the stencil
+// arithmetically from cell coordinates: coefficient s to every in-domain
+// neighbour (coarse levels carry beta = 1 inside the tank), no metadata loads.
+
+struct UStencil {
+  int diagn;     // number of in-domain neighbours
+  int nb[6];     // neighbour cell indices, -1 = tank wall
+  int cls;       // block colour * 2 + cell colour
+};
+
+__device__ __forceinline__ void uniform_stencil(const TailLevel& T, int i, UStencil& u) {
+  const int lr = T.ulr, r = 1 << lr, sh = 3 * lr;
+  const int B = i >> sh, li = i & ((1 << sh) - 1);
+  int Bc[3];
+  block_coords(T.L.D, B, Bc);
+  const int l[3] = {li & (r - 1), (li >> lr) & (r - 1), li >> (2 * lr)};
+  u.cls = (((Bc[0] + Bc[1] + Bc[2]) & 1) << 1) | ((l[0] + l[1] + l[2]) & 1);
+  u.diagn = 0;
 #pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const int i = threadIdx.x + k * TPB;
-    ok[k] = i < ncells;
-    liq[k] = false;
-    if (ok[k]) {
-      // locate the cell's block by scanning offsets (the coarsest level is tiny)
-      int B = 0;
-      while (L.coff[B + 1] <= (uint32_t)i) ++B;
-      CellRef& c = cr[k];
-      c.B = B;
-      c.r = L.bs >> L.e[B];
-      c.lr = 31 - __clz(c.r);
-      c.li = i - L.coff[B];
-      c.q = 0;
-      c.l[0] = c.li & (c.r - 1);
-      c.l[1] = (c.li >> c.lr) & (c.r - 1);
-      c.l[2] = c.li >> (2 * c.lr);
-      c.c = i;
-      int Bc[3];
-      block_coords(L.D, B, Bc);
-      bcol[k] = (Bc[0] + Bc[1] + Bc[2]) & 1;
-      sx[i] = x[i];
-      bb[k] = b[i];
-      liq[k] = L.liquid[i] != 0;
+  for (int d = 0; d < 6; ++d) {
+    const int a = d >> 1;
+    int G[3] = {Bc[0] * r + l[0], Bc[1] * r + l[1], Bc[2] * r + l[2]};
+    G[a] += (d & 1) ? 1 : -1;
+    if (G[a] < 0 || G[a] >= T.L.D[a] * r) {
+      u.nb[d] = -1;
+      continue;
+    }
+    const int q[3] = {G[0] >> lr, G[1] >> lr, G[2] >> lr};
+    const int lq[3] = {G[0] & (r - 1), G[1] & (r - 1), G[2] & (r - 1)};
+    u.nb[d] = (block_id(T.L.D, q) << sh) + cell_idx(lq, r);
+    ++u.diagn;
+  }
+}
+
+// class of a cell of a generic level (block colour * 2 + cell colour)
+__device__ __forceinline__ int generic_class(const TailLevel& T, int i, CellRef& c) {
+  const int B = T.cblk[i];
+  cell_ref(T.L, B, i - T.L.coff[B], c);
+  int Bc[3];
+  block_coords(T.L.D, B, Bc);
+  return (((Bc[0] + Bc[1] + Bc[2]) & 1) << 1) | ((c.l[0] + c.l[1] + c.l[2]) & 1);
+}
+
+// A x (own row) of cell i on a tail level, from LDS values xs
+__device__ __forceinline__ float tail_ax(const TailLevel& T, const float* xs, int i, int* cls_out) {
+  if (T.ulr >= 0) {
+    UStencil u;
+    uniform_stencil(T, i, u);
+    const float s = (float)(1 << T.ue);
+    float off = 0.f;
+#pragma unroll
+    for (int d = 0; d < 6; ++d)
+      if (u.nb[d] >= 0) off += s * xs[u.nb[d]];
+    if (cls_out) *cls_out = u.cls;
+    return s * (float)u.diagn * xs[i] - off;
+  }
+  CellRef c;
+  const int cls = generic_class(T, i, c);
+  if (cls_out) *cls_out = cls;
+  float dg, ex, cw[6];
+  stencil_split(
+      T.L, c, [&](int, const int*, int, uint32_t q) { return xs[q]; }, dg, ex, cw);
+  const int base = i - c.li;
+  return dg * xs[i] - (ex + inblock(cw, c.li, c.r, [&](int li) { return xs[base + li]; }));
+}
+
+__device__ __forceinline__ float tail_diag(const TailLevel& T, int i) {
+  if (T.ulr >= 0) {
+    UStencil u;
+    uniform_stencil(T, i, u);
+    return (float)(1 << T.ue) * (float)u.diagn;
+  }
+  CellRef c;
+  generic_class(T, i, c);
+  float dg, ex, cw[6];
+  stencil_split(
+      T.L, c, [&](int, const int*, int, uint32_t) { return 0.f; }, dg, ex, cw);
+  return dg;
+}
+
+__device__ __forceinline__ void tail_smooth(const TailLevel& T, float* sm, const uint8_t* liq, int iters,
+                                            bool reverse, float omega, int local_sweeps) {
+  float* xs = sm + T.off;
+  const float* bs = sm + T.off + T.ncells;
+  for (int it = 0; it < iters; ++it)
+    for (int ph = 0; ph < 2; ++ph) {
+      const int bc = reverse ? 1 - ph : ph;
+      for (int ls = 0; ls < local_sweeps; ++ls)
+        for (int cp = 0; cp < 2; ++cp) {
+          const int cc = reverse ? 1 - cp : cp;
+          for (int i = threadIdx.x; i < T.ncells; i += TAIL_T) {
+            if (!liq[i]) continue;
+            int cls;
+            const float ax = tail_ax(T, xs, i, &cls);
+            if (cls != ((bc << 1) | cc)) continue;
+            const float dg = tail_diag(T, i);
+            if (dg > 0.f) xs[i] += omega * (bs[i] - ax) / dg;
+          }
+          __syncthreads();
+        }
+    }
+}
+
+__global__ __launch_bounds__(TAIL_T) void tail_kernel(TailArgs A) {
+  extern __shared__ float sm[];
+  // liquid masks of every tail level, as bytes after the float arrays
+  uint8_t* lq = reinterpret_cast<uint8_t*>(sm + A.fl);
+  for (int t = 0; t < A.n; ++t)
+    for (int i = threadIdx.x; i < A.lv[t].ncells; i += TAIL_T) lq[A.lv[t].loff + i] = A.lv[t].L.liquid[i];
+  const TailLevel& T0 = A.lv[0];
+  for (int i = threadIdx.x; i < T0.ncells; i += TAIL_T) {
+    sm[T0.off + i] = 0.f;
+    sm[T0.off + T0.ncells + i] = A.b0[i];
+  }
+  __syncthreads();
+  for (int t = 0; t + 1 < A.n; ++t) {
+    const TailLevel& F = A.lv[t];
+    const TailLevel& C = A.lv[t + 1];
+    const uint8_t* fl = lq + F.loff;
+    tail_smooth(F, sm, fl, A.pre, false, A.omega, A.local_sweeps);
+    float* xs = sm + F.off;
+    const float* bs = sm + F.off + F.ncells;
+    float* rs = sm + F.off + 2 * F.ncells;
+    for (int i = threadIdx.x; i < F.ncells; i += TAIL_T) rs[i] = fl[i] ? bs[i] - tail_ax(F, xs, i, nullptr) : 0.f;
+    __syncthreads();
+    for (int i = threadIdx.x; i < C.ncells; i += TAIL_T) {
+      const int B = C.cblk[i];
+      CellRef c;
+      cell_ref(C.L, B, i - C.L.coff[B], c);
+      sm[C.off + C.ncells + i] = children_sum(F.L, C.L, C.agg, c, [&](uint32_t q) { return rs[q]; });
+      sm[C.off + i] = 0.f;
+    }
+    __syncthreads();
+  }
+  const TailLevel& TC = A.lv[A.n - 1];
+  if (A.cinv) {
+    // exact coarsest solve: x = A^-1 b with the FP64 inverse (rows and
+    // columns of air or dropped unknowns are zero)
+    const int n = TC.ncells;
+    const float* bs = sm + TC.off + n;
+    for (int i = threadIdx.x; i < n; i += TAIL_T) {
+      double s = 0.0;
+      for (int j = 0; j < n; ++j) s += A.cinv[(size_t)i * n + j] * (double)bs[j];
+      sm[TC.off + i] = (float)s;
+    }
+    __syncthreads();
+  } else {
+    for (int sw = 0; sw < A.coarse_sweeps; ++sw) {
+      tail_smooth(TC, sm, lq + TC.loff, 1, false, A.omega, A.local_sweeps);
+      tail_smooth(TC, sm, lq + TC.loff, 1, true, A.omega, A.local_sweeps);
+    }
+  }
+  for (int t = A.n - 2; t >= 0; --t) {
+    const TailLevel& F = A.lv[t];
+    const TailLevel& C = A.lv[t + 1];
+    const uint8_t* fl = lq + F.loff;
+    for (int i = threadIdx.x; i < F.ncells; i += TAIL_T) {
+      if (!fl[i]) continue;
+      const int B = F.cblk[i];
+      CellRef c;
+      cell_ref(F.L, B, i - F.L.coff[B], c);
+      sm[F.off + i] += A.scale * sm[C.off + parent_cell(F.L, C.L, C.agg, c.B, c.l, c.r, c.li)];
+    }
+    __syncthreads();
+    tail_smooth(F, sm, fl, A.post, true, A.omega, A.local_sweeps);
+  }
+  for (int i = threadIdx.x; i < T0.ncells; i += TAIL_T)
+    if (lq[T0.loff + i]) A.x0[i] = sm[T0.off + i];
+}
+
+// FP64 inverse of the coarsest level restricted to its liquid cells
+// (Gauss-Jordan in LDS; the same rule as the exact coarsest solve of the
+// oracle: a pivot below 1e-12 of the largest diagonal drops its unknown).
+
+__global__ __launch_bounds__(TPB) void coarse_factor_kernel(LevelDev L, int n, double* __restrict__ inv) {
+  __shared__ double Am[DIRECT_MAX * DIRECT_MAX];
+  __shared__ double Im[DIRECT_MAX * DIRECT_MAX];
+  __shared__ double col[DIRECT_MAX];
+  __shared__ int dead[DIRECT_MAX];
+  __shared__ double dmax;
+  for (int k = threadIdx.x; k < n * n; k += TPB) {
+    Am[k] = 0.0;
+    Im[k] = (k / n == k % n) ? 1.0 : 0.0;
+  }
+  if (threadIdx.x == 0) dmax = 0.0;
+  __syncthreads();
+  for (int c = threadIdx.x; c < n; c += TPB) {
+    dead[c] = 0;
+    if (!L.liquid[c]) {
+      Am[c * n + c] = 1.0;
+      dead[c] = 1;
+      continue;
+    }
+    int B = 0;
+    while (L.coff[B + 1] <= (uint32_t)c) ++B;
+    CellRef cr;
+    cell_ref(L, B, c - L.coff[B], cr);
+    float dg, ex, cw[6];
+    // cross-block neighbours: the coefficient is recovered by reading a
+    // unit value at one neighbour at a time
+    stencil_split(
+        L, cr,
+        [&](int, const int*, int, uint32_t q) {
+          return 0.f;
+        },
+        dg, ex, cw);
+    Am[c * n + c] = dg;
+    const int st[3] = {1, cr.r, cr.r * cr.r};
+    for (int d = 0; d < 6; ++d)
+      if (cw[d] != 0.f) {
+        const int q = c + ((d & 1) ? st[d >> 1] : -st[d >> 1]);
+        if (L.liquid[q]) Am[c * n + q] = -cw[d];
+      }
+  }
+  __syncthreads();
+  // cross-block couplings: every cell visits its cross-block neighbours
+  for (int c = threadIdx.x; c < n; c += TPB) {
+    if (!L.liquid[c]) continue;
+    int B = 0;
+    while (L.coff[B + 1] <= (uint32_t)c) ++B;
+    CellRef cr;
+    cell_ref(L, B, c - L.coff[B], cr);
+    // coefficient of neighbour q = diag - (diag with q's term removed): use
+    // the linearity of ext in the neighbour values
+    uint32_t qs[24];
+    int nq = 0;
+    float dg, ex, cw[6];
+    stencil_split(
+        L, cr,
+        [&](int, const int*, int, uint32_t q) {
+          if (nq < 24) qs[nq++] = q;
+          return 0.f;
+        },
+        dg, ex, cw);
+    for (int k = 0; k < nq; ++k) {
+      float e1;
+      stencil_split(
+          L, cr, [&](int, const int*, int, uint32_t q) { return q == qs[k] ? 1.f : 0.f; }, dg, e1, cw);
+      if (L.liquid[qs[k]]) Am[c * n + qs[k]] -= (double)e1;
     }
   }
   __syncthreads();
-  for (int sw = 0; sw < sweeps; ++sw)
-    for (int dir = 0; dir < 2; ++dir)
-      for (int ph = 0; ph < 2; ++ph) {
-        const int bc = dir ? 1 - ph : ph;
-        for (int ls = 0; ls < local_sweeps; ++ls)
-          for (int cp = 0; cp < 2; ++cp) {
-            const int cc = dir ? 1 - cp : cp;
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-              if (!ok[k] || !liq[k] || bcol[k] != bc) continue;
-              const CellRef& c = cr[k];
-              if (((c.l[0] + c.l[1] + c.l[2]) & 1) != cc) continue;
-              const uint32_t co = L.coff[c.B];
-              float diag, off;
-              stencil(
-                  L, c, [&](int li) { return sx[co + li]; }, [&](uint32_t q) { return sx[q]; }, diag, off);
-              const int i = c.c;
-              const float res = bb[k] - (diag * sx[i] - off);
-              if (diag > 0.f) sx[i] += omega * res / diag;
-            }
-            __syncthreads();
-          }
-      }
-#pragma unroll
-  for (int k = 0; k < 2; ++k)
-    if (ok[k] && liq[k]) x[cr[k].c] = sx[cr[k].c];
+  for (int c = threadIdx.x; c < n; c += TPB)
+    if (!dead[c]) atomicMax((unsigned long long*)&dmax, __double_as_longlong(Am[c * n + c]));
+  __syncthreads();
+  for (int p = 0; p < n; ++p) {
+    const double piv = Am[p * n + p];
+    const bool drop = dead[p] || !(fabs(piv) > 1e-12 * dmax);
+    __syncthreads();
+    if (drop) {
+      if (threadIdx.x == 0) dead[p] = 1;
+      __syncthreads();
+      continue;
+    }
+    const double ip = 1.0 / piv;
+    for (int j = threadIdx.x; j < n; j += TPB) {
+      Am[p * n + j] *= ip;
+      Im[p * n + j] *= ip;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += TPB) col[i] = Am[i * n + p];
+    __syncthreads();
+    for (int k = threadIdx.x; k < n * n; k += TPB) {
+      const int i = k / n, j = k - i * n;
+      if (i == p || col[i] == 0.0) continue;
+      Am[k] -= col[i] * Am[p * n + j];
+      Im[k] -= col[i] * Im[p * n + j];
+    }
+    __syncthreads();
+  }
+  for (int k = threadIdx.x; k < n * n; k += TPB) {
+    const int i = k / n, j = k - i * n;
+    inv[k] = (dead[i] || dead[j]) ? 0.0 : Im[k];
+  }
 }
 
 // ------------------------------------------------------------- vectors
@@ -1126,7 +1571,7 @@ void k_init_beta(hipStream_t s, const LevelDev& L, float* bx, float* by, float* 
 
 void k_p2g_deposit(hipStream_t s, const GridDev& g, const float* spos, const float* svel, const uint32_t* pcount,
                    const uint32_t* poff, float* slab, uint8_t* liquid) {
-  p2g_deposit_kernel<<<g.nb, TPB, 0, s>>>(g, spos, svel, pcount, poff, slab, liquid);
+  p2g_deposit_kernel<<<g.nb, DEP_T, 0, s>>>(g, spos, svel, pcount, poff, slab, liquid);
 }
 
 void k_p2g_gather(hipStream_t s, const GridDev& g, const uint32_t* pcount, const float* slab, float* const* Macc,
@@ -1163,13 +1608,13 @@ void k_g2p_advect(hipStream_t s, const GridDev& g, const float* fine, const floa
 }
 
 void k_smooth(hipStream_t s, const LevelDev& L, float* x, const float* b, const WG* wg, int nwg,
-              const uint32_t* blist, float omega, int local_sweeps, int reverse) {
-  if (nwg > 0) smooth_kernel<<<nwg, TPB, 0, s>>>(L, x, b, wg, blist, omega, local_sweeps, reverse);
+              const uint32_t* blist, float omega, int local_sweeps, int reverse, const Pro& pro) {
+  if (nwg > 0) smooth_kernel<<<nwg, TPB, 0, s>>>(L, x, b, wg, blist, omega, local_sweeps, reverse, pro);
 }
 
 void k_residual(hipStream_t s, const LevelDev& L, const float* x, const float* b, float* r, const WG* wg, int nwg,
-                const uint32_t* blist) {
-  if (nwg > 0) residual_kernel<<<nwg, TPB, 0, s>>>(L, x, b, r, wg, blist);
+                const uint32_t* blist, const Restr& rs) {
+  if (nwg > 0) residual_kernel<<<nwg, TPB, 0, s>>>(L, x, b, r, wg, blist, rs);
 }
 
 void k_apply_dot(hipStream_t s, const LevelDev& L, const float* x, float* y, const WG* wg, int nwg,
@@ -1178,8 +1623,8 @@ void k_apply_dot(hipStream_t s, const LevelDev& L, const float* x, float* y, con
 }
 
 void k_restrict(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, const float* rf, float* bc,
-                const WG* wgc, int nwgc, const uint32_t* blistc) {
-  if (nwgc > 0) restrict_kernel<<<nwgc, TPB, 0, s>>>(F, C, aggregated, rf, bc, wgc, blistc);
+                float* xc, const WG* wgc, int nwgc, const uint32_t* blistc) {
+  if (nwgc > 0) restrict_kernel<<<nwgc, TPB, 0, s>>>(F, C, aggregated, rf, bc, xc, wgc, blistc);
 }
 
 void k_prolong(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, float* xf, const float* xc,
@@ -1192,9 +1637,12 @@ void k_coarsen_liquid(hipStream_t s, const LevelDev& F, const LevelDev& C, int a
   if (nwgc > 0) coarsen_liquid_kernel<<<nwgc, TPB, 0, s>>>(F, C, aggregated, liquid_c, wgc, blistc);
 }
 
-void k_coarse_solve(hipStream_t s, const LevelDev& L, float* x, const float* b, int ncells, int sweeps, float omega,
-                    int local_sweeps) {
-  coarse_solve_kernel<<<1, TPB, 0, s>>>(L, x, b, ncells, sweeps, omega, local_sweeps);
+void k_tail(hipStream_t s, const TailArgs& args, size_t lds_bytes) {
+  tail_kernel<<<1, TAIL_T, lds_bytes, s>>>(args);
+}
+
+void k_coarse_factor(hipStream_t s, const LevelDev& L, int n, double* inv) {
+  coarse_factor_kernel<<<1, TPB, 0, s>>>(L, n, inv);
 }
 
 void k_dot(hipStream_t s, const float* a, const float* b, const uint8_t* liquid, int n, double* acc) {

@@ -145,6 +145,11 @@ int aw_ctx_synchronize(aw_ctx* ctx);
 int aw_malloc(aw_ctx* ctx, uint64_t bytes, void** out);
 int aw_free(aw_ctx* ctx, void* p);
 int aw_copy(aw_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* Per-launch timing with HIP events on the context's stream (no reference
+ * counterpart).  enable clears earlier records; report synchronises and writes
+ * JSON {"kernel": {"launches": n, "ms": total, "bytes": algorithmic bytes}}. */
+int aw_profile_enable(aw_ctx* ctx, int on);
+int aw_profile_report(aw_ctx* ctx, char* buf, uint64_t len);
 
 int aw_grid_create(aw_ctx* ctx, const aw_grid_desc* desc, aw_grid** out);
 int aw_grid_destroy(aw_grid* grid);

@@ -461,13 +461,72 @@ void prolong_from(OGrid& g, int k) {  // prolongateCorrection: injection-add (Mu
     if (F.liquid[c]) F.x[c] += g.prolong_scale * C.x[F.parent[c]];
 }
 
+// Exact solve of a small coarsest level (<= 64 cells) restricted to its
+// liquid cells: Gauss-Jordan elimination in FP64 without pivot search (the
+// matrix is SPD when the level touches air); a pivot below 1e-12 of the
+// largest diagonal drops its unknown (x = 0), which keeps an all-Neumann
+// component solvable.
+constexpr int DIRECT_MAX = 64;
+
+void coarse_direct(Level& L) {
+  const int n = L.ncells;
+  std::vector<double> A((size_t)n * n, 0.0), I((size_t)n * n, 0.0);
+  double dmax = 0.0;
+  for (int c = 0; c < n; ++c) {
+    I[(size_t)c * n + c] = 1.0;
+    if (!L.liquid[c]) {
+      A[(size_t)c * n + c] = 1.0;
+      continue;
+    }
+    A[(size_t)c * n + c] = L.diag[c];
+    dmax = std::max(dmax, L.diag[c]);
+    for (int k = L.rowptr[c]; k < L.rowptr[c + 1]; ++k) A[(size_t)c * n + L.col[k]] = L.val[k];
+  }
+  std::vector<char> dead(n, 0);
+  for (int p = 0; p < n; ++p) {
+    double piv = A[(size_t)p * n + p];
+    if (!(std::fabs(piv) > 1e-12 * dmax)) {
+      dead[p] = 1;
+      continue;
+    }
+    const double inv = 1.0 / piv;
+    for (int j = 0; j < n; ++j) {
+      A[(size_t)p * n + j] *= inv;
+      I[(size_t)p * n + j] *= inv;
+    }
+    for (int i = 0; i < n; ++i) {
+      if (i == p) continue;
+      const double f = A[(size_t)i * n + p];
+      if (f == 0.0) continue;
+      for (int j = 0; j < n; ++j) {
+        A[(size_t)i * n + j] -= f * A[(size_t)p * n + j];
+        I[(size_t)i * n + j] -= f * I[(size_t)p * n + j];
+      }
+    }
+  }
+  std::vector<double> x(n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    if (!L.liquid[i] || dead[i]) continue;
+    double s = 0.0;
+    for (int j = 0; j < n; ++j)
+      if (L.liquid[j] && !dead[j]) s += I[(size_t)i * n + j] * L.b[j];
+    x[i] = s;
+  }
+  L.x = x;
+}
+
 // Recursive V-cycle (VCycleSolver::runCycle, VCycle.cuh:251-304): pre-smooth,
 // residual, restrict, zero the coarse guess, recurse, prolongate, post-smooth.
-// The coarsest level runs `coarse_sweeps` symmetric sweeps (the reference runs
-// 50 one-sided ones, VCycle.cuh:252-256).
+// The coarsest level is solved exactly when it has at most 64 cells and
+// otherwise gets `coarse_sweeps` symmetric sweeps (the reference runs 50
+// one-sided sweeps, VCycle.cuh:252-256).
 void vcycle(OGrid& g, int k) {
   Level& L = g.mg[k];
   if (k + 1 == (int)g.mg.size()) {
+    if (L.ncells <= DIRECT_MAX) {
+      coarse_direct(L);
+      return;
+    }
     for (int i = 0; i < g.coarse_sweeps; ++i) {
       smooth(g, L, 1, false);
       smooth(g, L, 1, true);
