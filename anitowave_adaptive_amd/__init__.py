@@ -60,7 +60,7 @@ class StepStats(ctypes.Structure):
 class LevelInfo(ctypes.Structure):
     _fields_ = [("dims", ctypes.c_int * 3), ("num_blocks", ctypes.c_uint32), ("total_cells", ctypes.c_uint32),
                 ("total_faces", ctypes.c_uint32 * 3), ("aggregated", ctypes.c_int), ("d_e", _V),
-                ("d_cell_off", _V), ("d_x", _V), ("d_b", _V), ("d_r", _V), ("d_liquid", _V)]
+                ("d_cell_off", _V), ("d_x", _V), ("d_b", _V), ("d_r", _V), ("d_liquid", _V), ("d_x64", _V)]
 
 
 _I = ctypes.c_int
@@ -390,6 +390,10 @@ class VCycleSolver:
             return download(self.grid.ctx, L.d_e, np.uint8, L.num_blocks)
         if name == "coff":
             return download(self.grid.ctx, L.d_cell_off, np.uint32, L.num_blocks + 1)
+        if name == "x64":
+            if not L.d_x64:
+                raise ValueError("no FP64 solution (precision 0 or no solve yet)")
+            return download(self.grid.ctx, L.d_x64, np.float64, L.total_cells)
         return download(self.grid.ctx, {"x": L.d_x, "b": L.d_b, "r": L.d_r}[name], np.float32, L.total_cells)
 
     def level_set(self, k, name, vals):

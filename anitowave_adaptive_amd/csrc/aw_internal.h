@@ -152,6 +152,12 @@ void k_pcg_p(hipStream_t s, const float* z, float* p, int n, const double* sc, i
 void k_pcg_xr(hipStream_t s, float* x, float* r, const float* p, const float* q, const uint8_t* liquid, int n,
               double* sc);
 void k_mask(hipStream_t s, float* x, const uint8_t* liquid, int n);
+// FP64 iterative refinement
+void k_residual64(hipStream_t s, const LevelDev& L, const double* x, const float* b, float* r32, const WG* wg,
+                  int nwg, const uint32_t* blist, double* rr);
+void k_axpy64(hipStream_t s, double* x, const float* e, const uint8_t* liquid, int n);
+void k_to64(hipStream_t s, double* x, const float* p, const uint8_t* liquid, int n);
+void k_to32(hipStream_t s, float* p, const double* x, int n);
 void k_add(hipStream_t s, float* x, const float* z, const uint8_t* liquid, int n);
 
 }  // namespace aw

@@ -862,14 +862,13 @@ __device__ __forceinline__ void cell_ref(const LevelDev& L, int B, int li, CellR
 // Coefficients: beta * s between equal cells (s = cell edge in fine cells),
 // beta * s_f / 1.5 across a coarse-fine interface, one term per fine
 // sub-face; tank walls carry beta = 0.
-template <class Out>
-__device__ __forceinline__ void stencil_split(const LevelDev& L, const CellRef& cr, Out outb, float& diag, float& ext,
-                                              float* w) {
+template <class Out, typename T>
+__device__ __forceinline__ void stencil_split(const LevelDev& L, const CellRef& cr, Out outb, T& diag, T& ext, T* w) {
   const int B = cr.B, r = cr.r;
   const int e = L.e[B];
-  const float s = (float)(1 << e);
-  diag = 0.f;
-  ext = 0.f;
+  const T s = (T)(1 << e);
+  diag = 0;
+  ext = 0;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
@@ -877,12 +876,12 @@ __device__ __forceinline__ void stencil_split(const LevelDev& L, const CellRef& 
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
       const int d = 2 * a + side;
-      w[d] = 0.f;
+      w[d] = 0;
       const int nl = cr.l[a] + (side ? 1 : -1);
       int sl[3] = {cr.l[0], cr.l[1], cr.l[2]};
       sl[a] += side;
       if (nl >= 0 && nl < r) {
-        const float c = L.beta[a][fo + face_idx(a, sl, r)] * s;
+        const T c = (T)L.beta[a][fo + face_idx(a, sl, r)] * s;
         w[d] = c;
         diag += c;
         continue;
@@ -893,11 +892,11 @@ __device__ __forceinline__ void stencil_split(const LevelDev& L, const CellRef& 
       int nc[3] = {cr.l[0], cr.l[1], cr.l[2]};
       nc[a] = side ? 0 : rn - 1;
       if (en == e) {
-        const float c = L.beta[a][fo + face_idx(a, sl, r)] * s;
+        const T c = (T)L.beta[a][fo + face_idx(a, sl, r)] * s;
         diag += c;
         ext += c * outb(N, nc, rn, L.coff[N] + cell_idx(nc, rn));
       } else if (en > e) {
-        const float c = L.beta[a][fo + face_idx(a, sl, r)] * s / 1.5f;
+        const T c = (T)L.beta[a][fo + face_idx(a, sl, r)] * s / (T)1.5;
         nc[t1] = cr.l[t1] >> 1;
         nc[t2] = cr.l[t2] >> 1;
         diag += c;
@@ -906,14 +905,14 @@ __device__ __forceinline__ void stencil_split(const LevelDev& L, const CellRef& 
         int ps[3];
         ps[a] = side ? 0 : rn;
         const uint32_t fon = L.foff[a][N], con = L.coff[N];
-        const float sf = 0.5f * s;
+        const T sf = (T)0.5 * s;
 #pragma unroll
         for (int d1 = 0; d1 < 2; ++d1)
 #pragma unroll
           for (int d2 = 0; d2 < 2; ++d2) {
             nc[t1] = ps[t1] = 2 * cr.l[t1] + d1;
             nc[t2] = ps[t2] = 2 * cr.l[t2] + d2;
-            const float c = L.beta[a][fon + face_idx(a, ps, rn)] * sf / 1.5f;
+            const T c = (T)L.beta[a][fon + face_idx(a, ps, rn)] * sf / (T)1.5;
             diag += c;
             ext += c * outb(N, nc, rn, con + cell_idx(nc, rn));
           }
@@ -1460,6 +1459,56 @@ __global__ __launch_bounds__(TPB) void coarse_factor_kernel(LevelDev L, int n, d
   }
 }
 
+
+// ---- FP64 iterative refinement on level 0 ----
+// r = b - A x with x and the stencil in FP64 (b is the FP32 right-hand side);
+// r is handed to the FP32 inner solve, ||r||^2 accumulates in FP64.
+__global__ __launch_bounds__(TPB) void residual64_kernel(LevelDev L, const double* __restrict__ x,
+                                                         const float* __restrict__ b, float* __restrict__ r32,
+                                                         const WG* __restrict__ wgt,
+                                                         const uint32_t* __restrict__ blist, double* rr) {
+  const WG w = wgt[blockIdx.x];
+  double part = 0.0;
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    CellRef c;
+    if (!decode(L, w, blist, threadIdx.x + k * TPB, c)) continue;
+    if (!L.liquid[c.c]) {
+      r32[c.c] = 0.f;
+      continue;
+    }
+    double dg, ex, cw[6];
+    stencil_split(
+        L, c, [&](int, const int*, int, uint32_t cc) { return x[cc]; }, dg, ex, cw);
+    const uint32_t base = c.c - c.li;
+    double off = ex;
+    const int st[3] = {1, c.r, c.r * c.r};
+#pragma unroll
+    for (int d = 0; d < 6; ++d)
+      if (cw[d] != 0.0) off += cw[d] * x[base + c.li + ((d & 1) ? st[d >> 1] : -st[d >> 1])];
+    const double rv = (double)b[c.c] - (dg * x[c.c] - off);
+    r32[c.c] = (float)rv;
+    part += rv * rv;
+  }
+  const double s = block_reduce_sum(part);
+  if (threadIdx.x == 0) atomicAdd(rr, s);
+}
+
+__global__ void axpy64_kernel(double* __restrict__ x, const float* __restrict__ e, const uint8_t* __restrict__ liq,
+                              int n) {
+  for (int i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB)
+    if (liq[i]) x[i] += (double)e[i];
+}
+
+__global__ void to64_kernel(double* __restrict__ x, const float* __restrict__ p, const uint8_t* __restrict__ liq,
+                            int n) {
+  for (int i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) x[i] = liq[i] ? (double)p[i] : 0.0;
+}
+
+__global__ void to32_kernel(float* __restrict__ p, const double* __restrict__ x, int n) {
+  for (int i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) p[i] = (float)x[i];
+}
+
 // ------------------------------------------------------------- vectors
 __global__ __launch_bounds__(TPB) void dot_kernel(const float* __restrict__ a, const float* __restrict__ b,
                                                   const uint8_t* __restrict__ liq, int n, double* acc) {
@@ -1644,6 +1693,21 @@ void k_tail(hipStream_t s, const TailArgs& args, size_t lds_bytes) {
 void k_coarse_factor(hipStream_t s, const LevelDev& L, int n, double* inv) {
   coarse_factor_kernel<<<1, TPB, 0, s>>>(L, n, inv);
 }
+
+void k_residual64(hipStream_t s, const LevelDev& L, const double* x, const float* b, float* r32, const WG* wg,
+                  int nwg, const uint32_t* blist, double* rr) {
+  if (nwg > 0) residual64_kernel<<<nwg, TPB, 0, s>>>(L, x, b, r32, wg, blist, rr);
+}
+
+void k_axpy64(hipStream_t s, double* x, const float* e, const uint8_t* liquid, int n) {
+  axpy64_kernel<<<red_grid(n), TPB, 0, s>>>(x, e, liquid, n);
+}
+
+void k_to64(hipStream_t s, double* x, const float* p, const uint8_t* liquid, int n) {
+  to64_kernel<<<red_grid(n), TPB, 0, s>>>(x, p, liquid, n);
+}
+
+void k_to32(hipStream_t s, float* p, const double* x, int n) { to32_kernel<<<red_grid(n), TPB, 0, s>>>(p, x, n); }
 
 void k_dot(hipStream_t s, const float* a, const float* b, const uint8_t* liquid, int n, double* acc) {
   dot_kernel<<<red_grid(n), TPB, 0, s>>>(a, b, liquid, n, acc);

@@ -231,6 +231,7 @@ struct aw_solver {
   size_t tail_lds = 0;
   DevBuf cinv;           // FP64 inverse of a small coarsest level
   bool direct = false;
+  DevBuf x64, e32, r32;  // FP64 iterative refinement: solution, FP32 correction, FP32 residual
   DevBuf r0, z, pv, q, sc;
   double* h_sc = nullptr;
 };
@@ -726,13 +727,12 @@ int read_scalars(aw_solver* S) {
   return launch_status("solver");
 }
 
-int solve(aw_solver* S, double tol, int max_iters, aw_solve_stats* st) {
+// FP32 conjugate gradients (or stationary V-cycles) on level 0 for A x = b.
+int solve_fp32(aw_solver* S, float* x, const float* b, double tol, int max_iters, aw_solve_stats* st) {
   aw_grid* g = S->g;
   hipStream_t s = g->ctx->stream;
   MGLevel& L0 = *S->lv[0];
   const int n = (int)L0.ncells;
-  float* x = g->p.as<float>();
-  const float* b = g->rhs.as<float>();
   const uint8_t* liq = g->liquid.as<uint8_t>();
   float* r = S->r0.as<float>();
   float* z = S->z.as<float>();
@@ -835,6 +835,76 @@ int solve(aw_solver* S, double tol, int max_iters, aw_solve_stats* st) {
   }
   AW_TRY(launch_status("solve"));
   return rn <= tol * bn ? AW_OK : fail(AW_ERR_NOT_CONVERGED, "relative residual %.3e after %d iterations", rn / bn, it);
+}
+
+// FP32 V-cycle PCG inside FP64 iterative refinement: the fine-level
+// residual and the accumulated solution are FP64, every correction is an FP32
+// solve to relative accuracy 1e-4, the coarsest level is solved in FP64; the
+// FP32 pressure used by the projection is the rounded FP64 solution.
+int solve_mixed(aw_solver* S, double tol, int max_iters, aw_solve_stats* st) {
+  aw_grid* g = S->g;
+  hipStream_t s = g->ctx->stream;
+  MGLevel& L0 = *S->lv[0];
+  const int n = (int)L0.ncells;
+  const uint8_t* liq = g->liquid.as<uint8_t>();
+  double* sc = S->sc.as<double>();
+  AW_TRY(S->x64.ensure((size_t)n * sizeof(double)));
+  AW_TRY(S->e32.ensure((size_t)n * sizeof(float)));
+  AW_TRY(S->r32.ensure((size_t)n * sizeof(float)));
+  double* x64 = S->x64.as<double>();
+  float* e = S->e32.as<float>();
+  float* r = S->r32.as<float>();
+  k_to64(s, x64, g->p.as<float>(), liq, n);
+  k_fill_f64(s, sc, SC_N, 0.0);
+  k_dot(s, g->rhs.as<float>(), g->rhs.as<float>(), liq, n, sc + SC_BB);
+  AW_TRY(read_scalars(S));
+  const double bn = std::sqrt(S->h_sc[SC_BB]);
+  if (st) {
+    st->b_norm = bn;
+    st->iterations = 0;
+    st->outer_iterations = 0;
+    st->rel_residual = 0.0;
+  }
+  if (!(bn > 0.0)) {
+    k_fill_f32(s, g->p.as<float>(), n, 0.f);
+    k_fill_f64(s, x64, n, 0.0);
+    return launch_status("solve");
+  }
+  if (!std::isfinite(bn)) return fail(AW_ERR_NAN, "non-finite right-hand side");
+  int inner_total = 0, outer = 0;
+  double rn = 0.0;
+  for (;;) {
+    k_scalar_shift(s, sc);
+    {
+      Prof pf(g->ctx, "residual64_l0", 8.0 * n + 4.0 * n + 4.0 * n + 12.0 * L0.nfaces);
+      k_residual64(s, L0.view(), x64, g->rhs.as<float>(), r, L0.wgp(), L0.nwg, L0.bl(), sc + SC_RR);
+    }
+    AW_TRY(read_scalars(S));
+    rn = std::sqrt(S->h_sc[SC_RR]);
+    if (!std::isfinite(rn)) return fail(AW_ERR_NAN, "iterative refinement diverged");
+    if (rn <= tol * bn || inner_total >= max_iters || outer >= 30) break;
+    k_fill_f32(s, e, n, 0.f);
+    aw_solve_stats ist{};
+    int sti = solve_fp32(S, e, r, 1e-4, max_iters - inner_total, &ist);
+    if (sti != AW_OK && sti != AW_ERR_NOT_CONVERGED) return sti;
+    inner_total += std::max(ist.iterations, 1);
+    k_axpy64(s, x64, e, liq, n);
+    ++outer;
+  }
+  k_to32(s, g->p.as<float>(), x64, n);
+  if (st) {
+    st->iterations = inner_total;
+    st->outer_iterations = outer;
+    st->rel_residual = rn / bn;
+  }
+  AW_TRY(launch_status("solve"));
+  return rn <= tol * bn ? AW_OK
+                        : fail(AW_ERR_NOT_CONVERGED, "relative residual %.3e after %d refinements", rn / bn, outer);
+}
+
+int solve(aw_solver* S, double tol, int max_iters, aw_solve_stats* st) {
+  if (S->prm.precision == 1) return solve_mixed(S, tol, max_iters, st);
+  return solve_fp32(S, S->g->p.as<float>(), S->g->rhs.as<float>(), tol, max_iters, st);
 }
 
 bool valid_params(const aw_solver_params* p) {
@@ -1132,7 +1202,6 @@ int aw_solver_create(aw_grid* g, const aw_solver_params* p, aw_solver** out) {
   if (!g || !out) return fail(AW_ERR_INVALID, "NULL argument");
   *out = nullptr;
   if (!valid_params(p)) return fail(AW_ERR_INVALID, "invalid solver parameters");
-  if (p->precision != 0) return fail(AW_ERR_UNSUPPORTED, "mixed precision is not available in this build");
   auto S = std::make_unique<aw_solver>();
   S->g = g;
   S->prm = *p;
@@ -1173,6 +1242,7 @@ int aw_solver_level_info(aw_solver* S, int k, aw_level_info* o) {
   o->d_b = L.d_b;
   o->d_r = L.d_r;
   o->d_liquid = L.d_liq;
+  o->d_x64 = (k == 0 && S->prm.precision == 1 && S->x64.p) ? S->x64.as<double>() : nullptr;
   return AW_OK;
 }
 

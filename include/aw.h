@@ -131,6 +131,7 @@ typedef struct {
   float* d_b;
   float* d_r;
   uint8_t* d_liquid;
+  const double* d_x64;     /* level 0 with precision 1: the FP64 solution of the last solve */
 } aw_level_info;
 
 const char* aw_status_str(int status);
