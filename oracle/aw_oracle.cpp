@@ -21,7 +21,10 @@
 // brute-force topology on tiny grids, A = A^T and A SPD, a dense solve of the
 // single-level box problem, partition of unity / linear-field reproduction of
 // the transfers, mass conservation of P2G, hydrostatic column, divergence of
-// the projected field equal to the solver residual, RK3 exact translation.
+// the projected field equal to the solver residual, RK3 exact translation;
+// the adjoint: Pi / Pi^T transpose identity, whole-step and rollout
+// directional derivatives against central finite differences of the FP64
+// forward with frozen topology (tests/test_oracle_adjoint.py).
 
 #include <algorithm>
 #include <cmath>
@@ -90,6 +93,11 @@ struct OGrid {
   std::vector<uint32_t> pcount;
   // virtual fine lattice: [block][axis][512]
   std::vector<double> Fm, FM, Fnew, Fdel;
+  // adjoint fields (bar = derivative of the loss with respect to the field)
+  std::vector<double> Fbar_nd, Fbar_d;   // fine lattice: Fnew-bar + Fdel-bar, Fdel-bar
+  std::vector<double> FMbar, Fmbar;      // fine lattice: deposit accumulators
+  std::vector<double> ubar_new[3], ubar_old[3], ubar_star[3], Mbar[3], mbar[3];
+  std::vector<double> pbar, rhsbar;
   // multigrid
   std::vector<Level> mg;
   double omega = 0.85714f;
@@ -118,6 +126,10 @@ inline void fine_cell(const OGrid& g, const float* x, int* gc) {
     gc[a] = iclamp(v, 0, g.N[a] - 1);
   }
 }
+// FP64 positions (the finite-difference rollouts): x / h is exact as well.
+inline void fine_cell(const OGrid& g, const double* x, int* gc) {
+  for (int a = 0; a < 3; ++a) gc[a] = iclamp((int)std::floor(x[a] / g.h), 0, g.N[a] - 1);
+}
 
 // ------------------------------------------------------- native topology
 void set_topology(OGrid& g, const uint8_t* level) {
@@ -132,7 +144,9 @@ void set_topology(OGrid& g, const uint8_t* level) {
   g.ncells = g.coff[g.nb];
   for (int a = 0; a < 3; ++a) {
     g.nfaces[a] = g.foff[a][g.nb];
-    for (auto* v : {&g.Macc[a], &g.macc[a], &g.uold[a], &g.ustar[a], &g.unew[a], &g.mt[a]}) v->assign(g.nfaces[a], 0.0);
+    for (auto* v : {&g.Macc[a], &g.macc[a], &g.uold[a], &g.ustar[a], &g.unew[a], &g.mt[a], &g.ubar_new[a],
+                    &g.ubar_old[a], &g.ubar_star[a], &g.Mbar[a], &g.mbar[a]})
+      v->assign(g.nfaces[a], 0.0);
     // beta = 1/rho on every interior face, 0 on the tank walls (Neumann); the
     // reference sets 1 everywhere and treats the outside as Dirichlet air
     // (main.cu:294-296, VCycle.cuh:43-44).
@@ -156,6 +170,8 @@ void set_topology(OGrid& g, const uint8_t* level) {
   }
   g.p.assign(g.ncells, 0.0);
   g.rhs.assign(g.ncells, 0.0);
+  g.pbar.assign(g.ncells, 0.0);
+  g.rhsbar.assign(g.ncells, 0.0);
   g.liquid.assign(g.ncells, 0);
   g.pcount.assign(g.ncells, 0);
   g.mg.clear();
@@ -716,8 +732,12 @@ inline void to_units(const OGrid& g, const float* x, double* u) {
     u[a] = v;
   }
 }
+inline void to_units(const OGrid& g, const double* x, double* u) {
+  for (int a = 0; a < 3; ++a) u[a] = std::min(std::max(x[a] / g.h, 0.0), (double)g.N[a] - 0.0009765625);
+}
 
-void p2g(OGrid& g, const float* pos, const float* vel, int64_t n, double dt, const double* grav) {
+template <class T>
+void p2g(OGrid& g, const T* pos, const T* vel, int64_t n, double dt, const double* grav) {
   const size_t fl = (size_t)g.nb * 3 * 512;
   g.FM.assign(fl, 0.0);
   g.Fm.assign(fl, 0.0);
@@ -1000,7 +1020,8 @@ inline void sample(const OGrid& g, const std::vector<double>& F, const double* u
 // FLIP/PIC blend (g2pUpdateKernel, G2P.cuh:87-128) then Ralston RK3 through the
 // new field (advectParticlesKernel, G2P.cuh:131-173) with the wall clamp in
 // cells: [h/4, extent - h/4] (the reference clamps 0.1 world units, A9).
-void g2p_advect(const OGrid& g, const float* pos, const float* vel, int64_t n, double alpha, double dt,
+template <class T>
+void g2p_advect(const OGrid& g, const T* pos, const T* vel, int64_t n, double alpha, double dt,
                 double* pos_out, double* vel_out) {
   const double h = g.h;
 #pragma omp parallel for schedule(static)
@@ -1028,6 +1049,426 @@ void g2p_advect(const OGrid& g, const float* pos, const float* vel, int64_t n, d
       double xn = x[a] + dt * ((2.0 / 9.0) * vn[a] + (3.0 / 9.0) * v2[a] + (4.0 / 9.0) * v3[a]);
       double lo = 0.25 * h, hi = g.N[a] * h - 0.25 * h;
       pos_out[3 * p + a] = std::min(std::max(xn, lo), hi);
+    }
+  }
+}
+
+// ------------------------------------------------------------ adjoint
+// Reverse-mode derivative of one step with the topology, the liquid mask and
+// the mass thresholds held at their forward values (they are piecewise
+// constant in the particle state).  Every linear stage is transposed slot by
+// slot from its forward formula above; the nonlinear ones (u = M / m, the
+// trilinear weights as functions of position, the position clamps) are
+// differentiated at the forward point.  Accumulations into shared fields use
+// atomics so the transposes read exactly like the forward loops.
+
+// Trilinear stencil with d(weight)/d(fine units) along each axis.
+inline void stencil8_grad(const OGrid& g, int a, const double* u, int64_t* addr, double* w, double (*dw)[3]) {
+  int i0[3];
+  double t[3];
+  for (int d = 0; d < 3; ++d) {
+    double c = (d == a) ? u[d] : u[d] - 0.5;
+    i0[d] = (int)std::floor(c);
+    t[d] = c - i0[d];
+  }
+  int k = 0;
+  for (int dz = 0; dz < 2; ++dz)
+    for (int dy = 0; dy < 2; ++dy)
+      for (int dx = 0; dx < 2; ++dx, ++k) {
+        int dd[3] = {dx, dy, dz};
+        int gn[3];
+        double f[3], df[3];
+        for (int d = 0; d < 3; ++d) {
+          gn[d] = i0[d] + dd[d];
+          f[d] = dd[d] ? t[d] : 1.0 - t[d];
+          df[d] = dd[d] ? 1.0 : -1.0;
+        }
+        addr[k] = fine_addr(g, a, gn);
+        w[k] = f[0] * f[1] * f[2];
+        dw[k][0] = df[0] * f[1] * f[2];
+        dw[k][1] = f[0] * df[1] * f[2];
+        dw[k][2] = f[0] * f[1] * df[2];
+      }
+}
+
+// grad[d] += sum_a wbar[a] d S_a(F)(u) / du_d
+inline void sample_vjp(const OGrid& g, const std::vector<double>& F, const double* u, const double* wbar, double* grad) {
+  for (int a = 0; a < 3; ++a) {
+    if (wbar[a] == 0.0) continue;
+    int64_t ad[8];
+    double w[8], dw[8][3];
+    stencil8_grad(g, a, u, ad, w, dw);
+    for (int k = 0; k < 8; ++k) {
+      if (ad[k] < 0) continue;
+      for (int d = 0; d < 3; ++d) grad[d] += wbar[a] * F[ad[k]] * dw[k][d];
+    }
+  }
+}
+
+// F[node] += w(u) * val[a] on every component lattice (the transpose of sample)
+inline void deposit_atomic(const OGrid& g, std::vector<double>& F, const double* u, const double* val) {
+  for (int a = 0; a < 3; ++a) {
+    if (val[a] == 0.0) continue;
+    int64_t ad[8];
+    double w[8];
+    stencil8(g, a, u, ad, w);
+    for (int k = 0; k < 8; ++k) {
+      if (ad[k] < 0) continue;
+#pragma omp atomic
+      F[ad[k]] += w[k] * val[a];
+    }
+  }
+}
+
+// Pi^T of a fine-lattice field into per-slot accumulators, scaled by sgn.
+void pi_transpose(const OGrid& g, const std::vector<double>& F, std::vector<double>* out, double sgn) {
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int b = 0; b < g.nb; ++b) {
+    int e = g.level[b];
+    for (int a = 0; a < 3; ++a)
+      for (int li = 0; li < 512; ++li) {
+        double v = F[((int64_t)b * 3 + a) * 512 + li];
+        if (v == 0.0) continue;
+        int l[3] = {li & 7, (li >> 3) & 7, li >> 6};
+        PiW pw;
+        pi_weights(a, e, l, pw);
+        for (int k = 0; k < pw.n; ++k) out[a][g.foff[a][b] + pw.slot[k]] += sgn * pw.w[k] * v;
+      }
+  }
+}
+
+// Pi of per-slot fields onto the fine lattice.
+void pi_forward(const OGrid& g, const std::vector<double>* S, std::vector<double>& F) {
+  F.assign((size_t)g.nb * 3 * 512, 0.0);
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int b = 0; b < g.nb; ++b) {
+    int e = g.level[b];
+    for (int a = 0; a < 3; ++a)
+      for (int li = 0; li < 512; ++li) {
+        int l[3] = {li & 7, (li >> 3) & 7, li >> 6};
+        PiW pw;
+        pi_weights(a, e, l, pw);
+        double v = 0.0;
+        for (int k = 0; k < pw.n; ++k) v += pw.w[k] * S[a][g.foff[a][b] + pw.slot[k]];
+        F[((int64_t)b * 3 + a) * 512 + li] = v;
+      }
+  }
+}
+
+// Adjoint of g2p_advect: given xbo = dL/dx' and vbo = dL/dv', write the
+// particles' direct partials xbi, vbi and accumulate Fbar_nd (= Fnew-bar +
+// Fdel-bar) and Fbar_d on the fine lattice; then ubar_new = Pi^T Fbar_nd and
+// ubar_old = -Pi^T Fbar_d.
+template <class T>
+void g2p_advect_adjoint(OGrid& g, const T* pos, const T* vel, int64_t n, double alpha, double dt, const double* xbo,
+                        const double* vbo, double* xbi, double* vbi) {
+  const size_t fl = (size_t)g.nb * 3 * 512;
+  g.Fbar_nd.assign(fl, 0.0);
+  g.Fbar_d.assign(fl, 0.0);
+  const double h = g.h;
+#pragma omp parallel for schedule(static)
+  for (int64_t p = 0; p < n; ++p) {
+    double u[3], m1[3], vn[3], vd[3], x[3];
+    to_units(g, pos + 3 * p, u);
+    for (int a = 0; a < 3; ++a) {
+      x[a] = (double)pos[3 * p + a];
+      double raw = x[a] / h;
+      m1[a] = (raw > 0.0 && raw < (double)g.N[a] - 0.0009765625) ? 1.0 : 0.0;
+    }
+    sample(g, g.Fnew, u, vn);
+    sample(g, g.Fdel, u, vd);
+    auto units = [&](const double* xx, double* uu, double* mm) {
+      for (int a = 0; a < 3; ++a) {
+        double t = xx[a] / h;
+        mm[a] = (t > -1.0 && t < (double)g.N[a] + 1.0) ? 1.0 : 0.0;
+        uu[a] = std::min(std::max(t, -1.0), (double)g.N[a] + 1.0);
+      }
+    };
+    double x2[3], u2[3], m2[3], v2[3], x3[3], u3[3], m3[3], v3[3];
+    for (int a = 0; a < 3; ++a) x2[a] = x[a] + 0.5 * dt * vn[a];
+    units(x2, u2, m2);
+    sample(g, g.Fnew, u2, v2);
+    for (int a = 0; a < 3; ++a) x3[a] = x[a] + 0.75 * dt * v2[a];
+    units(x3, u3, m3);
+    sample(g, g.Fnew, u3, v3);
+    double xb[3], k1[3], k2[3], k3[3], xacc[3];
+    for (int a = 0; a < 3; ++a) {
+      double xn = x[a] + dt * ((2.0 / 9.0) * vn[a] + (3.0 / 9.0) * v2[a] + (4.0 / 9.0) * v3[a]);
+      double lo = 0.25 * h, hi = g.N[a] * h - 0.25 * h;
+      xb[a] = (xn > lo && xn < hi) ? xbo[3 * p + a] : 0.0;
+      k1[a] = dt * (2.0 / 9.0) * xb[a];
+      k2[a] = dt * (3.0 / 9.0) * xb[a];
+      k3[a] = dt * (4.0 / 9.0) * xb[a];
+      xacc[a] = xb[a];
+    }
+    // stage 3: v3 = S(Fnew)(x3 / h), x3 = x + 3/4 dt v2
+    double gr[3] = {0, 0, 0};
+    deposit_atomic(g, g.Fbar_nd, u3, k3);
+    sample_vjp(g, g.Fnew, u3, k3, gr);
+    for (int d = 0; d < 3; ++d) {
+      double x3b = gr[d] / h * m3[d];
+      xacc[d] += x3b;
+      k2[d] += 0.75 * dt * x3b;
+    }
+    // stage 2: v2 = S(Fnew)(x2 / h), x2 = x + 1/2 dt vn
+    gr[0] = gr[1] = gr[2] = 0.0;
+    deposit_atomic(g, g.Fbar_nd, u2, k2);
+    sample_vjp(g, g.Fnew, u2, k2, gr);
+    for (int d = 0; d < 3; ++d) {
+      double x2b = gr[d] / h * m2[d];
+      xacc[d] += x2b;
+      k1[d] += 0.5 * dt * x2b;
+    }
+    // G2P blend and stage 1 at x
+    double vnb[3], vdb[3], both[3];
+    for (int a = 0; a < 3; ++a) {
+      vnb[a] = k1[a] + (1.0 - alpha) * vbo[3 * p + a];
+      vdb[a] = alpha * vbo[3 * p + a];
+      both[a] = vnb[a] + vdb[a];
+    }
+    deposit_atomic(g, g.Fbar_nd, u, both);
+    deposit_atomic(g, g.Fbar_d, u, vdb);
+    gr[0] = gr[1] = gr[2] = 0.0;
+    sample_vjp(g, g.Fnew, u, vnb, gr);
+    sample_vjp(g, g.Fdel, u, vdb, gr);
+    for (int d = 0; d < 3; ++d) {
+      xbi[3 * p + d] = xacc[d] + gr[d] / h * m1[d];
+      vbi[3 * p + d] = alpha * vbo[3 * p + d];
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    std::fill(g.ubar_new[a].begin(), g.ubar_new[a].end(), 0.0);
+    std::fill(g.ubar_old[a].begin(), g.ubar_old[a].end(), 0.0);
+  }
+  pi_transpose(g, g.Fbar_nd, g.ubar_new, 1.0);
+  pi_transpose(g, g.Fbar_d, g.ubar_old, -1.0);
+}
+
+inline void atomic_add(double& t, double v) {
+#pragma omp atomic
+  t += v;
+}
+
+// Adjoint of project(): ubar_new -> ubar_star (the direct u* terms) and pbar
+// (the gradient terms), pbar kept on liquid cells only (air cells are not
+// unknowns).
+void project_adjoint(OGrid& g) {
+  for (int a = 0; a < 3; ++a) std::fill(g.ubar_star[a].begin(), g.ubar_star[a].end(), 0.0);
+  std::fill(g.pbar.begin(), g.pbar.end(), 0.0);
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int b = 0; b < g.nb; ++b) {
+    int c[3];
+    block_coords(g.D, b, c);
+    int e = g.level[b], r = B >> e;
+    double s = (double)(1 << e);
+    for (int a = 0; a < 3; ++a) {
+      int t1 = (a + 1) % 3, t2 = (a + 2) % 3;
+      int nn[3] = {r, r, r};
+      nn[a] = r + 1;
+      for (int z = 0; z < nn[2]; ++z)
+        for (int y = 0; y < nn[1]; ++y)
+          for (int x = 0; x < nn[0]; ++x) {
+            int sl[3] = {x, y, z};
+            int64_t id = g.foff[a][b] + face_index(a, x, y, z, r);
+            double ub = g.ubar_new[a][id], bt = g.beta[a][id];
+            if (ub == 0.0) continue;
+            if (sl[a] > 0 && sl[a] < r) {
+              int cm[3] = {x, y, z};
+              cm[a] -= 1;
+              atomic_add(g.ubar_star[a][id], ub);
+              atomic_add(g.pbar[g.coff[b] + cell_index(x, y, z, r)], -bt / s * ub);
+              atomic_add(g.pbar[g.coff[b] + cell_index(cm[0], cm[1], cm[2], r)], bt / s * ub);
+              continue;
+            }
+            bool lo = sl[a] == 0;
+            double sg = lo ? 1.0 : -1.0;  // out = us - bt * sg * (po - pn) / d
+            int cc[3] = {c[0], c[1], c[2]};
+            cc[a] += lo ? -1 : 1;
+            if (cc[a] < 0 || cc[a] >= g.D[a]) continue;
+            int nbb = block_id(g.D, cc[0], cc[1], cc[2]);
+            int en = g.level[nbb], rn = B >> en;
+            int own[3] = {x, y, z};
+            own[a] = lo ? 0 : r - 1;
+            int64_t co = g.coff[b] + cell_index(own[0], own[1], own[2], r);
+            if (en == e || en == e + 1) {
+              int q[3] = {x, y, z};
+              q[a] = lo ? rn - 1 : 0;
+              double d = s;
+              if (en == e + 1) {
+                q[t1] = sl[t1] >> 1;
+                q[t2] = sl[t2] >> 1;
+                d = 1.5 * s;
+              }
+              int64_t cn = g.coff[nbb] + cell_index(q[0], q[1], q[2], rn);
+              atomic_add(g.ubar_star[a][id], ub);
+              atomic_add(g.pbar[co], -bt * sg / d * ub);
+              atomic_add(g.pbar[cn], bt * sg / d * ub);
+            } else {
+              double sf = 0.5 * s;
+              for (int d1 = 0; d1 < 2; ++d1)
+                for (int d2 = 0; d2 < 2; ++d2) {
+                  int q[3], ps[3];
+                  q[a] = lo ? rn - 1 : 0;
+                  ps[a] = lo ? rn : 0;
+                  q[t1] = ps[t1] = 2 * sl[t1] + d1;
+                  q[t2] = ps[t2] = 2 * sl[t2] + d2;
+                  int64_t pid = g.foff[a][nbb] + face_index(a, ps[0], ps[1], ps[2], rn);
+                  int64_t cn = g.coff[nbb] + cell_index(q[0], q[1], q[2], rn);
+                  double k = 0.25 * g.beta[a][pid] * sg / (1.5 * sf);
+                  atomic_add(g.ubar_star[a][pid], 0.25 * ub);
+                  atomic_add(g.pbar[co], -k * ub);
+                  atomic_add(g.pbar[cn], k * ub);
+                }
+            }
+          }
+    }
+  }
+  for (int64_t c = 0; c < g.ncells; ++c)
+    if (!g.liquid[c]) g.pbar[c] = 0.0;
+}
+
+// Adjoint of divergence(): ubar_star += D^T rhsbar with rhs = -s^2 sum(u+ - u-).
+void divergence_adjoint(OGrid& g) {
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int b = 0; b < g.nb; ++b) {
+    int r = block_res(g, b);
+    double s = (double)(1 << g.level[b]);
+    for (int z = 0; z < r; ++z)
+      for (int y = 0; y < r; ++y)
+        for (int x = 0; x < r; ++x) {
+          int64_t c = g.coff[b] + cell_index(x, y, z, r);
+          if (!g.liquid[c]) continue;
+          double k = s * s * g.rhsbar[c];
+          g.ubar_star[0][g.foff[0][b] + face_index(0, x + 1, y, z, r)] -= k;
+          g.ubar_star[0][g.foff[0][b] + face_index(0, x, y, z, r)] += k;
+          g.ubar_star[1][g.foff[1][b] + face_index(1, x, y + 1, z, r)] -= k;
+          g.ubar_star[1][g.foff[1][b] + face_index(1, x, y, z, r)] += k;
+          g.ubar_star[2][g.foff[2][b] + face_index(2, x, y, z + 1, r)] -= k;
+          g.ubar_star[2][g.foff[2][b] + face_index(2, x, y, z, r)] += k;
+        }
+  }
+}
+
+// Adjoint pressure solve: A is symmetric, so rhsbar = A^{-1} pbar with the
+// forward hierarchy.  The forward p and rhs are kept.
+int solve_adjoint(OGrid& g, double tol, int max_iters, int* iters, double* rel) {
+  std::vector<double> p0 = g.p, r0 = g.rhs;
+  for (int64_t c = 0; c < g.ncells; ++c) {
+    g.rhs[c] = g.liquid[c] ? g.pbar[c] : 0.0;
+    g.p[c] = 0.0;
+  }
+  int st = solve(g, 0, tol, max_iters, iters, rel);
+  g.rhsbar = g.p;
+  g.p = p0;
+  g.rhs = r0;
+  return st;
+}
+
+// Adjoint of the fold (u_old, u_star as functions of Macc, macc): with
+// ub = ubar_old + ubar_star per slot, every slot formula u = M' / m' with
+// M' = sum_j c_j Macc[j], m' = sum_j c_j macc[j] sends c_j ub / m' to
+// Mbar[j] and -c_j ub u / m' to mbar[j] (m' > 1e-6; frozen branch).
+void fold_adjoint(OGrid& g) {
+  const double eps = 1e-6;
+  for (int a = 0; a < 3; ++a) {
+    std::fill(g.Mbar[a].begin(), g.Mbar[a].end(), 0.0);
+    std::fill(g.mbar[a].begin(), g.mbar[a].end(), 0.0);
+  }
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int b = 0; b < g.nb; ++b) {
+    int c[3];
+    block_coords(g.D, b, c);
+    int r = block_res(g, b), e = g.level[b];
+    for (int a = 0; a < 3; ++a) {
+      int t1 = (a + 1) % 3, t2 = (a + 2) % 3;
+      int nn[3] = {r, r, r};
+      nn[a] = r + 1;
+      for (int z = 0; z < nn[2]; ++z)
+        for (int y = 0; y < nn[1]; ++y)
+          for (int x = 0; x < nn[0]; ++x) {
+            int sl[3] = {x, y, z};
+            int64_t id = g.foff[a][b] + face_index(a, x, y, z, r);
+            double ub = g.ubar_old[a][id] + g.ubar_star[a][id];
+            if (ub == 0.0) continue;
+            double M = g.Macc[a][id], m = g.macc[a][id];
+            bool bnd = sl[a] == 0 || sl[a] == r;
+            if (!bnd) {
+              if (m > eps) {
+                atomic_add(g.Mbar[a][id], ub / m);
+                atomic_add(g.mbar[a][id], -ub * (M / m) / m);
+              }
+              continue;
+            }
+            int cc[3] = {c[0], c[1], c[2]};
+            cc[a] += sl[a] == 0 ? -1 : 1;
+            if (cc[a] < 0 || cc[a] >= g.D[a]) continue;
+            int nbb = block_id(g.D, cc[0], cc[1], cc[2]);
+            int en = g.level[nbb], rn = B >> en;
+            int pa = sl[a] == 0 ? rn : 0;
+            if (en == e || en == e + 1) {
+              int ps[3] = {x, y, z};
+              ps[a] = pa;
+              double cp = 1.0;
+              if (en == e + 1) {
+                ps[t1] = sl[t1] >> 1;
+                ps[t2] = sl[t2] >> 1;
+                cp = 0.25;
+              }
+              int64_t pid = g.foff[a][nbb] + face_index(a, ps[0], ps[1], ps[2], rn);
+              double mt = m + cp * g.macc[a][pid];
+              if (mt <= eps) continue;
+              double u = (M + cp * g.Macc[a][pid]) / mt;
+              atomic_add(g.Mbar[a][id], ub / mt);
+              atomic_add(g.mbar[a][id], -ub * u / mt);
+              atomic_add(g.Mbar[a][pid], cp * ub / mt);
+              atomic_add(g.mbar[a][pid], -cp * ub * u / mt);
+            } else {
+              for (int d1 = 0; d1 < 2; ++d1)
+                for (int d2 = 0; d2 < 2; ++d2) {
+                  int ps[3];
+                  ps[a] = pa;
+                  ps[t1] = 2 * sl[t1] + d1;
+                  ps[t2] = 2 * sl[t2] + d2;
+                  int64_t pid = g.foff[a][nbb] + face_index(a, ps[0], ps[1], ps[2], rn);
+                  double ms = g.macc[a][pid] + 0.25 * m;
+                  if (ms <= eps) continue;
+                  double u0 = (g.Macc[a][pid] + 0.25 * M) / ms;
+                  double uk = 0.25 * ub;
+                  atomic_add(g.Mbar[a][pid], uk / ms);
+                  atomic_add(g.mbar[a][pid], -uk * u0 / ms);
+                  atomic_add(g.Mbar[a][id], 0.25 * uk / ms);
+                  atomic_add(g.mbar[a][id], -0.25 * uk * u0 / ms);
+                }
+            }
+          }
+    }
+  }
+}
+
+// Adjoint of the deposit: FMbar = Pi Mbar, Fmbar = Pi mbar, then per particle
+// vbar += S(FMbar)(x / h) and xbar += (v . dS(FMbar) + dS(Fmbar)) / h.
+template <class T>
+void p2g_adjoint(OGrid& g, const T* pos, const T* vel, int64_t n, double* xbar, double* vbar) {
+  fold_adjoint(g);
+  pi_forward(g, g.Mbar, g.FMbar);
+  pi_forward(g, g.mbar, g.Fmbar);
+  const double h = g.h;
+  const double ones[3] = {1.0, 1.0, 1.0};
+#pragma omp parallel for schedule(static)
+  for (int64_t p = 0; p < n; ++p) {
+    double u[3], m1[3], v[3], sv[3], gr[3] = {0, 0, 0};
+    to_units(g, pos + 3 * p, u);
+    for (int a = 0; a < 3; ++a) {
+      double raw = (double)pos[3 * p + a] / h;
+      m1[a] = (raw > 0.0 && raw < (double)g.N[a] - 0.0009765625) ? 1.0 : 0.0;
+      v[a] = (double)vel[3 * p + a];
+    }
+    sample(g, g.FMbar, u, sv);
+    sample_vjp(g, g.FMbar, u, v, gr);
+    sample_vjp(g, g.Fmbar, u, ones, gr);
+    for (int a = 0; a < 3; ++a) {
+      vbar[3 * p + a] += sv[a];
+      xbar[3 * p + a] += gr[a] / h * m1[a];
     }
   }
 }
@@ -1275,6 +1716,11 @@ static std::vector<double>* face_field(OGrid& g, int which, int a) {
     case 4: return &g.unew[a];
     case 5: return &g.mt[a];
     case 6: return &g.beta[a];
+    case 7: return &g.ubar_new[a];
+    case 8: return &g.ubar_old[a];
+    case 9: return &g.ubar_star[a];
+    case 10: return &g.Mbar[a];
+    case 11: return &g.mbar[a];
   }
   return nullptr;
 }
@@ -1303,6 +1749,8 @@ int og_get_cell(void* p, int which, double* out) {
       case 1: out[c] = g.rhs[c]; break;
       case 2: out[c] = g.liquid[c]; break;
       case 3: out[c] = g.pcount[c]; break;
+      case 4: out[c] = g.pbar[c]; break;
+      case 5: out[c] = g.rhsbar[c]; break;
       default: return 1;
     }
   }
@@ -1316,6 +1764,8 @@ int og_set_cell(void* p, int which, const double* in) {
       case 0: g.p[c] = in[c]; break;
       case 1: g.rhs[c] = in[c]; break;
       case 2: g.liquid[c] = in[c] != 0.0; break;
+      case 4: g.pbar[c] = in[c]; break;
+      case 5: g.rhsbar[c] = in[c]; break;
       default: return 1;
     }
   }
@@ -1361,5 +1811,138 @@ int og_step(void* p, const float* pos, const float* vel, int64_t n, double dt, d
 }
 
 int og_num_threads(void) { return omp_get_max_threads(); }
+
+static std::vector<double>* fine_field(OGrid& g, int which) {
+  switch (which) {
+    case 0: return &g.Fnew;
+    case 1: return &g.Fdel;
+    case 2: return &g.Fbar_nd;
+    case 3: return &g.Fbar_d;
+    case 4: return &g.FMbar;
+    case 5: return &g.Fmbar;
+    case 6: return &g.FM;
+    case 7: return &g.Fm;
+  }
+  return nullptr;
+}
+
+int og_get_fine(void* p, int which, double* out) {
+  OGrid& g = *static_cast<OGrid*>(p);
+  auto* v = fine_field(g, which);
+  if (!v) return 1;
+  std::fill(out, out + (size_t)g.nb * 3 * 512, 0.0);
+  std::copy(v->begin(), v->end(), out);
+  return 0;
+}
+
+int og_set_fine(void* p, int which, const double* in) {
+  OGrid& g = *static_cast<OGrid*>(p);
+  auto* v = fine_field(g, which);
+  if (!v) return 1;
+  v->assign(in, in + (size_t)g.nb * 3 * 512);
+  return 0;
+}
+
+// Pi and Pi^T between per-slot fields (which: face field index) and fine
+// lattice fields, for transpose tests.
+int og_pi(void* p, int face_which, int fine_which) {
+  OGrid& g = *static_cast<OGrid*>(p);
+  std::vector<double> S[3];
+  for (int a = 0; a < 3; ++a) S[a] = *face_field(g, face_which, a);
+  pi_forward(g, S, *fine_field(g, fine_which));
+  return 0;
+}
+
+int og_pi_transpose(void* p, int fine_which, int face_which) {
+  OGrid& g = *static_cast<OGrid*>(p);
+  std::vector<double> S[3];
+  for (int a = 0; a < 3; ++a) S[a].assign(g.nfaces[a], 0.0);
+  pi_transpose(g, *fine_field(g, fine_which), S, 1.0);
+  for (int a = 0; a < 3; ++a) *face_field(g, face_which, a) = S[a];
+  return 0;
+}
+
+// One FP64 step on FP64 particles (the finite-difference and adjoint
+// rollouts).  levels_in / liquid_in, when given, replace the topology and the
+// liquid mask the particles would produce (frozen topology); levels_out /
+// liquid_out receive the ones used.  Intermediates stay in the grid for
+// og_step_adjoint.
+int og_step64(void* p, const double* pos, const double* vel, int64_t n, double dt, double gx, double gy, double gz,
+              double alpha, int method, double tol, int max_iters, const uint8_t* levels_in,
+              const uint8_t* liquid_in, uint8_t* levels_out, uint8_t* liquid_out, double* pos_out, double* vel_out,
+              int* iters, double* rel) {
+  OGrid& g = *static_cast<OGrid*>(p);
+  std::vector<uint8_t> lv(g.nb);
+  if (levels_in) {
+    std::copy(levels_in, levels_in + g.nb, lv.begin());
+  } else {
+    std::vector<uint32_t> cnt(g.nb, 0);
+    for (int64_t i = 0; i < n; ++i) {
+      int gc[3];
+      fine_cell(g, pos + 3 * i, gc);
+      cnt[block_id(g.D, gc[0] >> LOGB, gc[1] >> LOGB, gc[2] >> LOGB)]++;
+    }
+    levels_from_counts(g, cnt.data(), lv.data());
+  }
+  set_topology(g, lv.data());
+  double gr[3] = {gx, gy, gz};
+  p2g(g, pos, vel, n, dt, gr);
+  if (liquid_in)
+    for (int64_t c = 0; c < g.ncells; ++c) g.liquid[c] = liquid_in[c] != 0;
+  divergence(g);
+  mg_build(g);
+  mg_refresh(g);
+  std::fill(g.p.begin(), g.p.end(), 0.0);
+  int st = solve(g, method, tol, max_iters, iters, rel);
+  project(g);
+  to_fine(g);
+  g2p_advect(g, pos, vel, n, alpha, dt, pos_out, vel_out);
+  if (levels_out) std::copy(g.level.begin(), g.level.end(), levels_out);
+  if (liquid_out) std::copy(g.liquid.begin(), g.liquid.end(), liquid_out);
+  return st;
+}
+
+// Adjoint stages.  fp64 selects the particle type (0: float, 1: double).
+int og_adj_g2p(void* p, int fp64, const void* pos, const void* vel, int64_t n, double alpha, double dt,
+               const double* xbo, const double* vbo, double* xbi, double* vbi) {
+  OGrid& g = *static_cast<OGrid*>(p);
+  if (g.Fnew.empty()) return 1;
+  if (fp64)
+    g2p_advect_adjoint(g, (const double*)pos, (const double*)vel, n, alpha, dt, xbo, vbo, xbi, vbi);
+  else
+    g2p_advect_adjoint(g, (const float*)pos, (const float*)vel, n, alpha, dt, xbo, vbo, xbi, vbi);
+  return 0;
+}
+
+int og_adj_project(void* p, double tol, int max_iters, int* iters, double* rel) {
+  OGrid& g = *static_cast<OGrid*>(p);
+  if (g.mg.empty()) return 1;
+  project_adjoint(g);
+  int st = solve_adjoint(g, tol, max_iters, iters, rel);
+  divergence_adjoint(g);
+  return st;
+}
+
+int og_adj_p2g(void* p, int fp64, const void* pos, const void* vel, int64_t n, double* xbar, double* vbar) {
+  OGrid& g = *static_cast<OGrid*>(p);
+  if (fp64)
+    p2g_adjoint(g, (const double*)pos, (const double*)vel, n, xbar, vbar);
+  else
+    p2g_adjoint(g, (const float*)pos, (const float*)vel, n, xbar, vbar);
+  return 0;
+}
+
+// Whole-step adjoint on the intermediates of the last og_step / og_step64
+// with these particles: xbar, vbar hold dL/d(x', v') on entry and dL/d(x, v)
+// on return.
+int og_step_adjoint(void* p, int fp64, const void* pos, const void* vel, int64_t n, double alpha, double dt,
+                    double tol, int max_iters, double* xbar, double* vbar, int* iters, double* rel) {
+  std::vector<double> xo(xbar, xbar + 3 * n), vo(vbar, vbar + 3 * n);
+  int st = og_adj_g2p(p, fp64, pos, vel, n, alpha, dt, xo.data(), vo.data(), xbar, vbar);
+  if (st) return st;
+  st = og_adj_project(p, tol, max_iters, iters, rel);
+  og_adj_p2g(p, fp64, pos, vel, n, xbar, vbar);
+  return st;
+}
 
 }  // extern "C"

@@ -71,6 +71,18 @@ def lib():
             "og_step": (ctypes.c_int, [_V, _F, _F, ctypes.c_int64] + [ctypes.c_double] * 5
                         + [ctypes.c_int, ctypes.c_double, ctypes.c_int, _D, _D, _I, _D, _D]),
             "og_num_threads": (ctypes.c_int, []),
+            "og_get_fine": (ctypes.c_int, [_V, ctypes.c_int, _D]),
+            "og_set_fine": (ctypes.c_int, [_V, ctypes.c_int, _D]),
+            "og_pi": (ctypes.c_int, [_V, ctypes.c_int, ctypes.c_int]),
+            "og_pi_transpose": (ctypes.c_int, [_V, ctypes.c_int, ctypes.c_int]),
+            "og_step64": (ctypes.c_int, [_V, _D, _D, ctypes.c_int64] + [ctypes.c_double] * 5
+                          + [ctypes.c_int, ctypes.c_double, ctypes.c_int, _U8, _U8, _U8, _U8, _D, _D, _I, _D]),
+            "og_adj_g2p": (ctypes.c_int, [_V, ctypes.c_int, _V, _V, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
+                                          _D, _D, _D, _D]),
+            "og_adj_project": (ctypes.c_int, [_V, ctypes.c_double, ctypes.c_int, _I, _D]),
+            "og_adj_p2g": (ctypes.c_int, [_V, ctypes.c_int, _V, _V, ctypes.c_int64, _D, _D]),
+            "og_step_adjoint": (ctypes.c_int, [_V, ctypes.c_int, _V, _V, ctypes.c_int64, ctypes.c_double,
+                                               ctypes.c_double, ctypes.c_double, ctypes.c_int, _D, _D, _I, _D]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -84,8 +96,17 @@ def _p(a, t):
     return a.ctypes.data_as(t)
 
 
-FACE_FIELDS = {"Macc": 0, "macc": 1, "u_old": 2, "u_star": 3, "u_new": 4, "mass": 5, "beta": 6}
-CELL_FIELDS = {"p": 0, "rhs": 1, "liquid": 2, "pcount": 3}
+FACE_FIELDS = {"Macc": 0, "macc": 1, "u_old": 2, "u_star": 3, "u_new": 4, "mass": 5, "beta": 6,
+               "ubar_new": 7, "ubar_old": 8, "ubar_star": 9, "Mbar": 10, "mbar": 11}
+CELL_FIELDS = {"p": 0, "rhs": 1, "liquid": 2, "pcount": 3, "pbar": 4, "rhsbar": 5}
+FINE_FIELDS = {"new": 0, "delta": 1, "bar_nd": 2, "bar_d": 3, "Mbar": 4, "mbar": 5, "M": 6, "m": 7}
+
+
+def _particles(pos, vel):
+    """Contiguous (n, 3) particle arrays and the oracle's precision flag."""
+    fp64 = pos.dtype == np.float64
+    t = np.float64 if fp64 else np.float32
+    return np.ascontiguousarray(pos, t).reshape(-1, 3), np.ascontiguousarray(vel, t).reshape(-1, 3), int(fp64)
 MG_FIELDS = {"x": 0, "b": 1, "r": 2, "liquid": 3, "diag": 4, "parent": 5}
 
 
@@ -274,6 +295,110 @@ class OracleGrid:
                            float(alpha), int(method), float(tol), int(max_iters), _p(po, _D), _p(vo, _D),
                            ctypes.byref(it), ctypes.byref(rel), _p(ms, _D))
         return dict(status=st, pos=po, vel=vo, iters=it.value, relres=rel.value, stage_ms=ms)
+
+
+    # FP64 rollouts and the adjoint
+    def step64(self, pos, vel, dt, gravity, alpha, tol=1e-10, max_iters=400, levels=None, liquid=None, method=0):
+        """One FP64 step on FP64 particles; `levels` / `liquid` freeze the topology and the liquid mask."""
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        vel = np.ascontiguousarray(vel, np.float64).reshape(-1, 3)
+        po = np.zeros(pos.shape, np.float64)
+        vo = np.zeros(vel.shape, np.float64)
+        it = ctypes.c_int(0)
+        rel = ctypes.c_double(0.0)
+        lv_in = None if levels is None else np.ascontiguousarray(levels, np.uint8)
+        lq_in = None if liquid is None else np.ascontiguousarray(liquid, np.uint8)
+        lv_out = np.zeros(self.nb, np.uint8)
+        # the liquid mask size follows the topology: size it after the step from the totals
+        lq_buf = np.zeros(self.nb * 512, np.uint8)
+        st = lib().og_step64(self._h, _p(pos, _D), _p(vel, _D), len(pos), float(dt), *[float(g) for g in gravity],
+                             float(alpha), int(method), float(tol), int(max_iters),
+                             None if lv_in is None else _p(lv_in, _U8), None if lq_in is None else _p(lq_in, _U8),
+                             _p(lv_out, _U8), _p(lq_buf, _U8), _p(po, _D), _p(vo, _D), ctypes.byref(it),
+                             ctypes.byref(rel))
+        nc = int(self.totals()[0])
+        return dict(status=st, pos=po, vel=vo, iters=it.value, relres=rel.value, levels=lv_out,
+                    liquid=lq_buf[:nc].copy())
+
+    def adj_g2p(self, pos, vel, xbar_out, vbar_out, alpha, dt):
+        """Adjoint of G2P + RK3: particle partials (xbar, vbar); fills ubar_new, ubar_old."""
+        pos, vel, f = _particles(pos, vel)
+        xo = np.ascontiguousarray(xbar_out, np.float64).reshape(-1, 3)
+        vo = np.ascontiguousarray(vbar_out, np.float64).reshape(-1, 3)
+        xi = np.zeros_like(xo)
+        vi = np.zeros_like(vo)
+        if lib().og_adj_g2p(self._h, f, pos.ctypes.data, vel.ctypes.data, len(pos), float(alpha), float(dt),
+                            _p(xo, _D), _p(vo, _D), _p(xi, _D), _p(vi, _D)) != 0:
+            raise RuntimeError("no forward fine lattice")
+        return xi, vi
+
+    def adj_project(self, tol=1e-12, max_iters=400):
+        """Adjoint of the projection: ubar_new -> pbar -> A^{-1} -> rhsbar -> ubar_star."""
+        it = ctypes.c_int(0)
+        rel = ctypes.c_double(0.0)
+        st = lib().og_adj_project(self._h, float(tol), int(max_iters), ctypes.byref(it), ctypes.byref(rel))
+        return st, it.value, rel.value
+
+    def adj_p2g(self, pos, vel, xbar, vbar):
+        """Adjoint of P2G (fold, Pi, deposit): accumulates into xbar, vbar (modified in place)."""
+        pos, vel, f = _particles(pos, vel)
+        assert xbar.dtype == np.float64 and vbar.dtype == np.float64 and xbar.flags.c_contiguous
+        lib().og_adj_p2g(self._h, f, pos.ctypes.data, vel.ctypes.data, len(pos), _p(xbar, _D), _p(vbar, _D))
+
+    def step_adjoint(self, pos, vel, xbar, vbar, alpha, dt, tol=1e-12, max_iters=400):
+        """dL/d(x, v) of the last step run with these particles, from dL/d(x', v')."""
+        pos, vel, f = _particles(pos, vel)
+        xb = np.array(xbar, np.float64).reshape(-1, 3)
+        vb = np.array(vbar, np.float64).reshape(-1, 3)
+        it = ctypes.c_int(0)
+        rel = ctypes.c_double(0.0)
+        st = lib().og_step_adjoint(self._h, f, pos.ctypes.data, vel.ctypes.data, len(pos), float(alpha), float(dt),
+                                   float(tol), int(max_iters), _p(xb, _D), _p(vb, _D), ctypes.byref(it),
+                                   ctypes.byref(rel))
+        if st not in (0,):
+            raise RuntimeError(f"adjoint solve status {st}")
+        return xb, vb
+
+    def fine(self, name):
+        out = np.zeros(self.nb * 3 * 512, np.float64)
+        lib().og_get_fine(self._h, FINE_FIELDS[name], _p(out, _D))
+        return out
+
+    def set_fine(self, name, vals):
+        vals = np.ascontiguousarray(vals, np.float64)
+        lib().og_set_fine(self._h, FINE_FIELDS[name], _p(vals, _D))
+
+    def pi(self, face_name, fine_name):
+        lib().og_pi(self._h, FACE_FIELDS[face_name], FINE_FIELDS[fine_name])
+
+    def pi_transpose(self, fine_name, face_name):
+        lib().og_pi_transpose(self._h, FINE_FIELDS[fine_name], FACE_FIELDS[face_name])
+
+
+def rollout64(og, pos, vel, steps, dt, gravity, alpha, tol=1e-10, frozen=None):
+    """FP64 rollout; returns the states [(x_k, v_k)] and the (levels, liquid) each step used.
+    `frozen` (a list of (levels, liquid)) freezes every step's topology and mask."""
+    states = [(np.array(pos, np.float64), np.array(vel, np.float64))]
+    masks = []
+    for k in range(steps):
+        lv, lq = (None, None) if frozen is None else frozen[k]
+        r = og.step64(states[-1][0], states[-1][1], dt, gravity, alpha, tol=tol, levels=lv, liquid=lq)
+        if r["status"] != 0:
+            raise RuntimeError(f"oracle step {k} did not converge")
+        masks.append((r["levels"], r["liquid"]))
+        states.append((r["pos"], r["vel"]))
+    return states, masks
+
+
+def rollout_grad64(og, states, masks, xbar, vbar, dt, gravity, alpha, tol=1e-10, adj_tol=1e-12):
+    """Reverse pass over a stored FP64 rollout: re-runs each step's forward (frozen to its masks) to
+    restore the intermediates, then its adjoint.  Returns dL/d(x_0, v_0)."""
+    xb, vb = np.array(xbar, np.float64), np.array(vbar, np.float64)
+    for k in range(len(masks) - 1, -1, -1):
+        x, v = states[k]
+        og.step64(x, v, dt, gravity, alpha, tol=tol, levels=masks[k][0], liquid=masks[k][1])
+        xb, vb = og.step_adjoint(x, v, xb, vb, alpha, dt, tol=adj_tol)
+    return xb, vb
 
 
 def morton3d(x, y, z):
