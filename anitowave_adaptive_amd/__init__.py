@@ -63,6 +63,19 @@ class LevelInfo(ctypes.Structure):
                 ("d_cell_off", _V), ("d_x", _V), ("d_b", _V), ("d_r", _V), ("d_liquid", _V), ("d_x64", _V)]
 
 
+class AdjointLayout(ctypes.Structure):
+    _fields_ = [("d_ubar_new", _V * 3), ("d_ubar_old", _V * 3), ("d_ubar_star", _V * 3), ("d_Mbar", _V * 3),
+                ("d_mbar", _V * 3), ("d_pbar", _V), ("d_rhsbar", _V), ("d_fine_bar", _V), ("d_fine_mbar", _V),
+                ("d_sorted_ids", _V), ("num_sorted", ctypes.c_int64)]
+
+
+class RolloutStats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int), ("forward_ms", ctypes.c_double), ("recompute_ms", ctypes.c_double),
+                ("backward_ms", ctypes.c_double), ("forward_iterations", ctypes.c_int),
+                ("adjoint_iterations", ctypes.c_int), ("forward_rel_residual", ctypes.c_double),
+                ("adjoint_rel_residual", ctypes.c_double)]
+
+
 _I = ctypes.c_int
 _I64 = ctypes.c_int64
 _F = ctypes.c_float
@@ -103,6 +116,16 @@ SIGNATURES = {
     "aw_solver_vcycle": (_I, [_V, _I]),
     "aw_solver_solve": (_I, [_V, _D, _I, _P(SolveStats)]),
     "aw_step": (_I, [_V, _V, _V, _V, _V, _I64, _P(StepParams), _P(StepStats)]),
+    "aw_grid_adjoint_layout": (_I, [_V, _P(AdjointLayout)]),
+    "aw_g2p_advect_adjoint": (_I, [_V, _F, _F, _V, _V]),
+    "aw_project_adjoint": (_I, [_V, _D, _I, _P(SolveStats)]),
+    "aw_p2g_adjoint": (_I, [_V, _V, _V]),
+    "aw_step_adjoint": (_I, [_V, _V, _P(StepParams), _V, _V, _P(SolveStats)]),
+    "aw_tape_create": (_I, [_V, _V, _I64, _I, _I, _I, _P(_V)]),
+    "aw_tape_destroy": (_I, [_V]),
+    "aw_rollout_forward": (_I, [_V, _V, _V, _V, _P(StepParams), _P(RolloutStats)]),
+    "aw_rollout_backward": (_I, [_V, _V, _V, _P(RolloutStats)]),
+    "aw_tape_masks": (_I, [_V, _I, _V, _V, ctypes.c_uint32, _P(ctypes.c_uint32)]),
 }
 
 _lib = None
@@ -459,3 +482,90 @@ class FlipStep:
         if not (allow_not_converged and s == AW_ERR_NOT_CONVERGED):
             check(s, "aw_step")
         return st
+
+
+class Adjoint:
+    """Reverse-mode derivative of the step on a grid and solver (no reference
+    counterpart: the reference has no gradient code).  Particle adjoints are
+    float32 (n, 3) device buffers indexed by particle id."""
+
+    def __init__(self, grid, solver):
+        self.grid, self.solver = grid, solver
+
+    def layout(self):
+        L = AdjointLayout()
+        check(lib().aw_grid_adjoint_layout(self.grid.handle, ctypes.byref(L)), "aw_grid_adjoint_layout")
+        return L
+
+    def face(self, name, axis):
+        L = self.layout()
+        n = self.grid.layout().total_faces[axis]
+        ptr = {"ubar_new": L.d_ubar_new, "ubar_old": L.d_ubar_old, "ubar_star": L.d_ubar_star, "Mbar": L.d_Mbar,
+               "mbar": L.d_mbar}[name][axis]
+        return download(self.grid.ctx, ptr, np.float32, n)
+
+    def cell(self, name):
+        L = self.layout()
+        n = self.grid.layout().total_cells
+        return download(self.grid.ctx, {"pbar": L.d_pbar, "rhsbar": L.d_rhsbar}[name], np.float32, n)
+
+    def g2p_advect(self, d_xbar, d_vbar, alpha=0.95, dt=1.0 / 240.0):
+        check(lib().aw_g2p_advect_adjoint(self.grid.handle, float(alpha), float(dt), d_xbar, d_vbar),
+              "aw_g2p_advect_adjoint")
+
+    def project(self, tol=1e-6, max_iters=200):
+        st = SolveStats()
+        check(lib().aw_project_adjoint(self.solver.handle, float(tol), int(max_iters), ctypes.byref(st)),
+              "aw_project_adjoint")
+        return st
+
+    def p2g(self, d_xbar, d_vbar):
+        check(lib().aw_p2g_adjoint(self.grid.handle, d_xbar, d_vbar), "aw_p2g_adjoint")
+
+    def step(self, params, d_xbar, d_vbar):
+        st = SolveStats()
+        check(lib().aw_step_adjoint(self.grid.handle, self.solver.handle, ctypes.byref(params), d_xbar, d_vbar,
+                                    ctypes.byref(st)), "aw_step_adjoint")
+        return st
+
+
+class Rollout:
+    """Checkpointed rollout of `steps` FLIP steps with its reverse pass
+    (aw_tape_*, aw_rollout_*)."""
+
+    def __init__(self, grid, solver, n, steps, checkpoint_every=4, keep_masks=False):
+        self.grid, self.solver, self.n, self.steps = grid, solver, int(n), int(steps)
+        self._h = _V()
+        check(lib().aw_tape_create(grid.handle, solver.handle, self.n, self.steps, int(checkpoint_every),
+                                   int(bool(keep_masks)), ctypes.byref(self._h)), "aw_tape_create")
+        self.stats = RolloutStats()
+
+    def forward(self, step_params, d_pos, d_vel, d_ids):
+        check(lib().aw_rollout_forward(self._h, d_pos, d_vel, d_ids, ctypes.byref(step_params),
+                                       ctypes.byref(self.stats)), "aw_rollout_forward")
+        return self.stats
+
+    def backward(self, d_xbar, d_vbar):
+        check(lib().aw_rollout_backward(self._h, d_xbar, d_vbar, ctypes.byref(self.stats)), "aw_rollout_backward")
+        return self.stats
+
+    def masks(self, step):
+        lv = np.zeros(self.grid.num_blocks, np.uint8)
+        nc = ctypes.c_uint32(0)
+        check(lib().aw_tape_masks(self._h, int(step), lv.ctypes.data_as(_V), None, 0, ctypes.byref(nc)),
+              "aw_tape_masks")
+        lq = np.zeros(nc.value, np.uint8)
+        check(lib().aw_tape_masks(self._h, int(step), lv.ctypes.data_as(_V), lq.ctypes.data_as(_V), nc.value,
+                                  ctypes.byref(nc)), "aw_tape_masks")
+        return lv, lq
+
+    def close(self):
+        if self._h:
+            lib().aw_tape_destroy(self._h)
+            self._h = _V()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -13,6 +13,7 @@ constexpr int LOGB = 3;
 constexpr int PAD = 900;       // padded fine nodes per axis per block (9 x 10 x 10)
 constexpr int SLAB = 2 * 3 * PAD;  // floats per block in the P2G slab / fine lattice
 constexpr int FINE = 512;      // own fine nodes per axis per block
+constexpr int ADJ = 6 * FINE;  // floats per block of a two-field fine-lattice adjoint buffer
 
 // One multigrid level as the kernels see it.  Level 0 aliases the grid.
 struct LevelDev {
@@ -128,6 +129,23 @@ void k_project(hipStream_t s, const GridDev& g, const float* p, float* const* us
 void k_to_fine(hipStream_t s, const GridDev& g, float* const* unew, float* const* uold, float* fine);
 void k_g2p_advect(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const float* svel,
                   int64_t n, float alpha, float dt, float* pos_out, float* vel_out);
+
+// adjoint (fine-lattice adjoint buffers: [block][field][axis][512], stride ADJ)
+void k_g2p_advect_adjoint(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const float* svel,
+                          const int32_t* sids, int64_t n, float alpha, float dt, float* xbar, float* vbar,
+                          float* fbar);
+void k_pi_transpose(hipStream_t s, const GridDev& g, const float* fbar, float* const* unb, float* const* uob);
+void k_project_adjoint(hipStream_t s, const GridDev& g, float* const* beta, float* const* unb, float* const* usb,
+                       float* pbar);
+void k_divergence_adjoint(hipStream_t s, const GridDev& g, const uint8_t* liquid, const float* rhsbar,
+                          float* const* usb);
+void k_fold_adjoint(hipStream_t s, const GridDev& g, float* const* uold, float* const* mass, float* const* uob,
+                    float* const* usb, float* const* Mb, float* const* mb);
+void k_pi_forward2(hipStream_t s, const GridDev& g, float* const* A, float* const* Bf, float* out);
+void k_p2g_adjoint(hipStream_t s, const GridDev& g, const float* fm, const float* spos, const float* svel,
+                   const int32_t* sids, int64_t n, float* xbar, float* vbar);
+void k_masked_copy(hipStream_t s, float* dst, const float* src, const uint8_t* liquid, int n);
+void k_ids_range(hipStream_t s, const int32_t* ids, int64_t n, int* out);  // out: {min, max}, preset by the caller
 
 // solver
 void k_smooth(hipStream_t s, const LevelDev& L, float* x, const float* b, const WG* wg, int nwg,

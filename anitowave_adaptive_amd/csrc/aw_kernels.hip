@@ -820,6 +820,449 @@ __global__ __launch_bounds__(TPB) void g2p_advect_kernel(GridDev g, const float*
   }
 }
 
+// ------------------------------------------------------------- adjoint
+// Reverse-mode derivative of the step with the topology, the liquid mask and
+// the mass thresholds held at their forward values.  Fine-lattice adjoint
+// buffers are block-major [block][field][axis][512] with two fields and a
+// per-block stride of ADJ floats.  Scatters into the fine lattice use global
+// float atomics (no return); every other stage is a gather.
+
+// address of fine node q of component a (transverse indices clamp to the
+// domain), or -1 where the node is the +wall or outside
+__device__ __forceinline__ int64_t fine_node(const GridDev& g, int stride, int field, int a, int* q) {
+  if (q[a] < 0 || q[a] >= g.N[a]) return -1;
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    if (d != a) q[d] = iclamp(q[d], 0, g.N[d] - 1);
+  const int B = (q[0] >> LOGB) + g.D[0] * ((q[1] >> LOGB) + g.D[1] * (q[2] >> LOGB));
+  const int l = (q[0] & 7) + 8 * ((q[1] & 7) + 8 * (q[2] & 7));
+  return (int64_t)B * stride + field * 3 * FINE + a * FINE + l;
+}
+
+// Component a of the field at u (fine units); adds wb * d(value)/du_d to grad.
+__device__ __forceinline__ float sample_vjp(const GridDev& g, const float* __restrict__ F, int stride, int field,
+                                            int a, const float* u, float wb, float* grad) {
+  int i0[3];
+  float t[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float c = d == a ? u[d] : u[d] - 0.5f;
+    const float fl = floorf(c);
+    i0[d] = (int)fl;
+    t[d] = c - fl;
+  }
+  float s = 0.f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int dd[3] = {k & 1, (k >> 1) & 1, k >> 2};
+    int q[3] = {i0[0] + dd[0], i0[1] + dd[1], i0[2] + dd[2]};
+    const int64_t ad = fine_node(g, stride, field, a, q);
+    const float f = ad >= 0 ? F[ad] : 0.f;
+    const float f0 = dd[0] ? t[0] : 1.f - t[0], f1 = dd[1] ? t[1] : 1.f - t[1], f2 = dd[2] ? t[2] : 1.f - t[2];
+    s += f0 * f1 * f2 * f;
+    const float wf = wb * f;
+    grad[0] += wf * (dd[0] ? 1.f : -1.f) * f1 * f2;
+    grad[1] += wf * f0 * (dd[1] ? 1.f : -1.f) * f2;
+    grad[2] += wf * f0 * f1 * (dd[2] ? 1.f : -1.f);
+  }
+  return s;
+}
+
+// F[node] += w(u) * val on component a (the transpose of sampling)
+__device__ __forceinline__ void scatter_comp(const GridDev& g, float* F, int stride, int field, int a,
+                                             const float* u, float val) {
+  if (val == 0.f) return;
+  int i0[3];
+  float t[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float c = d == a ? u[d] : u[d] - 0.5f;
+    const float fl = floorf(c);
+    i0[d] = (int)fl;
+    t[d] = c - fl;
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int dd[3] = {k & 1, (k >> 1) & 1, k >> 2};
+    int q[3] = {i0[0] + dd[0], i0[1] + dd[1], i0[2] + dd[2]};
+    const int64_t ad = fine_node(g, stride, field, a, q);
+    if (ad < 0) continue;
+    const float w = (dd[0] ? t[0] : 1.f - t[0]) * (dd[1] ? t[1] : 1.f - t[1]) * (dd[2] ? t[2] : 1.f - t[2]);
+    unsafeAtomicAdd(F + ad, w * val);
+  }
+}
+
+// Adjoint of g2p_advect_kernel for the particle in sorted slot i (id sids[i]):
+// recomputes the forward stages, reads dL/dx', dL/dv' of the particle,
+// writes its direct partials dL/dx, dL/dv and scatters the field adjoints:
+// fbar field 0 = Fnew-bar + Fdel-bar, field 1 = Fdel-bar.
+__global__ __launch_bounds__(TPB) void g2p_advect_adjoint_kernel(GridDev g, const float* __restrict__ F,
+                                                                 const float* __restrict__ spos,
+                                                                 const float* __restrict__ svel,
+                                                                 const int32_t* __restrict__ sids, int64_t n,
+                                                                 float alpha, float dt, float* __restrict__ xbar,
+                                                                 float* __restrict__ vbar, float* fbar) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t id = sids[i];
+  const float umax[3] = {(float)g.N[0] - 0.0009765625f, (float)g.N[1] - 0.0009765625f,
+                         (float)g.N[2] - 0.0009765625f};
+  float x[3], u[3], m1[3], v1[3], vd[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    x[a] = spos[3 * i + a];
+    const float raw = x[a] * g.inv_h;
+    u[a] = fminf(fmaxf(raw, 0.f), umax[a]);
+    m1[a] = (raw > 0.f && raw < umax[a]) ? 1.f : 0.f;
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    v1[a] = sample_comp(g, F, 0, a, u);
+    vd[a] = sample_comp(g, F, 1, a, u);
+  }
+  float u2[3], m2[3], v2[3], u3[3], m3[3], v3[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float raw = (x[a] + 0.5f * dt * v1[a]) * g.inv_h;
+    u2[a] = fminf(fmaxf(raw, -1.f), (float)g.N[a] + 1.f);
+    m2[a] = (raw > -1.f && raw < (float)g.N[a] + 1.f) ? 1.f : 0.f;
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) v2[a] = sample_comp(g, F, 0, a, u2);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float raw = (x[a] + 0.75f * dt * v2[a]) * g.inv_h;
+    u3[a] = fminf(fmaxf(raw, -1.f), (float)g.N[a] + 1.f);
+    m3[a] = (raw > -1.f && raw < (float)g.N[a] + 1.f) ? 1.f : 0.f;
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) v3[a] = sample_comp(g, F, 0, a, u3);
+  float k1[3], k2[3], k3[3], xacc[3], vo[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float xn = x[a] + dt * ((2.f / 9.f) * v1[a] + (3.f / 9.f) * v2[a] + (4.f / 9.f) * v3[a]);
+    const float lo = 0.25f * g.h, hi = (float)g.N[a] * g.h - 0.25f * g.h;
+    const float xb = (xn > lo && xn < hi) ? xbar[3 * id + a] : 0.f;
+    vo[a] = vbar[3 * id + a];
+    k1[a] = dt * (2.f / 9.f) * xb;
+    k2[a] = dt * (3.f / 9.f) * xb;
+    k3[a] = dt * (4.f / 9.f) * xb;
+    xacc[a] = xb;
+  }
+  // stage 3: v3 = S(Fnew)(x3), x3 = x + 3/4 dt v2
+  float gr[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    scatter_comp(g, fbar, ADJ, 0, a, u3, k3[a]);
+    if (k3[a] != 0.f) sample_vjp(g, F, SLAB, 0, a, u3, k3[a], gr);
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float b = gr[d] * g.inv_h * m3[d];
+    xacc[d] += b;
+    k2[d] += 0.75f * dt * b;
+    gr[d] = 0.f;
+  }
+  // stage 2: v2 = S(Fnew)(x2), x2 = x + 1/2 dt v1
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    scatter_comp(g, fbar, ADJ, 0, a, u2, k2[a]);
+    if (k2[a] != 0.f) sample_vjp(g, F, SLAB, 0, a, u2, k2[a], gr);
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float b = gr[d] * g.inv_h * m2[d];
+    xacc[d] += b;
+    k1[d] += 0.5f * dt * b;
+    gr[d] = 0.f;
+  }
+  // FLIP/PIC blend and stage 1 at x
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float vnb = k1[a] + (1.f - alpha) * vo[a], vdb = alpha * vo[a];
+    scatter_comp(g, fbar, ADJ, 0, a, u, vnb + vdb);
+    scatter_comp(g, fbar, ADJ, 1, a, u, vdb);
+    if (vnb != 0.f) sample_vjp(g, F, SLAB, 0, a, u, vnb, gr);
+    if (vdb != 0.f) sample_vjp(g, F, SLAB, 1, a, u, vdb, gr);
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    xbar[3 * id + d] = xacc[d] + gr[d] * g.inv_h * m1[d];
+    vbar[3 * id + d] = alpha * vo[d];
+  }
+}
+
+// Pi^T of the two fine-lattice adjoint fields onto the block's own slots:
+// unb = Pi^T field 0, uob = -Pi^T field 1.
+__global__ __launch_bounds__(TPB) void pi_transpose_kernel(GridDev g, const float* __restrict__ fbar, FacePtrs unb,
+                                                           FacePtrs uob) {
+  __shared__ float nA[3 * 576], nB[3 * 576];
+  const int B = blockIdx.x, tid = threadIdx.x;
+  const int e = g.level[B], r = BASE >> e, nf = (r + 1) * r * r;
+  for (int k = tid; k < 3 * 576; k += TPB) {
+    nA[k] = 0.f;
+    nB[k] = 0.f;
+  }
+  __syncthreads();
+  const float* src = fbar + (size_t)B * ADJ;
+  for (int k = tid; k < 3 * FINE; k += TPB) {
+    const float fa = src[k], fb = src[3 * FINE + k];
+    if (fa == 0.f && fb == 0.f) continue;
+    const int a = k >> 9, li = k & 511;
+    const int l[3] = {li & 7, (li >> 3) & 7, li >> 6};
+    int slot[8];
+    float w[8];
+    const int nw = pi_weights(a, e, l, slot, w);
+    for (int j = 0; j < nw; ++j) {
+      atomicAdd(&nA[a * 576 + slot[j]], w[j] * fa);
+      atomicAdd(&nB[a * 576 + slot[j]], w[j] * fb);
+    }
+  }
+  __syncthreads();
+  for (int k = tid; k < 3 * nf; k += TPB) {
+    const int a = k / nf, f = k - a * nf;
+    unb.p[a][g.foff[a][B] + f] = nA[a * 576 + f];
+    uob.p[a][g.foff[a][B] + f] = -nB[a * 576 + f];
+  }
+}
+
+// Adjoint of project_kernel.  Per slot: the u* terms are gathered (own
+// formula, plus 1/4 of the coarse neighbour's average slot on the fine side
+// of a level change) into usb; the pressure terms are scattered into pbar.
+__global__ __launch_bounds__(TPB) void project_adjoint_kernel(GridDev g, FacePtrs beta, FacePtrs unb, FacePtrs usb,
+                                                              float* pbar) {
+  const int B = blockIdx.x;
+  const int e = g.level[B], r = BASE >> e, nf = (r + 1) * r * r;
+  const float s = (float)(1 << e);
+  const uint32_t co = g.coff[B];
+  for (int k = threadIdx.x; k < 3 * nf; k += blockDim.x) {
+    const int a = k / nf, f = k - a * nf;
+    const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+    int sl[3];
+    face_decode(a, f, r, sl);
+    const size_t id = g.foff[a][B] + f;
+    const float ub = unb.p[a][id], bt = beta.p[a][id];
+    float direct = 0.f;
+    if (sl[a] > 0 && sl[a] < r) {
+      direct = ub;
+      if (ub != 0.f) {
+        int cm[3] = {sl[0], sl[1], sl[2]};
+        cm[a] -= 1;
+        unsafeAtomicAdd(pbar + co + cell_idx(sl, r), -bt / s * ub);
+        unsafeAtomicAdd(pbar + co + cell_idx(cm, r), bt / s * ub);
+      }
+    } else {
+      const int side = sl[a] == r;
+      const int N = g.nbr[B * 6 + 2 * a + side];
+      if (N >= 0) {
+        const int en = g.level[N], rn = BASE >> en;
+        const float sg = side ? -1.f : 1.f;  // u = u* - bt sg (p_own - p_nbr) / d
+        int own[3] = {sl[0], sl[1], sl[2]};
+        own[a] = side ? r - 1 : 0;
+        const uint32_t po = co + cell_idx(own, r);
+        int q[3] = {sl[0], sl[1], sl[2]};
+        q[a] = side ? 0 : rn - 1;
+        if (en == e || en == e + 1) {
+          float d = s;
+          direct = ub;
+          if (en == e + 1) {
+            q[t1] = sl[t1] >> 1;
+            q[t2] = sl[t2] >> 1;
+            d = 1.5f * s;
+            int ps[3] = {sl[0], sl[1], sl[2]};
+            ps[a] = side ? 0 : rn;
+            ps[t1] = q[t1];
+            ps[t2] = q[t2];
+            direct += 0.25f * unb.p[a][g.foff[a][N] + face_idx(a, ps, rn)];
+          }
+          if (ub != 0.f) {
+            const float c = bt * sg / d * ub;
+            unsafeAtomicAdd(pbar + po, -c);
+            unsafeAtomicAdd(pbar + g.coff[N] + cell_idx(q, rn), c);
+          }
+        } else if (ub != 0.f) {
+          const float sf = 0.5f * s;
+          int ps[3];
+          ps[a] = side ? 0 : rn;
+          for (int d1 = 0; d1 < 2; ++d1)
+            for (int d2 = 0; d2 < 2; ++d2) {
+              q[t1] = ps[t1] = 2 * sl[t1] + d1;
+              q[t2] = ps[t2] = 2 * sl[t2] + d2;
+              const size_t pid = g.foff[a][N] + face_idx(a, ps, rn);
+              const float c = 0.25f * beta.p[a][pid] * sg / (1.5f * sf) * ub;
+              unsafeAtomicAdd(pbar + po, -c);
+              unsafeAtomicAdd(pbar + g.coff[N] + cell_idx(q, rn), c);
+            }
+        }
+      }
+    }
+    usb.p[a][id] = direct;
+  }
+}
+
+// usb += D^T rhsbar: rhs_c = -s^2 sum_a (u[c+] - u[c-]) on liquid cells
+__global__ __launch_bounds__(TPB) void divergence_adjoint_kernel(GridDev g, const uint8_t* __restrict__ liquid,
+                                                                 const float* __restrict__ rhsbar, FacePtrs usb) {
+  const int B = blockIdx.x;
+  const int e = g.level[B], r = BASE >> e, nf = (r + 1) * r * r;
+  const float s2 = (float)(1 << e) * (float)(1 << e);
+  const uint32_t co = g.coff[B];
+  for (int k = threadIdx.x; k < 3 * nf; k += blockDim.x) {
+    const int a = k / nf, f = k - a * nf;
+    int sl[3];
+    face_decode(a, f, r, sl);
+    float acc = 0.f;
+    if (sl[a] > 0) {
+      int cm[3] = {sl[0], sl[1], sl[2]};
+      cm[a] -= 1;
+      const uint32_t c = co + cell_idx(cm, r);
+      if (liquid[c]) acc -= s2 * rhsbar[c];
+    }
+    if (sl[a] < r) {
+      const uint32_t c = co + cell_idx(sl, r);
+      if (liquid[c]) acc += s2 * rhsbar[c];
+    }
+    usb.p[a][g.foff[a][B] + f] += acc;
+  }
+}
+
+// Adjoint of fold_kernel.  Each unknown face value u = M'/m' (m' > 1e-6)
+// collects the adjoints of every slot that copies it (lambda); the slots
+// whose accumulators enter M', m' with weight c receive c lambda / m' and
+// -c lambda u / m'.  u and m' are the forward's u_old and mass.
+__global__ __launch_bounds__(TPB) void fold_adjoint_kernel(GridDev g, FacePtrs uold, FacePtrs mass, FacePtrs uob,
+                                                           FacePtrs usb, FacePtrs Mb, FacePtrs mb) {
+  const int B = blockIdx.x;
+  const int e = g.level[B], r = BASE >> e, nf = (r + 1) * r * r;
+  for (int k = threadIdx.x; k < 3 * nf; k += blockDim.x) {
+    const int a = k / nf, f = k - a * nf;
+    const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+    int sl[3];
+    face_decode(a, f, r, sl);
+    const size_t id = g.foff[a][B] + f;
+    const float ub = uob.p[a][id] + usb.p[a][id];
+    float oM = 0.f, om = 0.f;
+    if (sl[a] > 0 && sl[a] < r) {
+      const float mt = mass.p[a][id];
+      if (mt > 1e-6f) {
+        oM = ub / mt;
+        om = -ub * uold.p[a][id] / mt;
+      }
+    } else {
+      const int side = sl[a] == r;
+      const int N = g.nbr[B * 6 + 2 * a + side];
+      if (N >= 0) {
+        const int en = g.level[N], rn = BASE >> en;
+        int ps[3] = {sl[0], sl[1], sl[2]};
+        ps[a] = side ? 0 : rn;
+        if (en == e || en == e + 1) {
+          float cp = 1.f;
+          if (en == e + 1) {
+            ps[t1] = sl[t1] >> 1;
+            ps[t2] = sl[t2] >> 1;
+            cp = 0.25f;
+          }
+          const size_t pid = g.foff[a][N] + face_idx(a, ps, rn);
+          const float lam = ub + cp * (uob.p[a][pid] + usb.p[a][pid]);
+          const float mt = mass.p[a][id];
+          if (mt > 1e-6f) {
+            oM = lam / mt;
+            om = -lam * uold.p[a][id] / mt;
+          }
+        } else {
+          for (int d1 = 0; d1 < 2; ++d1)
+            for (int d2 = 0; d2 < 2; ++d2) {
+              ps[t1] = 2 * sl[t1] + d1;
+              ps[t2] = 2 * sl[t2] + d2;
+              const size_t pid = g.foff[a][N] + face_idx(a, ps, rn);
+              const float mt = mass.p[a][pid];
+              if (mt <= 1e-6f) continue;
+              const float lam = uob.p[a][pid] + usb.p[a][pid] + 0.25f * ub;
+              oM += 0.25f * lam / mt;
+              om -= 0.25f * lam * uold.p[a][pid] / mt;
+            }
+        }
+      }
+    }
+    Mb.p[a][id] = oM;
+    mb.p[a][id] = om;
+  }
+}
+
+// Pi of two per-slot fields onto the block's own fine nodes (fields 0, 1)
+__global__ __launch_bounds__(TPB) void pi_forward2_kernel(GridDev g, FacePtrs A, FacePtrs Bf, float* __restrict__ out) {
+  const int B = blockIdx.x;
+  const int e = g.level[B];
+  float* dst = out + (size_t)B * ADJ;
+  for (int k = threadIdx.x; k < 3 * FINE; k += blockDim.x) {
+    const int a = k >> 9, li = k & 511;
+    const int l[3] = {li & 7, (li >> 3) & 7, li >> 6};
+    int slot[8];
+    float w[8];
+    const int nw = pi_weights(a, e, l, slot, w);
+    float va = 0.f, vb = 0.f;
+    for (int j = 0; j < nw; ++j) {
+      const size_t id = g.foff[a][B] + slot[j];
+      va += w[j] * A.p[a][id];
+      vb += w[j] * Bf.p[a][id];
+    }
+    dst[k] = va;
+    dst[3 * FINE + k] = vb;
+  }
+}
+
+// Adjoint of the deposit per particle: vbar += S(FMbar)(x / h),
+// xbar += (v . dS(FMbar) + dS(Fmbar)) / h  (fm field 0 = FMbar, 1 = Fmbar)
+__global__ __launch_bounds__(TPB) void p2g_adjoint_kernel(GridDev g, const float* __restrict__ fm,
+                                                          const float* __restrict__ spos,
+                                                          const float* __restrict__ svel,
+                                                          const int32_t* __restrict__ sids, int64_t n,
+                                                          float* __restrict__ xbar, float* __restrict__ vbar) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t id = sids[i];
+  float u[3], m1[3], gr[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float umax = (float)g.N[a] - 0.0009765625f;
+    const float raw = spos[3 * i + a] * g.inv_h;
+    u[a] = fminf(fmaxf(raw, 0.f), umax);
+    m1[a] = (raw > 0.f && raw < umax) ? 1.f : 0.f;
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float sv = sample_vjp(g, fm, ADJ, 0, a, u, svel[3 * i + a], gr);
+    sample_vjp(g, fm, ADJ, 1, a, u, 1.f, gr);
+    vbar[3 * id + a] += sv;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) xbar[3 * id + d] += gr[d] * g.inv_h * m1[d];
+}
+
+// out[0] = min id, out[1] = max id (out preset to INT_MAX, INT_MIN)
+__global__ __launch_bounds__(TPB) void ids_range_kernel(const int32_t* __restrict__ ids, int64_t n, int* out) {
+  int lo = 0x7FFFFFFF, hi = -0x7FFFFFFF - 1;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    lo = min(lo, ids[i]);
+    hi = max(hi, ids[i]);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = min(lo, __shfl_down(lo, off, 64));
+    hi = max(hi, __shfl_down(hi, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(out, lo);
+    atomicMax(out + 1, hi);
+  }
+}
+
+__global__ void masked_copy_kernel(float* __restrict__ dst, const float* __restrict__ src,
+                                   const uint8_t* __restrict__ liq, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = liq[i] ? src[i] : 0.f;
+}
+
 // ------------------------------------------------------------- solver
 struct CellRef {
   int B, r, lr, q, li;
@@ -1654,6 +2097,45 @@ void k_g2p_advect(hipStream_t s, const GridDev& g, const float* fine, const floa
   if (n > 0)
     g2p_advect_kernel<<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, s>>>(g, fine, spos, svel, n, alpha, dt, pos_out,
                                                                      vel_out);
+}
+
+void k_g2p_advect_adjoint(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const float* svel,
+                          const int32_t* sids, int64_t n, float alpha, float dt, float* xbar, float* vbar,
+                          float* fbar) {
+  if (n > 0)
+    g2p_advect_adjoint_kernel<<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, s>>>(g, fine, spos, svel, sids, n, alpha,
+                                                                             dt, xbar, vbar, fbar);
+}
+void k_pi_transpose(hipStream_t s, const GridDev& g, const float* fbar, float* const* unb, float* const* uob) {
+  pi_transpose_kernel<<<g.nb, TPB, 0, s>>>(g, fbar, fp(unb), fp(uob));
+}
+void k_project_adjoint(hipStream_t s, const GridDev& g, float* const* beta, float* const* unb, float* const* usb,
+                       float* pbar) {
+  project_adjoint_kernel<<<g.nb, TPB, 0, s>>>(g, fp(beta), fp(unb), fp(usb), pbar);
+}
+void k_divergence_adjoint(hipStream_t s, const GridDev& g, const uint8_t* liquid, const float* rhsbar,
+                          float* const* usb) {
+  divergence_adjoint_kernel<<<g.nb, TPB, 0, s>>>(g, liquid, rhsbar, fp(usb));
+}
+void k_fold_adjoint(hipStream_t s, const GridDev& g, float* const* uold, float* const* mass, float* const* uob,
+                    float* const* usb, float* const* Mb, float* const* mb) {
+  fold_adjoint_kernel<<<g.nb, TPB, 0, s>>>(g, fp(uold), fp(mass), fp(uob), fp(usb), fp(Mb), fp(mb));
+}
+void k_pi_forward2(hipStream_t s, const GridDev& g, float* const* A, float* const* Bf, float* out) {
+  pi_forward2_kernel<<<g.nb, TPB, 0, s>>>(g, fp(A), fp(Bf), out);
+}
+void k_p2g_adjoint(hipStream_t s, const GridDev& g, const float* fm, const float* spos, const float* svel,
+                   const int32_t* sids, int64_t n, float* xbar, float* vbar) {
+  if (n > 0)
+    p2g_adjoint_kernel<<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, s>>>(g, fm, spos, svel, sids, n, xbar, vbar);
+}
+void k_ids_range(hipStream_t s, const int32_t* ids, int64_t n, int* out) {
+  int64_t blocks = (n + TPB - 1) / TPB;
+  if (blocks > 2048) blocks = 2048;
+  if (n > 0) ids_range_kernel<<<(unsigned)blocks, TPB, 0, s>>>(ids, n, out);
+}
+void k_masked_copy(hipStream_t s, float* dst, const float* src, const uint8_t* liquid, int n) {
+  if (n > 0) masked_copy_kernel<<<(n + TPB - 1) / TPB, TPB, 0, s>>>(dst, src, liquid, n);
 }
 
 void k_smooth(hipStream_t s, const LevelDev& L, float* x, const float* b, const WG* wg, int nwg,

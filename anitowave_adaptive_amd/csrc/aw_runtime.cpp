@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -151,6 +152,10 @@ struct aw_grid {
   DevBuf spos, svel, sids;
   DevBuf liquid, p, rhs, slab;
   DevBuf beta[3], Macc[3], macc[3], uold[3], ustar[3], unew[3], mass[3];
+  // adjoint fields: fine-lattice adjoints (two fields each, stride ADJ), per-slot and per-cell adjoints
+  DevBuf fbar, fmbar, pbar, rhsbar, idrange;
+  DevBuf ubn[3], ubo[3], ubs[3], Mb[3], mb[3];
+  bool sorted_ids = false;  // the last binning carried particle ids
   std::vector<uint8_t> h_level;
   std::vector<uint32_t> h_coff;
   bool have_topology = false;
@@ -331,6 +336,7 @@ int sort_particles(aw_grid* g, const float* d_pos, const float* d_vel, const int
   k_scatter(s, d_pos, d_vel, d_ids, g->key.as<int32_t>(), g->cursor.as<uint32_t>(), n, g->spos.as<float>(),
             g->svel.as<float>(), d_ids ? g->sids.as<int32_t>() : nullptr);
   g->n_sorted = n;
+  g->sorted_ids = d_ids != nullptr;
   return launch_status("sort_particles");
 }
 
@@ -907,6 +913,225 @@ int solve(aw_solver* S, double tol, int max_iters, aw_solve_stats* st) {
   return solve_fp32(S, S->g->p.as<float>(), S->g->rhs.as<float>(), tol, max_iters, st);
 }
 
+// ---------------------------------------------------------------- adjoint
+// Reverse-mode derivative of aw_step with the topology, the liquid mask and
+// the mass thresholds held at their forward values.  It runs on the state the
+// last forward step left in the grid: the sorted particles (spos, svel, sids),
+// the fine lattice of Pi(u_new), Pi(u_new - u_old), u_old and the folded mass.
+// Particle adjoints are indexed by particle id.
+int ensure_adjoint(aw_grid* g) {
+  const size_t nf = (size_t)g->nfaces * sizeof(float);
+  AW_TRY(g->fbar.ensure((size_t)g->nb * ADJ * sizeof(float)));
+  AW_TRY(g->fmbar.ensure((size_t)g->nb * ADJ * sizeof(float)));
+  AW_TRY(g->pbar.ensure((size_t)g->ncells * sizeof(float)));
+  AW_TRY(g->rhsbar.ensure((size_t)g->ncells * sizeof(float)));
+  for (int a = 0; a < 3; ++a)
+    for (DevBuf* f : {g->ubn, g->ubo, g->ubs, g->Mb, g->mb}) AW_TRY(f[a].ensure(nf));
+  return AW_OK;
+}
+
+struct Faces {
+  float* p[3];
+  explicit Faces(DevBuf* f) {
+    for (int a = 0; a < 3; ++a) p[a] = f[a].as<float>();
+  }
+};
+
+// Particle adjoints are indexed by id: every id must lie in [0, n).
+int validate_ids(aw_grid* g, const int32_t* ids, int64_t n) {
+  if (n <= 0) return AW_OK;
+  hipStream_t s = g->ctx->stream;
+  AW_TRY(g->idrange.ensure(2 * sizeof(int)));
+  const int init[2] = {0x7FFFFFFF, -0x7FFFFFFF - 1};
+  int mm[2];
+  AW_HIP(hipMemcpyAsync(g->idrange.p, init, sizeof init, hipMemcpyHostToDevice, s));
+  k_ids_range(s, ids, n, g->idrange.as<int>());
+  AW_HIP(hipMemcpyAsync(mm, g->idrange.p, sizeof mm, hipMemcpyDeviceToHost, s));
+  AW_HIP(hipStreamSynchronize(s));
+  if (mm[0] < 0 || mm[1] >= n) return fail(AW_ERR_INVALID, "particle ids span [%d, %d], outside [0, %lld)", mm[0], mm[1], (long long)n);
+  return AW_OK;
+}
+
+int check_adjoint_state(aw_grid* g) {
+  if (!g->have_topology) return fail(AW_ERR_INVALID, "grid has no topology");
+  if (g->n_sorted < 0 || !g->sorted_ids)
+    return fail(AW_ERR_INVALID, "the adjoint needs a forward step binned with particle ids");
+  return ensure_adjoint(g);
+}
+
+// dL/d(x', v') -> direct particle partials; ubar_new = Pi^T (Fnew-bar + Fdel-bar), ubar_old = -Pi^T Fdel-bar
+int g2p_adjoint(aw_grid* g, float alpha, float dt, float* xbar, float* vbar) {
+  AW_TRY(check_adjoint_state(g));
+  hipStream_t s = g->ctx->stream;
+  const GridDev gv = g->view();
+  const int64_t n = g->n_sorted;
+  k_fill_f32(s, g->fbar.as<float>(), (size_t)g->nb * ADJ, 0.f);
+  {
+    // particle state 28 B + adjoints read and written 48 B; forward fine lattice and its adjoint per block
+    Prof pf(g->ctx, "g2p_advect_adjoint", 76.0 * n + 2.0 * 4.0 * 3 * FINE * g->nb + 4.0 * ADJ * g->nb);
+    k_g2p_advect_adjoint(s, gv, g->slab.as<float>(), g->spos.as<float>(), g->svel.as<float>(),
+                         g->sids.as<int32_t>(), n, alpha, dt, xbar, vbar, g->fbar.as<float>());
+  }
+  {
+    Prof pf(g->ctx, "pi_transpose", 4.0 * ADJ * g->nb + 8.0 * 3 * g->nfaces);
+    Faces un(g->ubn), uo(g->ubo);
+    k_pi_transpose(s, gv, g->fbar.as<float>(), un.p, uo.p);
+  }
+  return launch_status("g2p_advect_adjoint");
+}
+
+// ubar_new -> pbar -> rhsbar = A^{-1} pbar (same operator and hierarchy) -> ubar_star
+int project_adjoint(aw_solver* S, double tol, int max_iters, aw_solve_stats* st) {
+  aw_grid* g = S->g;
+  AW_TRY(ensure_adjoint(g));
+  hipStream_t s = g->ctx->stream;
+  const GridDev gv = g->view();
+  Faces bt(g->beta), un(g->ubn), us(g->ubs);
+  k_fill_f32(s, g->pbar.as<float>(), g->ncells, 0.f);
+  {
+    Prof pf(g->ctx, "project_adjoint", 12.0 * 3 * g->nfaces + 8.0 * g->ncells);
+    k_project_adjoint(s, gv, bt.p, un.p, us.p, g->pbar.as<float>());
+  }
+  k_masked_copy(s, g->rhs.as<float>(), g->pbar.as<float>(), g->liquid.as<uint8_t>(), (int)g->ncells);
+  k_fill_f32(s, g->p.as<float>(), g->ncells, 0.f);
+  AW_TRY(ensure_setup(S));
+  int rc = solve(S, tol, max_iters, st);
+  if (rc != AW_OK && rc != AW_ERR_NOT_CONVERGED) return rc;
+  std::string keep = g_err;
+  AW_HIP(hipMemcpyAsync(g->rhsbar.p, g->p.p, (size_t)g->ncells * sizeof(float), hipMemcpyDeviceToDevice, s));
+  {
+    Prof pf(g->ctx, "divergence_adjoint", 8.0 * 3 * g->nfaces + 5.0 * g->ncells);
+    k_divergence_adjoint(s, gv, g->liquid.as<uint8_t>(), g->rhsbar.as<float>(), us.p);
+  }
+  AW_TRY(launch_status("project_adjoint"));
+  if (rc != AW_OK) g_err = keep;
+  return rc;
+}
+
+// (ubar_old + ubar_star) -> Mbar, mbar -> Pi -> particle partials (accumulated)
+int p2g_adjoint(aw_grid* g, float* xbar, float* vbar) {
+  AW_TRY(check_adjoint_state(g));
+  hipStream_t s = g->ctx->stream;
+  const GridDev gv = g->view();
+  Faces uo(g->uold), ms(g->mass), ubo(g->ubo), ubs(g->ubs), Mb(g->Mb), mb(g->mb);
+  {
+    Prof pf(g->ctx, "fold_adjoint", 24.0 * 3 * g->nfaces);
+    k_fold_adjoint(s, gv, uo.p, ms.p, ubo.p, ubs.p, Mb.p, mb.p);
+  }
+  {
+    Prof pf(g->ctx, "pi_forward", 8.0 * 3 * g->nfaces + 4.0 * ADJ * g->nb);
+    k_pi_forward2(s, gv, Mb.p, mb.p, g->fmbar.as<float>());
+  }
+  {
+    Prof pf(g->ctx, "p2g_adjoint", 76.0 * g->n_sorted + 4.0 * ADJ * g->nb);
+    k_p2g_adjoint(s, gv, g->fmbar.as<float>(), g->spos.as<float>(), g->svel.as<float>(), g->sids.as<int32_t>(),
+                  g->n_sorted, xbar, vbar);
+  }
+  return launch_status("p2g_adjoint");
+}
+
+int step_adjoint(aw_grid* g, aw_solver* S, const aw_step_params* prm, float* xbar, float* vbar,
+                 aw_solve_stats* st) {
+  AW_TRY(g2p_adjoint(g, prm->flip_alpha, prm->dt, xbar, vbar));
+  int rc = project_adjoint(S, prm->rel_tol, prm->max_iters, st);
+  if (rc != AW_OK && rc != AW_ERR_NOT_CONVERGED) return rc;
+  std::string keep = g_err;
+  AW_TRY(p2g_adjoint(g, xbar, vbar));
+  if (rc != AW_OK) g_err = keep;
+  return rc;
+}
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+// Checkpointed rollout: (x, v, ids) every `every` steps in the forward pass;
+// the reverse pass re-runs each segment from its checkpoint and keeps what
+// the step adjoint needs per step (topology, liquid mask, u_old, mass, u_new
+// and the sorted particles), then walks the segment backwards.
+struct aw_tape {
+  struct Step {
+    std::vector<uint8_t> level;
+    uint32_t ncells = 0, nfaces = 0;
+    DevBuf liquid, uold[3], mass[3], unew[3], spos, svel, sids;
+  };
+  struct Ckpt {
+    DevBuf pos, vel, ids;
+  };
+  aw_grid* g = nullptr;
+  aw_solver* S = nullptr;
+  int64_t n = 0;
+  int steps = 0, every = 1;
+  bool keep_masks = false;
+  bool have_forward = false;
+  aw_step_params prm{};
+  std::vector<std::unique_ptr<Ckpt>> ck;
+  std::vector<std::unique_ptr<Step>> rec;
+  DevBuf wpos, wvel, wids;
+  std::vector<std::vector<uint8_t>> mask_level, mask_liquid;
+};
+
+namespace {
+
+int copy_d2d(hipStream_t s, DevBuf& dst, const void* src, size_t bytes) {
+  AW_TRY(dst.ensure(bytes));
+  if (bytes) AW_HIP(hipMemcpyAsync(dst.p, src, bytes, hipMemcpyDeviceToDevice, s));
+  return AW_OK;
+}
+
+int tape_record(aw_tape* t, aw_tape::Step& r) {
+  aw_grid* g = t->g;
+  hipStream_t s = g->ctx->stream;
+  const size_t nf = (size_t)g->nfaces * sizeof(float);
+  r.level = g->h_level;
+  r.ncells = g->ncells;
+  r.nfaces = g->nfaces;
+  AW_TRY(copy_d2d(s, r.liquid, g->liquid.p, g->ncells));
+  for (int a = 0; a < 3; ++a) {
+    AW_TRY(copy_d2d(s, r.uold[a], g->uold[a].p, nf));
+    AW_TRY(copy_d2d(s, r.mass[a], g->mass[a].p, nf));
+    AW_TRY(copy_d2d(s, r.unew[a], g->unew[a].p, nf));
+  }
+  AW_TRY(copy_d2d(s, r.spos, g->spos.p, (size_t)t->n * 3 * sizeof(float)));
+  AW_TRY(copy_d2d(s, r.svel, g->svel.p, (size_t)t->n * 3 * sizeof(float)));
+  AW_TRY(copy_d2d(s, r.sids, g->sids.p, (size_t)t->n * sizeof(int32_t)));
+  return AW_OK;
+}
+
+int tape_restore(aw_tape* t, const aw_tape::Step& r) {
+  aw_grid* g = t->g;
+  hipStream_t s = g->ctx->stream;
+  if (g->h_level != r.level) {
+    g->h_level = r.level;
+    AW_HIP(hipMemcpyAsync(g->level.p, g->h_level.data(), g->nb, hipMemcpyHostToDevice, s));
+    AW_TRY(apply_levels(g));
+  }
+  const size_t nf = (size_t)g->nfaces * sizeof(float);
+  AW_HIP(hipMemcpyAsync(g->liquid.p, r.liquid.p, g->ncells, hipMemcpyDeviceToDevice, s));
+  for (int a = 0; a < 3; ++a) {
+    AW_HIP(hipMemcpyAsync(g->uold[a].p, r.uold[a].p, nf, hipMemcpyDeviceToDevice, s));
+    AW_HIP(hipMemcpyAsync(g->mass[a].p, r.mass[a].p, nf, hipMemcpyDeviceToDevice, s));
+    AW_HIP(hipMemcpyAsync(g->unew[a].p, r.unew[a].p, nf, hipMemcpyDeviceToDevice, s));
+  }
+  AW_HIP(hipMemcpyAsync(g->spos.p, r.spos.p, (size_t)t->n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s));
+  AW_HIP(hipMemcpyAsync(g->svel.p, r.svel.p, (size_t)t->n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s));
+  AW_HIP(hipMemcpyAsync(g->sids.p, r.sids.p, (size_t)t->n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+  g->n_sorted = t->n;
+  g->sorted_ids = true;
+  Faces un(g->unew), uo(g->uold);
+  {
+    Prof pf(g->ctx, "to_fine", 24.0 * g->nfaces + 2.0 * 4.0 * 3 * FINE * g->nb);
+    k_to_fine(s, g->view(), un.p, uo.p, g->slab.as<float>());
+  }
+  return launch_status("tape restore");
+}
+
+}  // namespace
+
+namespace {
+
 bool valid_params(const aw_solver_params* p) {
   return p && (p->method == 0 || p->method == 1) && p->pre_sweeps >= 0 && p->post_sweeps >= 0 &&
          p->coarse_sweeps >= 0 && p->local_sweeps >= 1 && p->omega > 0.f && p->omega < 2.f &&
@@ -1342,6 +1567,197 @@ int aw_step(aw_grid* g, aw_solver* S, float* d_pos, float* d_vel, int32_t* d_ids
   }
   if (st != AW_OK) g_err = keep;
   return st;
+}
+
+int aw_grid_adjoint_layout(aw_grid* g, aw_adjoint_layout* o) {
+  if (!g || !o) return fail(AW_ERR_INVALID, "NULL argument");
+  if (!g->have_topology) return fail(AW_ERR_INVALID, "grid has no topology");
+  AW_TRY(ensure_adjoint(g));
+  std::memset(o, 0, sizeof *o);
+  for (int a = 0; a < 3; ++a) {
+    o->d_ubar_new[a] = g->ubn[a].as<float>();
+    o->d_ubar_old[a] = g->ubo[a].as<float>();
+    o->d_ubar_star[a] = g->ubs[a].as<float>();
+    o->d_Mbar[a] = g->Mb[a].as<float>();
+    o->d_mbar[a] = g->mb[a].as<float>();
+  }
+  o->d_pbar = g->pbar.as<float>();
+  o->d_rhsbar = g->rhsbar.as<float>();
+  o->d_fine_bar = g->fbar.as<float>();
+  o->d_fine_mbar = g->fmbar.as<float>();
+  o->d_sorted_ids = g->sids.as<int32_t>();
+  o->num_sorted = g->n_sorted;
+  return AW_OK;
+}
+
+int aw_g2p_advect_adjoint(aw_grid* g, float alpha, float dt, float* d_xbar, float* d_vbar) {
+  if (!g || !d_xbar || !d_vbar) return fail(AW_ERR_INVALID, "NULL argument");
+  AW_TRY(check_adjoint_state(g));
+  AW_TRY(validate_ids(g, g->sids.as<int32_t>(), g->n_sorted));
+  return g2p_adjoint(g, alpha, dt, d_xbar, d_vbar);
+}
+
+int aw_project_adjoint(aw_solver* S, double rel_tol, int max_iters, aw_solve_stats* stats) {
+  if (!S || !(rel_tol > 0.0) || max_iters < 1) return fail(AW_ERR_INVALID, "bad arguments");
+  AW_TRY(check_adjoint_state(S->g));
+  aw_solve_stats st{};
+  int rc = project_adjoint(S, rel_tol, max_iters, &st);
+  AW_HIP(hipStreamSynchronize(S->g->ctx->stream));
+  if (stats) *stats = st;
+  return rc;
+}
+
+int aw_p2g_adjoint(aw_grid* g, float* d_xbar, float* d_vbar) {
+  if (!g || !d_xbar || !d_vbar) return fail(AW_ERR_INVALID, "NULL argument");
+  AW_TRY(check_adjoint_state(g));
+  AW_TRY(validate_ids(g, g->sids.as<int32_t>(), g->n_sorted));
+  return p2g_adjoint(g, d_xbar, d_vbar);
+}
+
+int aw_step_adjoint(aw_grid* g, aw_solver* S, const aw_step_params* prm, float* d_xbar, float* d_vbar,
+                    aw_solve_stats* stats) {
+  if (!g || !S || S->g != g || !prm || !d_xbar || !d_vbar) return fail(AW_ERR_INVALID, "bad arguments");
+  AW_TRY(check_adjoint_state(g));
+  AW_TRY(validate_ids(g, g->sids.as<int32_t>(), g->n_sorted));
+  aw_solve_stats st{};
+  int rc = step_adjoint(g, S, prm, d_xbar, d_vbar, &st);
+  AW_HIP(hipStreamSynchronize(g->ctx->stream));
+  if (stats) *stats = st;
+  return rc;
+}
+
+int aw_tape_create(aw_grid* g, aw_solver* S, int64_t n, int num_steps, int checkpoint_every, int keep_masks,
+                   aw_tape** out) {
+  if (!out) return fail(AW_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  if (!g || !S || S->g != g || n < 1 || n > 0x7FFFFFFF || num_steps < 1 || checkpoint_every < 1)
+    return fail(AW_ERR_INVALID, "bad arguments");
+  auto t = std::make_unique<aw_tape>();
+  t->g = g;
+  t->S = S;
+  t->n = n;
+  t->steps = num_steps;
+  t->every = std::min(checkpoint_every, num_steps);
+  t->keep_masks = keep_masks != 0;
+  const int nck = (num_steps + t->every - 1) / t->every;
+  for (int i = 0; i < nck; ++i) t->ck.push_back(std::make_unique<aw_tape::Ckpt>());
+  for (int i = 0; i < t->every; ++i) t->rec.push_back(std::make_unique<aw_tape::Step>());
+  *out = t.release();
+  return AW_OK;
+}
+
+int aw_tape_destroy(aw_tape* t) {
+  delete t;
+  return AW_OK;
+}
+
+int aw_rollout_forward(aw_tape* t, float* d_pos, float* d_vel, int32_t* d_ids, const aw_step_params* prm,
+                       aw_rollout_stats* stats) {
+  if (!t || !d_pos || !d_vel || !d_ids || !prm) return fail(AW_ERR_INVALID, "NULL argument");
+  aw_grid* g = t->g;
+  hipStream_t s = g->ctx->stream;
+  const int64_t n = t->n;
+  t->prm = *prm;
+  t->have_forward = false;
+  t->mask_level.clear();
+  t->mask_liquid.clear();
+  AW_TRY(validate_ids(g, d_ids, n));
+  aw_rollout_stats rs{};
+  AW_HIP(hipStreamSynchronize(s));
+  const double t0 = now_ms();
+  int worst = AW_OK;
+  for (int k = 0; k < t->steps; ++k) {
+    if (k % t->every == 0) {
+      aw_tape::Ckpt& c = *t->ck[k / t->every];
+      AW_TRY(copy_d2d(s, c.pos, d_pos, (size_t)n * 3 * sizeof(float)));
+      AW_TRY(copy_d2d(s, c.vel, d_vel, (size_t)n * 3 * sizeof(float)));
+      AW_TRY(copy_d2d(s, c.ids, d_ids, (size_t)n * sizeof(int32_t)));
+    }
+    aw_step_stats ss{};
+    int rc = aw_step(g, t->S, d_pos, d_vel, d_ids, n, prm, &ss);
+    if (rc != AW_OK && rc != AW_ERR_NOT_CONVERGED) return rc;
+    if (rc != AW_OK) worst = rc;
+    rs.forward_iterations += ss.solve.iterations;
+    rs.forward_rel_residual = std::max(rs.forward_rel_residual, ss.solve.rel_residual);
+    if (t->keep_masks) {
+      t->mask_level.push_back(g->h_level);
+      std::vector<uint8_t> lq(g->ncells);
+      AW_HIP(hipMemcpy(lq.data(), g->liquid.p, g->ncells, hipMemcpyDeviceToHost));
+      t->mask_liquid.push_back(std::move(lq));
+    }
+  }
+  AW_HIP(hipStreamSynchronize(s));
+  rs.forward_ms = now_ms() - t0;
+  rs.steps = t->steps;
+  t->have_forward = true;
+  if (stats) *stats = rs;
+  return worst == AW_OK ? AW_OK : fail(worst, "a forward solve ended above its tolerance");
+}
+
+int aw_rollout_backward(aw_tape* t, float* d_xbar, float* d_vbar, aw_rollout_stats* stats) {
+  if (!t || !d_xbar || !d_vbar) return fail(AW_ERR_INVALID, "NULL argument");
+  if (!t->have_forward) return fail(AW_ERR_INVALID, "no forward rollout on this tape");
+  aw_grid* g = t->g;
+  hipStream_t s = g->ctx->stream;
+  const int64_t n = t->n;
+  AW_TRY(t->wpos.ensure((size_t)n * 3 * sizeof(float)));
+  AW_TRY(t->wvel.ensure((size_t)n * 3 * sizeof(float)));
+  AW_TRY(t->wids.ensure((size_t)n * sizeof(int32_t)));
+  aw_rollout_stats rs{};
+  if (stats) rs = *stats;
+  rs.recompute_ms = rs.backward_ms = 0.0;
+  rs.adjoint_iterations = 0;
+  rs.adjoint_rel_residual = 0.0;
+  int worst = AW_OK;
+  const int nseg = (int)t->ck.size();
+  for (int seg = nseg - 1; seg >= 0; --seg) {
+    const int k0 = seg * t->every, k1 = std::min(t->steps, k0 + t->every);
+    AW_HIP(hipStreamSynchronize(s));
+    double ta = now_ms();
+    aw_tape::Ckpt& c = *t->ck[seg];
+    AW_HIP(hipMemcpyAsync(t->wpos.p, c.pos.p, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s));
+    AW_HIP(hipMemcpyAsync(t->wvel.p, c.vel.p, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s));
+    AW_HIP(hipMemcpyAsync(t->wids.p, c.ids.p, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+    for (int k = k0; k < k1; ++k) {
+      aw_step_stats ss{};
+      int rc = aw_step(g, t->S, t->wpos.as<float>(), t->wvel.as<float>(), t->wids.as<int32_t>(), n, &t->prm, &ss);
+      if (rc != AW_OK && rc != AW_ERR_NOT_CONVERGED) return rc;
+      if (k < k1 - 1) AW_TRY(tape_record(t, *t->rec[k - k0]));
+    }
+    AW_HIP(hipStreamSynchronize(s));
+    double tb = now_ms();
+    rs.recompute_ms += tb - ta;
+    for (int k = k1 - 1; k >= k0; --k) {
+      if (k < k1 - 1) AW_TRY(tape_restore(t, *t->rec[k - k0]));
+      aw_solve_stats st{};
+      int rc = step_adjoint(g, t->S, &t->prm, d_xbar, d_vbar, &st);
+      if (rc != AW_OK && rc != AW_ERR_NOT_CONVERGED) return rc;
+      if (rc != AW_OK) worst = rc;
+      rs.adjoint_iterations += st.iterations;
+      rs.adjoint_rel_residual = std::max(rs.adjoint_rel_residual, st.rel_residual);
+    }
+    AW_HIP(hipStreamSynchronize(s));
+    rs.backward_ms += now_ms() - tb;
+  }
+  AW_TRY(launch_status("rollout_backward"));
+  rs.steps = t->steps;
+  if (stats) *stats = rs;
+  return worst == AW_OK ? AW_OK : fail(worst, "an adjoint solve ended above its tolerance");
+}
+
+int aw_tape_masks(aw_tape* t, int step, uint8_t* h_levels, uint8_t* h_liquid, uint32_t liquid_cap,
+                  uint32_t* ncells) {
+  if (!t || !h_levels || !ncells) return fail(AW_ERR_INVALID, "NULL argument");
+  if (!t->keep_masks || step < 0 || step >= (int)t->mask_level.size())
+    return fail(AW_ERR_INVALID, "no masks recorded for step %d", step);
+  std::copy(t->mask_level[step].begin(), t->mask_level[step].end(), h_levels);
+  const auto& lq = t->mask_liquid[step];
+  *ncells = (uint32_t)lq.size();
+  if (h_liquid) {
+    if (liquid_cap < lq.size()) return fail(AW_ERR_INVALID, "liquid buffer holds %u of %zu cells", liquid_cap, lq.size());
+    std::copy(lq.begin(), lq.end(), h_liquid);
+  }
+  return AW_OK;
 }
 
 }  // extern "C"

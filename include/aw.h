@@ -25,6 +25,8 @@
  *                                                                            include/MultigridTransfer.cuh:107-149
  *   aw_solver_solve                      main.cu's fixed V-cycle loop        src/main.cu:314-317
  *   aw_step                              the main.cu step sequence, completed with the projection
+ *   aw_*_adjoint, aw_tape_*, aw_rollout_* (no counterpart: the reference has no gradient code; these
+ *                                        are the reverse-mode derivative of aw_step and of a rollout)
  */
 #ifndef ANITOWAVE_AW_H
 #define ANITOWAVE_AW_H
@@ -195,6 +197,75 @@ int aw_solver_solve(aw_solver* s, double rel_tol, int max_iters, aw_solve_stats*
  * optional int32 ids permuted with them).  Synchronises. */
 int aw_step(aw_grid* grid, aw_solver* s, float* d_pos, float* d_vel, int32_t* d_ids, int64_t n,
             const aw_step_params* params, aw_step_stats* stats);
+
+/* ---- Adjoint (reverse mode) of the step ----
+ * Derivative of aw_step with the topology, the liquid mask and the mass
+ * thresholds held at their forward values (they are piecewise constant in
+ * the particle state).  The stage entry points run on the state the last
+ * forward step left in the grid (which must have been binned with particle
+ * ids); particle adjoints d_xbar, d_vbar are float[3 n] indexed by particle
+ * id, not by sorted slot. */
+typedef struct {
+  float* d_ubar_new[3];   /* dL/du_new per face slot */
+  float* d_ubar_old[3];   /* dL/du_old through G2P's FLIP term */
+  float* d_ubar_star[3];  /* dL/du_star (projection and divergence) */
+  float* d_Mbar[3];       /* dL/d(momentum accumulator) per slot */
+  float* d_mbar[3];       /* dL/d(mass accumulator) per slot */
+  float* d_pbar;          /* dL/dp on liquid cells */
+  float* d_rhsbar;        /* A^{-1} pbar */
+  float* d_fine_bar;      /* fine lattice [block][field][axis][512]: field 0 Fnew-bar + Fdel-bar, field 1 Fdel-bar */
+  float* d_fine_mbar;     /* fine lattice: field 0 Pi Mbar, field 1 Pi mbar */
+  const int32_t* d_sorted_ids; /* particle id of each sorted slot of the last binning */
+  int64_t num_sorted;
+} aw_adjoint_layout;
+
+int aw_grid_adjoint_layout(aw_grid* grid, aw_adjoint_layout* out);
+
+/* G2P + RK3 adjoint: d_xbar, d_vbar hold dL/dx', dL/dv' on entry and the
+ * direct partials dL/dx, dL/dv on return; fills ubar_new and ubar_old. */
+int aw_g2p_advect_adjoint(aw_grid* grid, float alpha, float dt, float* d_xbar, float* d_vbar);
+/* Projection adjoint: pbar from ubar_new, the adjoint Poisson solve
+ * A rhsbar = pbar with the forward operator and hierarchy (A is symmetric),
+ * then ubar_star.  Overwrites the grid's pressure and rhs.  Synchronises. */
+int aw_project_adjoint(aw_solver* s, double rel_tol, int max_iters, aw_solve_stats* stats);
+/* P2G adjoint: fold adjoint from ubar_old + ubar_star, then adds the deposit
+ * partials to d_xbar, d_vbar. */
+int aw_p2g_adjoint(aw_grid* grid, float* d_xbar, float* d_vbar);
+/* The three stages: dL/d(x', v') -> dL/d(x, v) of the last aw_step.  Synchronises. */
+int aw_step_adjoint(aw_grid* grid, aw_solver* s, const aw_step_params* params, float* d_xbar, float* d_vbar,
+                    aw_solve_stats* stats);
+
+/* Checkpointed rollout.  The forward pass keeps (x, v, ids) every
+ * checkpoint_every steps; the backward pass re-runs each segment from its
+ * checkpoint, keeps the per-step state the step adjoint needs, and walks the
+ * segment backwards.  keep_masks = 1 also records every step's native levels
+ * and liquid mask on the host (aw_tape_masks). */
+typedef struct aw_tape aw_tape;
+
+typedef struct {
+  int steps;
+  double forward_ms;        /* wall time of aw_rollout_forward */
+  double recompute_ms;      /* segment re-runs inside aw_rollout_backward */
+  double backward_ms;       /* step adjoints inside aw_rollout_backward */
+  int forward_iterations;   /* solver iterations summed over the forward steps */
+  int adjoint_iterations;   /* adjoint solver iterations summed over the steps */
+  double forward_rel_residual;  /* worst forward solve */
+  double adjoint_rel_residual;  /* worst adjoint solve */
+} aw_rollout_stats;
+
+int aw_tape_create(aw_grid* grid, aw_solver* s, int64_t n, int num_steps, int checkpoint_every, int keep_masks,
+                   aw_tape** out);
+int aw_tape_destroy(aw_tape* tape);
+/* num_steps steps of aw_step on the particles (ids required).  Synchronises. */
+int aw_rollout_forward(aw_tape* tape, float* d_pos, float* d_vel, int32_t* d_ids, const aw_step_params* params,
+                       aw_rollout_stats* stats);
+/* d_xbar, d_vbar: dL/d(x_K, v_K) on entry, dL/d(x_0, v_0) on return, indexed
+ * by particle id.  Leaves the grid on the state of step 0.  Synchronises. */
+int aw_rollout_backward(aw_tape* tape, float* d_xbar, float* d_vbar, aw_rollout_stats* stats);
+/* Native levels (num_blocks bytes) and liquid mask (*ncells bytes) that
+ * forward step `step` used; h_liquid may be NULL to query *ncells. */
+int aw_tape_masks(aw_tape* tape, int step, uint8_t* h_levels, uint8_t* h_liquid, uint32_t liquid_cap,
+                  uint32_t* ncells);
 
 #ifdef __cplusplus
 }

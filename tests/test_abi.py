@@ -44,3 +44,28 @@ def test_struct_sizes_match_c_layout():
     assert ctypes.sizeof(aw.SolverParams) == 8 * 4
     assert ctypes.sizeof(aw.SolveStats) == 24
     assert ctypes.sizeof(aw.StepParams) == 4 + 12 + 4 + 4 + 8 + 4 + 4
+
+
+def test_struct_layouts_match_the_c_compiler(tmp_path):
+    """sizeof and field offsets of every aw.h struct, from gcc, against the ctypes mirrors."""
+    import subprocess
+    structs = {"aw_grid_desc": aw.GridDesc, "aw_layout": aw.Layout, "aw_solver_params": aw.SolverParams,
+               "aw_solve_stats": aw.SolveStats, "aw_step_params": aw.StepParams, "aw_step_stats": aw.StepStats,
+               "aw_level_info": aw.LevelInfo, "aw_adjoint_layout": aw.AdjointLayout,
+               "aw_rollout_stats": aw.RolloutStats}
+    src = ['#include <stdio.h>', '#include <stddef.h>', '#include "aw.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        src.append(f'  printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            src.append(f'  printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    src.append("  return 0;\n}")
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", aw.INCLUDE_DIR, str(c), "-o", str(exe)], check=True)
+    out = dict(line.rsplit(" ", 1) for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                               check=True).stdout.split("\n") if line)
+    for cname, py in structs.items():
+        assert int(out[cname]) == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(out[f"{cname}.{f}"]) == getattr(py, f).offset, (cname, f)

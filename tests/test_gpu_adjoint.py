@@ -1,0 +1,146 @@
+"""GPU adjoint of the step against the FP64 CPU adjoint (oracle), stage by
+stage, and a checkpointed rollout gradient against the FP64 oracle's reverse
+pass and central finite differences of the FP64 forward.
+
+The oracle adjoint is itself pinned by central differences
+(tests/test_oracle_adjoint.py).  Bars:
+* stage fields and particle partials: 1e-4 of the field's max magnitude;
+* rollout gradient dL/dv0: 1e-3 relative (max norm) against the oracle's;
+* directional derivatives against central differences (eps 1e-3, 32 seeded
+  directions, topology and liquid masks frozen at the GPU's): 1e-3 relative.
+"""
+import numpy as np
+import pytest
+
+import anitowave_adaptive_amd as aw
+import oracle as O
+from anitowave_adaptive_amd import scenes
+from helpers import DT, GRAV, GpuParticles, gpu_grid, oracle_grid, particles, rel_err
+
+pytestmark = pytest.mark.gpu
+
+ALPHA = 0.95
+STAGE_TOL = 1e-4
+
+
+def _liquid_rel(a, b, liq):
+    a = np.asarray(a, np.float64)[liq]
+    b = np.asarray(b, np.float64)[liq]
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
+
+
+@pytest.mark.parametrize("name", ["dam32", "pool64", "slosh32"])
+def test_step_adjoint_stages_match_oracle(ctx, name):
+    print(f"[adjoint] stages {name}", flush=True)
+    pos, vel = particles(name)
+    rng = np.random.default_rng(21)
+    vel = (vel + 0.3 * rng.standard_normal(vel.shape)).astype(np.float32)
+    n = len(pos)
+    og = oracle_grid(name)
+    r = og.step(pos, vel, DT, GRAV, ALPHA, tol=1e-10, max_iters=400)
+    assert r["status"] == 0
+    gg = gpu_grid(ctx, name)
+    sv = aw.VCycleSolver(gg, precision=1)
+    gp = GpuParticles(ctx, pos, vel)
+    stepper = aw.FlipStep(gg, sv, dt=DT, gravity=GRAV, alpha=ALPHA, tol=1e-10, max_iters=400)
+    stepper(gp.pos.ptr, gp.vel.ptr, gp.ids.ptr, n)
+    np.testing.assert_array_equal(gg.cell("liquid"), og.cell("liquid").astype(np.uint8))
+
+    xo = rng.standard_normal((n, 3))
+    vo = rng.standard_normal((n, 3))
+    dxb = aw.DeviceArray.from_numpy(ctx, xo.astype(np.float32))
+    dvb = aw.DeviceArray.from_numpy(ctx, vo.astype(np.float32))
+    adj = aw.Adjoint(gg, sv)
+
+    # G2P + RK3
+    adj.g2p_advect(dxb.ptr, dvb.ptr, ALPHA, DT)
+    xi, vi = og.adj_g2p(pos, vel, xo, vo, ALPHA, DT)
+    for f in ("ubar_new", "ubar_old"):
+        for a in range(3):
+            e = rel_err(adj.face(f, a), og.face(f, a))
+            assert e < STAGE_TOL, (f, a, e)
+    gx = dxb.download(np.float32, 3 * n).reshape(-1, 3)
+    gv = dvb.download(np.float32, 3 * n).reshape(-1, 3)
+    assert rel_err(gx, xi) < STAGE_TOL, rel_err(gx, xi)
+    assert rel_err(gv, vi) < STAGE_TOL
+
+    # projection: pbar, adjoint solve, D^T
+    st = adj.project(tol=1e-10, max_iters=400)
+    og.adj_project(tol=1e-12)
+    liq = og.cell("liquid") > 0
+    assert _liquid_rel(adj.cell("pbar"), og.cell("pbar"), liq) < STAGE_TOL
+    e_rhs = rel_err(adj.cell("rhsbar"), og.cell("rhsbar"))
+    print(f"[adjoint] {name}: adjoint solve {st.iterations} iterations, rel res {st.rel_residual:.2e}, "
+          f"rhsbar err {e_rhs:.2e}", flush=True)
+    assert e_rhs < STAGE_TOL
+    for a in range(3):
+        assert rel_err(adj.face("ubar_star", a), og.face("ubar_star", a)) < STAGE_TOL, a
+
+    # P2G: fold adjoint and deposit partials
+    adj.p2g(dxb.ptr, dvb.ptr)
+    og.adj_p2g(pos, vel, xi, vi)
+    for f in ("Mbar", "mbar"):
+        for a in range(3):
+            e = rel_err(adj.face(f, a), og.face(f, a))
+            assert e < STAGE_TOL, (f, a, e)
+    gx = dxb.download(np.float32, 3 * n).reshape(-1, 3)
+    gv = dvb.download(np.float32, 3 * n).reshape(-1, 3)
+    ex, ev = rel_err(gx, xi), rel_err(gv, vi)
+    print(f"[adjoint] {name}: step dL/dx err {ex:.2e}, dL/dv err {ev:.2e}", flush=True)
+    assert ex < STAGE_TOL and ev < STAGE_TOL
+
+
+def _com_loss(x, target):
+    c = x.mean(axis=0) - target
+    return float(c @ c), np.broadcast_to(2.0 * c / len(x), x.shape).copy()
+
+
+def test_rollout_gradient_matches_oracle_and_central_differences(ctx):
+    """Sloshing pool 32^3, 2 native levels, 8 steps, checkpoint every 4, loss
+    |COM(x_8) - x*|^2, gradient w.r.t. v0 (mixed-precision solves to 1e-10)."""
+    n_cells, levels, band = 32, 2, (0, 1, 0, 0)
+    pos, vel = scenes.slosh(n=n_cells, ppc=2, seed=3)
+    rng = np.random.default_rng(4)
+    vel = (vel + 0.2 * rng.standard_normal(vel.shape)).astype(np.float32)
+    n = len(pos)
+    steps, every = 8, 4
+    target = np.array([0.6, 0.35, 0.5])
+    print(f"[adjoint] rollout: {n} particles, {steps} steps", flush=True)
+    gg = aw.MSBGManager(ctx, (n_cells // 8,) * 3, 1.0 / n_cells, num_levels=levels, band=band)
+    sv = aw.VCycleSolver(gg, precision=1)
+    prm = aw.FlipStep(gg, sv, dt=DT, gravity=GRAV, alpha=ALPHA, tol=1e-10, max_iters=400).params
+    gp = GpuParticles(ctx, pos, vel)
+    ro = aw.Rollout(gg, sv, n, steps, checkpoint_every=every, keep_masks=True)
+    ro.forward(prm, gp.pos.ptr, gp.vel.ptr, gp.ids.ptr)
+    xK, _ = gp.unpermuted()
+    L, xbar = _com_loss(xK.astype(np.float64), target)
+    dxb = aw.DeviceArray.from_numpy(ctx, xbar.astype(np.float32))
+    dvb = aw.DeviceArray.from_numpy(ctx, np.zeros((n, 3), np.float32))
+    st = ro.backward(dxb.ptr, dvb.ptr)
+    g_gpu = dvb.download(np.float32, 3 * n).reshape(-1, 3).astype(np.float64)
+    ratio = (st.recompute_ms + st.backward_ms) / st.forward_ms
+    print(f"[adjoint] forward {st.forward_ms:.1f} ms, recompute {st.recompute_ms:.1f} ms, backward "
+          f"{st.backward_ms:.1f} ms, adjoint/forward {ratio:.2f}", flush=True)
+    masks = [ro.masks(k) for k in range(steps)]
+
+    og = O.OracleGrid((n_cells // 8,) * 3, num_levels=levels, h=1.0 / n_cells, band=band)
+    og.set_params()
+    states, _ = O.rollout64(og, pos, vel, steps, DT, GRAV, ALPHA, tol=1e-12, frozen=masks)
+    _, xbar64 = _com_loss(states[-1][0], target)
+    _, g_or = O.rollout_grad64(og, states, masks, xbar64, np.zeros_like(xbar64), DT, GRAV, ALPHA, tol=1e-12)
+    e = rel_err(g_gpu, g_or)
+    print(f"[adjoint] rollout dL/dv0: GPU vs FP64 oracle adjoint {e:.2e}", flush=True)
+    assert e < 1e-3
+
+    eps = 1e-3
+    worst = 0.0
+    for k in range(32):
+        d = np.random.default_rng(100 + k).standard_normal(vel.shape)
+        d /= np.linalg.norm(d)
+        xp = O.rollout64(og, pos, vel + eps * d, steps, DT, GRAV, ALPHA, tol=1e-12, frozen=masks)[0][-1][0]
+        xm = O.rollout64(og, pos, vel - eps * d, steps, DT, GRAV, ALPHA, tol=1e-12, frozen=masks)[0][-1][0]
+        fd = (_com_loss(xp, target)[0] - _com_loss(xm, target)[0]) / (2 * eps)
+        ad = float((g_gpu * d).sum())
+        worst = max(worst, abs(fd - ad) / abs(fd))
+    print(f"[adjoint] 32 directions: worst |fd - adj| / |fd| = {worst:.2e}", flush=True)
+    assert worst < 1e-3
