@@ -100,6 +100,87 @@ def roofline(prof):
     return rl, per
 
 
+def run_mixed(args, w, pos, vel):
+    """The same workload with mixed precision (FP32 V-cycle PCG inside FP64
+    iterative refinement) to relative residual 1e-8."""
+    import anitowave_adaptive_amd as aw
+    ctx = aw.Context(0)
+    n = len(pos)
+    dpos = aw.DeviceArray.from_numpy(ctx, pos)
+    dvel = aw.DeviceArray.from_numpy(ctx, vel)
+    dids = aw.DeviceArray.from_numpy(ctx, np.arange(n, dtype=np.int32))
+    grid = aw.MSBGManager(ctx, w["dims"], 1.0 / w["n"], num_levels=w["levels"], band=w["band"])
+    solver = aw.VCycleSolver(grid, precision=1)
+    step = aw.FlipStep(grid, solver, dt=1.0 / 240.0, gravity=(0.0, -9.81, 0.0), alpha=0.95, tol=1e-8,
+                       max_iters=400, rebuild_topology=True)
+    step(dpos.ptr, dvel.ptr, dids.ptr, n)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    it, outer, rel = [], [], 0.0
+    for _ in range(args.mixed_steps):
+        st = step(dpos.ptr, dvel.ptr, dids.ptr, n)
+        it.append(st.solve.iterations)
+        outer.append(st.solve.outer_iterations)
+        rel = max(rel, st.solve.rel_residual)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    return {"steps_per_s": round(args.mixed_steps / dt, 3), "ms_per_step": round(1e3 * dt / args.mixed_steps, 3),
+            "rel_tol": 1e-8, "worst_rel_residual": rel, "steps": args.mixed_steps,
+            "iterations_mean": round(float(np.mean(it)), 2), "refinements_mean": round(float(np.mean(outer)), 2)}
+
+
+def run_adjoint(args):
+    """Differentiable rollout: 128^3 effective, 2 native levels, sloshing pool
+    (y < 0.4 plus a seeded sinusoidal perturbation of amplitude 0.05), 16
+    steps, checkpoint every 4; loss |COM(x_16) - x*|^2, gradient with respect
+    to the initial velocities; forward and adjoint solves mixed precision to
+    1e-8.  Ratio = (recompute + backward) / forward."""
+    import anitowave_adaptive_amd as aw
+    from anitowave_adaptive_amd import scenes
+    w = scenes.WORKLOADS["slosh128"]
+    pos, vel = w["scene"](n=w["n"])
+    n = len(pos)
+    ctx = aw.Context(0)
+    grid = aw.MSBGManager(ctx, w["dims"], 1.0 / w["n"], num_levels=w["levels"], band=w["band"])
+    solver = aw.VCycleSolver(grid, precision=1)
+    prm = aw.FlipStep(grid, solver, dt=1.0 / 240.0, gravity=(0.0, -9.81, 0.0), alpha=0.95, tol=1e-8,
+                      max_iters=400).params
+    steps, every = w["steps"], 4
+    ro = aw.Rollout(grid, solver, n, steps, checkpoint_every=every)
+    target = np.array([0.6, 0.35, 0.5])
+    dpos = aw.DeviceArray.from_numpy(ctx, pos)
+    dvel = aw.DeviceArray.from_numpy(ctx, vel)
+    dids = aw.DeviceArray(ctx, 4 * n)
+    dxb = aw.DeviceArray(ctx, 12 * n)
+    dvb = aw.DeviceArray(ctx, 12 * n)
+    res = None
+    for rep in range(2):  # the first pass warms the allocations up
+        dpos.upload(pos)
+        dvel.upload(vel)
+        dids.upload(np.arange(n, dtype=np.int32))
+        st = ro.forward(prm, dpos.ptr, dvel.ptr, dids.ptr)
+        fwd = st.forward_ms
+        x = dpos.download(np.float32, 3 * n).reshape(-1, 3)
+        ids = dids.download(np.int32, n)
+        xs = np.empty_like(x)
+        xs[ids] = x
+        c = xs.astype(np.float64).mean(axis=0) - target
+        dxb.upload(np.broadcast_to((2.0 * c / n).astype(np.float32), (n, 3)).copy())
+        dvb.upload(np.zeros((n, 3), np.float32))
+        st = ro.backward(dxb.ptr, dvb.ptr)
+        g = dvb.download(np.float32, 3 * n)
+        res = {"workload": f"slosh128: 128^3 effective, 2 levels, {n} particles, {steps} steps, checkpoint every "
+                           f"{every}, mixed precision to 1e-8",
+               "forward_ms": round(fwd, 2), "recompute_ms": round(st.recompute_ms, 2),
+               "backward_ms": round(st.backward_ms, 2),
+               "ratio": round((st.recompute_ms + st.backward_ms) / fwd, 3),
+               "forward_iterations": st.forward_iterations, "adjoint_iterations": st.adjoint_iterations,
+               "worst_adjoint_rel_residual": st.adjoint_rel_residual,
+               "grad_norm": float(np.linalg.norm(g)), "loss": float(c @ c)}
+        log(f"[bench] adjoint pass {rep}: {res}")
+    return res
+
+
 def run_cpu_baseline(args, w, pos, vel):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -122,6 +203,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--no-profile", dest="profile", action="store_false")
     ap.add_argument("--no-cpu-baseline", dest="cpu", action="store_false")
+    ap.add_argument("--no-adjoint", dest="adjoint", action="store_false")
+    ap.add_argument("--mixed-steps", type=int, default=5, help="steps of the mixed-precision 1e-8 leg (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only",
@@ -134,6 +217,11 @@ def main():
     gpu = run_gpu(args, w, pos, vel)
     log(f"[bench] gpu {args.steps} steps in {gpu['seconds']:.3f}s, pcg iters {gpu['iters']}")
     rl, per = roofline(gpu["profile"])
+    mixed = None
+    if args.mixed_steps > 0:
+        mixed = run_mixed(args, w, pos, vel)
+        log(f"[bench] mixed precision 1e-8: {mixed}")
+    adj = run_adjoint(args) if args.adjoint else None
     cpu = None
     if args.cpu:
         log("[bench] cpu baseline: one oracle step")
@@ -167,7 +255,9 @@ def main():
         "smoother": None if smoother is None else {"gbs": smoother["gbs"], "frac_hbm_peak": smoother["frac"],
                                                    "us_per_launch": smoother["us_per_launch"]},
         "residual": None if resid is None else {"gbs": resid["gbs"], "frac_hbm_peak": resid["frac"]},
-        "adjoint_forward_ratio": None,
+        "mixed_precision_1e-8": mixed,
+        "adjoint_forward_ratio": None if adj is None else adj["ratio"],
+        "adjoint": adj,
         "kernels": per,
         "cpu_baseline": cpu,
     }
