@@ -131,9 +131,10 @@ void k_g2p_advect(hipStream_t s, const GridDev& g, const float* fine, const floa
                   int64_t n, float alpha, float dt, float* pos_out, float* vel_out);
 
 // adjoint (fine-lattice adjoint buffers: [block][field][axis][512], stride ADJ)
+// one work-group per block over the block's sorted particles [poff[b], poff[b] + pcount[b])
 void k_g2p_advect_adjoint(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const float* svel,
-                          const int32_t* sids, int64_t n, float alpha, float dt, float* xbar, float* vbar,
-                          float* fbar);
+                          const int32_t* sids, const uint32_t* pcount, const uint32_t* poff, float alpha, float dt,
+                          float* xbar, float* vbar, float* fbar);
 void k_pi_transpose(hipStream_t s, const GridDev& g, const float* fbar, float* const* unb, float* const* uob);
 void k_project_adjoint(hipStream_t s, const GridDev& g, float* const* beta, float* const* unb, float* const* usb,
                        float* pbar);

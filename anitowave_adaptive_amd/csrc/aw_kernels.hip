@@ -868,9 +868,16 @@ __device__ __forceinline__ float sample_vjp(const GridDev& g, const float* __res
   return s;
 }
 
-// F[node] += w(u) * val on component a (the transpose of sampling)
-__device__ __forceinline__ void scatter_comp(const GridDev& g, float* F, int stride, int field, int a,
-                                             const float* u, float val) {
+// LDS tile of the fine-lattice adjoint around one block: nodes at block-local
+// coordinates [-AH, 8 + AH) on every axis, per field and component.
+constexpr int AH = 2, AT = BASE + 2 * AH, AT3 = AT * AT * AT;
+
+// F[node] += w(u) * val on component a, into the block's LDS tile where the
+// node lies in it and straight into the global buffer (stride ADJ) otherwise.
+// Node indices are resolved as in sampling: normal index outside [0, N) is a
+// dead node, transverse indices clamp to [0, N - 1].
+__device__ __forceinline__ void tile_scatter(const GridDev& g, float* tile, float* fbar, const int* o, int field,
+                                             int a, const float* u, float val) {
   if (val == 0.f) return;
   int i0[3];
   float t[3];
@@ -885,110 +892,146 @@ __device__ __forceinline__ void scatter_comp(const GridDev& g, float* F, int str
   for (int k = 0; k < 8; ++k) {
     const int dd[3] = {k & 1, (k >> 1) & 1, k >> 2};
     int q[3] = {i0[0] + dd[0], i0[1] + dd[1], i0[2] + dd[2]};
-    const int64_t ad = fine_node(g, stride, field, a, q);
-    if (ad < 0) continue;
+    if (q[a] < 0 || q[a] >= g.N[a]) continue;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      if (d != a) q[d] = iclamp(q[d], 0, g.N[d] - 1);
     const float w = (dd[0] ? t[0] : 1.f - t[0]) * (dd[1] ? t[1] : 1.f - t[1]) * (dd[2] ? t[2] : 1.f - t[2]);
-    unsafeAtomicAdd(F + ad, w * val);
+    const int t0 = q[0] - o[0], t1 = q[1] - o[1], t2 = q[2] - o[2];
+    if ((unsigned)t0 < (unsigned)AT && (unsigned)t1 < (unsigned)AT && (unsigned)t2 < (unsigned)AT) {
+      atomicAdd(&tile[(field * 3 + a) * AT3 + t0 + AT * (t1 + AT * t2)], w * val);
+    } else {
+      const int B = (q[0] >> LOGB) + g.D[0] * ((q[1] >> LOGB) + g.D[1] * (q[2] >> LOGB));
+      const int l = (q[0] & 7) + 8 * ((q[1] & 7) + 8 * (q[2] & 7));
+      unsafeAtomicAdd(fbar + (int64_t)B * ADJ + field * 3 * FINE + a * FINE + l, w * val);
+    }
   }
 }
 
-// Adjoint of g2p_advect_kernel for the particle in sorted slot i (id sids[i]):
-// recomputes the forward stages, reads dL/dx', dL/dv' of the particle,
-// writes its direct partials dL/dx, dL/dv and scatters the field adjoints:
-// fbar field 0 = Fnew-bar + Fdel-bar, field 1 = Fdel-bar.
+// Adjoint of g2p_advect_kernel, one work-group per block over the block's
+// sorted particles: recomputes the forward stages, reads dL/dx', dL/dv' of
+// each particle (by id), writes its direct partials dL/dx, dL/dv and scatters
+// the field adjoints (field 0 = Fnew-bar + Fdel-bar, field 1 = Fdel-bar) into
+// an LDS tile around the block, flushed once per node at the end.
 __global__ __launch_bounds__(TPB) void g2p_advect_adjoint_kernel(GridDev g, const float* __restrict__ F,
                                                                  const float* __restrict__ spos,
                                                                  const float* __restrict__ svel,
-                                                                 const int32_t* __restrict__ sids, int64_t n,
-                                                                 float alpha, float dt, float* __restrict__ xbar,
+                                                                 const int32_t* __restrict__ sids,
+                                                                 const uint32_t* __restrict__ pcount,
+                                                                 const uint32_t* __restrict__ poff, float alpha,
+                                                                 float dt, float* __restrict__ xbar,
                                                                  float* __restrict__ vbar, float* fbar) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int64_t id = sids[i];
+  __shared__ float tile[2 * 3 * AT3];
+  const int B = blockIdx.x;
+  const uint32_t cnt = pcount[B];
+  if (cnt == 0) return;
+  for (int k = threadIdx.x; k < 2 * 3 * AT3; k += TPB) tile[k] = 0.f;
+  int o[3];
+  block_coords(g.D, B, o);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) o[d] = o[d] * BASE - AH;
+  __syncthreads();
   const float umax[3] = {(float)g.N[0] - 0.0009765625f, (float)g.N[1] - 0.0009765625f,
                          (float)g.N[2] - 0.0009765625f};
-  float x[3], u[3], m1[3], v1[3], vd[3];
+  for (uint32_t j = threadIdx.x; j < cnt; j += TPB) {
+    const int64_t i = (int64_t)poff[B] + j;
+    const int64_t id = sids[i];
+    float x[3], u[3], m1[3], v1[3], vd[3];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    x[a] = spos[3 * i + a];
-    const float raw = x[a] * g.inv_h;
-    u[a] = fminf(fmaxf(raw, 0.f), umax[a]);
-    m1[a] = (raw > 0.f && raw < umax[a]) ? 1.f : 0.f;
+    for (int a = 0; a < 3; ++a) {
+      x[a] = spos[3 * i + a];
+      const float raw = x[a] * g.inv_h;
+      u[a] = fminf(fmaxf(raw, 0.f), umax[a]);
+      m1[a] = (raw > 0.f && raw < umax[a]) ? 1.f : 0.f;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      v1[a] = sample_comp(g, F, 0, a, u);
+      vd[a] = sample_comp(g, F, 1, a, u);
+    }
+    float u2[3], m2[3], v2[3], u3[3], m3[3], v3[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float raw = (x[a] + 0.5f * dt * v1[a]) * g.inv_h;
+      u2[a] = fminf(fmaxf(raw, -1.f), (float)g.N[a] + 1.f);
+      m2[a] = (raw > -1.f && raw < (float)g.N[a] + 1.f) ? 1.f : 0.f;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) v2[a] = sample_comp(g, F, 0, a, u2);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float raw = (x[a] + 0.75f * dt * v2[a]) * g.inv_h;
+      u3[a] = fminf(fmaxf(raw, -1.f), (float)g.N[a] + 1.f);
+      m3[a] = (raw > -1.f && raw < (float)g.N[a] + 1.f) ? 1.f : 0.f;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) v3[a] = sample_comp(g, F, 0, a, u3);
+    float k1[3], k2[3], k3[3], xacc[3], vo[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float xn = x[a] + dt * ((2.f / 9.f) * v1[a] + (3.f / 9.f) * v2[a] + (4.f / 9.f) * v3[a]);
+      const float lo = 0.25f * g.h, hi = (float)g.N[a] * g.h - 0.25f * g.h;
+      const float xb = (xn > lo && xn < hi) ? xbar[3 * id + a] : 0.f;
+      vo[a] = vbar[3 * id + a];
+      k1[a] = dt * (2.f / 9.f) * xb;
+      k2[a] = dt * (3.f / 9.f) * xb;
+      k3[a] = dt * (4.f / 9.f) * xb;
+      xacc[a] = xb;
+    }
+    // stage 3: v3 = S(Fnew)(x3), x3 = x + 3/4 dt v2
+    float gr[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      tile_scatter(g, tile, fbar, o, 0, a, u3, k3[a]);
+      if (k3[a] != 0.f) sample_vjp(g, F, SLAB, 0, a, u3, k3[a], gr);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const float b = gr[d] * g.inv_h * m3[d];
+      xacc[d] += b;
+      k2[d] += 0.75f * dt * b;
+      gr[d] = 0.f;
+    }
+    // stage 2: v2 = S(Fnew)(x2), x2 = x + 1/2 dt v1
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      tile_scatter(g, tile, fbar, o, 0, a, u2, k2[a]);
+      if (k2[a] != 0.f) sample_vjp(g, F, SLAB, 0, a, u2, k2[a], gr);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const float b = gr[d] * g.inv_h * m2[d];
+      xacc[d] += b;
+      k1[d] += 0.5f * dt * b;
+      gr[d] = 0.f;
+    }
+    // FLIP/PIC blend and stage 1 at x
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float vnb = k1[a] + (1.f - alpha) * vo[a], vdb = alpha * vo[a];
+      tile_scatter(g, tile, fbar, o, 0, a, u, vnb + vdb);
+      tile_scatter(g, tile, fbar, o, 1, a, u, vdb);
+      if (vnb != 0.f) sample_vjp(g, F, SLAB, 0, a, u, vnb, gr);
+      if (vdb != 0.f) sample_vjp(g, F, SLAB, 1, a, u, vdb, gr);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      xbar[3 * id + d] = xacc[d] + gr[d] * g.inv_h * m1[d];
+      vbar[3 * id + d] = alpha * vo[d];
+    }
   }
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    v1[a] = sample_comp(g, F, 0, a, u);
-    vd[a] = sample_comp(g, F, 1, a, u);
-  }
-  float u2[3], m2[3], v2[3], u3[3], m3[3], v3[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const float raw = (x[a] + 0.5f * dt * v1[a]) * g.inv_h;
-    u2[a] = fminf(fmaxf(raw, -1.f), (float)g.N[a] + 1.f);
-    m2[a] = (raw > -1.f && raw < (float)g.N[a] + 1.f) ? 1.f : 0.f;
-  }
-#pragma unroll
-  for (int a = 0; a < 3; ++a) v2[a] = sample_comp(g, F, 0, a, u2);
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const float raw = (x[a] + 0.75f * dt * v2[a]) * g.inv_h;
-    u3[a] = fminf(fmaxf(raw, -1.f), (float)g.N[a] + 1.f);
-    m3[a] = (raw > -1.f && raw < (float)g.N[a] + 1.f) ? 1.f : 0.f;
-  }
-#pragma unroll
-  for (int a = 0; a < 3; ++a) v3[a] = sample_comp(g, F, 0, a, u3);
-  float k1[3], k2[3], k3[3], xacc[3], vo[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const float xn = x[a] + dt * ((2.f / 9.f) * v1[a] + (3.f / 9.f) * v2[a] + (4.f / 9.f) * v3[a]);
-    const float lo = 0.25f * g.h, hi = (float)g.N[a] * g.h - 0.25f * g.h;
-    const float xb = (xn > lo && xn < hi) ? xbar[3 * id + a] : 0.f;
-    vo[a] = vbar[3 * id + a];
-    k1[a] = dt * (2.f / 9.f) * xb;
-    k2[a] = dt * (3.f / 9.f) * xb;
-    k3[a] = dt * (4.f / 9.f) * xb;
-    xacc[a] = xb;
-  }
-  // stage 3: v3 = S(Fnew)(x3), x3 = x + 3/4 dt v2
-  float gr[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    scatter_comp(g, fbar, ADJ, 0, a, u3, k3[a]);
-    if (k3[a] != 0.f) sample_vjp(g, F, SLAB, 0, a, u3, k3[a], gr);
-  }
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const float b = gr[d] * g.inv_h * m3[d];
-    xacc[d] += b;
-    k2[d] += 0.75f * dt * b;
-    gr[d] = 0.f;
-  }
-  // stage 2: v2 = S(Fnew)(x2), x2 = x + 1/2 dt v1
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    scatter_comp(g, fbar, ADJ, 0, a, u2, k2[a]);
-    if (k2[a] != 0.f) sample_vjp(g, F, SLAB, 0, a, u2, k2[a], gr);
-  }
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const float b = gr[d] * g.inv_h * m2[d];
-    xacc[d] += b;
-    k1[d] += 0.5f * dt * b;
-    gr[d] = 0.f;
-  }
-  // FLIP/PIC blend and stage 1 at x
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const float vnb = k1[a] + (1.f - alpha) * vo[a], vdb = alpha * vo[a];
-    scatter_comp(g, fbar, ADJ, 0, a, u, vnb + vdb);
-    scatter_comp(g, fbar, ADJ, 1, a, u, vdb);
-    if (vnb != 0.f) sample_vjp(g, F, SLAB, 0, a, u, vnb, gr);
-    if (vdb != 0.f) sample_vjp(g, F, SLAB, 1, a, u, vdb, gr);
-  }
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    xbar[3 * id + d] = xacc[d] + gr[d] * g.inv_h * m1[d];
-    vbar[3 * id + d] = alpha * vo[d];
+  __syncthreads();
+  // flush: tile nodes are in-domain (clamped) by construction except the
+  // normal-axis ones past the +wall, which were never written
+  for (int k = threadIdx.x; k < 2 * 3 * AT3; k += TPB) {
+    const float v = tile[k];
+    if (v == 0.f) continue;
+    const int fa = k / AT3, r = k - fa * AT3;
+    const int q[3] = {o[0] + r % AT, o[1] + (r / AT) % AT, o[2] + r / (AT * AT)};
+    const int Bq = (q[0] >> LOGB) + g.D[0] * ((q[1] >> LOGB) + g.D[1] * (q[2] >> LOGB));
+    const int l = (q[0] & 7) + 8 * ((q[1] & 7) + 8 * (q[2] & 7));
+    const int field = fa / 3, a = fa - 3 * field;
+    unsafeAtomicAdd(fbar + (int64_t)Bq * ADJ + field * 3 * FINE + a * FINE + l, v);
   }
 }
 
@@ -2100,11 +2143,10 @@ void k_g2p_advect(hipStream_t s, const GridDev& g, const float* fine, const floa
 }
 
 void k_g2p_advect_adjoint(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const float* svel,
-                          const int32_t* sids, int64_t n, float alpha, float dt, float* xbar, float* vbar,
-                          float* fbar) {
-  if (n > 0)
-    g2p_advect_adjoint_kernel<<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, s>>>(g, fine, spos, svel, sids, n, alpha,
-                                                                             dt, xbar, vbar, fbar);
+                          const int32_t* sids, const uint32_t* pcount, const uint32_t* poff, float alpha, float dt,
+                          float* xbar, float* vbar, float* fbar) {
+  g2p_advect_adjoint_kernel<<<g.nb, TPB, 0, s>>>(g, fine, spos, svel, sids, pcount, poff, alpha, dt, xbar, vbar,
+                                                 fbar);
 }
 void k_pi_transpose(hipStream_t s, const GridDev& g, const float* fbar, float* const* unb, float* const* uob) {
   pi_transpose_kernel<<<g.nb, TPB, 0, s>>>(g, fbar, fp(unb), fp(uob));

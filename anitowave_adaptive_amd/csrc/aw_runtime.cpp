@@ -970,7 +970,8 @@ int g2p_adjoint(aw_grid* g, float alpha, float dt, float* xbar, float* vbar) {
     // particle state 28 B + adjoints read and written 48 B; forward fine lattice and its adjoint per block
     Prof pf(g->ctx, "g2p_advect_adjoint", 76.0 * n + 2.0 * 4.0 * 3 * FINE * g->nb + 4.0 * ADJ * g->nb);
     k_g2p_advect_adjoint(s, gv, g->slab.as<float>(), g->spos.as<float>(), g->svel.as<float>(),
-                         g->sids.as<int32_t>(), n, alpha, dt, xbar, vbar, g->fbar.as<float>());
+                         g->sids.as<int32_t>(), g->count.as<uint32_t>(), g->poff.as<uint32_t>(), alpha, dt, xbar,
+                         vbar, g->fbar.as<float>());
   }
   {
     Prof pf(g->ctx, "pi_transpose", 4.0 * ADJ * g->nb + 8.0 * 3 * g->nfaces);
@@ -1055,7 +1056,7 @@ struct aw_tape {
   struct Step {
     std::vector<uint8_t> level;
     uint32_t ncells = 0, nfaces = 0;
-    DevBuf liquid, uold[3], mass[3], unew[3], spos, svel, sids;
+    DevBuf liquid, uold[3], mass[3], unew[3], spos, svel, sids, count, poff;
   };
   struct Ckpt {
     DevBuf pos, vel, ids;
@@ -1097,6 +1098,8 @@ int tape_record(aw_tape* t, aw_tape::Step& r) {
   AW_TRY(copy_d2d(s, r.spos, g->spos.p, (size_t)t->n * 3 * sizeof(float)));
   AW_TRY(copy_d2d(s, r.svel, g->svel.p, (size_t)t->n * 3 * sizeof(float)));
   AW_TRY(copy_d2d(s, r.sids, g->sids.p, (size_t)t->n * sizeof(int32_t)));
+  AW_TRY(copy_d2d(s, r.count, g->count.p, (size_t)g->nb * sizeof(uint32_t)));
+  AW_TRY(copy_d2d(s, r.poff, g->poff.p, (size_t)(g->nb + 1) * sizeof(uint32_t)));
   return AW_OK;
 }
 
@@ -1118,6 +1121,8 @@ int tape_restore(aw_tape* t, const aw_tape::Step& r) {
   AW_HIP(hipMemcpyAsync(g->spos.p, r.spos.p, (size_t)t->n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s));
   AW_HIP(hipMemcpyAsync(g->svel.p, r.svel.p, (size_t)t->n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s));
   AW_HIP(hipMemcpyAsync(g->sids.p, r.sids.p, (size_t)t->n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+  AW_HIP(hipMemcpyAsync(g->count.p, r.count.p, (size_t)g->nb * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+  AW_HIP(hipMemcpyAsync(g->poff.p, r.poff.p, (size_t)(g->nb + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
   g->n_sorted = t->n;
   g->sorted_ids = true;
   Faces un(g->unew), uo(g->uold);
