@@ -26,7 +26,16 @@ struct LevelDev {
   const int32_t* nbr;          // 6 face neighbours per block (-x,+x,-y,+y,-z,+z), -1 = tank wall
   const uint8_t* liquid;       // per cell
   const float* beta[3];        // per face slot
+  const uint32_t* bd = nullptr;  // block descriptors in work-list order (BD_WORDS each), or null
 };
+
+// Block descriptor of one work-list entry, so that a stencil kernel reaches
+// every offset it needs with one dependent load:
+// [0] block, [1] cell offset, [2] face-slot offset, [3..8] neighbour blocks
+// (-1 = tank wall), [9..14] neighbour cell offsets, [15..20] neighbour
+// face-slot offsets, [21] e | ne0 << 8 | ne1 << 16 | ne2 << 24,
+// [22] ne3 | ne4 << 8 | ne5 << 16 (e = log2 cell edge), [23] unused.
+constexpr int BD_WORDS = 24;
 
 // The native grid as the transfer kernels see it.
 struct GridDev {

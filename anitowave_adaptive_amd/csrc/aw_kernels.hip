@@ -1407,6 +1407,87 @@ __device__ __forceinline__ void stencil_split(const LevelDev& L, const CellRef& 
   }
 }
 
+// Work-list cell i of group w with its block descriptor (one dependent load
+// for every offset the stencil needs).
+__device__ __forceinline__ bool decode_bd(const LevelDev& L, const WG& w, int i, CellRef& o, const uint32_t*& bd) {
+  const int sh = 3 * w.lr;
+  if (i >= (w.count << sh)) return false;
+  o.q = i >> sh;
+  o.li = i & ((1 << sh) - 1);
+  bd = L.bd + (size_t)(w.start + o.q) * BD_WORDS;
+  o.B = (int)bd[0];
+  o.lr = w.lr;
+  o.r = 1 << w.lr;
+  o.l[0] = o.li & (o.r - 1);
+  o.l[1] = (o.li >> w.lr) & (o.r - 1);
+  o.l[2] = o.li >> (2 * w.lr);
+  o.c = bd[1] + o.li;
+  return true;
+}
+
+// stencil_split with the block's metadata read from its descriptor
+template <class Out, typename T>
+__device__ __forceinline__ void stencil_split_bd(const LevelDev& L, const CellRef& cr, const uint32_t* bd, Out outb,
+                                                 T& diag, T& ext, T* w) {
+  const int r = cr.r;
+  const uint32_t m0 = bd[21], m1 = bd[22];
+  const int e = (int)(m0 & 255u);
+  const T s = (T)(1 << e);
+  const uint32_t fo = bd[2];
+  diag = 0;
+  ext = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const int d = 2 * a + side;
+      w[d] = 0;
+      const int nl = cr.l[a] + (side ? 1 : -1);
+      int sl[3] = {cr.l[0], cr.l[1], cr.l[2]};
+      sl[a] += side;
+      if (nl >= 0 && nl < r) {
+        const T c = (T)L.beta[a][fo + face_idx(a, sl, r)] * s;
+        w[d] = c;
+        diag += c;
+        continue;
+      }
+      const int N = (int)bd[3 + d];
+      if (N < 0) continue;
+      const int en = (int)((d < 3 ? m0 >> (8 * (d + 1)) : m1 >> (8 * (d - 3))) & 255u), rn = L.bs >> en;
+      const uint32_t con = bd[9 + d];
+      int nc[3] = {cr.l[0], cr.l[1], cr.l[2]};
+      nc[a] = side ? 0 : rn - 1;
+      if (en == e) {
+        const T c = (T)L.beta[a][fo + face_idx(a, sl, r)] * s;
+        diag += c;
+        ext += c * outb(N, nc, rn, con + cell_idx(nc, rn));
+      } else if (en > e) {
+        const T c = (T)L.beta[a][fo + face_idx(a, sl, r)] * s / (T)1.5;
+        nc[t1] = cr.l[t1] >> 1;
+        nc[t2] = cr.l[t2] >> 1;
+        diag += c;
+        ext += c * outb(N, nc, rn, con + cell_idx(nc, rn));
+      } else {
+        int ps[3];
+        ps[a] = side ? 0 : rn;
+        const uint32_t fon = bd[15 + d];
+        const T sf = (T)0.5 * s;
+#pragma unroll
+        for (int d1 = 0; d1 < 2; ++d1)
+#pragma unroll
+          for (int d2 = 0; d2 < 2; ++d2) {
+            nc[t1] = ps[t1] = 2 * cr.l[t1] + d1;
+            nc[t2] = ps[t2] = 2 * cr.l[t2] + d2;
+            const T c = (T)L.beta[a][fon + face_idx(a, ps, rn)] * sf / (T)1.5;
+            diag += c;
+            ext += c * outb(N, nc, rn, con + cell_idx(nc, rn));
+          }
+      }
+    }
+  }
+}
+
 // in-block part: sum of w[d] * value of the in-block neighbour d
 template <class In>
 __device__ __forceinline__ float inblock(const float* w, int li, int r, In inb) {
@@ -1451,7 +1532,8 @@ __global__ __launch_bounds__(TPB) void smooth_kernel(LevelDev L, float* __restri
   for (int k = 0; k < 2; ++k) {
     const int i = threadIdx.x + k * TPB;
     act[k] = false;
-    if (!decode(L, w, blist, i, cr[k])) continue;
+    const uint32_t* bd;
+    if (!decode_bd(L, w, i, cr[k], bd)) continue;
     const uint32_t c = cr[k].c;
     float xv = x[c];
     const bool liq = L.liquid[c] != 0;
@@ -1460,8 +1542,8 @@ __global__ __launch_bounds__(TPB) void smooth_kernel(LevelDev L, float* __restri
     if (!liq) continue;
     act[k] = true;
     bb[k] = b[c];
-    stencil_split(
-        L, cr[k],
+    stencil_split_bd(
+        L, cr[k], bd,
         [&](int N, const int* nc, int rn, uint32_t cc) {
           float v = x[cc];
           if (pro.nbr && L.liquid[cc])
@@ -1505,10 +1587,11 @@ __global__ __launch_bounds__(TPB) void residual_kernel(LevelDev L, const float* 
   const WG w = wgt[blockIdx.x];
   CellRef cr[2];
   bool ok[2];
+  const uint32_t* bd[2];
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const int i = threadIdx.x + k * TPB;
-    ok[k] = decode(L, w, blist, i, cr[k]);
+    ok[k] = decode_bd(L, w, i, cr[k], bd[k]);
     if (ok[k]) sx[i] = x[cr[k].c];
   }
   __syncthreads();
@@ -1520,8 +1603,8 @@ __global__ __launch_bounds__(TPB) void residual_kernel(LevelDev L, const float* 
     float rv = 0.f;
     if (L.liquid[c]) {
       float dg, ex, cw[6];
-      stencil_split(
-          L, cr[k], [&](int, const int*, int, uint32_t cc) { return x[cc]; }, dg, ex, cw);
+      stencil_split_bd(
+          L, cr[k], bd[k], [&](int, const int*, int, uint32_t cc) { return x[cc]; }, dg, ex, cw);
       const int base = i - cr[k].li;
       const float off = ex + inblock(cw, cr[k].li, cr[k].r, [&](int li) { return sx[base + li]; });
       rv = b[c] - (dg * sx[i] - off);
@@ -1564,10 +1647,11 @@ __global__ __launch_bounds__(TPB) void apply_dot_kernel(LevelDev L, const float*
   const WG w = wgt[blockIdx.x];
   CellRef cr[2];
   bool ok[2];
+  const uint32_t* bd[2];
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const int i = threadIdx.x + k * TPB;
-    ok[k] = decode(L, w, blist, i, cr[k]);
+    ok[k] = decode_bd(L, w, i, cr[k], bd[k]);
     if (ok[k]) sx[i] = x[cr[k].c];
   }
   __syncthreads();
@@ -1582,8 +1666,8 @@ __global__ __launch_bounds__(TPB) void apply_dot_kernel(LevelDev L, const float*
       continue;
     }
     float dg, ex, cw[6];
-    stencil_split(
-        L, cr[k], [&](int, const int*, int, uint32_t cc) { return x[cc]; }, dg, ex, cw);
+    stencil_split_bd(
+        L, cr[k], bd[k], [&](int, const int*, int, uint32_t cc) { return x[cc]; }, dg, ex, cw);
     const int base = i - cr[k].li;
     const float off = ex + inblock(cw, cr[k].li, cr[k].r, [&](int li) { return sx[base + li]; });
     const float v = dg * sx[i] - off;
@@ -1958,14 +2042,15 @@ __global__ __launch_bounds__(TPB) void residual64_kernel(LevelDev L, const doubl
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     CellRef c;
-    if (!decode(L, w, blist, threadIdx.x + k * TPB, c)) continue;
+    const uint32_t* bd;
+    if (!decode_bd(L, w, threadIdx.x + k * TPB, c, bd)) continue;
     if (!L.liquid[c.c]) {
       r32[c.c] = 0.f;
       continue;
     }
     double dg, ex, cw[6];
-    stencil_split(
-        L, c, [&](int, const int*, int, uint32_t cc) { return x[cc]; }, dg, ex, cw);
+    stencil_split_bd(
+        L, c, bd, [&](int, const int*, int, uint32_t cc) { return x[cc]; }, dg, ex, cw);
     const uint32_t base = c.c - c.li;
     double off = ex;
     const int st[3] = {1, c.r, c.r * c.r};

@@ -105,7 +105,14 @@ struct ProfRec {
   double bytes;
 };
 
+// Handles are reference-counted by the handles made from them (grids hold
+// their context, solvers their grid, tapes their solver, aw_malloc buffers
+// their context): destroying a handle that others still use only marks it
+// closed, and it is freed with its last user, whatever order callers (or a
+// garbage collector) destroy them in.
 struct aw_ctx {
+  int refs = 0;
+  bool closed = false;
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
@@ -143,6 +150,8 @@ struct Prof {
 }  // namespace
 
 struct aw_grid {
+  int refs = 0;
+  bool closed = false;
   aw_ctx* ctx = nullptr;
   aw_grid_desc desc{};
   int D[3]{}, N[3]{};
@@ -190,7 +199,12 @@ struct MGLevel {
   uint32_t ncells = 0, nfaces = 0;
   std::vector<uint8_t> h_e;
   std::vector<uint32_t> h_coff;
-  DevBuf e, coff, foff, nbr, liquid, beta[3], x, b, r, blist, wg, cblk;
+  // host sources of the uploaded tables, kept for the level's lifetime so no
+  // asynchronous copy can outlive its source
+  std::vector<uint32_t> h_blist, h_bd, h_cblk, h_foff;
+  std::vector<WG> h_wg;
+  std::vector<int32_t> h_nbr;
+  DevBuf e, coff, foff, nbr, liquid, beta[3], x, b, r, blist, wg, cblk, bdesc;
   int nwg = 0, nwg0 = 0;  // work-group table: colour 0 = [0, nwg0), colour 1 = [nwg0, nwg)
   // algorithmic bytes of one stencil pass over colour 0, colour 1 (x read +
   // x write + b read = 12 B per cell, beta 3 x 4 B per face slot)
@@ -219,6 +233,7 @@ struct MGLevel {
     L.coff = d_coff;
     L.nbr = d_nbr;
     L.liquid = d_liq;
+    L.bd = bdesc.as<uint32_t>();
     return L;
   }
   const WG* wgp() const { return wg.as<WG>(); }
@@ -228,6 +243,8 @@ struct MGLevel {
 }  // namespace
 
 struct aw_solver {
+  int refs = 0;
+  bool closed = false;
   aw_grid* g = nullptr;
   aw_solver_params prm{};
   std::vector<std::unique_ptr<MGLevel>> lv;
@@ -422,7 +439,8 @@ void build_wg(MGLevel& L, std::vector<uint32_t>& blist, std::vector<WG>& wg) {
 int upload_level(hipStream_t s, MGLevel& L, bool own_topology) {
   L.nb = L.D[0] * L.D[1] * L.D[2];
   L.h_coff.assign(L.nb + 1, 0);
-  std::vector<uint32_t> foff(L.nb + 1, 0);
+  std::vector<uint32_t>& foff = L.h_foff;
+  foff.assign(L.nb + 1, 0);
   for (int b = 0; b < L.nb; ++b) {
     const int r = L.bs >> L.h_e[b];
     L.h_coff[b + 1] = L.h_coff[b] + r * r * r;
@@ -436,9 +454,37 @@ int upload_level(hipStream_t s, MGLevel& L, bool own_topology) {
     const int c[3] = {b % L.D[0], (b / L.D[0]) % L.D[1], b / (L.D[0] * L.D[1])};
     L.sweep_bytes[(c[0] + c[1] + c[2]) & 1] += 12.0 * r * r * r + 12.0 * (r + 1) * r * r;
   }
-  std::vector<uint32_t> blist;
-  std::vector<WG> wg;
+  std::vector<uint32_t>& blist = L.h_blist;
+  std::vector<WG>& wg = L.h_wg;
   build_wg(L, blist, wg);
+  {
+    // one descriptor per work-list entry: the block's and its face neighbours' offsets and levels
+    std::vector<int32_t>& nb6 = L.h_nbr;
+    neighbour_table(L.D, nb6);
+    std::vector<uint32_t>& bd = L.h_bd;
+    bd.assign(blist.size() * BD_WORDS, 0u);
+    for (size_t k = 0; k < blist.size(); ++k) {
+      const uint32_t B = blist[k];
+      uint32_t* d = &bd[k * BD_WORDS];
+      d[0] = B;
+      d[1] = L.h_coff[B];
+      d[2] = foff[B];
+      uint32_t ne[6] = {0, 0, 0, 0, 0, 0};
+      for (int j = 0; j < 6; ++j) {
+        const int32_t N = nb6[(size_t)B * 6 + j];
+        d[3 + j] = (uint32_t)N;
+        if (N >= 0) {
+          d[9 + j] = L.h_coff[N];
+          d[15 + j] = foff[N];
+          ne[j] = L.h_e[N];
+        }
+      }
+      d[21] = (uint32_t)L.h_e[B] | (ne[0] << 8) | (ne[1] << 16) | (ne[2] << 24);
+      d[22] = ne[3] | (ne[4] << 8) | (ne[5] << 16);
+    }
+    AW_TRY(L.bdesc.ensure(bd.size() * sizeof(uint32_t)));
+    AW_HIP(hipMemcpyAsync(L.bdesc.p, bd.data(), bd.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  }
   AW_TRY(L.blist.ensure(blist.size() * sizeof(uint32_t)));
   AW_TRY(L.wg.ensure(wg.size() * sizeof(WG)));
   AW_HIP(hipMemcpyAsync(L.blist.p, blist.data(), blist.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
@@ -446,15 +492,15 @@ int upload_level(hipStream_t s, MGLevel& L, bool own_topology) {
   AW_TRY(L.r.ensure((size_t)L.ncells * sizeof(float)));
   L.d_r = L.r.as<float>();
   if ((size_t)L.ncells * 13 <= TAIL_LDS_BYTES) {
-    std::vector<uint32_t> cb(L.ncells);
+    std::vector<uint32_t>& cb = L.h_cblk;
+    cb.assign(L.ncells, 0u);
     for (int b = 0; b < L.nb; ++b)
       for (uint32_t c = L.h_coff[b]; c < L.h_coff[b + 1]; ++c) cb[c] = (uint32_t)b;
     AW_TRY(L.cblk.ensure(cb.size() * sizeof(uint32_t)));
     AW_HIP(hipMemcpyAsync(L.cblk.p, cb.data(), cb.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
   }
   if (!own_topology) return AW_OK;
-  std::vector<int32_t> nbr;
-  neighbour_table(L.D, nbr);
+  const std::vector<int32_t>& nbr = L.h_nbr;
   AW_TRY(L.e.ensure(L.nb));
   AW_TRY(L.coff.ensure((L.nb + 1) * sizeof(uint32_t)));
   AW_TRY(L.foff.ensure((L.nb + 1) * sizeof(uint32_t)));
@@ -1137,6 +1183,35 @@ int tape_restore(aw_tape* t, const aw_tape::Step& r) {
 
 namespace {
 
+void ctx_free(aw_ctx* c) {
+  (void)hipStreamSynchronize(c->stream);
+  if (c->own_stream) (void)hipStreamDestroy(c->stream);
+  for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
+  delete c;
+}
+void ctx_release(aw_ctx* c) {
+  if (--c->refs == 0 && c->closed) ctx_free(c);
+}
+void grid_free(aw_grid* g) {
+  aw_ctx* c = g->ctx;
+  (void)hipStreamSynchronize(c->stream);
+  delete g;
+  ctx_release(c);
+}
+void grid_release(aw_grid* g) {
+  if (--g->refs == 0 && g->closed) grid_free(g);
+}
+void solver_free(aw_solver* S) {
+  aw_grid* g = S->g;
+  (void)hipStreamSynchronize(g->ctx->stream);
+  if (S->h_sc) (void)hipHostFree(S->h_sc);
+  delete S;
+  grid_release(g);
+}
+void solver_release(aw_solver* S) {
+  if (--S->refs == 0 && S->closed) solver_free(S);
+}
+
 bool valid_params(const aw_solver_params* p) {
   return p && (p->method == 0 || p->method == 1) && p->pre_sweeps >= 0 && p->post_sweeps >= 0 &&
          p->coarse_sweeps >= 0 && p->local_sweeps >= 1 && p->omega > 0.f && p->omega < 2.f &&
@@ -1191,10 +1266,8 @@ int aw_ctx_create(int device, void* stream, aw_ctx** out) {
 
 int aw_ctx_destroy(aw_ctx* c) {
   if (!c) return fail(AW_ERR_INVALID, "ctx is NULL");
-  (void)hipStreamSynchronize(c->stream);
-  if (c->own_stream) (void)hipStreamDestroy(c->stream);
-  for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
-  delete c;
+  c->closed = true;
+  if (c->refs == 0) ctx_free(c);
   return AW_OK;
 }
 
@@ -1261,14 +1334,17 @@ int aw_malloc(aw_ctx* c, uint64_t bytes, void** out) {
     *out = nullptr;
     return fail(AW_ERR_OOM, "hipMalloc of %llu bytes failed", (unsigned long long)bytes);
   }
+  c->refs++;
   return AW_OK;
 }
 
 int aw_free(aw_ctx* c, void* p) {
   if (!c) return fail(AW_ERR_INVALID, "ctx is NULL");
   if (p) {
-    AW_HIP(hipStreamSynchronize(c->stream));
-    AW_HIP(hipFree(p));
+    (void)hipStreamSynchronize(c->stream);
+    const hipError_t e = hipFree(p);
+    ctx_release(c);
+    if (e != hipSuccess) return fail(AW_ERR_DEVICE, "hipFree: %s", hipGetErrorString(e));
   }
   return AW_OK;
 }
@@ -1321,14 +1397,15 @@ int aw_grid_create(aw_ctx* ctx, const aw_grid_desc* d, aw_grid** out) {
   AW_TRY(g->nbr.ensure(nbr.size() * sizeof(int32_t)));
   AW_HIP(hipMemcpyAsync(g->nbr.p, nbr.data(), nbr.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
   AW_HIP(hipStreamSynchronize(s));
+  ctx->refs++;
   *out = g.release();
   return AW_OK;
 }
 
 int aw_grid_destroy(aw_grid* g) {
   if (!g) return fail(AW_ERR_INVALID, "grid is NULL");
-  (void)hipStreamSynchronize(g->ctx->stream);
-  delete g;
+  g->closed = true;
+  if (g->refs == 0) grid_free(g);
   return AW_OK;
 }
 
@@ -1437,15 +1514,15 @@ int aw_solver_create(aw_grid* g, const aw_solver_params* p, aw_solver** out) {
   S->prm = *p;
   AW_TRY(S->sc.ensure(SC_N * sizeof(double)));
   AW_HIP(hipHostMalloc((void**)&S->h_sc, SC_N * sizeof(double), hipHostMallocDefault));
+  g->refs++;
   *out = S.release();
   return AW_OK;
 }
 
 int aw_solver_destroy(aw_solver* S) {
   if (!S) return fail(AW_ERR_INVALID, "solver is NULL");
-  (void)hipStreamSynchronize(S->g->ctx->stream);
-  if (S->h_sc) (void)hipHostFree(S->h_sc);
-  delete S;
+  S->closed = true;
+  if (S->refs == 0) solver_free(S);
   return AW_OK;
 }
 
@@ -1647,12 +1724,17 @@ int aw_tape_create(aw_grid* g, aw_solver* S, int64_t n, int num_steps, int check
   const int nck = (num_steps + t->every - 1) / t->every;
   for (int i = 0; i < nck; ++i) t->ck.push_back(std::make_unique<aw_tape::Ckpt>());
   for (int i = 0; i < t->every; ++i) t->rec.push_back(std::make_unique<aw_tape::Step>());
+  S->refs++;
   *out = t.release();
   return AW_OK;
 }
 
 int aw_tape_destroy(aw_tape* t) {
+  if (!t) return fail(AW_ERR_INVALID, "tape is NULL");
+  aw_solver* S = t->S;
+  (void)hipStreamSynchronize(t->g->ctx->stream);
   delete t;
+  solver_release(S);
   return AW_OK;
 }
 

@@ -6,6 +6,9 @@
  * and is asynchronous unless the function says it synchronises.  Handles are
  * not thread-safe: one host thread per context, as the reference's classes.
  * The caller owns particle buffers; the library owns grid and solver memory.
+ * Destroying a handle that others were made from (a context with live grids
+ * or aw_malloc buffers, a grid with live solvers, a solver with live tapes)
+ * only closes it; it is released together with the last of them.
  *
  * Reference interfaces each entry point replaces (paths relative to the
  * reference checkout):

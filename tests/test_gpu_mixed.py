@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import anitowave_adaptive_amd as aw
-from helpers import DT, GRAV, GpuParticles, gpu_grid, oracle_grid, particles
+from helpers import DT, GRAV, GpuParticles, gpu_grid, oracle_grid, particles, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -91,8 +91,14 @@ def test_mixed_step_errors_vs_oracle(ctx, name):
         P, V = gp.unpermuted()
         ex = np.abs(P - r["pos"]).max()
         ev = np.abs(V - r["vel"]).max() / np.abs(r["vel"]).max()
-        out[(prec, tol)] = (ex, ev)
+        ep = rel_err(gg.cell("p"), og.cell("p"))
+        out[(prec, tol)] = (ex, ev, ep)
         print(f"[mixed] step precision {prec} tol {tol:g}: iters {st.solve.iterations} "
-              f"outer {st.solve.outer_iterations} |dx| {ex:.2e} rel |dv| {ev:.2e}", flush=True)
-    assert out[(1, 1e-10)][1] <= out[(0, 1e-6)][1]
-    assert out[(1, 1e-10)][0] < 1e-5
+              f"outer {st.solve.outer_iterations} |dx| {ex:.2e} rel |dv| {ev:.2e} rel |dp| {ep:.2e}", flush=True)
+    # the pressure error against the FP64 oracle falls with the solve
+    # tolerance until the FP32 right-hand side (FP32 P2G) bounds it near 1e-7
+    # (measured: 1.1e-6 at FP32 1e-6, 1.4e-7 mixed 1e-10); particle errors
+    # sit at the FP32 floor of the transfers for every solve precision
+    assert out[(1, 1e-10)][2] < 0.5 * out[(0, 1e-6)][2] and out[(1, 1e-10)][2] < 1e-6
+    for k, (ex, ev, ep) in out.items():
+        assert ex < 1e-5 and ev < 1e-5, (k, ex, ev)
