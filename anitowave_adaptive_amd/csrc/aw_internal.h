@@ -107,7 +107,10 @@ struct TailArgs {
 
 constexpr int DIRECT_MAX = 64;  // coarsest levels up to this many cells are solved exactly
 
+// PCG scalars in device memory: SC_SLOTS accumulators each (a reduction's
+// work-groups spread their atomics over the slots; readers add them).
 enum Scalar { SC_RZ = 0, SC_RZOLD = 1, SC_PQ = 2, SC_RR = 3, SC_BB = 4, SC_N = 8 };
+constexpr int SC_SLOTS = 32;
 
 // ---- launchers (aw_kernels.hip) ----
 void k_fill_f32(hipStream_t s, float* p, size_t n, float v);
@@ -163,7 +166,7 @@ void k_smooth(hipStream_t s, const LevelDev& L, float* x, const float* b, const 
 void k_residual(hipStream_t s, const LevelDev& L, const float* x, const float* b, float* r, const WG* wg, int nwg,
                 const uint32_t* blist, const Restr& rs);
 void k_apply_dot(hipStream_t s, const LevelDev& L, const float* x, float* y, const WG* wg, int nwg,
-                 const uint32_t* blist, double* acc);
+                 const uint32_t* blist, double* acc);  // acc: SC_SLOTS accumulators, or null
 void k_restrict(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, const float* rf, float* bc,
                 float* xc, const WG* wgc, int nwgc, const uint32_t* blistc);
 void k_prolong(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, float* xf, const float* xc,

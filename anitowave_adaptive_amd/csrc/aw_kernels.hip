@@ -101,6 +101,29 @@ __device__ __forceinline__ T block_reduce_sum(T v) {
   return s;
 }
 
+// Grid-wide sums go to SC_SLOTS accumulators per scalar (work-group b adds
+// into slot b % SC_SLOTS), so that thousands of work-groups do not serialise
+// on one address; readers add the slots.  Call from every thread.
+__device__ __forceinline__ void grid_sum(double v, double* acc) {
+  const double s = block_reduce_sum(v);
+  if (threadIdx.x == 0) atomicAdd(acc + (blockIdx.x & (SC_SLOTS - 1)), s);
+}
+
+// value of a slotted scalar, for every thread of the work-group
+__device__ __forceinline__ double sc_get(const double* sc, int idx) {
+  __shared__ double v;
+  if (threadIdx.x < 64) {
+    double t = threadIdx.x < SC_SLOTS ? sc[idx * SC_SLOTS + threadIdx.x] : 0.0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+    if (threadIdx.x == 0) v = t;
+  }
+  __syncthreads();
+  const double r = v;
+  __syncthreads();
+  return r;
+}
+
 // ------------------------------------------------------------- fills
 template <typename T>
 __global__ void fill_kernel(T* p, size_t n, T v) {
@@ -1425,65 +1448,163 @@ __device__ __forceinline__ bool decode_bd(const LevelDev& L, const WG& w, int i,
   return true;
 }
 
+// Coefficient and neighbour term of face d (2 axis + side) of a cell on the
+// block boundary: c and c * value summed over the neighbour cells it touches
+// (one equal or coarser cell, or four finer sub-faces).  dg = ex = 0 on walls.
+template <class Out, typename T>
+__device__ __forceinline__ void boundary_term(const LevelDev& L, const CellRef& cr, const uint32_t* bd, int d,
+                                              Out outb, T& dg, T& ex) {
+  const int a = d >> 1, side = d & 1;
+  const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+  const int r = cr.r;
+  dg = 0;
+  ex = 0;
+  const int N = (int)bd[3 + d];
+  if (N < 0) return;
+  const uint32_t m0 = bd[21], m1 = bd[22];
+  const int e = (int)(m0 & 255u);
+  const T s = (T)(1 << e);
+  const uint32_t fo = bd[2];
+  const int en = (int)((d < 3 ? m0 >> (8 * (d + 1)) : m1 >> (8 * (d - 3))) & 255u), rn = L.bs >> en;
+  const uint32_t con = bd[9 + d];
+  int sl[3] = {cr.l[0], cr.l[1], cr.l[2]};
+  sl[a] += side;
+  int nc[3] = {cr.l[0], cr.l[1], cr.l[2]};
+  nc[a] = side ? 0 : rn - 1;
+  if (en == e) {
+    const T c = (T)L.beta[a][fo + face_idx(a, sl, r)] * s;
+    dg = c;
+    ex = c * outb(N, nc, rn, con + cell_idx(nc, rn));
+  } else if (en > e) {
+    const T c = (T)L.beta[a][fo + face_idx(a, sl, r)] * s / (T)1.5;
+    nc[t1] = cr.l[t1] >> 1;
+    nc[t2] = cr.l[t2] >> 1;
+    dg = c;
+    ex = c * outb(N, nc, rn, con + cell_idx(nc, rn));
+  } else {
+    int ps[3];
+    ps[a] = side ? 0 : rn;
+    const uint32_t fon = bd[15 + d];
+    const T sf = (T)0.5 * s;
+#pragma unroll
+    for (int d1 = 0; d1 < 2; ++d1)
+#pragma unroll
+      for (int d2 = 0; d2 < 2; ++d2) {
+        nc[t1] = ps[t1] = 2 * cr.l[t1] + d1;
+        nc[t2] = ps[t2] = 2 * cr.l[t2] + d2;
+        const T c = (T)L.beta[a][fon + face_idx(a, ps, rn)] * sf / (T)1.5;
+        dg += c;
+        ex += c * outb(N, nc, rn, con + cell_idx(nc, rn));
+      }
+  }
+}
+
 // stencil_split with the block's metadata read from its descriptor
 template <class Out, typename T>
 __device__ __forceinline__ void stencil_split_bd(const LevelDev& L, const CellRef& cr, const uint32_t* bd, Out outb,
                                                  T& diag, T& ext, T* w) {
   const int r = cr.r;
-  const uint32_t m0 = bd[21], m1 = bd[22];
-  const int e = (int)(m0 & 255u);
-  const T s = (T)(1 << e);
+  const T s = (T)(1 << (int)(bd[21] & 255u));
   const uint32_t fo = bd[2];
   diag = 0;
   ext = 0;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
       const int d = 2 * a + side;
       w[d] = 0;
       const int nl = cr.l[a] + (side ? 1 : -1);
-      int sl[3] = {cr.l[0], cr.l[1], cr.l[2]};
-      sl[a] += side;
       if (nl >= 0 && nl < r) {
+        int sl[3] = {cr.l[0], cr.l[1], cr.l[2]};
+        sl[a] += side;
         const T c = (T)L.beta[a][fo + face_idx(a, sl, r)] * s;
         w[d] = c;
         diag += c;
         continue;
       }
-      const int N = (int)bd[3 + d];
-      if (N < 0) continue;
+      T dg, ex;
+      boundary_term(L, cr, bd, d, outb, dg, ex);
+      diag += dg;
+      ext += ex;
+    }
+  }
+}
+
+// The same stencil with every face's loads issued before any is used: the
+// coefficient from the block's own face slot, and one neighbour value per
+// boundary face from an address chosen with selects (the cell itself where
+// the face has no single neighbour).  Only a face with four finer neighbours
+// branches into loads of its own.  rd(cell) reads the value of a cell.
+// Sums run in face order as in stencil_split.
+template <typename T, class Rd>
+__device__ __forceinline__ void stencil_fast(const LevelDev& L, const CellRef& cr, const uint32_t* bd, Rd rd,
+                                             T& diag, T& ext, T* w) {
+  const int r = cr.r;
+  const uint32_t m0 = bd[21], m1 = bd[22];
+  const int e = (int)(m0 & 255u);
+  const T s = (T)(1 << e);
+  const uint32_t fo = bd[2];
+  T bt[6], xn[6];
+  int kind[6];  // 0 in-block, 1 wall, 2 equal neighbour, 3 coarser neighbour, 4 finer neighbours
+#pragma unroll
+  for (int d = 0; d < 6; ++d) {
+    const int a = d >> 1, side = d & 1;
+    const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+    const int nl = cr.l[a] + (side ? 1 : -1);
+    const bool inb = nl >= 0 && nl < r;
+    int sl[3] = {cr.l[0], cr.l[1], cr.l[2]};
+    sl[a] += side;
+    bt[d] = (T)L.beta[a][fo + face_idx(a, sl, r)];
+    const int N = inb ? -1 : (int)bd[3 + d];
+    const int en = (int)((d < 3 ? m0 >> (8 * (d + 1)) : m1 >> (8 * (d - 3))) & 255u);
+    const int rn = L.bs >> en;
+    const bool coarser = en > e;
+    int nc[3] = {cr.l[0], cr.l[1], cr.l[2]};
+    nc[a] = side ? 0 : rn - 1;
+    nc[t1] = coarser ? cr.l[t1] >> 1 : cr.l[t1];
+    nc[t2] = coarser ? cr.l[t2] >> 1 : cr.l[t2];
+    const bool single = !inb && N >= 0 && en >= e;
+    const uint32_t cc = single ? bd[9 + d] + cell_idx(nc, rn) : cr.c;
+    xn[d] = rd(cc);
+    kind[d] = inb ? 0 : (N < 0 ? 1 : (en == e ? 2 : (coarser ? 3 : 4)));
+  }
+  diag = 0;
+  ext = 0;
+#pragma unroll
+  for (int d = 0; d < 6; ++d) {
+    w[d] = 0;
+    if (kind[d] == 0) {
+      const T c = bt[d] * s;
+      w[d] = c;
+      diag += c;
+    } else if (kind[d] == 2) {
+      const T c = bt[d] * s;
+      diag += c;
+      ext += c * xn[d];
+    } else if (kind[d] == 3) {
+      const T c = bt[d] * s / (T)1.5;
+      diag += c;
+      ext += c * xn[d];
+    } else if (kind[d] == 4) {
+      const int a = d >> 1, side = d & 1;
+      const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
       const int en = (int)((d < 3 ? m0 >> (8 * (d + 1)) : m1 >> (8 * (d - 3))) & 255u), rn = L.bs >> en;
-      const uint32_t con = bd[9 + d];
-      int nc[3] = {cr.l[0], cr.l[1], cr.l[2]};
+      const uint32_t con = bd[9 + d], fon = bd[15 + d];
+      int nc[3], ps[3];
       nc[a] = side ? 0 : rn - 1;
-      if (en == e) {
-        const T c = (T)L.beta[a][fo + face_idx(a, sl, r)] * s;
-        diag += c;
-        ext += c * outb(N, nc, rn, con + cell_idx(nc, rn));
-      } else if (en > e) {
-        const T c = (T)L.beta[a][fo + face_idx(a, sl, r)] * s / (T)1.5;
-        nc[t1] = cr.l[t1] >> 1;
-        nc[t2] = cr.l[t2] >> 1;
-        diag += c;
-        ext += c * outb(N, nc, rn, con + cell_idx(nc, rn));
-      } else {
-        int ps[3];
-        ps[a] = side ? 0 : rn;
-        const uint32_t fon = bd[15 + d];
-        const T sf = (T)0.5 * s;
+      ps[a] = side ? 0 : rn;
+      const T sf = (T)0.5 * s;
 #pragma unroll
-        for (int d1 = 0; d1 < 2; ++d1)
+      for (int d1 = 0; d1 < 2; ++d1)
 #pragma unroll
-          for (int d2 = 0; d2 < 2; ++d2) {
-            nc[t1] = ps[t1] = 2 * cr.l[t1] + d1;
-            nc[t2] = ps[t2] = 2 * cr.l[t2] + d2;
-            const T c = (T)L.beta[a][fon + face_idx(a, ps, rn)] * sf / (T)1.5;
-            diag += c;
-            ext += c * outb(N, nc, rn, con + cell_idx(nc, rn));
-          }
-      }
+        for (int d2 = 0; d2 < 2; ++d2) {
+          nc[t1] = ps[t1] = 2 * cr.l[t1] + d1;
+          nc[t2] = ps[t2] = 2 * cr.l[t2] + d2;
+          const T c = (T)L.beta[a][fon + face_idx(a, ps, rn)] * sf / (T)1.5;
+          diag += c;
+          ext += c * rd(con + cell_idx(nc, rn));
+        }
     }
   }
 }
@@ -1520,14 +1641,24 @@ __device__ __forceinline__ uint32_t parent_cell(const LevelDev& F, const LevelDe
 // b, diag and the contribution of the frozen other-colour neighbour blocks;
 // then `local_sweeps` x (cell colour 0, 1) sweeps entirely in LDS; store.
 // `reverse` runs the cell colours in the opposite order (post-smoother).
-__global__ __launch_bounds__(TPB) void smooth_kernel(LevelDev L, float* __restrict__ x, const float* __restrict__ b,
-                                                     const WG* __restrict__ wgt, const uint32_t* __restrict__ blist,
-                                                     float omega, int local_sweeps, int reverse, Pro pro) {
+// The stencil kernels are latency-bound (one round of work-groups): they are
+// held to 64 VGPRs for eight waves per SIMD.
+#define AW_STENCIL_OCCUPANCY __attribute__((amdgpu_waves_per_eu(8, 8)))
+
+__global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void smooth_kernel(
+    LevelDev L, float* __restrict__ x, const float* __restrict__ b, const WG* __restrict__ wgt,
+    const uint32_t* __restrict__ blist, float omega, int local_sweeps, int reverse, Pro pro) {
   __shared__ float sx[512];
   const WG w = wgt[blockIdx.x];
   CellRef cr[2];
   bool act[2];
   float bb[2], dg[2], ex[2], cw[2][6];
+  auto outb = [&](int N, const int* nc, int rn, uint32_t cc) {
+    float v = x[cc];
+    if (pro.nbr && L.liquid[cc])
+      v += pro.scale * pro.xc[parent_cell(L, pro.C, pro.aggregated, N, nc, rn, cell_idx(nc, rn))];
+    return v;
+  };
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const int i = threadIdx.x + k * TPB;
@@ -1542,15 +1673,10 @@ __global__ __launch_bounds__(TPB) void smooth_kernel(LevelDev L, float* __restri
     if (!liq) continue;
     act[k] = true;
     bb[k] = b[c];
-    stencil_split_bd(
-        L, cr[k], bd,
-        [&](int N, const int* nc, int rn, uint32_t cc) {
-          float v = x[cc];
-          if (pro.nbr && L.liquid[cc])
-            v += pro.scale * pro.xc[parent_cell(L, pro.C, pro.aggregated, N, nc, rn, cell_idx(nc, rn))];
-          return v;
-        },
-        dg[k], ex[k], cw[k]);
+    if (pro.nbr)
+      stencil_split_bd(L, cr[k], bd, outb, dg[k], ex[k], cw[k]);
+    else
+      stencil_fast(L, cr[k], bd, [&](uint32_t cc) { return x[cc]; }, dg[k], ex[k], cw[k]);
   }
   __syncthreads();
   for (int ls = 0; ls < local_sweeps; ++ls)
@@ -1578,10 +1704,9 @@ __global__ __launch_bounds__(TPB) void smooth_kernel(LevelDev L, float* __restri
 // r = b - A x (computeExplicitResidual, VCycle.cuh:81-131), 0 on air cells;
 // optionally fused with the restriction b_c = sum of children
 // (restrictResidualsKernel, MultigridTransfer.cuh:9-59) and x_c = 0.
-__global__ __launch_bounds__(TPB) void residual_kernel(LevelDev L, const float* __restrict__ x,
-                                                       const float* __restrict__ b, float* __restrict__ r,
-                                                       const WG* __restrict__ wgt,
-                                                       const uint32_t* __restrict__ blist, Restr rs) {
+__global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void residual_kernel(
+    LevelDev L, const float* __restrict__ x, const float* __restrict__ b, float* __restrict__ r,
+    const WG* __restrict__ wgt, const uint32_t* __restrict__ blist, Restr rs) {
   __shared__ float sx[512];
   __shared__ float sr[512];
   const WG w = wgt[blockIdx.x];
@@ -1603,8 +1728,7 @@ __global__ __launch_bounds__(TPB) void residual_kernel(LevelDev L, const float* 
     float rv = 0.f;
     if (L.liquid[c]) {
       float dg, ex, cw[6];
-      stencil_split_bd(
-          L, cr[k], bd[k], [&](int, const int*, int, uint32_t cc) { return x[cc]; }, dg, ex, cw);
+      stencil_fast(L, cr[k], bd[k], [&](uint32_t cc) { return x[cc]; }, dg, ex, cw);
       const int base = i - cr[k].li;
       const float off = ex + inblock(cw, cr[k].li, cr[k].r, [&](int li) { return sx[base + li]; });
       rv = b[c] - (dg * sx[i] - off);
@@ -1639,10 +1763,10 @@ __global__ __launch_bounds__(TPB) void residual_kernel(LevelDev L, const float* 
   }
 }
 
-// y = A x with acc += x . y
-__global__ __launch_bounds__(TPB) void apply_dot_kernel(LevelDev L, const float* __restrict__ x, float* __restrict__ y,
-                                                        const WG* __restrict__ wgt,
-                                                        const uint32_t* __restrict__ blist, double* acc) {
+// y = A x with sum.acc += x . y
+__global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void apply_dot_kernel(
+    LevelDev L, const float* __restrict__ x, float* __restrict__ y, const WG* __restrict__ wgt,
+    const uint32_t* __restrict__ blist, double* acc) {
   __shared__ float sx[512];
   const WG w = wgt[blockIdx.x];
   CellRef cr[2];
@@ -1666,18 +1790,14 @@ __global__ __launch_bounds__(TPB) void apply_dot_kernel(LevelDev L, const float*
       continue;
     }
     float dg, ex, cw[6];
-    stencil_split_bd(
-        L, cr[k], bd[k], [&](int, const int*, int, uint32_t cc) { return x[cc]; }, dg, ex, cw);
+    stencil_fast(L, cr[k], bd[k], [&](uint32_t cc) { return x[cc]; }, dg, ex, cw);
     const int base = i - cr[k].li;
     const float off = ex + inblock(cw, cr[k].li, cr[k].r, [&](int li) { return sx[base + li]; });
     const float v = dg * sx[i] - off;
     y[c] = v;
     part += (double)sx[i] * (double)v;
   }
-  if (acc) {
-    const double s = block_reduce_sum(part);
-    if (threadIdx.x == 0) atomicAdd(acc, s);
-  }
+  if (acc) grid_sum(part, acc);
 }
 
 // children sum of one coarse cell (halving / keeping block, or aggregation)
@@ -2049,8 +2169,7 @@ __global__ __launch_bounds__(TPB) void residual64_kernel(LevelDev L, const doubl
       continue;
     }
     double dg, ex, cw[6];
-    stencil_split_bd(
-        L, c, bd, [&](int, const int*, int, uint32_t cc) { return x[cc]; }, dg, ex, cw);
+    stencil_fast(L, c, bd, [&](uint32_t cc) { return x[cc]; }, dg, ex, cw);
     const uint32_t base = c.c - c.li;
     double off = ex;
     const int st[3] = {1, c.r, c.r * c.r};
@@ -2061,8 +2180,7 @@ __global__ __launch_bounds__(TPB) void residual64_kernel(LevelDev L, const doubl
     r32[c.c] = (float)rv;
     part += rv * rv;
   }
-  const double s = block_reduce_sum(part);
-  if (threadIdx.x == 0) atomicAdd(rr, s);
+  grid_sum(part, rr);
 }
 
 __global__ void axpy64_kernel(double* __restrict__ x, const float* __restrict__ e, const uint8_t* __restrict__ liq,
@@ -2086,28 +2204,34 @@ __global__ __launch_bounds__(TPB) void dot_kernel(const float* __restrict__ a, c
   double s = 0.0;
   for (int i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB)
     if (liq[i]) s += (double)a[i] * (double)b[i];
-  s = block_reduce_sum(s);
-  if (threadIdx.x == 0) atomicAdd(acc, s);
+  grid_sum(s, acc);
 }
 
+// RZOLD = RZ (summed into slot 0), then RZ, PQ, RR cleared; one wave
 __global__ void scalar_shift_kernel(double* sc) {
-  sc[SC_RZOLD] = sc[SC_RZ];
-  sc[SC_RZ] = 0.0;
-  sc[SC_PQ] = 0.0;
-  sc[SC_RR] = 0.0;
+  const int i = threadIdx.x;
+  double t = i < SC_SLOTS ? sc[SC_RZ * SC_SLOTS + i] : 0.0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+  if (i < SC_SLOTS) {
+    sc[SC_RZOLD * SC_SLOTS + i] = i == 0 ? t : 0.0;
+    sc[SC_RZ * SC_SLOTS + i] = 0.0;
+    sc[SC_PQ * SC_SLOTS + i] = 0.0;
+    sc[SC_RR * SC_SLOTS + i] = 0.0;
+  }
 }
 
 __global__ __launch_bounds__(TPB) void pcg_p_kernel(const float* __restrict__ z, float* __restrict__ p, int n,
                                                     const double* __restrict__ sc, int first) {
-  const float beta = first ? 0.f : (float)(sc[SC_RZ] / sc[SC_RZOLD]);
+  const float beta = first ? 0.f : (float)(sc_get(sc, SC_RZ) / sc_get(sc, SC_RZOLD));
   for (int i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) p[i] = z[i] + beta * p[i];
 }
 
 __global__ __launch_bounds__(TPB) void pcg_xr_kernel(float* __restrict__ x, float* __restrict__ r,
                                                      const float* __restrict__ p, const float* __restrict__ q,
                                                      const uint8_t* __restrict__ liq, int n, double* sc) {
-  const double pq = sc[SC_PQ];
-  const float alpha = pq > 0.0 ? (float)(sc[SC_RZ] / pq) : 0.f;
+  const double pq = sc_get(sc, SC_PQ);
+  const float alpha = pq > 0.0 ? (float)(sc_get(sc, SC_RZ) / pq) : 0.f;
   double s = 0.0;
   for (int i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) {
     x[i] += alpha * p[i];
@@ -2115,8 +2239,7 @@ __global__ __launch_bounds__(TPB) void pcg_xr_kernel(float* __restrict__ x, floa
     r[i] = ri;
     if (liq[i]) s += (double)ri * (double)ri;
   }
-  s = block_reduce_sum(s);
-  if (threadIdx.x == 0) atomicAdd(&sc[SC_RR], s);
+  grid_sum(s, sc + SC_RR * SC_SLOTS);
 }
 
 __global__ void mask_kernel(float* __restrict__ x, const uint8_t* __restrict__ liq, int n) {
@@ -2322,7 +2445,7 @@ void k_dot(hipStream_t s, const float* a, const float* b, const uint8_t* liquid,
   dot_kernel<<<red_grid(n), TPB, 0, s>>>(a, b, liquid, n, acc);
 }
 
-void k_scalar_shift(hipStream_t s, double* sc) { scalar_shift_kernel<<<1, 1, 0, s>>>(sc); }
+void k_scalar_shift(hipStream_t s, double* sc) { scalar_shift_kernel<<<1, 64, 0, s>>>(sc); }
 
 void k_pcg_p(hipStream_t s, const float* z, float* p, int n, const double* sc, int first) {
   pcg_p_kernel<<<red_grid(n), TPB, 0, s>>>(z, p, n, sc, first);

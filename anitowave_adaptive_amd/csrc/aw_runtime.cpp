@@ -255,7 +255,9 @@ struct aw_solver {
   bool direct = false;
   DevBuf x64, e32, r32;  // FP64 iterative refinement: solution, FP32 correction, FP32 residual
   DevBuf r0, z, pv, q, sc;
-  double* h_sc = nullptr;
+  double* h_sc = nullptr;       // host scalars (slots added), points at h_sc_sum
+  double h_sc_sum[SC_N] = {};
+  double* h_slots = nullptr;    // pinned host copy of the slotted device scalars
 };
 
 namespace {
@@ -772,10 +774,19 @@ void vcycle(aw_solver* S, int k, float* x, const float* b) {
   }
 }
 
+// slot group of scalar idx
+double* scp(aw_solver* S, int idx) { return S->sc.as<double>() + idx * SC_SLOTS; }
+
+// copies the slotted scalars to the host and adds each one's slots into h_sc[idx]
 int read_scalars(aw_solver* S) {
   hipStream_t s = S->g->ctx->stream;
-  AW_HIP(hipMemcpyAsync(S->h_sc, S->sc.p, SC_N * sizeof(double), hipMemcpyDeviceToHost, s));
+  AW_HIP(hipMemcpyAsync(S->h_slots, S->sc.p, SC_N * SC_SLOTS * sizeof(double), hipMemcpyDeviceToHost, s));
   AW_HIP(hipStreamSynchronize(s));
+  for (int i = 0; i < SC_N; ++i) {
+    double t = 0.0;
+    for (int k = 0; k < SC_SLOTS; ++k) t += S->h_slots[i * SC_SLOTS + k];
+    S->h_sc[i] = t;
+  }
   return launch_status("solver");
 }
 
@@ -793,10 +804,10 @@ int solve_fp32(aw_solver* S, float* x, const float* b, double tol, int max_iters
   double* sc = S->sc.as<double>();
   const LevelDev V0 = L0.view();
   k_mask(s, x, liq, n);
-  k_fill_f64(s, sc, SC_N, 0.0);
+  k_fill_f64(s, sc, SC_N * SC_SLOTS, 0.0);
   residual_level(S, 0, x, b, r);
-  k_dot(s, b, b, liq, n, sc + SC_BB);
-  k_dot(s, r, r, liq, n, sc + SC_RR);
+  k_dot(s, b, b, liq, n, scp(S, SC_BB));
+  k_dot(s, r, r, liq, n, scp(S, SC_RR));
   AW_TRY(read_scalars(S));
   const double bn = std::sqrt(S->h_sc[SC_BB]);
   double rn = std::sqrt(S->h_sc[SC_RR]);
@@ -821,7 +832,7 @@ int solve_fp32(aw_solver* S, float* x, const float* b, double tol, int max_iters
       k_add(s, x, z, liq, n);
       residual_level(S, 0, x, b, r);
       k_scalar_shift(s, sc);
-      k_dot(s, r, r, liq, n, sc + SC_RR);
+      k_dot(s, r, r, liq, n, scp(S, SC_RR));
       AW_TRY(read_scalars(S));
       rn = std::sqrt(S->h_sc[SC_RR]);
       ++it;
@@ -844,13 +855,13 @@ int solve_fp32(aw_solver* S, float* x, const float* b, double tol, int max_iters
       vcycle(S, 0, z, r);
       {
         Prof pf(g->ctx, "pcg_dot_p", 21.0 * n);
-        k_dot(s, r, z, liq, n, sc + SC_RZ);
+        k_dot(s, r, z, liq, n, scp(S, SC_RZ));
         k_pcg_p(s, z, pv, n, sc, first);
       }
       first = 0;
       {
         Prof pf(g->ctx, "apply_l0", L0.sweep_bytes[0] + L0.sweep_bytes[1] - 4.0 * n);
-        k_apply_dot(s, V0, pv, q, L0.wgp(), L0.nwg, L0.bl(), sc + SC_PQ);
+        k_apply_dot(s, V0, pv, q, L0.wgp(), L0.nwg, L0.bl(), scp(S, SC_PQ));
       }
       {
         Prof pf(g->ctx, "pcg_update", 28.0 * n);
@@ -863,7 +874,7 @@ int solve_fp32(aw_solver* S, float* x, const float* b, double tol, int max_iters
       if (rn > tol * bn) continue;
       residual_level(S, 0, x, b, r);
       k_scalar_shift(s, sc);
-      k_dot(s, r, r, liq, n, sc + SC_RR);
+      k_dot(s, r, r, liq, n, scp(S, SC_RR));
       AW_TRY(read_scalars(S));
       rn = std::sqrt(S->h_sc[SC_RR]);
       if (rn <= tol * bn || restarts == 1) {
@@ -907,8 +918,8 @@ int solve_mixed(aw_solver* S, double tol, int max_iters, aw_solve_stats* st) {
   float* e = S->e32.as<float>();
   float* r = S->r32.as<float>();
   k_to64(s, x64, g->p.as<float>(), liq, n);
-  k_fill_f64(s, sc, SC_N, 0.0);
-  k_dot(s, g->rhs.as<float>(), g->rhs.as<float>(), liq, n, sc + SC_BB);
+  k_fill_f64(s, sc, SC_N * SC_SLOTS, 0.0);
+  k_dot(s, g->rhs.as<float>(), g->rhs.as<float>(), liq, n, scp(S, SC_BB));
   AW_TRY(read_scalars(S));
   const double bn = std::sqrt(S->h_sc[SC_BB]);
   if (st) {
@@ -929,7 +940,7 @@ int solve_mixed(aw_solver* S, double tol, int max_iters, aw_solve_stats* st) {
     k_scalar_shift(s, sc);
     {
       Prof pf(g->ctx, "residual64_l0", 8.0 * n + 4.0 * n + 4.0 * n + 12.0 * L0.nfaces);
-      k_residual64(s, L0.view(), x64, g->rhs.as<float>(), r, L0.wgp(), L0.nwg, L0.bl(), sc + SC_RR);
+      k_residual64(s, L0.view(), x64, g->rhs.as<float>(), r, L0.wgp(), L0.nwg, L0.bl(), scp(S, SC_RR));
     }
     AW_TRY(read_scalars(S));
     rn = std::sqrt(S->h_sc[SC_RR]);
@@ -1204,7 +1215,7 @@ void grid_release(aw_grid* g) {
 void solver_free(aw_solver* S) {
   aw_grid* g = S->g;
   (void)hipStreamSynchronize(g->ctx->stream);
-  if (S->h_sc) (void)hipHostFree(S->h_sc);
+  if (S->h_slots) (void)hipHostFree(S->h_slots);
   delete S;
   grid_release(g);
 }
@@ -1512,8 +1523,9 @@ int aw_solver_create(aw_grid* g, const aw_solver_params* p, aw_solver** out) {
   auto S = std::make_unique<aw_solver>();
   S->g = g;
   S->prm = *p;
-  AW_TRY(S->sc.ensure(SC_N * sizeof(double)));
-  AW_HIP(hipHostMalloc((void**)&S->h_sc, SC_N * sizeof(double), hipHostMallocDefault));
+  AW_TRY(S->sc.ensure(SC_N * SC_SLOTS * sizeof(double)));
+  AW_HIP(hipHostMalloc((void**)&S->h_slots, SC_N * SC_SLOTS * sizeof(double), hipHostMallocDefault));
+  S->h_sc = S->h_sc_sum;
   g->refs++;
   *out = S.release();
   return AW_OK;
