@@ -80,8 +80,11 @@ constexpr int TAIL_T = 1024;
 constexpr int TAIL_MAX_LEVELS = 8;
 constexpr size_t TAIL_LDS_BYTES = 64 * 1024;
 // one work-group evaluates every cell of a tail level per phase: beyond about
-// a thousand cells separate launches are faster
+// a thousand cells of a mixed-resolution level separate launches are faster;
+// a level of one resolution (arithmetic stencil, no table lookups) stays
+// cheaper in the tail up to a few thousand cells
 constexpr uint32_t TAIL_CELLS_MAX = 1024;
+constexpr uint32_t TAIL_UNIFORM_CELLS_MAX = 4096;
 
 struct TailLevel {
   LevelDev L;

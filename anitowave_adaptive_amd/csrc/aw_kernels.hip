@@ -1939,38 +1939,61 @@ __device__ __forceinline__ float tail_ax(const TailLevel& T, const float* xs, in
   return dg * xs[i] - (ex + inblock(cw, c.li, c.r, [&](int li) { return xs[base + li]; }));
 }
 
-__device__ __forceinline__ float tail_diag(const TailLevel& T, int i) {
-  if (T.ulr >= 0) {
-    UStencil u;
-    uniform_stencil(T, i, u);
-    return (float)(1 << T.ue) * (float)u.diagn;
+// colour class of cell i of a uniform level, without the stencil
+__device__ __forceinline__ int uniform_class(const TailLevel& T, int i) {
+  const int lr = T.ulr, r = 1 << lr, sh = 3 * lr;
+  const int B = i >> sh, li = i & ((1 << sh) - 1);
+  int Bc[3];
+  block_coords(T.L.D, B, Bc);
+  return (((Bc[0] + Bc[1] + Bc[2]) & 1) << 1) | (((li & (r - 1)) + ((li >> lr) & (r - 1)) + (li >> (2 * lr))) & 1);
+}
+
+// One weighted Gauss-Seidel update of the cells of class `want`: the class is
+// decided first, and the stencil (row and diagonal) evaluated once, only for
+// the cells that update.
+__device__ __forceinline__ void tail_update(const TailLevel& T, float* xs, const float* bs, const uint8_t* liq,
+                                            int want, float omega) {
+  for (int i = threadIdx.x; i < T.ncells; i += TAIL_T) {
+    if (!liq[i]) continue;
+    if (T.ulr >= 0) {
+      if (uniform_class(T, i) != want) continue;
+      UStencil u;
+      uniform_stencil(T, i, u);
+      const float s = (float)(1 << T.ue);
+      float off = 0.f;
+#pragma unroll
+      for (int d = 0; d < 6; ++d)
+        if (u.nb[d] >= 0) off += s * xs[u.nb[d]];
+      const float dg = s * (float)u.diagn;
+      const float ax = dg * xs[i] - off;
+      if (dg > 0.f) xs[i] += omega * (bs[i] - ax) / dg;
+    } else {
+      CellRef c;
+      if (generic_class(T, i, c) != want) continue;
+      float dg, ex, cw[6];
+      stencil_split(
+          T.L, c, [&](int, const int*, int, uint32_t q) { return xs[q]; }, dg, ex, cw);
+      const int base = i - c.li;
+      const float ax = dg * xs[i] - (ex + inblock(cw, c.li, c.r, [&](int li) { return xs[base + li]; }));
+      if (dg > 0.f) xs[i] += omega * (bs[i] - ax) / dg;
+    }
   }
-  CellRef c;
-  generic_class(T, i, c);
-  float dg, ex, cw[6];
-  stencil_split(
-      T.L, c, [&](int, const int*, int, uint32_t) { return 0.f; }, dg, ex, cw);
-  return dg;
 }
 
 __device__ __forceinline__ void tail_smooth(const TailLevel& T, float* sm, const uint8_t* liq, int iters,
                                             bool reverse, float omega, int local_sweeps) {
   float* xs = sm + T.off;
   const float* bs = sm + T.off + T.ncells;
+  // blocks of one cell have a single cell colour: its second sub-phase is empty
+  const bool one_cell = T.ulr == 0;
   for (int it = 0; it < iters; ++it)
     for (int ph = 0; ph < 2; ++ph) {
       const int bc = reverse ? 1 - ph : ph;
       for (int ls = 0; ls < local_sweeps; ++ls)
         for (int cp = 0; cp < 2; ++cp) {
           const int cc = reverse ? 1 - cp : cp;
-          for (int i = threadIdx.x; i < T.ncells; i += TAIL_T) {
-            if (!liq[i]) continue;
-            int cls;
-            const float ax = tail_ax(T, xs, i, &cls);
-            if (cls != ((bc << 1) | cc)) continue;
-            const float dg = tail_diag(T, i);
-            if (dg > 0.f) xs[i] += omega * (bs[i] - ax) / dg;
-          }
+          if (one_cell && cc == 1) continue;
+          tail_update(T, xs, bs, liq, (bc << 1) | cc, omega);
           __syncthreads();
         }
     }
@@ -2223,7 +2246,12 @@ __global__ void scalar_shift_kernel(double* sc) {
 
 __global__ __launch_bounds__(TPB) void pcg_p_kernel(const float* __restrict__ z, float* __restrict__ p, int n,
                                                     const double* __restrict__ sc, int first) {
-  const float beta = first ? 0.f : (float)(sc_get(sc, SC_RZ) / sc_get(sc, SC_RZOLD));
+  // the first direction is z itself: p may hold anything (a fresh buffer), and 0 * NaN is NaN
+  if (first) {
+    for (int i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) p[i] = z[i];
+    return;
+  }
+  const float beta = (float)(sc_get(sc, SC_RZ) / sc_get(sc, SC_RZOLD));
   for (int i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) p[i] = z[i] + beta * p[i];
 }
 

@@ -588,7 +588,13 @@ int build_hierarchy(aw_solver* S) {
   {
     size_t bytes = 0;
     int t = (int)S->lv.size();
-    while (t > 0 && (int)S->lv.size() - t < TAIL_MAX_LEVELS && S->lv[t - 1]->ncells <= TAIL_CELLS_MAX &&
+    auto fits = [](const MGLevel& L) {
+      if (L.ncells <= TAIL_CELLS_MAX) return true;
+      bool uni = true;
+      for (uint8_t e : L.h_e) uni = uni && e == L.h_e[0];
+      return uni && L.ncells <= TAIL_UNIFORM_CELLS_MAX;
+    };
+    while (t > 0 && (int)S->lv.size() - t < TAIL_MAX_LEVELS && fits(*S->lv[t - 1]) &&
            bytes + (size_t)S->lv[t - 1]->ncells * 13 <= TAIL_LDS_BYTES) {
       bytes += (size_t)S->lv[t - 1]->ncells * 13;
       --t;
