@@ -351,7 +351,9 @@ __global__ __launch_bounds__(DEP_T) void p2g_deposit_kernel(GridDev g, const flo
                                                           const uint32_t* __restrict__ poff, float* __restrict__ slab,
                                                           uint8_t* __restrict__ liquid) {
   __shared__ uint32_t off[4097];
-  __shared__ float su[3][DEP_CH];
+  // fractional coordinates of the chunk's particles: [d] = frac(u_d) (face-normal
+  // axis), [3 + d] = frac(u_d - 1/2) (transverse axes)
+  __shared__ float sf[6][DEP_CH];
   __shared__ float sv[3][DEP_CH];
   __shared__ uint32_t ccnt[512];
   __shared__ uint32_t wsum[DEP_T / 64];
@@ -450,7 +452,9 @@ __global__ __launch_bounds__(DEP_T) void p2g_deposit_kernel(GridDev g, const flo
       const uint32_t sl = off[hb[j]] + rk[j];
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        su[a][sl] = u[j][a];
+        const float un = u[j][a], ut = u[j][a] - 0.5f;
+        sf[a][sl] = un - floorf(un);
+        sf[3 + a][sl] = ut - floorf(ut);
         sv[a][sl] = v[j][a];
       }
     }
@@ -489,7 +493,13 @@ __global__ __launch_bounds__(DEP_T) void p2g_deposit_kernel(GridDev g, const flo
       // the chunk's particles cover half-cell z in [zlo, zhi]
       if (h0[2] + 3 < zlo || h0[2] > zhi) continue;
       float sM = 0.f, sm = 0.f;
-      const float sh0 = a == 0 ? 0.f : 0.5f, sh1 = a == 1 ? 0.f : 0.5f, sh2 = a == 2 ? 0.f : 0.5f;
+      const float* fx = sf[a == 0 ? 0 : 3];
+      const float* fy = sf[a == 1 ? 1 : 4];
+      const float* fz = sf[a == 2 ? 2 : 5];
+      const float* va = sv[a];
+      // the four half-cells of a row are contiguous in the sorted chunk; the
+      // lower pair weighs with (ca, cb)[0], the upper pair with (ca, cb)[1]
+      const int hx_lo = max(h0[0], 0), hx_mid = min(max(h0[0] + 2, 0), 16), hx_hi = min(h0[0] + 4, 16);
       for (int dz = 0; dz < 4; ++dz) {
         const int hz = h0[2] + dz;
         if (hz < 0 || hz > 15) continue;
@@ -499,19 +509,13 @@ __global__ __launch_bounds__(DEP_T) void p2g_deposit_kernel(GridDev g, const flo
           if (hy < 0 || hy > 15) continue;
           const float ya = ca[1][dy >> 1], yb = cb[1][dy >> 1];
           const int rb = 16 * (hy + 16 * hz);
-#pragma unroll
-          for (int px = 0; px < 2; ++px) {
-            const int hx_lo = max(h0[0] + 2 * px, 0), hx_hi = min(h0[0] + 2 * px + 2, 16);
-            if (hx_lo >= hx_hi) continue;
-            const float xa = ca[0][px], xb = cb[0][px];
-            const uint32_t pa = off[rb + hx_lo], pb = off[rb + hx_hi];
-            for (uint32_t q = pa; q < pb; ++q) {
-              const float ux = su[0][q] - sh0, uy = su[1][q] - sh1, uz = su[2][q] - sh2;
-              const float tx = ux - floorf(ux), ty = uy - floorf(uy), tz = uz - floorf(uz);
-              const float w = (xa + xb * tx) * (ya + yb * ty) * (za + zb * tz);
-              sM += w * sv[a][q];
-              sm += w;
-            }
+          const uint32_t pa = off[rb + hx_lo], pm = off[rb + hx_mid], pb = off[rb + hx_hi];
+          for (uint32_t q = pa; q < pb; ++q) {
+            const bool upper = q >= pm;
+            const float xa = upper ? ca[0][1] : ca[0][0], xb = upper ? cb[0][1] : cb[0][0];
+            const float w = (xa + xb * fx[q]) * (ya + yb * fy[q]) * (za + zb * fz[q]);
+            sM += w * va[q];
+            sm += w;
           }
         }
       }
