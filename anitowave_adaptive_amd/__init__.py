@@ -92,6 +92,7 @@ SIGNATURES = {
     "aw_malloc": (_I, [_V, ctypes.c_uint64, _P(_V)]),
     "aw_free": (_I, [_V, _V]),
     "aw_copy": (_I, [_V, _V, _V, ctypes.c_uint64]),
+    "aw_memory_stats": (_I, [_P(ctypes.c_uint64), _P(ctypes.c_uint64), _I]),
     "aw_profile_enable": (_I, [_V, _I]),
     "aw_profile_report": (_I, [_V, ctypes.c_char_p, ctypes.c_uint64]),
     "aw_grid_create": (_I, [_V, _P(GridDesc), _P(_V)]),
@@ -227,6 +228,13 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+def memory_stats(reset_peak=False):
+    """(current, peak) bytes of device memory the library holds in this process."""
+    cur, peak = ctypes.c_uint64(), ctypes.c_uint64()
+    check(lib().aw_memory_stats(ctypes.byref(cur), ctypes.byref(peak), int(bool(reset_peak))), "aw_memory_stats")
+    return cur.value, peak.value
 
 
 def download(ctx, ptr, dtype, count):

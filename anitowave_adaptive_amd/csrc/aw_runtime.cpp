@@ -6,13 +6,16 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/aw.h"
@@ -46,6 +49,18 @@ int fail(int code, const char* fmt, ...) {
     if (s_ != AW_OK) return s_; \
   } while (0)
 
+// device memory the library holds (all contexts), and its high-water mark
+std::atomic<uint64_t> g_mem_cur{0}, g_mem_peak{0};
+void mem_add(uint64_t b) {
+  const uint64_t now = g_mem_cur.fetch_add(b) + b;
+  uint64_t pk = g_mem_peak.load();
+  while (now > pk && !g_mem_peak.compare_exchange_weak(pk, now)) {
+  }
+}
+void mem_sub(uint64_t b) { g_mem_cur.fetch_sub(b); }
+std::mutex g_user_mu;
+std::unordered_map<void*, uint64_t> g_user_alloc;  // aw_malloc'd buffers and their sizes
+
 // grow-only device allocation
 struct DevBuf {
   void* p = nullptr;
@@ -54,12 +69,18 @@ struct DevBuf {
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      (void)hipFree(p);
+      mem_sub(cap);
+    }
   }
   int ensure(size_t bytes) {
     if (bytes == 0) bytes = 256;
     if (bytes <= cap) return AW_OK;
-    if (p) (void)hipFree(p);
+    if (p) {
+      (void)hipFree(p);
+      mem_sub(cap);
+    }
     p = nullptr;
     cap = 0;
     if (hipMalloc(&p, bytes) != hipSuccess) {
@@ -67,6 +88,7 @@ struct DevBuf {
       return fail(AW_ERR_OOM, "hipMalloc of %zu bytes failed", bytes);
     }
     cap = bytes;
+    mem_add(cap);
     return AW_OK;
   }
   template <class T>
@@ -1342,6 +1364,13 @@ int aw_ctx_synchronize(aw_ctx* c) {
   return launch_status("synchronize");
 }
 
+int aw_memory_stats(uint64_t* current_bytes, uint64_t* peak_bytes, int reset_peak) {
+  if (current_bytes) *current_bytes = g_mem_cur.load();
+  if (peak_bytes) *peak_bytes = g_mem_peak.load();
+  if (reset_peak) g_mem_peak.store(g_mem_cur.load());
+  return AW_OK;
+}
+
 int aw_malloc(aw_ctx* c, uint64_t bytes, void** out) {
   if (!c || !out) return fail(AW_ERR_INVALID, "NULL argument");
   *out = nullptr;
@@ -1352,6 +1381,11 @@ int aw_malloc(aw_ctx* c, uint64_t bytes, void** out) {
     return fail(AW_ERR_OOM, "hipMalloc of %llu bytes failed", (unsigned long long)bytes);
   }
   c->refs++;
+  {
+    std::lock_guard<std::mutex> lk(g_user_mu);
+    g_user_alloc[*out] = bytes ? bytes : 1;
+  }
+  mem_add(bytes ? bytes : 1);
   return AW_OK;
 }
 
@@ -1359,6 +1393,14 @@ int aw_free(aw_ctx* c, void* p) {
   if (!c) return fail(AW_ERR_INVALID, "ctx is NULL");
   if (p) {
     (void)hipStreamSynchronize(c->stream);
+    {
+      std::lock_guard<std::mutex> lk(g_user_mu);
+      auto it = g_user_alloc.find(p);
+      if (it != g_user_alloc.end()) {
+        mem_sub(it->second);
+        g_user_alloc.erase(it);
+      }
+    }
     const hipError_t e = hipFree(p);
     ctx_release(c);
     if (e != hipSuccess) return fail(AW_ERR_DEVICE, "hipFree: %s", hipGetErrorString(e));

@@ -41,6 +41,7 @@ def workload(name):
 
 def run_gpu(args, w, pos, vel):
     import anitowave_adaptive_amd as aw
+    aw.memory_stats(reset_peak=True)
     ctx = aw.Context(0)
     n = len(pos)
     dpos = aw.DeviceArray.from_numpy(ctx, pos)
@@ -75,7 +76,8 @@ def run_gpu(args, w, pos, vel):
         ctx.profile(False)
     lv = list(st.num_blocks_by_level)
     res = dict(seconds=t1 - t0, iters=iters, topology_changes=changed, blocks_by_level=lv,
-               total_cells=int(st.total_cells), rel_residual=st.solve.rel_residual, profile=prof)
+               total_cells=int(st.total_cells), rel_residual=st.solve.rel_residual, profile=prof,
+               peak_bytes=aw.memory_stats()[1])
     return res
 
 
@@ -110,6 +112,7 @@ def run_mixed(args, w, pos, vel):
     """The same workload with mixed precision (FP32 V-cycle PCG inside FP64
     iterative refinement) to relative residual 1e-8."""
     import anitowave_adaptive_amd as aw
+    aw.memory_stats(reset_peak=True)
     ctx = aw.Context(0)
     n = len(pos)
     dpos = aw.DeviceArray.from_numpy(ctx, pos)
@@ -132,26 +135,29 @@ def run_mixed(args, w, pos, vel):
     dt = time.perf_counter() - t0
     return {"steps_per_s": round(args.mixed_steps / dt, 3), "ms_per_step": round(1e3 * dt / args.mixed_steps, 3),
             "rel_tol": 1e-8, "worst_rel_residual": rel, "steps": args.mixed_steps,
-            "iterations_mean": round(float(np.mean(it)), 2), "refinements_mean": round(float(np.mean(outer)), 2)}
+            "iterations_mean": round(float(np.mean(it)), 2), "refinements_mean": round(float(np.mean(outer)), 2),
+            "peak_hbm_gib": round(aw.memory_stats()[1] / 2**30, 3)}
 
 
 def run_adjoint(args):
-    """Differentiable rollout: 128^3 effective, 2 native levels, sloshing pool
-    (y < 0.4 plus a seeded sinusoidal perturbation of amplitude 0.05), 16
-    steps, checkpoint every 4; loss |COM(x_16) - x*|^2, gradient with respect
-    to the initial velocities; forward and adjoint solves mixed precision to
-    1e-8.  Ratio = (recompute + backward) / forward."""
+    """Differentiable rollout, by default slosh128: 128^3 effective, 2 native
+    levels, sloshing pool (y < 0.4 plus a seeded sinusoidal perturbation of
+    amplitude 0.05), 16 steps, checkpoint every 4; loss |COM(x_K) - x*|^2,
+    gradient with respect to the initial velocities; forward and adjoint
+    solves mixed precision to 1e-8.  Ratio = (recompute + backward) / forward."""
     import anitowave_adaptive_amd as aw
     from anitowave_adaptive_amd import scenes
-    w = scenes.WORKLOADS["slosh128"]
+    name = args.adjoint_workload
+    w = scenes.WORKLOADS[name]
     pos, vel = w["scene"](n=w["n"])
     n = len(pos)
+    aw.memory_stats(reset_peak=True)
     ctx = aw.Context(0)
     grid = aw.MSBGManager(ctx, w["dims"], 1.0 / w["n"], num_levels=w["levels"], band=w["band"])
     solver = aw.VCycleSolver(grid, precision=1)
     prm = aw.FlipStep(grid, solver, dt=1.0 / 240.0, gravity=(0.0, -9.81, 0.0), alpha=0.95, tol=1e-8,
                       max_iters=400).params
-    steps, every = w["steps"], 4
+    steps, every = (args.adjoint_steps or w["steps"]), 4
     ro = aw.Rollout(grid, solver, n, steps, checkpoint_every=every)
     target = np.array([0.6, 0.35, 0.5])
     dpos = aw.DeviceArray.from_numpy(ctx, pos)
@@ -175,14 +181,15 @@ def run_adjoint(args):
         dvb.upload(np.zeros((n, 3), np.float32))
         st = ro.backward(dxb.ptr, dvb.ptr)
         g = dvb.download(np.float32, 3 * n)
-        res = {"workload": f"slosh128: 128^3 effective, 2 levels, {n} particles, {steps} steps, checkpoint every "
-                           f"{every}, mixed precision to 1e-8",
+        res = {"workload": f"{name}: {w['n']}^3 effective, {w['levels']} levels, {n} particles, {steps} steps, "
+                           f"checkpoint every {every}, mixed precision to 1e-8",
                "forward_ms": round(fwd, 2), "recompute_ms": round(st.recompute_ms, 2),
                "backward_ms": round(st.backward_ms, 2),
                "ratio": round((st.recompute_ms + st.backward_ms) / fwd, 3),
                "forward_iterations": st.forward_iterations, "adjoint_iterations": st.adjoint_iterations,
                "worst_adjoint_rel_residual": st.adjoint_rel_residual,
-               "grad_norm": float(np.linalg.norm(g)), "loss": float(c @ c)}
+               "grad_norm": float(np.linalg.norm(g)), "loss": float(c @ c),
+               "peak_hbm_gib": round(aw.memory_stats()[1] / 2**30, 3)}
         log(f"[bench] adjoint pass {rep}: {res}")
     return res
 
@@ -212,6 +219,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", dest="cpu", action="store_false")
     ap.add_argument("--no-adjoint", dest="adjoint", action="store_false")
     ap.add_argument("--mixed-steps", type=int, default=5, help="steps of the mixed-precision 1e-8 leg (0: skip)")
+    ap.add_argument("--adjoint-workload", default="slosh128")
+    ap.add_argument("--adjoint-steps", type=int, default=0, help="rollout length (0: the workload's own)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only",
@@ -257,6 +266,7 @@ def main():
             "mg0_cells": gpu["total_cells"],
             "pcg_iters_mean": round(float(np.mean(gpu["iters"])), 2),
             "topology_changes": gpu["topology_changes"],
+            "peak_hbm_gib": round(gpu["peak_bytes"] / 2**30, 3),
         },
         "roofline": rl,
         "smoother": None if smoother is None else {"gbs": smoother["gbs"], "frac_hbm_peak": smoother["frac"],

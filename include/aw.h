@@ -151,6 +151,11 @@ int aw_ctx_synchronize(aw_ctx* ctx);
 int aw_malloc(aw_ctx* ctx, uint64_t bytes, void** out);
 int aw_free(aw_ctx* ctx, void* p);
 int aw_copy(aw_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* Device memory the library holds in this process (grids, solvers, tapes,
+ * work buffers and aw_malloc buffers, over all contexts) and its high-water
+ * mark since load or the last reset (no reference counterpart).  reset_peak
+ * sets the mark to the current figure after reading it. */
+int aw_memory_stats(uint64_t* current_bytes, uint64_t* peak_bytes, int reset_peak);
 /* Per-launch timing with HIP events on the context's stream (no reference
  * counterpart).  enable clears earlier records; report synchronises and writes
  * JSON {"kernel": {"launches": n, "ms": total, "bytes": algorithmic bytes}}. */

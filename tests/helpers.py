@@ -10,6 +10,7 @@ SCENES = {
     "dam32": (32, 1, (0, 0, 0, 0), scenes.dam_break),
     "pool64": (64, 3, (1, 2, 0, 0), scenes.pool_sphere),
     "slosh32": (32, 2, (1, 0, 0, 0), scenes.slosh),
+    "double64": (64, 4, (0, 1, 1, 0), scenes.double_dam),  # all four native levels, r = 8, 4, 2, 1
 }
 
 DT = 1.0 / 240.0

@@ -5,6 +5,11 @@ pass and central finite differences of the FP64 forward.
 The oracle adjoint is itself pinned by central differences
 (tests/test_oracle_adjoint.py).  Bars:
 * stage fields and particle partials: 1e-4 of the field's max magnitude;
+  the position partials of G2P + RK3 are piecewise smooth (trilinear weights
+  kink on lattice planes), so a particle whose FP32 stage position lies within
+  rounding of a kink may take the other one-sided derivative: such a particle
+  must match the oracle's partial at the position shifted by -/+3e-7 along
+  (1, 1, 1) instead, and at most 1 in 10^4 particles (at least 2) may do so;
 * rollout gradient dL/dv0: 1e-3 relative (max norm) against the oracle's;
 * directional derivatives against central differences (eps 1e-3, 32 seeded
   directions, topology and liquid masks frozen at the GPU's): 1e-3 relative.
@@ -23,13 +28,39 @@ ALPHA = 0.95
 STAGE_TOL = 1e-4
 
 
+KINK_DELTA = 3e-7  # absolute shift, 2e-5 cells at h = 1/64: beyond FP32 rounding, below the smooth change
+
+
+def _one_sided(og, pos, vel, xo, vo):
+    """Oracle G2P + RK3 position partials at pos shifted by -/+KINK_DELTA along
+    (1, 1, 1).  Call before the base-point adjoint: adj_g2p overwrites the
+    oracle's ubar fields."""
+    p64, v64 = pos.astype(np.float64), vel.astype(np.float64)
+    return [og.adj_g2p(p64 + sgn * KINK_DELTA, v64, xo, vo, ALPHA, DT)[0] for sgn in (-1.0, 1.0)]
+
+
+def _kink_aware(got, base, alts, tol, what):
+    """Per-particle check of (n, 3) partials against the oracle's base point,
+    with the one-sided alternatives for the few particles on a kink; returns
+    the indices that needed an alternative."""
+    got = np.asarray(got, np.float64)
+    scale = max(np.abs(base).max(), 1e-30)
+    err = np.abs(got - base).max(axis=1) / scale
+    bad = np.flatnonzero(err >= tol)
+    for i in bad:
+        alt = min(np.abs(got[i] - a[i]).max() for a in alts) / scale
+        assert alt < tol, (what, int(i), float(err[i]), float(alt))
+    assert len(bad) <= max(2, len(got) // 10000), (what, len(bad))
+    return bad
+
+
 def _liquid_rel(a, b, liq):
     a = np.asarray(a, np.float64)[liq]
     b = np.asarray(b, np.float64)[liq]
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
 
 
-@pytest.mark.parametrize("name", ["dam32", "pool64", "slosh32"])
+@pytest.mark.parametrize("name", ["dam32", "pool64", "slosh32", "double64"])
 def test_step_adjoint_stages_match_oracle(ctx, name):
     print(f"[adjoint] stages {name}", flush=True)
     pos, vel = particles(name)
@@ -54,6 +85,7 @@ def test_step_adjoint_stages_match_oracle(ctx, name):
 
     # G2P + RK3
     adj.g2p_advect(dxb.ptr, dvb.ptr, ALPHA, DT)
+    alts = _one_sided(og, pos, vel, xo, vo)
     xi, vi = og.adj_g2p(pos, vel, xo, vo, ALPHA, DT)
     for f in ("ubar_new", "ubar_old"):
         for a in range(3):
@@ -61,8 +93,12 @@ def test_step_adjoint_stages_match_oracle(ctx, name):
             assert e < STAGE_TOL, (f, a, e)
     gx = dxb.download(np.float32, 3 * n).reshape(-1, 3)
     gv = dvb.download(np.float32, 3 * n).reshape(-1, 3)
-    assert rel_err(gx, xi) < STAGE_TOL, rel_err(gx, xi)
+    kinks = _kink_aware(gx, xi, alts, STAGE_TOL, "g2p dL/dx")
+    print(f"[adjoint] {name}: g2p dL/dx err {rel_err(gx, xi):.2e}, {len(kinks)} particle(s) on a kink", flush=True)
     assert rel_err(gv, vi) < STAGE_TOL
+    keep = np.ones(n, bool)
+    keep[kinks] = False
+    gx_before, gv_before = gx.astype(np.float64), gv.astype(np.float64)
 
     # projection: pbar, adjoint solve, D^T
     st = adj.project(tol=1e-10, max_iters=400)
@@ -76,8 +112,10 @@ def test_step_adjoint_stages_match_oracle(ctx, name):
     for a in range(3):
         assert rel_err(adj.face("ubar_star", a), og.face("ubar_star", a)) < STAGE_TOL, a
 
-    # P2G: fold adjoint and deposit partials
+    # P2G: fold adjoint and deposit partials, alone (the oracle's P2G adjoint
+    # accumulated onto the GPU's own G2P partials) and for the whole step
     adj.p2g(dxb.ptr, dvb.ptr)
+    og.adj_p2g(pos, vel, gx_before, gv_before)
     og.adj_p2g(pos, vel, xi, vi)
     for f in ("Mbar", "mbar"):
         for a in range(3):
@@ -85,8 +123,11 @@ def test_step_adjoint_stages_match_oracle(ctx, name):
             assert e < STAGE_TOL, (f, a, e)
     gx = dxb.download(np.float32, 3 * n).reshape(-1, 3)
     gv = dvb.download(np.float32, 3 * n).reshape(-1, 3)
-    ex, ev = rel_err(gx, xi), rel_err(gv, vi)
-    print(f"[adjoint] {name}: step dL/dx err {ex:.2e}, dL/dv err {ev:.2e}", flush=True)
+    sx, sv_ = rel_err(gx, gx_before), rel_err(gv, gv_before)
+    ex, ev = rel_err(gx[keep], xi[keep]), rel_err(gv, vi)
+    print(f"[adjoint] {name}: p2g stage err x {sx:.2e} v {sv_:.2e}; step dL/dx err {ex:.2e}, dL/dv err {ev:.2e}",
+          flush=True)
+    assert sx < STAGE_TOL and sv_ < STAGE_TOL
     assert ex < STAGE_TOL and ev < STAGE_TOL
 
 

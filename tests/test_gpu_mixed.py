@@ -31,7 +31,7 @@ def _setup(ctx, name, precision):
     return og, gg, gp, sv
 
 
-@pytest.mark.parametrize("name", ["dam32", "pool64", "slosh32"])
+@pytest.mark.parametrize("name", ["dam32", "pool64", "slosh32", "double64"])
 @pytest.mark.parametrize("tol", [1e-8, 1e-10])
 def test_mixed_precision_reaches_fp64_tolerance(ctx, name, tol):
     print(f"[mixed] {name} tol {tol}", flush=True)

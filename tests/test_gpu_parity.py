@@ -12,7 +12,7 @@ from helpers import DT, GRAV, GpuParticles, gpu_grid, oracle_grid, particles, re
 
 pytestmark = pytest.mark.gpu
 
-NAMES = ["dam32", "pool64", "slosh32"]
+NAMES = ["dam32", "pool64", "slosh32", "double64"]
 TOL = 1e-5
 
 
