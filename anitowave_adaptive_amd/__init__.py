@@ -107,6 +107,7 @@ SIGNATURES = {
     "aw_g2p_advect": (_I, [_V, _V, _V, _I64, _F, _F]),
     "aw_solver_create": (_I, [_V, _P(SolverParams), _P(_V)]),
     "aw_solver_destroy": (_I, [_V]),
+    "aw_solver_set_graph": (_I, [_V, _I]),
     "aw_solver_setup": (_I, [_V]),
     "aw_solver_level_info": (_I, [_V, _I, _P(LevelInfo)]),
     "aw_solver_smooth": (_I, [_V, _I, _I, _I]),
@@ -396,6 +397,10 @@ class VCycleSolver:
     @property
     def handle(self):
         return self._h
+
+    def use_graph(self, on=True):
+        """Replay each PCG iteration from a captured HIP graph (default off)."""
+        check(lib().aw_solver_set_graph(self._h, int(bool(on))), "aw_solver_set_graph")
 
     def setup(self):
         check(lib().aw_solver_setup(self._h), "aw_solver_setup")

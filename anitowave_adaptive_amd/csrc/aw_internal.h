@@ -49,10 +49,25 @@ struct GridDev {
   const int32_t* nbr;
 };
 
-// Work-group table entry: `count` blocks of resolution 1 << lr starting at
-// blist[start].  A group holds at most 512 cells.
+// Work-group of a stencil launch: `count` blocks of resolution 1 << lr
+// starting at work-list entry `start`.  A group holds at most 512 cells.
 struct WG {
   int start, count, lr, pad;
+};
+
+// Work-groups of one launch, decoded arithmetically from blockIdx (no table
+// load on the kernels' critical path).  The work list holds runs of blocks of
+// one resolution (per colour, resolutions from coarse cells to fine); run k
+// covers work-list entries [s0[k], s0[k] + cnt[k]) packed 512 >> 3 lr[k]
+// blocks per group, and its groups start at launch index g0[k].  Unused runs
+// have g0 = INT_MAX.
+constexpr int WGMAP_RUNS = 8;
+struct WGMap {
+  int n;                     // work-groups in the launch
+  int g0[WGMAP_RUNS];
+  int s0[WGMAP_RUNS];
+  int cnt[WGMAP_RUNS];
+  int lr[WGMAP_RUNS];
 };
 
 // Coarse correction fused into a smoothing phase: x += scale * xc[parent]
@@ -164,18 +179,18 @@ void k_masked_copy(hipStream_t s, float* dst, const float* src, const uint8_t* l
 void k_ids_range(hipStream_t s, const int32_t* ids, int64_t n, int* out);  // out: {min, max}, preset by the caller
 
 // solver
-void k_smooth(hipStream_t s, const LevelDev& L, float* x, const float* b, const WG* wg, int nwg,
+void k_smooth(hipStream_t s, const LevelDev& L, float* x, const float* b, const WGMap& wg,
               const uint32_t* blist, float omega, int local_sweeps, int reverse, const Pro& pro);
-void k_residual(hipStream_t s, const LevelDev& L, const float* x, const float* b, float* r, const WG* wg, int nwg,
+void k_residual(hipStream_t s, const LevelDev& L, const float* x, const float* b, float* r, const WGMap& wg,
                 const uint32_t* blist, const Restr& rs);
-void k_apply_dot(hipStream_t s, const LevelDev& L, const float* x, float* y, const WG* wg, int nwg,
+void k_apply_dot(hipStream_t s, const LevelDev& L, const float* x, float* y, const WGMap& wg,
                  const uint32_t* blist, double* acc);  // acc: SC_SLOTS accumulators, or null
 void k_restrict(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, const float* rf, float* bc,
-                float* xc, const WG* wgc, int nwgc, const uint32_t* blistc);
+                float* xc, const WGMap& wgc, const uint32_t* blistc);
 void k_prolong(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, float* xf, const float* xc,
-               float scale, const WG* wgf, int nwgf, const uint32_t* blistf);
+               float scale, const WGMap& wgf, const uint32_t* blistf);
 void k_coarsen_liquid(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, uint8_t* liquid_c,
-                      const WG* wgc, int nwgc, const uint32_t* blistc);
+                      const WGMap& wgc, const uint32_t* blistc);
 void k_tail(hipStream_t s, const TailArgs& args, size_t lds_bytes);
 void k_coarse_factor(hipStream_t s, const LevelDev& L, int n, double* inv);
 
@@ -187,8 +202,8 @@ void k_pcg_xr(hipStream_t s, float* x, float* r, const float* p, const float* q,
               double* sc);
 void k_mask(hipStream_t s, float* x, const uint8_t* liquid, int n);
 // FP64 iterative refinement
-void k_residual64(hipStream_t s, const LevelDev& L, const double* x, const float* b, float* r32, const WG* wg,
-                  int nwg, const uint32_t* blist, double* rr);
+void k_residual64(hipStream_t s, const LevelDev& L, const double* x, const float* b, float* r32, const WGMap& wg,
+                  const uint32_t* blist, double* rr);
 void k_axpy64(hipStream_t s, double* x, const float* e, const uint8_t* liquid, int n);
 void k_to64(hipStream_t s, double* x, const float* p, const uint8_t* liquid, int n);
 void k_to32(hipStream_t s, float* p, const double* x, int n);

@@ -333,17 +333,19 @@ __global__ void init_beta_kernel(LevelDev L, float* __restrict__ bx, float* __re
 }
 
 // ------------------------------------------------------------- P2G
-// One work-group per block, without float atomics: particles of the block are
-// counting-sorted by half-cell (16^3 bins of h/2) in LDS, chunk by chunk, and
-// every padded fine node of the block gathers the particles of the 4x4x4
+// One work-group per block, without float atomics: the block's particles
+// are counting-sorted by half-cell (16^3 bins of h/2) into LDS, and every
+// padded fine node of the block gathers the particles of the 4x4x4
 // half-cells that cover its trilinear support (P2G.cuh:16-135 scatters with
-// global float atomics instead).  The node weights are the same hat products
-// the scatter form uses, so the result is identical up to summation order.
-constexpr int DEP_CH = 1024;          // particles per chunk
-constexpr int DEP_T = 512;            // threads per work-group
+// global float atomics instead).  A block's particles fit in LDS at once
+// (8 per fine cell is 4096); more are taken in chunks.  The node weights are
+// the hat products of the scatter form, so the result is identical up to
+// summation order.  (LDS float atomics serialise per lane on this part, which
+// makes the scatter form several times slower than this gather.)
+constexpr int DEP_CH = 4096;          // particles per chunk
+constexpr int DEP_T = 1024;           // threads per work-group
 constexpr int DEP_NODES = 3 * PAD;    // padded nodes of all three axes
-
-__device__ __forceinline__ float hat(float d) { return fmaxf(0.f, 1.f - fabsf(d)); }
+constexpr int DEP_NPT = (DEP_NODES + DEP_T - 1) / DEP_T;  // nodes per thread
 
 __global__ __launch_bounds__(DEP_T) void p2g_deposit_kernel(GridDev g, const float* __restrict__ spos,
                                                           const float* __restrict__ svel,
@@ -351,14 +353,10 @@ __global__ __launch_bounds__(DEP_T) void p2g_deposit_kernel(GridDev g, const flo
                                                           const uint32_t* __restrict__ poff, float* __restrict__ slab,
                                                           uint8_t* __restrict__ liquid) {
   __shared__ uint32_t off[4097];
-  // fractional coordinates of the chunk's particles: [d] = frac(u_d) (face-normal
-  // axis), [3 + d] = frac(u_d - 1/2) (transverse axes)
-  __shared__ float sf[6][DEP_CH];
+  __shared__ float su[3][DEP_CH];   // particle coordinates in the block frame (cells), half-cell order
   __shared__ float sv[3][DEP_CH];
   __shared__ uint32_t ccnt[512];
   __shared__ uint32_t wsum[DEP_T / 64];
-  __shared__ float acc[2 * DEP_NODES];   // node sums (M, m), each node owned by one thread
-  __shared__ int zrange[2];
   const int B = blockIdx.x, tid = threadIdx.x;
   const uint32_t cnt = pcount[B];
   const int e = g.level[B], r = BASE >> e, nc = r * r * r;
@@ -369,16 +367,12 @@ __global__ __launch_bounds__(DEP_T) void p2g_deposit_kernel(GridDev g, const flo
   }
   int Bc[3];
   block_coords(g.D, B, Bc);
-  for (int k = tid; k < 2 * DEP_NODES; k += DEP_T) acc[k] = 0.f;
   for (int k = tid; k < 512; k += DEP_T) ccnt[k] = 0;
+  float* dst = slab + (size_t)B * SLAB;
   const uint32_t p0 = poff[B];
   for (uint32_t c0 = 0; c0 < cnt; c0 += DEP_CH) {
     const int m = (int)min((uint32_t)DEP_CH, cnt - c0);
     for (int k = tid; k < 4097; k += DEP_T) off[k] = 0;
-    if (tid == 0) {
-      zrange[0] = 16;
-      zrange[1] = -1;
-    }
     __syncthreads();
     // 1. bin the chunk's particles by half-cell; the returned count is the rank
     constexpr int PPT = DEP_CH / DEP_T;
@@ -405,11 +399,8 @@ __global__ __launch_bounds__(DEP_T) void p2g_deposit_kernel(GridDev g, const flo
       if (!inside) continue;
       hb[j] = h[0] + 16 * (h[1] + 16 * h[2]);
       rk[j] = atomicAdd(&off[hb[j]], 1u);
-      atomicMin(&zrange[0], h[2]);
-      atomicMax(&zrange[1], h[2]);
     }
     __syncthreads();
-    const int zlo = zrange[0], zhi = zrange[1];
     // 2. fine-cell particle counts (for the liquid mask), then an exclusive
     //    scan of the 4096 bin counts in place
     for (int c = tid; c < 512; c += DEP_T) {
@@ -452,26 +443,28 @@ __global__ __launch_bounds__(DEP_T) void p2g_deposit_kernel(GridDev g, const flo
       const uint32_t sl = off[hb[j]] + rk[j];
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        const float un = u[j][a], ut = u[j][a] - 0.5f;
-        sf[a][sl] = un - floorf(un);
-        sf[3 + a][sl] = ut - floorf(ut);
+        su[a][sl] = u[j][a];
         sv[a][sl] = v[j][a];
       }
     }
     __syncthreads();
-    // 4. node gather.  The half-cell (hx, hy, hz) of a particle fixes its
-    //    stencil base on every axis, so for a node the weight factor of each
-    //    axis is the same over whole half-cell ranges: t or 1 - t of the
-    //    particle's fractional coordinate (t = frac(u) on the face-normal
-    //    axis, frac(u - 1/2) on the others), or 1 for the outermost half-cell
-    //    of a node clamped at a tank wall.  A node's support is 4 half-cells
-    //    per axis: the lower two weigh t, the upper two 1 - t.
-    for (int k = tid; k < DEP_NODES; k += DEP_T) {
+    // 4. node gather.  A node's support is 4 half-cells per axis; on each
+    //    axis the lower two weigh the hat's rising side and the upper two its
+    //    falling side, both linear in the particle coordinate u: weight =
+    //    ca + cb * u.  Rising side: u - (n - 1), falling side: (n + 1) - u,
+    //    for the node at coordinate n (integer on the face-normal axis, half
+    //    integer on the others); 1 for the outermost half-cell of a node
+    //    clamped at a tank wall.  Each node is owned by one thread, which
+    //    stores its sums after the first chunk and adds those of later ones.
+#pragma unroll 1
+    for (int j = 0; j < DEP_NPT; ++j) {
+      const int k = tid + j * DEP_T;
+      if (k >= DEP_NODES) break;
       const int a = k / PAD, pi = k - a * PAD;
       const int n0 = a == 0 ? 9 : 10, n1 = a == 1 ? 9 : 10;
       int l[3] = {pi % n0, (pi / n0) % n1, pi / (n0 * n1)};
       int h0[3];            // first half-cell of the support (may be < 0)
-      float ca[3][2], cb[3][2];  // weight = ca + cb * t for the lower / upper pair
+      float ca[3][2], cb[3][2];
       bool dead = false;
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
@@ -480,8 +473,9 @@ __global__ __launch_bounds__(DEP_T) void p2g_deposit_kernel(GridDev g, const flo
         const bool normal = d == a;
         if (normal ? gn >= g.N[d] : (gn < 0 || gn >= g.N[d])) dead = true;
         h0[d] = normal ? 2 * l[d] - 2 : 2 * l[d] - 1;
-        ca[d][0] = 0.f;  cb[d][0] = 1.f;   // lower pair: t
-        ca[d][1] = 1.f;  cb[d][1] = -1.f;  // upper pair: 1 - t
+        const float pos = normal ? (float)l[d] : (float)l[d] + 0.5f;   // node coordinate in the block frame
+        ca[d][0] = 1.f - pos;  cb[d][0] = 1.f;    // rising side
+        ca[d][1] = 1.f + pos;  cb[d][1] = -1.f;   // falling side
         if (!normal && gn == 0) {          // clamped node below: outermost half-cell weighs 1
           ca[d][0] = 1.f;  cb[d][0] = 0.f;
         }
@@ -489,18 +483,15 @@ __global__ __launch_bounds__(DEP_T) void p2g_deposit_kernel(GridDev g, const flo
           ca[d][1] = 1.f;  cb[d][1] = 0.f;
         }
       }
-      if (dead) continue;
-      // the chunk's particles cover half-cell z in [zlo, zhi]
-      if (h0[2] + 3 < zlo || h0[2] > zhi) continue;
       float sM = 0.f, sm = 0.f;
-      const float* fx = sf[a == 0 ? 0 : 3];
-      const float* fy = sf[a == 1 ? 1 : 4];
-      const float* fz = sf[a == 2 ? 2 : 5];
+      const float* fx = su[0];
+      const float* fy = su[1];
+      const float* fz = su[2];
       const float* va = sv[a];
       // the four half-cells of a row are contiguous in the sorted chunk; the
       // lower pair weighs with (ca, cb)[0], the upper pair with (ca, cb)[1]
       const int hx_lo = max(h0[0], 0), hx_mid = min(max(h0[0] + 2, 0), 16), hx_hi = min(h0[0] + 4, 16);
-      for (int dz = 0; dz < 4; ++dz) {
+      for (int dz = 0; dz < (dead ? 0 : 4); ++dz) {
         const int hz = h0[2] + dz;
         if (hz < 0 || hz > 15) continue;
         const float za = ca[2][dz >> 1], zb = cb[2][dz >> 1];
@@ -519,13 +510,16 @@ __global__ __launch_bounds__(DEP_T) void p2g_deposit_kernel(GridDev g, const flo
           }
         }
       }
-      acc[k] += sM;
-      acc[DEP_NODES + k] += sm;
+      if (c0 == 0) {
+        dst[k] = sM;
+        dst[DEP_NODES + k] = sm;
+      } else {
+        dst[k] += sM;
+        dst[DEP_NODES + k] += sm;
+      }
     }
     __syncthreads();
   }
-  float* dst = slab + (size_t)B * SLAB;
-  for (int k = tid; k < 2 * DEP_NODES; k += DEP_T) dst[k] = acc[k];
   // native-cell liquid mask from the fine-cell counts
   for (int k = tid; k < nc; k += DEP_T) {
     const int x = k % r, y = (k / r) % r, z = k / (r * r);
@@ -1434,6 +1428,21 @@ __device__ __forceinline__ void stencil_split(const LevelDev& L, const CellRef& 
   }
 }
 
+// group g of a launch from its run table (scalar arithmetic only)
+__device__ __forceinline__ WG wg_at(const WGMap& m, int g) {
+  int k = 0;
+#pragma unroll
+  for (int j = 1; j < WGMAP_RUNS; ++j) k += g >= m.g0[j] ? 1 : 0;
+  const int lr = m.lr[k], per = 512 >> (3 * lr);
+  const int q0 = m.s0[k] + (g - m.g0[k]) * per;
+  WG w;
+  w.start = q0;
+  w.count = min(per, m.s0[k] + m.cnt[k] - q0);
+  w.lr = lr;
+  w.pad = 0;
+  return w;
+}
+
 // Work-list cell i of group w with its block descriptor (one dependent load
 // for every offset the stencil needs).
 __device__ __forceinline__ bool decode_bd(const LevelDev& L, const WG& w, int i, CellRef& o, const uint32_t*& bd) {
@@ -1649,17 +1658,21 @@ __device__ __forceinline__ uint32_t parent_cell(const LevelDev& F, const LevelDe
 // held to 64 VGPRs for eight waves per SIMD.
 #define AW_STENCIL_OCCUPANCY __attribute__((amdgpu_waves_per_eu(8, 8)))
 
+// PRO: 0 no coarse correction, 1 on the group's own cells, 2 also on the
+// neighbour values read (one instantiation per case keeps each kernel's code
+// small).
+template <int PRO>
 __global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void smooth_kernel(
-    LevelDev L, float* __restrict__ x, const float* __restrict__ b, const WG* __restrict__ wgt,
+    LevelDev L, float* __restrict__ x, const float* __restrict__ b, WGMap wgm,
     const uint32_t* __restrict__ blist, float omega, int local_sweeps, int reverse, Pro pro) {
   __shared__ float sx[512];
-  const WG w = wgt[blockIdx.x];
+  const WG w = wg_at(wgm, blockIdx.x);
   CellRef cr[2];
   bool act[2];
   float bb[2], dg[2], ex[2], cw[2][6];
   auto outb = [&](int N, const int* nc, int rn, uint32_t cc) {
     float v = x[cc];
-    if (pro.nbr && L.liquid[cc])
+    if (PRO == 2 && L.liquid[cc])
       v += pro.scale * pro.xc[parent_cell(L, pro.C, pro.aggregated, N, nc, rn, cell_idx(nc, rn))];
     return v;
   };
@@ -1672,12 +1685,12 @@ __global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void smooth_kernel(
     const uint32_t c = cr[k].c;
     float xv = x[c];
     const bool liq = L.liquid[c] != 0;
-    if (liq && pro.own) xv += pro.scale * pro.xc[parent_cell(L, pro.C, pro.aggregated, cr[k].B, cr[k].l, cr[k].r, cr[k].li)];
+    if (PRO >= 1 && liq) xv += pro.scale * pro.xc[parent_cell(L, pro.C, pro.aggregated, cr[k].B, cr[k].l, cr[k].r, cr[k].li)];
     sx[i] = xv;
     if (!liq) continue;
     act[k] = true;
     bb[k] = b[c];
-    if (pro.nbr)
+    if constexpr (PRO == 2)
       stencil_split_bd(L, cr[k], bd, outb, dg[k], ex[k], cw[k]);
     else
       stencil_fast(L, cr[k], bd, [&](uint32_t cc) { return x[cc]; }, dg[k], ex[k], cw[k]);
@@ -1710,10 +1723,10 @@ __global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void smooth_kernel(
 // (restrictResidualsKernel, MultigridTransfer.cuh:9-59) and x_c = 0.
 __global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void residual_kernel(
     LevelDev L, const float* __restrict__ x, const float* __restrict__ b, float* __restrict__ r,
-    const WG* __restrict__ wgt, const uint32_t* __restrict__ blist, Restr rs) {
+    WGMap wgm, const uint32_t* __restrict__ blist, Restr rs) {
   __shared__ float sx[512];
   __shared__ float sr[512];
-  const WG w = wgt[blockIdx.x];
+  const WG w = wg_at(wgm, blockIdx.x);
   CellRef cr[2];
   bool ok[2];
   const uint32_t* bd[2];
@@ -1769,10 +1782,10 @@ __global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void residual_kernel(
 
 // y = A x with sum.acc += x . y
 __global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void apply_dot_kernel(
-    LevelDev L, const float* __restrict__ x, float* __restrict__ y, const WG* __restrict__ wgt,
+    LevelDev L, const float* __restrict__ x, float* __restrict__ y, WGMap wgm,
     const uint32_t* __restrict__ blist, double* acc) {
   __shared__ float sx[512];
-  const WG w = wgt[blockIdx.x];
+  const WG w = wg_at(wgm, blockIdx.x);
   CellRef cr[2];
   bool ok[2];
   const uint32_t* bd[2];
@@ -1833,9 +1846,9 @@ __device__ __forceinline__ float children_sum(const LevelDev& F, const LevelDev&
 // children, x_c = 0
 __global__ __launch_bounds__(TPB) void restrict_kernel(LevelDev F, LevelDev C, int aggregated,
                                                        const float* __restrict__ rf, float* __restrict__ bc,
-                                                       float* __restrict__ xc, const WG* __restrict__ wgt,
+                                                       float* __restrict__ xc, WGMap wgm,
                                                        const uint32_t* __restrict__ blist) {
-  const WG w = wgt[blockIdx.x];
+  const WG w = wg_at(wgm, blockIdx.x);
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     CellRef c;
@@ -1847,9 +1860,9 @@ __global__ __launch_bounds__(TPB) void restrict_kernel(LevelDev F, LevelDev C, i
 
 // liquid_c = all children liquid (Dirichlet air dominates on coarse levels)
 __global__ __launch_bounds__(TPB) void coarsen_liquid_kernel(LevelDev F, LevelDev C, int aggregated,
-                                                             uint8_t* __restrict__ liq, const WG* __restrict__ wgt,
+                                                             uint8_t* __restrict__ liq, WGMap wgm,
                                                              const uint32_t* __restrict__ blist) {
-  const WG w = wgt[blockIdx.x];
+  const WG w = wg_at(wgm, blockIdx.x);
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     CellRef c;
@@ -1863,9 +1876,9 @@ __global__ __launch_bounds__(TPB) void coarsen_liquid_kernel(LevelDev F, LevelDe
 // MultigridTransfer.cuh:63-102)
 __global__ __launch_bounds__(TPB) void prolong_kernel(LevelDev F, LevelDev C, int aggregated, float* __restrict__ xf,
                                                       const float* __restrict__ xc, float scale,
-                                                      const WG* __restrict__ wgt,
+                                                      WGMap wgm,
                                                       const uint32_t* __restrict__ blist) {
-  const WG w = wgt[blockIdx.x];
+  const WG w = wg_at(wgm, blockIdx.x);
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     CellRef c;
@@ -2182,9 +2195,9 @@ __global__ __launch_bounds__(TPB) void coarse_factor_kernel(LevelDev L, int n, d
 // r is handed to the FP32 inner solve, ||r||^2 accumulates in FP64.
 __global__ __launch_bounds__(TPB) void residual64_kernel(LevelDev L, const double* __restrict__ x,
                                                          const float* __restrict__ b, float* __restrict__ r32,
-                                                         const WG* __restrict__ wgt,
+                                                         WGMap wgm,
                                                          const uint32_t* __restrict__ blist, double* rr) {
-  const WG w = wgt[blockIdx.x];
+  const WG w = wg_at(wgm, blockIdx.x);
   double part = 0.0;
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
@@ -2420,34 +2433,40 @@ void k_masked_copy(hipStream_t s, float* dst, const float* src, const uint8_t* l
   if (n > 0) masked_copy_kernel<<<(n + TPB - 1) / TPB, TPB, 0, s>>>(dst, src, liquid, n);
 }
 
-void k_smooth(hipStream_t s, const LevelDev& L, float* x, const float* b, const WG* wg, int nwg,
+void k_smooth(hipStream_t s, const LevelDev& L, float* x, const float* b, const WGMap& wg,
               const uint32_t* blist, float omega, int local_sweeps, int reverse, const Pro& pro) {
-  if (nwg > 0) smooth_kernel<<<nwg, TPB, 0, s>>>(L, x, b, wg, blist, omega, local_sweeps, reverse, pro);
+  if (wg.n <= 0) return;
+  if (pro.nbr)
+    smooth_kernel<2><<<wg.n, TPB, 0, s>>>(L, x, b, wg, blist, omega, local_sweeps, reverse, pro);
+  else if (pro.own)
+    smooth_kernel<1><<<wg.n, TPB, 0, s>>>(L, x, b, wg, blist, omega, local_sweeps, reverse, pro);
+  else
+    smooth_kernel<0><<<wg.n, TPB, 0, s>>>(L, x, b, wg, blist, omega, local_sweeps, reverse, pro);
 }
 
-void k_residual(hipStream_t s, const LevelDev& L, const float* x, const float* b, float* r, const WG* wg, int nwg,
+void k_residual(hipStream_t s, const LevelDev& L, const float* x, const float* b, float* r, const WGMap& wg,
                 const uint32_t* blist, const Restr& rs) {
-  if (nwg > 0) residual_kernel<<<nwg, TPB, 0, s>>>(L, x, b, r, wg, blist, rs);
+  if (wg.n > 0) residual_kernel<<<wg.n, TPB, 0, s>>>(L, x, b, r, wg, blist, rs);
 }
 
-void k_apply_dot(hipStream_t s, const LevelDev& L, const float* x, float* y, const WG* wg, int nwg,
+void k_apply_dot(hipStream_t s, const LevelDev& L, const float* x, float* y, const WGMap& wg,
                  const uint32_t* blist, double* acc) {
-  if (nwg > 0) apply_dot_kernel<<<nwg, TPB, 0, s>>>(L, x, y, wg, blist, acc);
+  if (wg.n > 0) apply_dot_kernel<<<wg.n, TPB, 0, s>>>(L, x, y, wg, blist, acc);
 }
 
 void k_restrict(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, const float* rf, float* bc,
-                float* xc, const WG* wgc, int nwgc, const uint32_t* blistc) {
-  if (nwgc > 0) restrict_kernel<<<nwgc, TPB, 0, s>>>(F, C, aggregated, rf, bc, xc, wgc, blistc);
+                float* xc, const WGMap& wgc, const uint32_t* blistc) {
+  if (wgc.n > 0) restrict_kernel<<<wgc.n, TPB, 0, s>>>(F, C, aggregated, rf, bc, xc, wgc, blistc);
 }
 
 void k_prolong(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, float* xf, const float* xc,
-               float scale, const WG* wgf, int nwgf, const uint32_t* blistf) {
-  if (nwgf > 0) prolong_kernel<<<nwgf, TPB, 0, s>>>(F, C, aggregated, xf, xc, scale, wgf, blistf);
+               float scale, const WGMap& wgf, const uint32_t* blistf) {
+  if (wgf.n > 0) prolong_kernel<<<wgf.n, TPB, 0, s>>>(F, C, aggregated, xf, xc, scale, wgf, blistf);
 }
 
 void k_coarsen_liquid(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, uint8_t* liquid_c,
-                      const WG* wgc, int nwgc, const uint32_t* blistc) {
-  if (nwgc > 0) coarsen_liquid_kernel<<<nwgc, TPB, 0, s>>>(F, C, aggregated, liquid_c, wgc, blistc);
+                      const WGMap& wgc, const uint32_t* blistc) {
+  if (wgc.n > 0) coarsen_liquid_kernel<<<wgc.n, TPB, 0, s>>>(F, C, aggregated, liquid_c, wgc, blistc);
 }
 
 void k_tail(hipStream_t s, const TailArgs& args, size_t lds_bytes) {
@@ -2458,9 +2477,9 @@ void k_coarse_factor(hipStream_t s, const LevelDev& L, int n, double* inv) {
   coarse_factor_kernel<<<1, TPB, 0, s>>>(L, n, inv);
 }
 
-void k_residual64(hipStream_t s, const LevelDev& L, const double* x, const float* b, float* r32, const WG* wg,
-                  int nwg, const uint32_t* blist, double* rr) {
-  if (nwg > 0) residual64_kernel<<<nwg, TPB, 0, s>>>(L, x, b, r32, wg, blist, rr);
+void k_residual64(hipStream_t s, const LevelDev& L, const double* x, const float* b, float* r32, const WGMap& wg,
+                  const uint32_t* blist, double* rr) {
+  if (wg.n > 0) residual64_kernel<<<wg.n, TPB, 0, s>>>(L, x, b, r32, wg, blist, rr);
 }
 
 void k_axpy64(hipStream_t s, double* x, const float* e, const uint8_t* liquid, int n) {

@@ -7,10 +7,12 @@
 
 #include <algorithm>
 #include <atomic>
+#include <climits>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -81,13 +83,16 @@ struct DevBuf {
       (void)hipFree(p);
       mem_sub(cap);
     }
+    // a buffer that has to grow gets 1/8 headroom, so that sizes that creep
+    // up over topology changes do not reallocate every time
+    const size_t want = cap ? bytes + bytes / 8 : bytes;
     p = nullptr;
     cap = 0;
-    if (hipMalloc(&p, bytes) != hipSuccess) {
+    if (hipMalloc(&p, want) != hipSuccess) {
       (void)hipGetLastError();
-      return fail(AW_ERR_OOM, "hipMalloc of %zu bytes failed", bytes);
+      return fail(AW_ERR_OOM, "hipMalloc of %zu bytes failed", want);
     }
-    cap = bytes;
+    cap = want;
     mem_add(cap);
     return AW_OK;
   }
@@ -106,16 +111,18 @@ int launch_status(const char* what) {
 void neighbour_table(const int* D, std::vector<int32_t>& nbr) {
   const int nb = D[0] * D[1] * D[2];
   nbr.assign((size_t)nb * 6, -1);
-  for (int b = 0; b < nb; ++b) {
-    int c[3] = {b % D[0], (b / D[0]) % D[1], b / (D[0] * D[1])};
-    for (int a = 0; a < 3; ++a)
-      for (int side = 0; side < 2; ++side) {
-        int q[3] = {c[0], c[1], c[2]};
-        q[a] += side ? 1 : -1;
-        if (q[a] < 0 || q[a] >= D[a]) continue;
-        nbr[(size_t)b * 6 + 2 * a + side] = q[0] + D[0] * (q[1] + D[1] * q[2]);
+  const int st[3] = {1, D[0], D[0] * D[1]};
+  int b = 0;
+  for (int z = 0; z < D[2]; ++z)
+    for (int y = 0; y < D[1]; ++y)
+      for (int x = 0; x < D[0]; ++x, ++b) {
+        const int c[3] = {x, y, z};
+        int32_t* n = &nbr[(size_t)b * 6];
+        for (int a = 0; a < 3; ++a) {
+          if (c[a] > 0) n[2 * a] = b - st[a];
+          if (c[a] + 1 < D[a]) n[2 * a + 1] = b + st[a];
+        }
       }
-  }
 }
 
 }  // namespace
@@ -224,10 +231,11 @@ struct MGLevel {
   // host sources of the uploaded tables, kept for the level's lifetime so no
   // asynchronous copy can outlive its source
   std::vector<uint32_t> h_blist, h_bd, h_cblk, h_foff;
-  std::vector<WG> h_wg;
   std::vector<int32_t> h_nbr;
-  DevBuf e, coff, foff, nbr, liquid, beta[3], x, b, r, blist, wg, cblk, bdesc;
-  int nwg = 0, nwg0 = 0;  // work-group table: colour 0 = [0, nwg0), colour 1 = [nwg0, nwg)
+  int nbr_dims[3] = {0, 0, 0};   // dims h_nbr was built for
+  bool valid = false;            // tables and device buffers built for h_e
+  DevBuf e, coff, foff, nbr, liquid, beta[3], x, b, r, blist, cblk, bdesc;
+  WGMap wgm[3]{};  // stencil launches over block colour 0, colour 1, both
   // algorithmic bytes of one stencil pass over colour 0, colour 1 (x read +
   // x write + b read = 12 B per cell, beta 3 x 4 B per face slot)
   double sweep_bytes[2] = {0.0, 0.0};
@@ -258,7 +266,6 @@ struct MGLevel {
     L.bd = bdesc.as<uint32_t>();
     return L;
   }
-  const WG* wgp() const { return wg.as<WG>(); }
   const uint32_t* bl() const { return blist.as<uint32_t>(); }
 };
 
@@ -280,6 +287,22 @@ struct aw_solver {
   double* h_sc = nullptr;       // host scalars (slots added), points at h_sc_sum
   double h_sc_sum[SC_N] = {};
   double* h_slots = nullptr;    // pinned host copy of the slotted device scalars
+  // The PCG iteration body (V-cycle preconditioner and vector updates, all
+  // between two scalar readbacks) replayed from a captured HIP graph:
+  // [0] continuing iteration, [1] first iteration after a (re)start.  The
+  // kernels' arguments are baked in, so a graph belongs to one hierarchy
+  // build (graph_epoch) and one solution buffer.
+  struct Captured {
+    hipGraphExec_t exec = nullptr;
+    int epoch = -1;
+    const void* x = nullptr;
+  };
+  Captured pcg_body[2];
+  int graph_epoch = 0;
+  hipEvent_t upload_event = nullptr;  // after the last hierarchy build's table uploads
+  bool upload_event_set = false;
+  bool use_graph = false;  // aw_solver_set_graph
+  bool capture_ok = true;  // false once capture failed (e.g. on the legacy null stream): run eagerly
 };
 
 namespace {
@@ -432,32 +455,64 @@ int project(aw_grid* g) {
 }
 
 // ---------------------------------------------------------------- multigrid
-void build_wg(MGLevel& L, std::vector<uint32_t>& blist, std::vector<WG>& wg) {
+// Work list: per block colour, the blocks of each resolution (8^3 cells
+// first), 512 cells per work-group; the launch maps of colour 0, colour 1
+// and both describe its runs.
+void build_wg(MGLevel& L, std::vector<uint32_t>& blist) {
   blist.clear();
-  wg.clear();
+  for (WGMap& m : L.wgm) {
+    m.n = 0;
+    for (int k = 0; k < WGMAP_RUNS; ++k) {
+      m.g0[k] = INT_MAX;
+      m.s0[k] = m.cnt[k] = m.lr[k] = 0;
+    }
+  }
+  // one pass: bucket (colour, resolution) of every block, then a stable
+  // counting sort into the work list
+  int lbs = 0;
+  while ((1 << lbs) < L.bs) ++lbs;
+  std::vector<uint8_t> key((size_t)L.nb);
+  int count[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  {
+    int b = 0;
+    for (int z = 0; z < L.D[2]; ++z)
+      for (int y = 0; y < L.D[1]; ++y)
+        for (int x = 0; x < L.D[0]; ++x, ++b) {
+          const int lr = lbs - (int)L.h_e[b];
+          const int k = ((x + y + z) & 1) * 4 + (3 - lr);
+          key[b] = (uint8_t)k;
+          ++count[k];
+        }
+  }
+  int first[8];
+  for (int k = 0, acc = 0; k < 8; ++k) {
+    first[k] = acc;
+    acc += count[k];
+  }
+  blist.assign((size_t)L.nb, 0u);
+  {
+    int pos[8];
+    std::copy(first, first + 8, pos);
+    for (int b = 0; b < L.nb; ++b) blist[pos[key[b]]++] = (uint32_t)b;
+  }
+  int nrun[3] = {0, 0, 0};
   for (int col = 0; col < 2; ++col) {
     for (int lr = 3; lr >= 0; --lr) {
       const int per = 512 >> (3 * lr);
-      int start = (int)blist.size(), cnt = 0;
-      for (int b = 0; b < L.nb; ++b) {
-        int c[3] = {b % L.D[0], (b / L.D[0]) % L.D[1], b / (L.D[0] * L.D[1])};
-        if (((c[0] + c[1] + c[2]) & 1) != col) continue;
-        const int r = L.bs >> L.h_e[b];
-        int l2 = 0;
-        while ((1 << l2) < r) ++l2;
-        if (l2 != lr) continue;
-        blist.push_back((uint32_t)b);
-        if (++cnt == per) {
-          wg.push_back(WG{start, cnt, lr, 0});
-          start = (int)blist.size();
-          cnt = 0;
-        }
+      const int start = first[col * 4 + 3 - lr];
+      const int cnt = count[col * 4 + 3 - lr];
+      if (cnt == 0) continue;
+      for (int mi : {col, 2}) {
+        WGMap& m = L.wgm[mi];
+        const int k = nrun[mi]++;
+        m.g0[k] = m.n;
+        m.s0[k] = start;
+        m.cnt[k] = cnt;
+        m.lr[k] = lr;
+        m.n += (cnt + per - 1) / per;
       }
-      if (cnt) wg.push_back(WG{start, cnt, lr, 0});
     }
-    if (col == 0) L.nwg0 = (int)wg.size();
   }
-  L.nwg = (int)wg.size();
 }
 
 int upload_level(hipStream_t s, MGLevel& L, bool own_topology) {
@@ -473,18 +528,25 @@ int upload_level(hipStream_t s, MGLevel& L, bool own_topology) {
   L.ncells = L.h_coff[L.nb];
   L.nfaces = foff[L.nb];
   L.sweep_bytes[0] = L.sweep_bytes[1] = 0.0;
-  for (int b = 0; b < L.nb; ++b) {
-    const int r = L.bs >> L.h_e[b];
-    const int c[3] = {b % L.D[0], (b / L.D[0]) % L.D[1], b / (L.D[0] * L.D[1])};
-    L.sweep_bytes[(c[0] + c[1] + c[2]) & 1] += 12.0 * r * r * r + 12.0 * (r + 1) * r * r;
+  {
+    int b = 0;
+    for (int z = 0; z < L.D[2]; ++z)
+      for (int y = 0; y < L.D[1]; ++y)
+        for (int x = 0; x < L.D[0]; ++x, ++b) {
+          const int r = L.bs >> L.h_e[b];
+          L.sweep_bytes[(x + y + z) & 1] += 12.0 * r * r * r + 12.0 * (r + 1) * r * r;
+        }
   }
   std::vector<uint32_t>& blist = L.h_blist;
-  std::vector<WG>& wg = L.h_wg;
-  build_wg(L, blist, wg);
+  build_wg(L, blist);
   {
     // one descriptor per work-list entry: the block's and its face neighbours' offsets and levels
     std::vector<int32_t>& nb6 = L.h_nbr;
-    neighbour_table(L.D, nb6);
+    if (nb6.size() != (size_t)L.nb * 6 || L.nbr_dims[0] != L.D[0] || L.nbr_dims[1] != L.D[1] ||
+        L.nbr_dims[2] != L.D[2]) {
+      neighbour_table(L.D, nb6);
+      for (int a = 0; a < 3; ++a) L.nbr_dims[a] = L.D[a];
+    }
     std::vector<uint32_t>& bd = L.h_bd;
     bd.assign(blist.size() * BD_WORDS, 0u);
     for (size_t k = 0; k < blist.size(); ++k) {
@@ -510,9 +572,7 @@ int upload_level(hipStream_t s, MGLevel& L, bool own_topology) {
     AW_HIP(hipMemcpyAsync(L.bdesc.p, bd.data(), bd.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
   }
   AW_TRY(L.blist.ensure(blist.size() * sizeof(uint32_t)));
-  AW_TRY(L.wg.ensure(wg.size() * sizeof(WG)));
   AW_HIP(hipMemcpyAsync(L.blist.p, blist.data(), blist.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-  AW_HIP(hipMemcpyAsync(L.wg.p, wg.data(), wg.size() * sizeof(WG), hipMemcpyHostToDevice, s));
   AW_TRY(L.r.ensure((size_t)L.ncells * sizeof(float)));
   L.d_r = L.r.as<float>();
   if ((size_t)L.ncells * 13 <= TAIL_LDS_BYTES) {
@@ -558,8 +618,36 @@ int build_hierarchy(aw_solver* S) {
   aw_grid* g = S->g;
   hipStream_t s = g->ctx->stream;
   std::vector<std::unique_ptr<MGLevel>> lv;
+  // The previous build's levels are reused in order: their device buffers
+  // (grow-only) are not freed and allocated again on every topology change,
+  // and a level whose block resolutions did not change keeps its tables as
+  // they are.  Host tables are rewritten only after the previous build's
+  // copies from them have completed.
+  std::vector<std::unique_ptr<MGLevel>> pool;
+  pool.swap(S->lv);
+  if (S->upload_event_set) AW_HIP(hipEventSynchronize(S->upload_event));
+  size_t reuse = 0;
+  auto level = [&]() {
+    return reuse < pool.size() ? std::move(pool[reuse++]) : std::make_unique<MGLevel>();
+  };
+  // (re)builds L for dims D, block size bs and resolutions he unless it holds them already
+  auto prepare = [&](MGLevel& L, const int* D, int bs, int aggregated, std::vector<uint8_t>& he) -> int {
+    if (L.valid && L.bs == bs && L.aggregated == aggregated && L.D[0] == D[0] && L.D[1] == D[1] &&
+        L.D[2] == D[2] && L.h_e == he)
+      return AW_OK;
+    L.valid = false;
+    for (int a = 0; a < 3; ++a) L.D[a] = D[a];
+    L.bs = bs;
+    L.aggregated = aggregated;
+    L.h_e.swap(he);
+    AW_TRY(upload_level(s, L, true));
+    L.valid = true;
+    return AW_OK;
+  };
   {
-    auto L0 = std::make_unique<MGLevel>();
+    auto L0 = level();
+    L0->valid = false;
+    L0->aggregated = 0;
     for (int a = 0; a < 3; ++a) L0->D[a] = g->D[a];
     L0->bs = BASE;
     L0->h_e = g->h_level;
@@ -579,16 +667,14 @@ int build_hierarchy(aw_solver* S) {
   bool all1 = true;
   for (uint8_t l : g->h_level) all1 = all1 && l == LOGB;
   for (int k = 1; !all1; ++k) {
-    auto L = std::make_unique<MGLevel>();
-    for (int a = 0; a < 3; ++a) L->D[a] = g->D[a];
-    L->bs = BASE;
-    L->h_e.resize(g->nb);
+    auto L = level();
+    std::vector<uint8_t> he((size_t)g->nb);
     all1 = true;
     for (int b = 0; b < g->nb; ++b) {
-      L->h_e[b] = (uint8_t)std::min(std::max((int)g->h_level[b], lmin + k), LOGB);
-      all1 = all1 && L->h_e[b] == LOGB;
+      he[b] = (uint8_t)std::min(std::max((int)g->h_level[b], lmin + k), LOGB);
+      all1 = all1 && he[b] == LOGB;
     }
-    AW_TRY(upload_level(s, *L, true));
+    AW_TRY(prepare(*L, g->D, BASE, 0, he));
     lv.push_back(std::move(L));
   }
   for (;;) {
@@ -596,16 +682,18 @@ int build_hierarchy(aw_solver* S) {
     const int mn = std::min(P.D[0], std::min(P.D[1], P.D[2]));
     if (mn < 4 || (P.D[0] & 1) || (P.D[1] & 1) || (P.D[2] & 1)) break;
     if ((int)lv.size() >= AW_MAX_MG_LEVELS) break;
-    auto L = std::make_unique<MGLevel>();
-    for (int a = 0; a < 3; ++a) L->D[a] = P.D[a] / 2;
-    L->bs = P.bs * 2;
-    L->aggregated = 1;
-    L->h_e.assign((size_t)L->D[0] * L->D[1] * L->D[2], (uint8_t)(P.h_e[0] + 1));
-    AW_TRY(upload_level(s, *L, true));
+    auto L = level();
+    const int D[3] = {P.D[0] / 2, P.D[1] / 2, P.D[2] / 2};
+    std::vector<uint8_t> he((size_t)D[0] * D[1] * D[2], (uint8_t)(P.h_e[0] + 1));
+    AW_TRY(prepare(*L, D, P.bs * 2, 1, he));
     lv.push_back(std::move(L));
   }
   S->lv.swap(lv);
   S->built_version = g->topo_version;
+  S->graph_epoch++;
+  if (!S->upload_event) AW_HIP(hipEventCreateWithFlags(&S->upload_event, hipEventDisableTiming));
+  AW_HIP(hipEventRecord(S->upload_event, s));
+  S->upload_event_set = true;
   // tail: the longest suffix of levels whose x, b, r fit the LDS budget
   {
     size_t bytes = 0;
@@ -641,7 +729,7 @@ void coarsen_masks(aw_solver* S) {
   for (size_t k = 0; k + 1 < S->lv.size(); ++k) {
     const MGLevel& F = *S->lv[k];
     MGLevel& C = *S->lv[k + 1];
-    k_coarsen_liquid(s, F.view(), C.view(), C.aggregated, C.d_liq, C.wgp(), C.nwg, C.bl());
+    k_coarsen_liquid(s, F.view(), C.view(), C.aggregated, C.d_liq, C.wgm[2], C.bl());
   }
 }
 
@@ -670,7 +758,6 @@ void smooth_level(aw_solver* S, int k, float* x, const float* b, int iters, bool
   for (int it = 0; it < iters; ++it)
     for (int ph = 0; ph < 2; ++ph) {
       const int col = reverse ? 1 - ph : ph;
-      const int off = col ? L.nwg0 : 0, cnt = col ? L.nwg - L.nwg0 : L.nwg0;
       Pro pro = no_pro();
       if (xc && it == 0) {
         // the first post-smoothing pass carries the coarse correction: the
@@ -685,7 +772,7 @@ void smooth_level(aw_solver* S, int k, float* x, const float* b, int iters, bool
         pro.nbr = ph == 0 ? 1 : 0;
       }
       Prof pf(S->g->ctx, k == 0 ? "smooth_l0" : "smooth_coarse", L.sweep_bytes[col]);
-      k_smooth(s, V, x, b, L.wgp() + off, cnt, L.bl(), S->prm.omega, S->prm.local_sweeps, reverse ? 1 : 0, pro);
+      k_smooth(s, V, x, b, L.wgm[col], L.bl(), S->prm.omega, S->prm.local_sweeps, reverse ? 1 : 0, pro);
     }
 }
 
@@ -757,21 +844,21 @@ void residual_level(aw_solver* S, int k, const float* x, const float* b, float* 
           k == 0 ? (fuse_restrict ? "residual_restrict_l0" : "residual_l0")
                  : (fuse_restrict ? "residual_restrict_coarse" : "residual_coarse"),
           bytes);
-  k_residual(S->g->ctx->stream, L.view(), x, b, fuse_restrict ? nullptr : r, L.wgp(), L.nwg, L.bl(), rs);
+  k_residual(S->g->ctx->stream, L.view(), x, b, fuse_restrict ? nullptr : r, L.wgm[2], L.bl(), rs);
 }
 
 void restrict_level(aw_solver* S, int k, const float* r, float* bc, float* xc = nullptr) {
   const MGLevel& F = *S->lv[k];
   const MGLevel& C = *S->lv[k + 1];
   Prof pf(S->g->ctx, "restrict", 4.0 * F.ncells + 4.0 * C.ncells);
-  k_restrict(S->g->ctx->stream, F.view(), C.view(), C.aggregated, r, bc, xc, C.wgp(), C.nwg, C.bl());
+  k_restrict(S->g->ctx->stream, F.view(), C.view(), C.aggregated, r, bc, xc, C.wgm[2], C.bl());
 }
 
 void prolong_level(aw_solver* S, int k, float* x, const float* xc) {
   const MGLevel& F = *S->lv[k];
   const MGLevel& C = *S->lv[k + 1];
   Prof pf(S->g->ctx, "prolong", 8.0 * F.ncells + 4.0 * C.ncells);
-  k_prolong(S->g->ctx->stream, F.view(), C.view(), C.aggregated, x, xc, S->prm.prolong_scale, F.wgp(), F.nwg,
+  k_prolong(S->g->ctx->stream, F.view(), C.view(), C.aggregated, x, xc, S->prm.prolong_scale, F.wgm[2],
             F.bl());
 }
 
@@ -816,6 +903,81 @@ int read_scalars(aw_solver* S) {
     S->h_sc[i] = t;
   }
   return launch_status("solver");
+}
+
+// One PCG iteration between scalar readbacks: z = V-cycle(r), beta from
+// (r.z)/(r.z)_old, p = z + beta p, q = A p with p.q, x += alpha p, r -= alpha q.
+void pcg_body(aw_solver* S, float* x, int first) {
+  aw_grid* g = S->g;
+  hipStream_t s = g->ctx->stream;
+  MGLevel& L0 = *S->lv[0];
+  const int n = (int)L0.ncells;
+  const uint8_t* liq = g->liquid.as<uint8_t>();
+  float* r = S->r0.as<float>();
+  float* z = S->z.as<float>();
+  float* pv = S->pv.as<float>();
+  float* q = S->q.as<float>();
+  double* sc = S->sc.as<double>();
+  k_scalar_shift(s, sc);
+  k_fill_f32(s, z, n, 0.f);
+  vcycle(S, 0, z, r);
+  {
+    Prof pf(g->ctx, "pcg_dot_p", 21.0 * n);
+    k_dot(s, r, z, liq, n, scp(S, SC_RZ));
+    k_pcg_p(s, z, pv, n, sc, first);
+  }
+  {
+    Prof pf(g->ctx, "apply_l0", L0.sweep_bytes[0] + L0.sweep_bytes[1] - 4.0 * n);
+    k_apply_dot(s, L0.view(), pv, q, L0.wgm[2], L0.bl(), scp(S, SC_PQ));
+  }
+  {
+    Prof pf(g->ctx, "pcg_update", 28.0 * n);
+    k_pcg_xr(s, x, r, pv, q, liq, n, sc);
+  }
+}
+
+// pcg_body through its captured graph (captured on first use after a
+// hierarchy build); eager when the per-launch profiler is on or capture is
+// unavailable on the context's stream.
+int run_pcg_body(aw_solver* S, float* x, int first) {
+  hipStream_t s = S->g->ctx->stream;
+  if (S->g->ctx->prof || !S->use_graph || !S->capture_ok) {
+    pcg_body(S, x, first);
+    return launch_status("pcg iteration");
+  }
+  aw_solver::Captured& c = S->pcg_body[first ? 1 : 0];
+  if (!c.exec || c.epoch != S->graph_epoch || c.x != x) {
+    if (c.exec) {
+      (void)hipGraphExecDestroy(c.exec);
+      c.exec = nullptr;
+    }
+    AW_TRY(launch_status("before capture"));
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      (void)hipGetLastError();
+      S->capture_ok = false;
+      pcg_body(S, x, first);
+      return launch_status("pcg iteration");
+    }
+    pcg_body(S, x, first);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s, &graph);
+    if (ec != hipSuccess || !graph) {
+      (void)hipGetLastError();
+      if (graph) (void)hipGraphDestroy(graph);
+      return fail(AW_ERR_DEVICE, "capture of the PCG iteration failed: %s", hipGetErrorString(ec));
+    }
+    const hipError_t ei = hipGraphInstantiate(&c.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+      (void)hipGetLastError();
+      c.exec = nullptr;
+      return fail(AW_ERR_DEVICE, "instantiation of the PCG iteration graph failed: %s", hipGetErrorString(ei));
+    }
+    c.epoch = S->graph_epoch;
+    c.x = x;
+  }
+  AW_HIP(hipGraphLaunch(c.exec, s));
+  return AW_OK;
 }
 
 // FP32 conjugate gradients (or stationary V-cycles) on level 0 for A x = b.
@@ -878,23 +1040,8 @@ int solve_fp32(aw_solver* S, float* x, const float* b, double tol, int max_iters
     bool recurrence_ok = false;
     int first = 1;
     while (it < max_iters) {
-      k_scalar_shift(s, sc);
-      k_fill_f32(s, z, n, 0.f);
-      vcycle(S, 0, z, r);
-      {
-        Prof pf(g->ctx, "pcg_dot_p", 21.0 * n);
-        k_dot(s, r, z, liq, n, scp(S, SC_RZ));
-        k_pcg_p(s, z, pv, n, sc, first);
-      }
+      AW_TRY(run_pcg_body(S, x, first));
       first = 0;
-      {
-        Prof pf(g->ctx, "apply_l0", L0.sweep_bytes[0] + L0.sweep_bytes[1] - 4.0 * n);
-        k_apply_dot(s, V0, pv, q, L0.wgp(), L0.nwg, L0.bl(), scp(S, SC_PQ));
-      }
-      {
-        Prof pf(g->ctx, "pcg_update", 28.0 * n);
-        k_pcg_xr(s, x, r, pv, q, liq, n, sc);
-      }
       AW_TRY(read_scalars(S));
       rn = std::sqrt(S->h_sc[SC_RR]);
       ++it;
@@ -968,7 +1115,7 @@ int solve_mixed(aw_solver* S, double tol, int max_iters, aw_solve_stats* st) {
     k_scalar_shift(s, sc);
     {
       Prof pf(g->ctx, "residual64_l0", 8.0 * n + 4.0 * n + 4.0 * n + 12.0 * L0.nfaces);
-      k_residual64(s, L0.view(), x64, g->rhs.as<float>(), r, L0.wgp(), L0.nwg, L0.bl(), scp(S, SC_RR));
+      k_residual64(s, L0.view(), x64, g->rhs.as<float>(), r, L0.wgm[2], L0.bl(), scp(S, SC_RR));
     }
     AW_TRY(read_scalars(S));
     rn = std::sqrt(S->h_sc[SC_RR]);
@@ -1243,6 +1390,9 @@ void grid_release(aw_grid* g) {
 void solver_free(aw_solver* S) {
   aw_grid* g = S->g;
   (void)hipStreamSynchronize(g->ctx->stream);
+  for (auto& c : S->pcg_body)
+    if (c.exec) (void)hipGraphExecDestroy(c.exec);
+  if (S->upload_event) (void)hipEventDestroy(S->upload_event);
   if (S->h_slots) (void)hipHostFree(S->h_slots);
   delete S;
   grid_release(g);
@@ -1586,6 +1736,12 @@ int aw_solver_destroy(aw_solver* S) {
   return AW_OK;
 }
 
+int aw_solver_set_graph(aw_solver* S, int on) {
+  if (!S) return fail(AW_ERR_INVALID, "solver is NULL");
+  S->use_graph = on != 0;
+  return AW_OK;
+}
+
 int aw_solver_setup(aw_solver* S) {
   if (!S) return fail(AW_ERR_INVALID, "solver is NULL");
   return ensure_setup(S);
@@ -1640,7 +1796,7 @@ int aw_solver_apply(aw_solver* S, int k, const float* d_x, float* d_y) {
   AW_TRY(check_level(S, k, false));
   if (!d_x || !d_y) return fail(AW_ERR_INVALID, "NULL vector");
   MGLevel& L = *S->lv[k];
-  k_apply_dot(S->g->ctx->stream, L.view(), d_x, d_y, L.wgp(), L.nwg, L.bl(), nullptr);
+  k_apply_dot(S->g->ctx->stream, L.view(), d_x, d_y, L.wgm[2], L.bl(), nullptr);
   return launch_status("apply");
 }
 

@@ -200,6 +200,13 @@ int aw_solver_vcycle(aw_solver* s, int level);    /* one V-cycle rooted at level
 /* Solves A p = rhs on level 0 into the grid's pressure (initial guess: its
  * current content).  Synchronises. */
 int aw_solver_solve(aw_solver* s, double rel_tol, int max_iters, aw_solve_stats* stats);
+/* Replays each PCG iteration (V-cycle preconditioner and vector updates
+ * between two scalar readbacks) from a captured HIP graph when on != 0; the
+ * default (0) launches its kernels one by one, which measures faster on
+ * ROCm 7 for this launch sequence.  The graph is captured on first use after
+ * each hierarchy build.  No reference counterpart (the reference launches
+ * every kernel from the host, VCycle.cuh:251-304). */
+int aw_solver_set_graph(aw_solver* s, int on);
 
 /* One FLIP step on caller-owned AoS float3 particles (positions, velocities,
  * optional int32 ids permuted with them).  Synchronises. */

@@ -1,0 +1,60 @@
+"""The PCG iteration body replayed from its captured HIP graph
+(aw_solver_set_graph) against the same body launched kernel by kernel.
+
+Several solves alternate between the two paths on one hierarchy, with eager
+work and host synchronisations between the graph replays, for the FP32 PCG
+and the mixed-precision solver (whose inner solves replay the graph with a
+different solution buffer).  The dot products sum into slotted FP64
+accumulators in arrival order, so the two paths agree to rounding, not
+bit for bit: bar 1e-5 relative on p, iteration counts within one.
+"""
+import numpy as np
+import pytest
+
+import anitowave_adaptive_amd as aw
+from helpers import DT, GRAV, GpuParticles, gpu_grid, particles, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _rhs_ready(ctx, name):
+    pos, vel = particles(name)
+    gg = gpu_grid(ctx, name)
+    gp = GpuParticles(ctx, pos, vel)
+    gg.build_topology(gp.pos.ptr, gp.n)
+    aw.P2GTransfer.bin(gg, gp.pos.ptr, gp.vel.ptr, gp.ids.ptr, gp.n)
+    aw.P2GTransfer.run(gg, gp.pos.ptr, gp.vel.ptr, gp.n, DT, GRAV)
+    aw.G2PManager.divergence(gg)
+    return gg, gp
+
+
+@pytest.mark.parametrize("name,precision,tol", [("pool64", 0, 1e-6), ("double64", 0, 1e-6), ("slosh32", 1, 1e-10)])
+def test_graph_replays_match_eager(ctx, name, precision, tol):
+    print(f"[graph] {name} precision {precision}", flush=True)
+    gg, gp = _rhs_ready(ctx, name)
+    sv = aw.VCycleSolver(gg, precision=precision)
+    sv.setup()
+    n = len(gg.cell("rhs"))
+    runs = []
+    for rep in range(3):
+        for eager in (True, False):
+            sv.use_graph(not eager)
+            gg.set_cell("p", np.zeros(n, np.float32))
+            st = sv.solve(tol=tol, max_iters=400)
+            ctx.synchronize()
+            runs.append((eager, st.iterations, st.rel_residual, gg.cell("p").astype(np.float64)))
+            # eager work between replays: a residual pass on the finest level
+            sv.residual(0)
+            ctx.synchronize()
+    sv.use_graph(False)
+    # FP32 PCG reports the recomputed residual, which the FP32 pressure itself
+    # floors near 1e-5; the mixed solver reaches its tolerance
+    bar = tol if precision == 1 else 2e-5
+    ref = runs[0]
+    assert ref[2] <= bar
+    for eager, it, rel, p in runs[1:]:
+        assert abs(it - ref[1]) <= 1, (eager, it, ref[1])
+        assert rel <= bar
+        e = rel_err(p, ref[3])
+        assert e < 1e-5, (eager, e)
+    print(f"[graph] {name}: {len(runs)} solves, {ref[1]} iterations each", flush=True)
