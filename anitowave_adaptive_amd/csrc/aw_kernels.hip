@@ -742,6 +742,35 @@ __global__ __launch_bounds__(TPB) void project_kernel(GridDev g, const float* __
   }
 }
 
+// The forward fine lattice is padded per block: for component a, the
+// block's 8^3 nodes and a one-node halo, so that the 2x2x2 nodes of any
+// trilinear sample at a point inside the block lie in that block's array
+// (face-normal axis nodes 0..8, transverse axes -1..8, index pad_idx, PAD
+// floats per component, 3 PAD per field; field 0 = u_new, 1 = u_new - u_old).
+// Halo nodes hold their owner block's values, or the sampling rule's value
+// past the tank: 0 across a face-normal wall, the nearest node inside
+// transversally.  Halos are filled for the blocks within one block of a
+// particle (where RK3 stage points fall); a sample elsewhere reads the
+// owners' nodes one by one.
+
+// halo[B] = a block of the 3x3x3 neighbourhood of B holds particles
+__global__ __launch_bounds__(TPB) void halo_flags_kernel(GridDev g, const uint32_t* __restrict__ count,
+                                                         uint8_t* __restrict__ halo) {
+  const int B = blockIdx.x * TPB + threadIdx.x;
+  if (B >= g.nb) return;
+  int Bc[3];
+  block_coords(g.D, B, Bc);
+  bool any = false;
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int q[3] = {Bc[0] + dx, Bc[1] + dy, Bc[2] + dz};
+        if (q[0] < 0 || q[1] < 0 || q[2] < 0 || q[0] >= g.D[0] || q[1] >= g.D[1] || q[2] >= g.D[2]) continue;
+        any = any || count[q[0] + g.D[0] * (q[1] + g.D[1] * q[2])] > 0;
+      }
+  halo[B] = any ? 1 : 0;
+}
+
 // Pi of u_new and of (u_new - u_old) on the block's own fine nodes
 __global__ __launch_bounds__(TPB) void to_fine_kernel(GridDev g, FacePtrs unew, FacePtrs uold, float* __restrict__ fine) {
   const int B = blockIdx.x;
@@ -760,64 +789,173 @@ __global__ __launch_bounds__(TPB) void to_fine_kernel(GridDev g, FacePtrs unew, 
       vn += w[j] * un;
       vd += w[j] * (un - uold.p[a][id]);
     }
-    dst[k] = vn;
-    dst[3 * FINE + k] = vd;
+    const int pi = a * PAD + pad_idx(a, l);
+    dst[pi] = vn;
+    dst[3 * PAD + pi] = vd;
   }
 }
 
-__device__ __forceinline__ float fine_at(const GridDev& g, const float* __restrict__ F, int field, int a, int* q) {
-  if (q[a] < 0 || q[a] >= g.N[a]) return 0.f;
+// halo nodes of the flagged blocks' padded lattices, copied from their owners
+__global__ __launch_bounds__(TPB) void fine_halo_kernel(GridDev g, float* __restrict__ fine) {
+  const int B = blockIdx.x;
+  if (!g.halo[B]) return;
+  int Bc[3];
+  block_coords(g.D, B, Bc);
+  float* dst = fine + (size_t)B * SLAB;
+  for (int k = threadIdx.x; k < 3 * PAD; k += blockDim.x) {
+    const int a = k / PAD, pi = k - a * PAD;
+    const int n0 = a == 0 ? 9 : 10, n1 = a == 1 ? 9 : 10;
+    const int lp[3] = {pi % n0, (pi / n0) % n1, pi / (n0 * n1)};
+    int q[3];
+    bool own = true, outside = false;
 #pragma unroll
-  for (int d = 0; d < 3; ++d)
-    if (d != a) q[d] = iclamp(q[d], 0, g.N[d] - 1);
-  const int B = (q[0] >> LOGB) + g.D[0] * ((q[1] >> LOGB) + g.D[1] * (q[2] >> LOGB));
-  const int l = (q[0] & 7) + 8 * ((q[1] & 7) + 8 * (q[2] & 7));
-  return F[(size_t)B * SLAB + field * 3 * FINE + a * FINE + l];
+    for (int d = 0; d < 3; ++d) {
+      const int l = lp[d] - (d == a ? 0 : 1);   // block-local node index
+      own = own && l >= 0 && l < BASE;
+      q[d] = BASE * Bc[d] + l;
+      if (d == a)
+        outside = outside || q[d] >= g.N[d];
+      else
+        q[d] = iclamp(q[d], 0, g.N[d] - 1);
+    }
+    if (own) continue;
+    float vn = 0.f, vd = 0.f;
+    if (!outside) {
+      const int Bo = (q[0] >> LOGB) + g.D[0] * ((q[1] >> LOGB) + g.D[1] * (q[2] >> LOGB));
+      const int l[3] = {q[0] & 7, q[1] & 7, q[2] & 7};
+      const float* src = fine + (size_t)Bo * SLAB + a * PAD + pad_idx(a, l);
+      vn = src[0];
+      vd = src[3 * PAD];
+    }
+    dst[k] = vn;
+    dst[3 * PAD + k] = vd;
+  }
+}
+
+// Trilinear sample of component A of a field of the padded fine lattice at u
+// (fine units): the 2x2x2 nodes lie in one block's padded array, at fixed
+// offsets from the first.  Node rule: a face-normal index outside [0, N) is
+// 0 (a tank wall or beyond), a transverse index clamps to [0, N - 1].  The
+// coordinate is clamped first, which reproduces that rule up to rounding (a
+// point past a wall normal to A is only sampled for u_new, 0 on the wall).
+// The block of a binned particle and whether it holds binned particles: then
+// every block of its 3x3x3 neighbourhood has its halo filled, known without
+// reading the flags.
+struct Near {
+  int bc[3];
+  bool own;
+};
+
+__device__ __forceinline__ Near near_of(const GridDev& g, const float* u) {
+  Near nr;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) nr.bc[d] = (int)u[d] >> LOGB;
+  nr.own = g.count[nr.bc[0] + g.D[0] * (nr.bc[1] + g.D[1] * nr.bc[2])] > 0;
+  return nr;
+}
+
+// SLOW: a sample whose block has no halo reads each node from its owner
+// block; otherwise such a sample clears ok (and its value is not used).
+template <int A, bool SLOW>
+__device__ __forceinline__ float sample_comp(const GridDev& g, const float* __restrict__ F, int field,
+                                             const float* u, const Near& nr, bool& ok) {
+  constexpr int n0 = A == 0 ? 9 : 10, n1 = A == 1 ? 9 : 10;
+  int bc[3], lp[3];
+  float t[3];
+  bool zero = false;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    int i;
+    if (d == A) {
+      zero = zero || u[d] < 0.f || u[d] > (float)g.N[d];
+      const float c = fminf(fmaxf(u[d], 0.f), (float)g.N[d]);
+      i = min((int)floorf(c), g.N[d] - 1);
+      t[d] = c - (float)i;
+    } else {
+      const float c = fminf(fmaxf(u[d] - 0.5f, 0.f), (float)(g.N[d] - 1));
+      i = (int)floorf(c);
+      t[d] = c - (float)i;
+    }
+    bc[d] = i >> LOGB;
+    lp[d] = (i & 7) + (d == A ? 0 : 1);
+  }
+  const int B = bc[0] + g.D[0] * (bc[1] + g.D[1] * bc[2]);
+  bool fast = nr.own;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) fast = fast && abs(bc[d] - nr.bc[d]) <= 1;
+  if (!fast) fast = g.halo[B] != 0;
+  float s = 0.f;
+  if (!SLOW && !fast) ok = false;
+  if (!SLOW || fast) {
+    const float* f = F + (size_t)B * SLAB + field * 3 * PAD + A * PAD + lp[0] + n0 * (lp[1] + n1 * lp[2]);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int dd[3] = {k & 1, (k >> 1) & 1, k >> 2};
+      const float w = (dd[0] ? t[0] : 1.f - t[0]) * (dd[1] ? t[1] : 1.f - t[1]) * (dd[2] ? t[2] : 1.f - t[2]);
+      s += w * f[dd[0] + n0 * (dd[1] + n1 * dd[2])];
+    }
+  } else if (SLOW) {
+    // no halo: each node from its owner block's own nodes
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int dd[3] = {k & 1, (k >> 1) & 1, k >> 2};
+      const float w = (dd[0] ? t[0] : 1.f - t[0]) * (dd[1] ? t[1] : 1.f - t[1]) * (dd[2] ? t[2] : 1.f - t[2]);
+      int q[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) q[d] = BASE * bc[d] + lp[d] - (d == A ? 0 : 1) + dd[d];
+      float v = 0.f;
+      if (q[A] < g.N[A]) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          if (d != A) q[d] = iclamp(q[d], 0, g.N[d] - 1);
+        const int Bo = (q[0] >> LOGB) + g.D[0] * ((q[1] >> LOGB) + g.D[1] * (q[2] >> LOGB));
+        const int l[3] = {q[0] & 7, q[1] & 7, q[2] & 7};
+        v = F[(size_t)Bo * SLAB + field * 3 * PAD + A * PAD + pad_idx(A, l)];
+      }
+      s += w * v;
+    }
+  }
+  return zero ? 0.f : s;
+}
+
+template <bool SLOW>
+__device__ __forceinline__ float sample_comp(const GridDev& g, const float* __restrict__ F, int field, int a,
+                                             const float* u, const Near& nr, bool& ok) {
+  return a == 0 ? sample_comp<0, SLOW>(g, F, field, u, nr, ok)
+                : (a == 1 ? sample_comp<1, SLOW>(g, F, field, u, nr, ok)
+                          : sample_comp<2, SLOW>(g, F, field, u, nr, ok));
 }
 
 __device__ __forceinline__ float sample_comp(const GridDev& g, const float* __restrict__ F, int field, int a,
-                                             const float* u) {
-  int i0[3];
-  float t[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const float c = d == a ? u[d] : u[d] - 0.5f;
-    const float fl = floorf(c);
-    i0[d] = (int)fl;
-    t[d] = c - fl;
-  }
-  float s = 0.f;
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int dd[3] = {k & 1, (k >> 1) & 1, k >> 2};
-    int q[3] = {i0[0] + dd[0], i0[1] + dd[1], i0[2] + dd[2]};
-    const float w = (dd[0] ? t[0] : 1.f - t[0]) * (dd[1] ? t[1] : 1.f - t[1]) * (dd[2] ? t[2] : 1.f - t[2]);
-    s += w * fine_at(g, F, field, a, q);
-  }
-  return s;
+                                             const float* u, const Near& nr) {
+  bool ok = true;
+  return sample_comp<true>(g, F, field, a, u, nr, ok);
 }
 
-// FLIP/PIC blend (G2P.cuh:87-128) and Ralston RK3 (G2P.cuh:131-173) per particle
-__global__ __launch_bounds__(TPB) void g2p_advect_kernel(GridDev g, const float* __restrict__ F,
-                                                         const float* __restrict__ spos,
-                                                         const float* __restrict__ svel, int64_t n, float alpha,
-                                                         float dt, float* __restrict__ pos_out,
-                                                         float* __restrict__ vel_out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// FLIP/PIC blend (G2P.cuh:87-128) and Ralston RK3 (G2P.cuh:131-173) of
+// particle i.  Without SLOW a particle with a stage point in a block without
+// halo is appended to list (list[0] counts) and nothing is written for it.
+template <bool SLOW>
+__device__ __forceinline__ void g2p_advect_one(const GridDev& g, const float* __restrict__ F,
+                                               const float* __restrict__ spos, const float* __restrict__ svel,
+                                               int64_t i, float alpha, float dt, float* __restrict__ pos_out,
+                                               float* __restrict__ vel_out, uint32_t* __restrict__ list) {
+  bool ok = true;
   float x[3], u[3], v1[3], vd[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     x[a] = spos[3 * i + a];
     u[a] = fminf(fmaxf(x[a] * g.inv_h, 0.f), (float)g.N[a] - 0.0009765625f);
   }
+  const Near nr = near_of(g, u);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    v1[a] = sample_comp(g, F, 0, a, u);
-    vd[a] = sample_comp(g, F, 1, a, u);
+    v1[a] = sample_comp<SLOW>(g, F, 0, a, u, nr, ok);
+    vd[a] = sample_comp<SLOW>(g, F, 1, a, u, nr, ok);
   }
+  float vn[3];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) vel_out[3 * i + a] = alpha * (svel[3 * i + a] + vd[a]) + (1.f - alpha) * v1[a];
+  for (int a = 0; a < 3; ++a) vn[a] = alpha * (svel[3 * i + a] + vd[a]) + (1.f - alpha) * v1[a];
   float xx[3], v2[3], v3[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -825,20 +963,45 @@ __global__ __launch_bounds__(TPB) void g2p_advect_kernel(GridDev g, const float*
     u[a] = fminf(fmaxf(xx[a] * g.inv_h, -1.f), (float)g.N[a] + 1.f);
   }
 #pragma unroll
-  for (int a = 0; a < 3; ++a) v2[a] = sample_comp(g, F, 0, a, u);
+  for (int a = 0; a < 3; ++a) v2[a] = sample_comp<SLOW>(g, F, 0, a, u, nr, ok);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     xx[a] = x[a] + 0.75f * dt * v2[a];
     u[a] = fminf(fmaxf(xx[a] * g.inv_h, -1.f), (float)g.N[a] + 1.f);
   }
 #pragma unroll
-  for (int a = 0; a < 3; ++a) v3[a] = sample_comp(g, F, 0, a, u);
+  for (int a = 0; a < 3; ++a) v3[a] = sample_comp<SLOW>(g, F, 0, a, u, nr, ok);
+  if (!SLOW && !ok) {
+    list[1 + atomicAdd(&list[0], 1u)] = (uint32_t)i;
+    return;
+  }
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const float xn = x[a] + dt * ((2.f / 9.f) * v1[a] + (3.f / 9.f) * v2[a] + (4.f / 9.f) * v3[a]);
     const float lo = 0.25f * g.h, hi = (float)g.N[a] * g.h - 0.25f * g.h;
+    vel_out[3 * i + a] = vn[a];
     pos_out[3 * i + a] = fminf(fmaxf(xn, lo), hi);
   }
+}
+
+// FIXUP = false: every particle; FIXUP = true: the particles listed by the
+// first pass (a grid-stride loop over list[0] entries, so the host needs no
+// count)
+template <bool FIXUP>
+__global__ __launch_bounds__(TPB) void g2p_advect_kernel(GridDev g, const float* __restrict__ F,
+                                                         const float* __restrict__ spos,
+                                                         const float* __restrict__ svel, int64_t n, float alpha,
+                                                         float dt, float* __restrict__ pos_out,
+                                                         float* __restrict__ vel_out, uint32_t* __restrict__ list) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (FIXUP) {
+    const int64_t m = (int64_t)list[0];
+    for (int64_t j = i; j < m; j += (int64_t)gridDim.x * blockDim.x)
+      g2p_advect_one<true>(g, F, spos, svel, (int64_t)list[1 + j], alpha, dt, pos_out, vel_out, list);
+    return;
+  }
+  if (i >= n) return;
+  g2p_advect_one<false>(g, F, spos, svel, i, alpha, dt, pos_out, vel_out, list);
 }
 
 // ------------------------------------------------------------- adjoint
@@ -849,13 +1012,19 @@ __global__ __launch_bounds__(TPB) void g2p_advect_kernel(GridDev g, const float*
 // float atomics (no return); every other stage is a gather.
 
 // address of fine node q of component a (transverse indices clamp to the
-// domain), or -1 where the node is the +wall or outside
+// domain), or -1 where the node is the +wall or outside; stride SLAB is the
+// padded forward lattice (the node in its owner block's array), any other
+// stride a buffer of the blocks' own nodes
 __device__ __forceinline__ int64_t fine_node(const GridDev& g, int stride, int field, int a, int* q) {
   if (q[a] < 0 || q[a] >= g.N[a]) return -1;
 #pragma unroll
   for (int d = 0; d < 3; ++d)
     if (d != a) q[d] = iclamp(q[d], 0, g.N[d] - 1);
   const int B = (q[0] >> LOGB) + g.D[0] * ((q[1] >> LOGB) + g.D[1] * (q[2] >> LOGB));
+  if (stride == SLAB) {
+    const int l[3] = {q[0] & 7, q[1] & 7, q[2] & 7};
+    return (int64_t)B * SLAB + field * 3 * PAD + a * PAD + pad_idx(a, l);
+  }
   const int l = (q[0] & 7) + 8 * ((q[1] & 7) + 8 * (q[2] & 7));
   return (int64_t)B * stride + field * 3 * FINE + a * FINE + l;
 }
@@ -965,10 +1134,11 @@ __global__ __launch_bounds__(TPB) void g2p_advect_adjoint_kernel(GridDev g, cons
       u[a] = fminf(fmaxf(raw, 0.f), umax[a]);
       m1[a] = (raw > 0.f && raw < umax[a]) ? 1.f : 0.f;
     }
+    const Near nr = near_of(g, u);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      v1[a] = sample_comp(g, F, 0, a, u);
-      vd[a] = sample_comp(g, F, 1, a, u);
+      v1[a] = sample_comp(g, F, 0, a, u, nr);
+      vd[a] = sample_comp(g, F, 1, a, u, nr);
     }
     float u2[3], m2[3], v2[3], u3[3], m3[3], v3[3];
 #pragma unroll
@@ -978,7 +1148,7 @@ __global__ __launch_bounds__(TPB) void g2p_advect_adjoint_kernel(GridDev g, cons
       m2[a] = (raw > -1.f && raw < (float)g.N[a] + 1.f) ? 1.f : 0.f;
     }
 #pragma unroll
-    for (int a = 0; a < 3; ++a) v2[a] = sample_comp(g, F, 0, a, u2);
+    for (int a = 0; a < 3; ++a) v2[a] = sample_comp(g, F, 0, a, u2, nr);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const float raw = (x[a] + 0.75f * dt * v2[a]) * g.inv_h;
@@ -986,7 +1156,7 @@ __global__ __launch_bounds__(TPB) void g2p_advect_adjoint_kernel(GridDev g, cons
       m3[a] = (raw > -1.f && raw < (float)g.N[a] + 1.f) ? 1.f : 0.f;
     }
 #pragma unroll
-    for (int a = 0; a < 3; ++a) v3[a] = sample_comp(g, F, 0, a, u3);
+    for (int a = 0; a < 3; ++a) v3[a] = sample_comp(g, F, 0, a, u3, nr);
     float k1[3], k2[3], k3[3], xacc[3], vo[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -2384,15 +2554,22 @@ void k_project(hipStream_t s, const GridDev& g, const float* p, float* const* us
   project_kernel<<<g.nb, TPB, 0, s>>>(g, p, fp(ustar), fp(beta), fp(unew));
 }
 
-void k_to_fine(hipStream_t s, const GridDev& g, float* const* unew, float* const* uold, float* fine) {
+void k_to_fine(hipStream_t s, const GridDev& g, float* const* unew, float* const* uold, float* fine,
+               const uint32_t* count, uint8_t* halo) {
+  halo_flags_kernel<<<(g.nb + TPB - 1) / TPB, TPB, 0, s>>>(g, count, halo);
   to_fine_kernel<<<g.nb, TPB, 0, s>>>(g, fp(unew), fp(uold), fine);
+  fine_halo_kernel<<<g.nb, TPB, 0, s>>>(g, fine);
 }
 
 void k_g2p_advect(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const float* svel,
-                  int64_t n, float alpha, float dt, float* pos_out, float* vel_out) {
-  if (n > 0)
-    g2p_advect_kernel<<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, s>>>(g, fine, spos, svel, n, alpha, dt, pos_out,
-                                                                     vel_out);
+                  int64_t n, float alpha, float dt, float* pos_out, float* vel_out, uint32_t* list) {
+  if (n <= 0) return;
+  fill_kernel<uint32_t><<<1, 64, 0, s>>>(list, 1, 0u);
+  g2p_advect_kernel<false><<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, s>>>(g, fine, spos, svel, n, alpha, dt,
+                                                                            pos_out, vel_out, list);
+  const int64_t nwg = (n + TPB - 1) / TPB;
+  const unsigned fix = (unsigned)(nwg < 1024 ? nwg : 1024);
+  g2p_advect_kernel<true><<<fix, TPB, 0, s>>>(g, fine, spos, svel, n, alpha, dt, pos_out, vel_out, list);
 }
 
 void k_g2p_advect_adjoint(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const float* svel,

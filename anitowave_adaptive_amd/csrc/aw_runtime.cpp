@@ -186,7 +186,7 @@ struct aw_grid {
   int D[3]{}, N[3]{};
   int nb = 0;
   float h = 0.f, inv_h = 0.f;
-  DevBuf level, surf, ltmp, count, poff, cursor, coff, foff, nbr, key;
+  DevBuf level, surf, ltmp, count, poff, cursor, coff, foff, nbr, key, halo, g2p_list;
   DevBuf spos, svel, sids;
   DevBuf liquid, p, rhs, slab;
   DevBuf beta[3], Macc[3], macc[3], uold[3], ustar[3], unew[3], mass[3];
@@ -215,6 +215,8 @@ struct aw_grid {
     g.level = level.as<uint8_t>();
     g.coff = coff.as<uint32_t>();
     g.nbr = nbr.as<int32_t>();
+    g.halo = halo.as<uint8_t>();
+    g.count = count.as<uint32_t>();
     return g;
   }
   float* face(DevBuf* f, int a) const { return f[a].as<float>(); }
@@ -333,6 +335,7 @@ int apply_levels(aw_grid* g) {
     for (DevBuf* f : {g->beta, g->Macc, g->macc, g->uold, g->ustar, g->unew, g->mass})
       AW_TRY(f[a].ensure(nf * sizeof(float)));
   AW_TRY(g->slab.ensure((size_t)nb * SLAB * sizeof(float)));
+  AW_TRY(g->halo.ensure((size_t)nb));
   LevelDev L;
   for (int a = 0; a < 3; ++a) {
     L.D[a] = g->D[a];
@@ -448,8 +451,8 @@ int project(aw_grid* g) {
     k_project(s, gv, g->p.as<float>(), us, bt, un);
   }
   {
-    Prof pf(g->ctx, "to_fine", 24.0 * g->nfaces + 2.0 * 4.0 * 3 * FINE * g->nb);
-    k_to_fine(s, gv, un, uo, g->slab.as<float>());
+    Prof pf(g->ctx, "to_fine", 24.0 * g->nfaces + 2.0 * 4.0 * 3 * FINE * g->nb + 4.0 * g->nb);
+    k_to_fine(s, gv, un, uo, g->slab.as<float>(), g->count.as<uint32_t>(), g->halo.as<uint8_t>());
   }
   return launch_status("project");
 }
@@ -1359,8 +1362,8 @@ int tape_restore(aw_tape* t, const aw_tape::Step& r) {
   g->sorted_ids = true;
   Faces un(g->unew), uo(g->uold);
   {
-    Prof pf(g->ctx, "to_fine", 24.0 * g->nfaces + 2.0 * 4.0 * 3 * FINE * g->nb);
-    k_to_fine(s, g->view(), un.p, uo.p, g->slab.as<float>());
+    Prof pf(g->ctx, "to_fine", 24.0 * g->nfaces + 2.0 * 4.0 * 3 * FINE * g->nb + 4.0 * g->nb);
+    k_to_fine(s, g->view(), un.p, uo.p, g->slab.as<float>(), g->count.as<uint32_t>(), g->halo.as<uint8_t>());
   }
   return launch_status("tape restore");
 }
@@ -1710,7 +1713,9 @@ int aw_project_velocity(aw_grid* g) {
 int aw_g2p_advect(aw_grid* g, float* d_pos, float* d_vel, int64_t n, float alpha, float dt) {
   if (!g || n < 0 || (n > 0 && (!d_pos || !d_vel))) return fail(AW_ERR_INVALID, "bad arguments");
   if (!g->have_topology) return fail(AW_ERR_INVALID, "grid has no topology");
-  k_g2p_advect(g->ctx->stream, g->view(), g->slab.as<float>(), d_pos, d_vel, n, alpha, dt, d_pos, d_vel);
+  AW_TRY(g->g2p_list.ensure((size_t)(n + 1) * sizeof(uint32_t)));
+  k_g2p_advect(g->ctx->stream, g->view(), g->slab.as<float>(), d_pos, d_vel, n, alpha, dt, d_pos, d_vel,
+               g->g2p_list.as<uint32_t>());
   return launch_status("g2p_advect");
 }
 
@@ -1850,8 +1855,9 @@ int aw_step(aw_grid* g, aw_solver* S, float* d_pos, float* d_vel, int32_t* d_ids
   AW_TRY(project(g));
   {
     Prof pf(g->ctx, "g2p_advect", 48.0 * n + 2.0 * 4.0 * 3 * FINE * g->occupied);
+    AW_TRY(g->g2p_list.ensure((size_t)(n + 1) * sizeof(uint32_t)));
     k_g2p_advect(s, g->view(), g->slab.as<float>(), g->spos.as<float>(), g->svel.as<float>(), n, prm->flip_alpha,
-                 prm->dt, d_pos, d_vel);
+                 prm->dt, d_pos, d_vel, g->g2p_list.as<uint32_t>());
   }
   if (d_ids && n > 0) AW_HIP(hipMemcpyAsync(d_ids, g->sids.p, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
   AW_HIP(hipStreamSynchronize(s));

@@ -200,6 +200,21 @@ def test_projection_and_g2p_parity(ctx, name):
     gpos, gvel, _ = gp.get()
     assert np.abs(gpos - opos).max() < 1e-6
     assert rel_err(gvel, ovel) < TOL
+    # probe particles all over the tank, fast enough that RK3 stage points
+    # land far from the liquid, in blocks whose fine-lattice halo is not
+    # filled (samples there read each node from its owner block)
+    rng = np.random.default_rng(5)
+    m = 4096
+    ppos = rng.uniform(0.0, 1.0, (m, 3)).astype(np.float32)
+    pvel = (40.0 * rng.standard_normal((m, 3))).astype(np.float32)
+    opos, ovel = og.g2p_advect(ppos, pvel, 0.95, DT)
+    dp = aw.DeviceArray.from_numpy(ctx, ppos)
+    dv = aw.DeviceArray.from_numpy(ctx, pvel)
+    aw.G2PManager.g2p_advect(gg, dp.ptr, dv.ptr, m, 0.95, DT)
+    gpos = dp.download(np.float32, 3 * m).reshape(-1, 3)
+    gvel = dv.download(np.float32, 3 * m).reshape(-1, 3)
+    assert np.abs(gpos - opos).max() < 1e-5
+    assert rel_err(gvel, ovel) < TOL
 
 
 @pytest.mark.parametrize("name", NAMES)
