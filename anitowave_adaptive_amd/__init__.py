@@ -174,8 +174,10 @@ class Context:
     def synchronize(self):
         check(lib().aw_ctx_synchronize(self._h), "aw_ctx_synchronize")
 
-    def profile(self, on=True):
-        check(lib().aw_profile_enable(self._h, int(bool(on))), "aw_profile_enable")
+    def profile(self, on=True, runs=False):
+        """Kernel timing with HIP events: per launch, or (runs=True) per run of
+        back-to-back launches of one kernel group."""
+        check(lib().aw_profile_enable(self._h, (2 if runs else 1) if on else 0), "aw_profile_enable")
 
     def profile_report(self):
         import json

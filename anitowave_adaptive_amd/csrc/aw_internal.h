@@ -72,17 +72,6 @@ struct WGMap {
   int lr[WGMAP_RUNS];
 };
 
-// Coarse correction fused into a smoothing phase: x += scale * xc[parent]
-// on the phase's own cells (own) and on values read from neighbour blocks
-// that have not been corrected yet (nbr).
-struct Pro {
-  const float* xc;
-  LevelDev C;
-  int aggregated;
-  float scale;
-  int own, nbr;
-};
-
 // Restriction fused into the residual pass (coarse level with the same block
 // dims): b_c = sum of children, x_c = 0.
 struct Restr {
@@ -187,7 +176,7 @@ void k_ids_range(hipStream_t s, const int32_t* ids, int64_t n, int* out);  // ou
 
 // solver
 void k_smooth(hipStream_t s, const LevelDev& L, float* x, const float* b, const WGMap& wg,
-              const uint32_t* blist, float omega, int local_sweeps, int reverse, const Pro& pro);
+              const uint32_t* blist, float omega, int local_sweeps, int reverse);
 void k_residual(hipStream_t s, const LevelDev& L, const float* x, const float* b, float* r, const WGMap& wg,
                 const uint32_t* blist, const Restr& rs);
 void k_apply_dot(hipStream_t s, const LevelDev& L, const float* x, float* y, const WGMap& wg,

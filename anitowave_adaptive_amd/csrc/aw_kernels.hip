@@ -1631,89 +1631,6 @@ __device__ __forceinline__ bool decode_bd(const LevelDev& L, const WG& w, int i,
   return true;
 }
 
-// Coefficient and neighbour term of face d (2 axis + side) of a cell on the
-// block boundary: c and c * value summed over the neighbour cells it touches
-// (one equal or coarser cell, or four finer sub-faces).  dg = ex = 0 on walls.
-template <class Out, typename T>
-__device__ __forceinline__ void boundary_term(const LevelDev& L, const CellRef& cr, const uint32_t* bd, int d,
-                                              Out outb, T& dg, T& ex) {
-  const int a = d >> 1, side = d & 1;
-  const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
-  const int r = cr.r;
-  dg = 0;
-  ex = 0;
-  const int N = (int)bd[3 + d];
-  if (N < 0) return;
-  const uint32_t m0 = bd[21], m1 = bd[22];
-  const int e = (int)(m0 & 255u);
-  const T s = (T)(1 << e);
-  const uint32_t fo = bd[2];
-  const int en = (int)((d < 3 ? m0 >> (8 * (d + 1)) : m1 >> (8 * (d - 3))) & 255u), rn = L.bs >> en;
-  const uint32_t con = bd[9 + d];
-  int sl[3] = {cr.l[0], cr.l[1], cr.l[2]};
-  sl[a] += side;
-  int nc[3] = {cr.l[0], cr.l[1], cr.l[2]};
-  nc[a] = side ? 0 : rn - 1;
-  if (en == e) {
-    const T c = (T)L.beta[a][fo + face_idx(a, sl, r)] * s;
-    dg = c;
-    ex = c * outb(N, nc, rn, con + cell_idx(nc, rn));
-  } else if (en > e) {
-    const T c = (T)L.beta[a][fo + face_idx(a, sl, r)] * s / (T)1.5;
-    nc[t1] = cr.l[t1] >> 1;
-    nc[t2] = cr.l[t2] >> 1;
-    dg = c;
-    ex = c * outb(N, nc, rn, con + cell_idx(nc, rn));
-  } else {
-    int ps[3];
-    ps[a] = side ? 0 : rn;
-    const uint32_t fon = bd[15 + d];
-    const T sf = (T)0.5 * s;
-#pragma unroll
-    for (int d1 = 0; d1 < 2; ++d1)
-#pragma unroll
-      for (int d2 = 0; d2 < 2; ++d2) {
-        nc[t1] = ps[t1] = 2 * cr.l[t1] + d1;
-        nc[t2] = ps[t2] = 2 * cr.l[t2] + d2;
-        const T c = (T)L.beta[a][fon + face_idx(a, ps, rn)] * sf / (T)1.5;
-        dg += c;
-        ex += c * outb(N, nc, rn, con + cell_idx(nc, rn));
-      }
-  }
-}
-
-// stencil_split with the block's metadata read from its descriptor
-template <class Out, typename T>
-__device__ __forceinline__ void stencil_split_bd(const LevelDev& L, const CellRef& cr, const uint32_t* bd, Out outb,
-                                                 T& diag, T& ext, T* w) {
-  const int r = cr.r;
-  const T s = (T)(1 << (int)(bd[21] & 255u));
-  const uint32_t fo = bd[2];
-  diag = 0;
-  ext = 0;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-#pragma unroll
-    for (int side = 0; side < 2; ++side) {
-      const int d = 2 * a + side;
-      w[d] = 0;
-      const int nl = cr.l[a] + (side ? 1 : -1);
-      if (nl >= 0 && nl < r) {
-        int sl[3] = {cr.l[0], cr.l[1], cr.l[2]};
-        sl[a] += side;
-        const T c = (T)L.beta[a][fo + face_idx(a, sl, r)] * s;
-        w[d] = c;
-        diag += c;
-        continue;
-      }
-      T dg, ex;
-      boundary_term(L, cr, bd, d, outb, dg, ex);
-      diag += dg;
-      ext += ex;
-    }
-  }
-}
-
 // The same stencil with every face's loads issued before any is used: the
 // coefficient from the block's own face slot, and one neighbour value per
 // boundary face from an address chosen with selects (the cell itself where
@@ -1828,24 +1745,14 @@ __device__ __forceinline__ uint32_t parent_cell(const LevelDev& F, const LevelDe
 // held to 64 VGPRs for eight waves per SIMD.
 #define AW_STENCIL_OCCUPANCY __attribute__((amdgpu_waves_per_eu(8, 8)))
 
-// PRO: 0 no coarse correction, 1 on the group's own cells, 2 also on the
-// neighbour values read (one instantiation per case keeps each kernel's code
-// small).
-template <int PRO>
 __global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void smooth_kernel(
     LevelDev L, float* __restrict__ x, const float* __restrict__ b, WGMap wgm,
-    const uint32_t* __restrict__ blist, float omega, int local_sweeps, int reverse, Pro pro) {
+    const uint32_t* __restrict__ blist, float omega, int local_sweeps, int reverse) {
   __shared__ float sx[512];
   const WG w = wg_at(wgm, blockIdx.x);
   CellRef cr[2];
   bool act[2];
   float bb[2], dg[2], ex[2], cw[2][6];
-  auto outb = [&](int N, const int* nc, int rn, uint32_t cc) {
-    float v = x[cc];
-    if (PRO == 2 && L.liquid[cc])
-      v += pro.scale * pro.xc[parent_cell(L, pro.C, pro.aggregated, N, nc, rn, cell_idx(nc, rn))];
-    return v;
-  };
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const int i = threadIdx.x + k * TPB;
@@ -1853,17 +1760,11 @@ __global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void smooth_kernel(
     const uint32_t* bd;
     if (!decode_bd(L, w, i, cr[k], bd)) continue;
     const uint32_t c = cr[k].c;
-    float xv = x[c];
-    const bool liq = L.liquid[c] != 0;
-    if (PRO >= 1 && liq) xv += pro.scale * pro.xc[parent_cell(L, pro.C, pro.aggregated, cr[k].B, cr[k].l, cr[k].r, cr[k].li)];
-    sx[i] = xv;
-    if (!liq) continue;
+    sx[i] = x[c];
+    if (!L.liquid[c]) continue;
     act[k] = true;
     bb[k] = b[c];
-    if constexpr (PRO == 2)
-      stencil_split_bd(L, cr[k], bd, outb, dg[k], ex[k], cw[k]);
-    else
-      stencil_fast(L, cr[k], bd, [&](uint32_t cc) { return x[cc]; }, dg[k], ex[k], cw[k]);
+    stencil_fast(L, cr[k], bd, [&](uint32_t cc) { return x[cc]; }, dg[k], ex[k], cw[k]);
   }
   __syncthreads();
   for (int ls = 0; ls < local_sweeps; ++ls)
@@ -2611,14 +2512,8 @@ void k_masked_copy(hipStream_t s, float* dst, const float* src, const uint8_t* l
 }
 
 void k_smooth(hipStream_t s, const LevelDev& L, float* x, const float* b, const WGMap& wg,
-              const uint32_t* blist, float omega, int local_sweeps, int reverse, const Pro& pro) {
-  if (wg.n <= 0) return;
-  if (pro.nbr)
-    smooth_kernel<2><<<wg.n, TPB, 0, s>>>(L, x, b, wg, blist, omega, local_sweeps, reverse, pro);
-  else if (pro.own)
-    smooth_kernel<1><<<wg.n, TPB, 0, s>>>(L, x, b, wg, blist, omega, local_sweeps, reverse, pro);
-  else
-    smooth_kernel<0><<<wg.n, TPB, 0, s>>>(L, x, b, wg, blist, omega, local_sweeps, reverse, pro);
+              const uint32_t* blist, float omega, int local_sweeps, int reverse) {
+  if (wg.n > 0) smooth_kernel<<<wg.n, TPB, 0, s>>>(L, x, b, wg, blist, omega, local_sweeps, reverse);
 }
 
 void k_residual(hipStream_t s, const LevelDev& L, const float* x, const float* b, float* r, const WGMap& wg,

@@ -132,6 +132,7 @@ struct ProfRec {
   const char* name;
   hipEvent_t a, b;
   double bytes;
+  long launches;
 };
 
 // Handles are reference-counted by the handles made from them (grids hold
@@ -146,6 +147,7 @@ struct aw_ctx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   bool prof = false;
+  bool prof_runs = false;   // consecutive launches of one name share an event pair
   std::vector<ProfRec> recs;
   std::vector<hipEvent_t> pool;
   size_t used = 0;
@@ -167,7 +169,14 @@ struct Prof {
   long idx = -1;
   Prof(aw_ctx* ctx, const char* name, double bytes) : c(ctx) {
     if (!c->prof) return;
-    ProfRec r{name, c->event(), c->event(), bytes};
+    if (c->prof_runs && !c->recs.empty() && std::strcmp(c->recs.back().name, name) == 0) {
+      // extends the run: its end event is recorded again after this launch
+      idx = (long)c->recs.size() - 1;
+      c->recs[idx].bytes += bytes;
+      c->recs[idx].launches++;
+      return;
+    }
+    ProfRec r{name, c->event(), c->event(), bytes, 1};
     (void)hipEventRecord(r.a, c->stream);
     c->recs.push_back(r);
     idx = (long)c->recs.size() - 1;
@@ -748,34 +757,15 @@ int ensure_setup(aw_solver* S) {
   return launch_status("solver setup");
 }
 
-Pro no_pro() {
-  Pro p;
-  std::memset(&p, 0, sizeof p);
-  return p;
-}
-
-void smooth_level(aw_solver* S, int k, float* x, const float* b, int iters, bool reverse, const float* xc = nullptr) {
+void smooth_level(aw_solver* S, int k, float* x, const float* b, int iters, bool reverse) {
   const MGLevel& L = *S->lv[k];
   hipStream_t s = S->g->ctx->stream;
   const LevelDev V = L.view();
   for (int it = 0; it < iters; ++it)
     for (int ph = 0; ph < 2; ++ph) {
       const int col = reverse ? 1 - ph : ph;
-      Pro pro = no_pro();
-      if (xc && it == 0) {
-        // the first post-smoothing pass carries the coarse correction: the
-        // first block colour adds it to its own cells and to the neighbour
-        // values it reads; the second colour's neighbours are corrected by then
-        const MGLevel& C = *S->lv[k + 1];
-        pro.xc = xc;
-        pro.C = C.view();
-        pro.aggregated = C.aggregated;
-        pro.scale = S->prm.prolong_scale;
-        pro.own = 1;
-        pro.nbr = ph == 0 ? 1 : 0;
-      }
       Prof pf(S->g->ctx, k == 0 ? "smooth_l0" : "smooth_coarse", L.sweep_bytes[col]);
-      k_smooth(s, V, x, b, L.wgm[col], L.bl(), S->prm.omega, S->prm.local_sweeps, reverse ? 1 : 0, pro);
+      k_smooth(s, V, x, b, L.wgm[col], L.bl(), S->prm.omega, S->prm.local_sweeps, reverse ? 1 : 0);
     }
 }
 
@@ -867,9 +857,10 @@ void prolong_level(aw_solver* S, int k, float* x, const float* xc) {
 
 // recursive V-cycle (VCycleSolver::runCycle, VCycle.cuh:251-304): the
 // restriction is fused into the residual pass (across an aggregation the
-// restriction kernel zeroes the coarse guess itself), the prolongation into
-// the first post-smoothing pass, and the trailing small levels run in one
-// work-group.
+// restriction kernel zeroes the coarse guess itself), the prolongation is a
+// pass of its own before post-smoothing (fusing it into the first smoothing
+// phase lengthens that phase's dependent loads more than the pass costs), and
+// the trailing small levels run in one work-group.
 void vcycle(aw_solver* S, int k, float* x, const float* b) {
   if (k >= S->tail) {
     tail_vcycle(S, k, x, b);
@@ -885,11 +876,8 @@ void vcycle(aw_solver* S, int k, float* x, const float* b) {
     restrict_level(S, k, L.d_r, C.d_b, C.d_x);
   }
   vcycle(S, k + 1, C.d_x, C.d_b);
-  if (S->prm.post_sweeps > 0) {
-    smooth_level(S, k, x, b, S->prm.post_sweeps, true, C.d_x);
-  } else {
-    prolong_level(S, k, x, C.d_x);
-  }
+  prolong_level(S, k, x, C.d_x);
+  smooth_level(S, k, x, b, S->prm.post_sweeps, true);
 }
 
 // slot group of scalar idx
@@ -1469,6 +1457,7 @@ int aw_profile_enable(aw_ctx* c, int on) {
   c->recs.clear();
   c->used = 0;
   c->prof = on != 0;
+  c->prof_runs = on == 2;
   return AW_OK;
 }
 
@@ -1494,7 +1483,7 @@ int aw_profile_report(aw_ctx* c, char* buf, uint64_t len) {
       agg.push_back(Agg{r.name});
       a = &agg.back();
     }
-    a->n++;
+    a->n += r.launches;
     a->ms += ms;
     a->bytes += r.bytes;
   }

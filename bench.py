@@ -68,7 +68,7 @@ def run_gpu(args, w, pos, vel):
     # do not slow the timed region down
     prof = {}
     if args.profile:
-        ctx.profile(True)
+        ctx.profile(True, runs=True)
         for i in range(args.profile_steps):
             step(dpos.ptr, dvel.ptr, dids.ptr, n, allow_not_converged=True)
         ctx.synchronize()

@@ -156,9 +156,12 @@ int aw_copy(aw_ctx* ctx, void* dst, const void* src, uint64_t bytes);
  * mark since load or the last reset (no reference counterpart).  reset_peak
  * sets the mark to the current figure after reading it. */
 int aw_memory_stats(uint64_t* current_bytes, uint64_t* peak_bytes, int reset_peak);
-/* Per-launch timing with HIP events on the context's stream (no reference
- * counterpart).  enable clears earlier records; report synchronises and writes
- * JSON {"kernel": {"launches": n, "ms": total, "bytes": algorithmic bytes}}. */
+/* Kernel timing with HIP events on the context's stream (no reference
+ * counterpart).  on = 1: an event pair around every launch; on = 2: one pair
+ * around each run of consecutive launches of one kernel group (no events
+ * between back-to-back launches, so short kernels are not stretched by them).
+ * enable clears earlier records; report synchronises and writes JSON
+ * {"kernel": {"launches": n, "ms": total, "bytes": algorithmic bytes}}. */
 int aw_profile_enable(aw_ctx* ctx, int on);
 int aw_profile_report(aw_ctx* ctx, char* buf, uint64_t len);
 
