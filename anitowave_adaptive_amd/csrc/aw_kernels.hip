@@ -52,6 +52,21 @@ __device__ __forceinline__ int pad_idx(int a, const int* l) {
 
 __device__ __forceinline__ float norm_u(float M, float m) { return m > 1e-6f ? M / m : 0.f; }
 
+// LDS float add through compare-and-swap.  On gfx950 the LDS float atomic
+// (ds_add_f32) retires about 0.33 lanes per clock per CU, while a read plus
+// ds_cmpst_rtn_b32 loop sustains about 8x that when lanes rarely hit the same
+// word; with many lanes on one word the retries make it far slower, so it is
+// used only where addresses are spread.
+__device__ __forceinline__ void lds_add_spread(float* p, float v) {
+  unsigned* a = reinterpret_cast<unsigned*>(p);
+  unsigned old = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  while (true) {
+    const unsigned got = atomicCAS(a, old, __float_as_uint(__uint_as_float(old) + v));
+    if (got == old) break;
+    old = got;
+  }
+}
+
 // Pi: the native slots (and weights) of a block of level e that define its
 // own fine node l on axis a.  Normal coordinate l / s, transverse
 // (l + 1/2) / s - 1/2 clamped to [0, r-1]; trilinear.
@@ -581,6 +596,12 @@ __global__ __launch_bounds__(TPB) void p2g_gather_kernel(GridDev g, const uint32
     int slot[8];
     float w[8];
     const int n = pi_weights(a, e, l, slot, w);
+    if (e == 0) {
+      // a level-0 block's fine nodes are its native slots, one to one
+      nM[a * 576 + slot[0]] += w[0] * M;
+      nm[a * 576 + slot[0]] += w[0] * m;
+      continue;
+    }
     for (int j = 0; j < n; ++j) {
       atomicAdd(&nM[a * 576 + slot[j]], w[j] * M);
       atomicAdd(&nm[a * 576 + slot[j]], w[j] * m);
@@ -926,10 +947,135 @@ __device__ __forceinline__ float sample_comp(const GridDev& g, const float* __re
                           : sample_comp<2, SLOW>(g, F, field, u, nr, ok));
 }
 
+// the per-node path of sample_comp, out of line
+template <int A>
+__device__ __attribute__((noinline)) float sample_comp_slow(const GridDev& g, const float* __restrict__ F, int field,
+                                                            float u0, float u1, float u2) {
+  const float u[3] = {u0, u1, u2};
+  const Near far = {{-2, -2, -2}, false};
+  bool ok = true;
+  return sample_comp<A, true>(g, F, field, u, far, ok);
+}
+
+template <int A>
+__device__ __forceinline__ float sample_comp_any(const GridDev& g, const float* __restrict__ F, int field,
+                                                 const float* u, const Near& nr) {
+  bool ok = true;
+  const float v = sample_comp<A, false>(g, F, field, u, nr, ok);
+  return ok ? v : sample_comp_slow<A>(g, F, field, u[0], u[1], u[2]);
+}
+
+// any sample point: the halo path, or the per-node path out of line
 __device__ __forceinline__ float sample_comp(const GridDev& g, const float* __restrict__ F, int field, int a,
                                              const float* u, const Near& nr) {
-  bool ok = true;
-  return sample_comp<true>(g, F, field, a, u, nr, ok);
+  return a == 0 ? sample_comp_any<0>(g, F, field, u, nr)
+                : (a == 1 ? sample_comp_any<1>(g, F, field, u, nr) : sample_comp_any<2>(g, F, field, u, nr));
+}
+
+// sample_comp<A, true> with the gradient of the interpolant with respect to
+// the sampled coordinate (fine units): G = 0 along an axis where the
+// coordinate was clamped (the sample is flat there), as for the node rule of
+// sample_comp; the value is computed exactly as sample_comp computes it.
+struct SG {
+  float s, g0, g1, g2;
+};
+
+template <int A, bool FAST>
+__device__ __forceinline__ SG sample_grad_impl(const GridDev& g, const float* __restrict__ F, int field,
+                                               const float* u) {
+  constexpr int n0 = A == 0 ? 9 : 10, n1 = A == 1 ? 9 : 10;
+  int bc[3], lp[3];
+  float t[3], dm[3];
+  bool zero = false;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    int i;
+    if (d == A) {
+      zero = zero || u[d] < 0.f || u[d] > (float)g.N[d];
+      const float c = fminf(fmaxf(u[d], 0.f), (float)g.N[d]);
+      i = min((int)floorf(c), g.N[d] - 1);
+      t[d] = c - (float)i;
+      dm[d] = 1.f;
+    } else {
+      const float cr = u[d] - 0.5f;
+      const float c = fminf(fmaxf(cr, 0.f), (float)(g.N[d] - 1));
+      i = (int)floorf(c);
+      t[d] = c - (float)i;
+      dm[d] = (cr >= 0.f && cr < (float)(g.N[d] - 1)) ? 1.f : 0.f;
+    }
+    bc[d] = i >> LOGB;
+    lp[d] = (i & 7) + (d == A ? 0 : 1);
+  }
+  const int B = bc[0] + g.D[0] * (bc[1] + g.D[1] * bc[2]);
+  const float* p = F + (size_t)B * SLAB + field * 3 * PAD + A * PAD + lp[0] + n0 * (lp[1] + n1 * lp[2]);
+  SG r = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int dd[3] = {k & 1, (k >> 1) & 1, k >> 2};
+    float f;
+    if (FAST) {
+      f = p[dd[0] + n0 * (dd[1] + n1 * dd[2])];
+    } else {
+      int q[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) q[d] = BASE * bc[d] + lp[d] - (d == A ? 0 : 1) + dd[d];
+      f = 0.f;
+      if (q[A] < g.N[A]) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          if (d != A) q[d] = iclamp(q[d], 0, g.N[d] - 1);
+        const int Bo = (q[0] >> LOGB) + g.D[0] * ((q[1] >> LOGB) + g.D[1] * (q[2] >> LOGB));
+        const int l[3] = {q[0] & 7, q[1] & 7, q[2] & 7};
+        f = F[(size_t)Bo * SLAB + field * 3 * PAD + A * PAD + pad_idx(A, l)];
+      }
+    }
+    const float w0 = dd[0] ? t[0] : 1.f - t[0], w1 = dd[1] ? t[1] : 1.f - t[1], w2 = dd[2] ? t[2] : 1.f - t[2];
+    r.s += w0 * w1 * w2 * f;
+    r.g0 += (dd[0] ? f : -f) * w1 * w2;
+    r.g1 += w0 * (dd[1] ? f : -f) * w2;
+    r.g2 += w0 * w1 * (dd[2] ? f : -f);
+  }
+  if (zero) return SG{0.f, 0.f, 0.f, 0.f};
+  r.g0 *= dm[0];
+  r.g1 *= dm[1];
+  r.g2 *= dm[2];
+  return r;
+}
+
+// the per-node path, out of line so that its registers do not weigh on the
+// callers' fast path
+template <int A>
+__device__ __attribute__((noinline)) SG sample_grad_slow(const GridDev& g, const float* __restrict__ F, int field,
+                                                         float u0, float u1, float u2) {
+  const float u[3] = {u0, u1, u2};
+  return sample_grad_impl<A, false>(g, F, field, u);
+}
+
+template <int A>
+__device__ __forceinline__ float sample_grad(const GridDev& g, const float* __restrict__ F, int field,
+                                             const float* u, const Near& nr, float* G) {
+  int bc[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float c = d == A ? fminf(fmaxf(u[d], 0.f), (float)g.N[d]) : fminf(fmaxf(u[d] - 0.5f, 0.f), (float)(g.N[d] - 1));
+    const int i = d == A ? min((int)floorf(c), g.N[d] - 1) : (int)floorf(c);
+    bc[d] = i >> LOGB;
+  }
+  bool fast = nr.own;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) fast = fast && abs(bc[d] - nr.bc[d]) <= 1;
+  if (!fast) fast = g.halo[bc[0] + g.D[0] * (bc[1] + g.D[1] * bc[2])] != 0;
+  const SG r = fast ? sample_grad_impl<A, true>(g, F, field, u) : sample_grad_slow<A>(g, F, field, u[0], u[1], u[2]);
+  G[0] = r.g0;
+  G[1] = r.g1;
+  G[2] = r.g2;
+  return r.s;
+}
+
+__device__ __forceinline__ float sample_grad(const GridDev& g, const float* __restrict__ F, int field, int a,
+                                             const float* u, const Near& nr, float* G) {
+  return a == 0 ? sample_grad<0>(g, F, field, u, nr, G)
+                : (a == 1 ? sample_grad<1>(g, F, field, u, nr, G) : sample_grad<2>(g, F, field, u, nr, G));
 }
 
 // FLIP/PIC blend (G2P.cuh:87-128) and Ralston RK3 (G2P.cuh:131-173) of
@@ -1089,7 +1235,7 @@ __device__ __forceinline__ void tile_scatter(const GridDev& g, float* tile, floa
     const float w = (dd[0] ? t[0] : 1.f - t[0]) * (dd[1] ? t[1] : 1.f - t[1]) * (dd[2] ? t[2] : 1.f - t[2]);
     const int t0 = q[0] - o[0], t1 = q[1] - o[1], t2 = q[2] - o[2];
     if ((unsigned)t0 < (unsigned)AT && (unsigned)t1 < (unsigned)AT && (unsigned)t2 < (unsigned)AT) {
-      atomicAdd(&tile[(field * 3 + a) * AT3 + t0 + AT * (t1 + AT * t2)], w * val);
+      lds_add_spread(&tile[(field * 3 + a) * AT3 + t0 + AT * (t1 + AT * t2)], w * val);
     } else {
       const int B = (q[0] >> LOGB) + g.D[0] * ((q[1] >> LOGB) + g.D[1] * (q[2] >> LOGB));
       const int l = (q[0] & 7) + 8 * ((q[1] & 7) + 8 * (q[2] & 7));
@@ -1123,7 +1269,13 @@ __global__ __launch_bounds__(TPB) void g2p_advect_adjoint_kernel(GridDev g, cons
   __syncthreads();
   const float umax[3] = {(float)g.N[0] - 0.0009765625f, (float)g.N[1] - 0.0009765625f,
                          (float)g.N[2] - 0.0009765625f};
-  for (uint32_t j = threadIdx.x; j < cnt; j += TPB) {
+  // each thread takes a contiguous run of the block's particles, so the lanes
+  // of a wave hold particles far apart in the sorted order (particles of one
+  // cell are often neighbours there) and their tile adds rarely meet on a word
+  const uint32_t per = (cnt + TPB - 1) / TPB;
+  for (uint32_t it = 0; it < per; ++it) {
+    const uint32_t j = threadIdx.x * per + it;
+    if (j >= cnt) break;
     const int64_t i = (int64_t)poff[B] + j;
     const int64_t id = sids[i];
     float x[3], u[3], m1[3], v1[3], vd[3];
@@ -1135,6 +1287,9 @@ __global__ __launch_bounds__(TPB) void g2p_advect_adjoint_kernel(GridDev g, cons
       m1[a] = (raw > 0.f && raw < umax[a]) ? 1.f : 0.f;
     }
     const Near nr = near_of(g, u);
+    // forward stages (values), then the backward pass, which samples the
+    // gradient of each stage's field (G[a][d] = d v_a / d u_d) when its
+    // partial is known
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       v1[a] = sample_comp(g, F, 0, a, u, nr);
@@ -1155,8 +1310,9 @@ __global__ __launch_bounds__(TPB) void g2p_advect_adjoint_kernel(GridDev g, cons
       u3[a] = fminf(fmaxf(raw, -1.f), (float)g.N[a] + 1.f);
       m3[a] = (raw > -1.f && raw < (float)g.N[a] + 1.f) ? 1.f : 0.f;
     }
+    float G[3][3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) v3[a] = sample_comp(g, F, 0, a, u3, nr);
+    for (int a = 0; a < 3; ++a) v3[a] = sample_grad(g, F, 0, a, u3, nr, G[a]);
     float k1[3], k2[3], k3[3], xacc[3], vo[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -1170,40 +1326,38 @@ __global__ __launch_bounds__(TPB) void g2p_advect_adjoint_kernel(GridDev g, cons
       xacc[a] = xb;
     }
     // stage 3: v3 = S(Fnew)(x3), x3 = x + 3/4 dt v2
-    float gr[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      tile_scatter(g, tile, fbar, o, 0, a, u3, k3[a]);
-      if (k3[a] != 0.f) sample_vjp(g, F, SLAB, 0, a, u3, k3[a], gr);
-    }
+    for (int a = 0; a < 3; ++a) tile_scatter(g, tile, fbar, o, 0, a, u3, k3[a]);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const float b = gr[d] * g.inv_h * m3[d];
+      const float b = (k3[0] * G[0][d] + k3[1] * G[1][d] + k3[2] * G[2][d]) * g.inv_h * m3[d];
       xacc[d] += b;
       k2[d] += 0.75f * dt * b;
-      gr[d] = 0.f;
     }
     // stage 2: v2 = S(Fnew)(x2), x2 = x + 1/2 dt v1
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       tile_scatter(g, tile, fbar, o, 0, a, u2, k2[a]);
-      if (k2[a] != 0.f) sample_vjp(g, F, SLAB, 0, a, u2, k2[a], gr);
+      (void)sample_grad(g, F, 0, a, u2, nr, G[a]);
     }
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const float b = gr[d] * g.inv_h * m2[d];
+      const float b = (k2[0] * G[0][d] + k2[1] * G[1][d] + k2[2] * G[2][d]) * g.inv_h * m2[d];
       xacc[d] += b;
       k1[d] += 0.5f * dt * b;
-      gr[d] = 0.f;
     }
     // FLIP/PIC blend and stage 1 at x
+    float gr[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const float vnb = k1[a] + (1.f - alpha) * vo[a], vdb = alpha * vo[a];
       tile_scatter(g, tile, fbar, o, 0, a, u, vnb + vdb);
       tile_scatter(g, tile, fbar, o, 1, a, u, vdb);
-      if (vnb != 0.f) sample_vjp(g, F, SLAB, 0, a, u, vnb, gr);
-      if (vdb != 0.f) sample_vjp(g, F, SLAB, 1, a, u, vdb, gr);
+      float Gn[3], Gd[3];
+      (void)sample_grad(g, F, 0, a, u, nr, Gn);
+      (void)sample_grad(g, F, 1, a, u, nr, Gd);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) gr[d] += vnb * Gn[d] + vdb * Gd[d];
     }
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -1247,6 +1401,11 @@ __global__ __launch_bounds__(TPB) void pi_transpose_kernel(GridDev g, const floa
     int slot[8];
     float w[8];
     const int nw = pi_weights(a, e, l, slot, w);
+    if (e == 0) {
+      nA[a * 576 + slot[0]] += w[0] * fa;
+      nB[a * 576 + slot[0]] += w[0] * fb;
+      continue;
+    }
     for (int j = 0; j < nw; ++j) {
       atomicAdd(&nA[a * 576 + slot[j]], w[j] * fa);
       atomicAdd(&nB[a * 576 + slot[j]], w[j] * fb);
