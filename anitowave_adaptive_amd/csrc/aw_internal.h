@@ -101,6 +101,7 @@ struct TailLevel {
   int loff;              // byte offset of the liquid mask after the float arrays
   int ulr;               // log2 block resolution when every block has it (arithmetic stencil), else -1
   int ue;                // log2 cell edge of such a uniform level
+  int sh[3];             // log2 dims when the level is a lattice of 1^3 blocks with power-of-two dims, else -1
 };
 
 struct TailArgs {

@@ -2163,8 +2163,37 @@ __device__ __forceinline__ int generic_class(const TailLevel& T, int i, CellRef&
   return (((Bc[0] + Bc[1] + Bc[2]) & 1) << 1) | ((c.l[0] + c.l[1] + c.l[2]) & 1);
 }
 
+// Lattice levels (one-cell blocks, power-of-two dims): the uniform stencil
+// from the cell's coordinates by shifts, neighbours in the order of
+// uniform_stencil (same sums).
+__device__ __forceinline__ float lattice_ax(const TailLevel& T, const float* xs, int i, int x, int y, int z) {
+  const int D0 = 1 << T.sh[0], D1 = 1 << T.sh[1], D2 = 1 << T.sh[2];
+  const int sy = D0, sz = D0 * D1;
+  const float s = (float)(1 << T.ue);
+  float off = 0.f;
+  int n = 0;
+  if (x > 0) { off += s * xs[i - 1]; ++n; }
+  if (x + 1 < D0) { off += s * xs[i + 1]; ++n; }
+  if (y > 0) { off += s * xs[i - sy]; ++n; }
+  if (y + 1 < D1) { off += s * xs[i + sy]; ++n; }
+  if (z > 0) { off += s * xs[i - sz]; ++n; }
+  if (z + 1 < D2) { off += s * xs[i + sz]; ++n; }
+  return s * (float)n * xs[i] - off;
+}
+
+__device__ __forceinline__ void lattice_xyz(const TailLevel& T, int i, int& x, int& y, int& z) {
+  x = i & ((1 << T.sh[0]) - 1);
+  y = (i >> T.sh[0]) & ((1 << T.sh[1]) - 1);
+  z = i >> (T.sh[0] + T.sh[1]);
+}
+
 // A x (own row) of cell i on a tail level, from LDS values xs
 __device__ __forceinline__ float tail_ax(const TailLevel& T, const float* xs, int i, int* cls_out) {
+  if (T.sh[0] >= 0 && !cls_out) {
+    int x, y, z;
+    lattice_xyz(T, i, x, y, z);
+    return lattice_ax(T, xs, i, x, y, z);
+  }
   if (T.ulr >= 0) {
     UStencil u;
     uniform_stencil(T, i, u);
@@ -2200,6 +2229,28 @@ __device__ __forceinline__ int uniform_class(const TailLevel& T, int i) {
 // the cells that update.
 __device__ __forceinline__ void tail_update(const TailLevel& T, float* xs, const float* bs, const uint8_t* liq,
                                             int want, float omega) {
+  if (T.sh[0] >= 0) {
+    // lattice: the cells of block colour want >> 1 directly, two per row
+    // pair (x even or odd with the row's parity); one-cell blocks have only
+    // cell colour 0
+    if (want & 1) return;
+    const int bcol = want >> 1;
+    const int hs = T.sh[0] - 1;
+    for (int t = threadIdx.x; t < (T.ncells >> 1); t += TAIL_T) {
+      const int row = t >> hs;
+      const int y = row & ((1 << T.sh[1]) - 1), z = row >> T.sh[1];
+      const int x = 2 * (t & ((1 << hs) - 1)) + ((bcol + y + z) & 1);
+      const int i = x + (row << T.sh[0]);
+      if (!liq[i]) continue;
+      const float s = (float)(1 << T.ue);
+      const int n = (x > 0) + (x + 1 < (1 << T.sh[0])) + (y > 0) + (y + 1 < (1 << T.sh[1])) + (z > 0) +
+                    (z + 1 < (1 << T.sh[2]));
+      const float dg = s * (float)n;
+      const float ax = lattice_ax(T, xs, i, x, y, z);
+      if (dg > 0.f) xs[i] += omega * (bs[i] - ax) / dg;
+    }
+    return;
+  }
   for (int i = threadIdx.x; i < T.ncells; i += TAIL_T) {
     if (!liq[i]) continue;
     if (T.ulr >= 0) {
@@ -2268,7 +2319,22 @@ __global__ __launch_bounds__(TAIL_T) void tail_kernel(TailArgs A) {
     float* rs = sm + F.off + 2 * F.ncells;
     for (int i = threadIdx.x; i < F.ncells; i += TAIL_T) rs[i] = fl[i] ? bs[i] - tail_ax(F, xs, i, nullptr) : 0.f;
     __syncthreads();
+    const bool lat = C.agg && F.sh[0] >= 0 && C.sh[0] >= 0;
     for (int i = threadIdx.x; i < C.ncells; i += TAIL_T) {
+      if (lat) {
+        // children of lattice cell (X, Y, Z): (2X + dx, 2Y + dy, 2Z + dz), in children_sum's order
+        int X, Y, Z;
+        lattice_xyz(C, i, X, Y, Z);
+        float s = 0.f;
+        for (int d = 0; d < 8; ++d) {
+          const int q = (2 * X + (d & 1)) + ((2 * Y + ((d >> 1) & 1)) << F.sh[0]) +
+                        ((2 * Z + (d >> 2)) << (F.sh[0] + F.sh[1]));
+          s += rs[q];
+        }
+        sm[C.off + C.ncells + i] = s;
+        sm[C.off + i] = 0.f;
+        continue;
+      }
       const int B = C.cblk[i];
       CellRef c;
       cell_ref(C.L, B, i - C.L.coff[B], c);
@@ -2299,8 +2365,15 @@ __global__ __launch_bounds__(TAIL_T) void tail_kernel(TailArgs A) {
     const TailLevel& F = A.lv[t];
     const TailLevel& C = A.lv[t + 1];
     const uint8_t* fl = lq + F.loff;
+    const bool lat = C.agg && F.sh[0] >= 0 && C.sh[0] >= 0;
     for (int i = threadIdx.x; i < F.ncells; i += TAIL_T) {
       if (!fl[i]) continue;
+      if (lat) {
+        int x, y, z;
+        lattice_xyz(F, i, x, y, z);
+        sm[F.off + i] += A.scale * sm[C.off + (x >> 1) + ((y >> 1) << C.sh[0]) + ((z >> 1) << (C.sh[0] + C.sh[1]))];
+        continue;
+      }
       const int B = F.cblk[i];
       CellRef c;
       cell_ref(F.L, B, i - F.L.coff[B], c);

@@ -800,6 +800,20 @@ void tail_vcycle(aw_solver* S, int t0, float* x, const float* b) {
         T.ue = L.h_e[0];
       }
     }
+    // a lattice of one-cell blocks with power-of-two dims (cell index = block
+    // index): neighbours, colours and parents by shifts
+    for (int d = 0; d < 3; ++d) T.sh[d] = -1;
+    if (T.ulr == 0) {
+      int sh[3];
+      bool p2 = true;
+      for (int d = 0; d < 3; ++d) {
+        sh[d] = 0;
+        while ((1 << sh[d]) < L.D[d]) ++sh[d];
+        p2 = p2 && (1 << sh[d]) == L.D[d];
+      }
+      if (p2 && sh[0] >= 1)
+        for (int d = 0; d < 3; ++d) T.sh[d] = sh[d];
+    }
   }
   A.fl = off;
   int loff = 0;
