@@ -178,6 +178,10 @@ void k_ids_range(hipStream_t s, const int32_t* ids, int64_t n, int* out);  // ou
 // solver
 void k_smooth(hipStream_t s, const LevelDev& L, float* x, const float* b, const WGMap& wg,
               const uint32_t* blist, float omega, int local_sweeps, int reverse);
+// the same sweep on a uniform coarse level with power-of-two block dims sh
+// (log2), block resolution 1 << lr and cell edge 1 << e, colour col
+void k_smooth_uniform(hipStream_t s, const int* sh, int lr, int e, const uint8_t* liquid, float* x, const float* b,
+                      int col, float omega, int local_sweeps, int reverse);
 void k_residual(hipStream_t s, const LevelDev& L, const float* x, const float* b, float* r, const WGMap& wg,
                 const uint32_t* blist, const Restr& rs);
 void k_apply_dot(hipStream_t s, const LevelDev& L, const float* x, float* y, const WGMap& wg,

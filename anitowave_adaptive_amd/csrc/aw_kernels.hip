@@ -1948,6 +1948,99 @@ __global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void smooth_kernel(
 }
 
 
+// smooth_kernel for a uniform coarse level (every block of resolution
+// 1 << lr, beta 1 inside the tank and 0 on its walls, power-of-two block
+// dims): the blocks of one colour are enumerated and every neighbour address
+// and coefficient comes from the cell's coordinates, so the only loads are
+// x, b and the liquid mask.  Same sums, in the same order, as smooth_kernel.
+struct ULevel {
+  int sh[3];   // log2 block dims
+  int lr, e;   // log2 block resolution, log2 cell edge
+};
+
+__global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void uniform_smooth_kernel(
+    ULevel U, const uint8_t* __restrict__ liquid, float* __restrict__ x, const float* __restrict__ b, int col,
+    float omega, int local_sweeps, int reverse) {
+  __shared__ float sx[512];
+  const int lr = U.lr, r = 1 << lr, sh3 = 3 * lr;
+  const int D[3] = {1 << U.sh[0], 1 << U.sh[1], 1 << U.sh[2]};
+  const int ncol = (D[0] * D[1] * D[2]) >> 1;       // blocks of one colour
+  const int per = 512 >> sh3;                        // blocks per work-group
+  const float s = (float)(1 << U.e);
+  int c[2], lo[2][3];
+  bool act[2];
+  float bb[2], dg[2], ex[2];
+  uint8_t inb[2];   // bit d: the face-d neighbour is in the block
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int i = threadIdx.x + k * TPB;
+    act[k] = false;
+    c[k] = -1;
+    const int kb = blockIdx.x * per + (i >> sh3);
+    if (kb >= ncol) continue;
+    const int hs = U.sh[0] - 1;
+    const int row = kb >> hs;
+    const int by = row & (D[1] - 1), bz = row >> U.sh[1];
+    const int bx = 2 * (kb & ((1 << hs) - 1)) + ((col + by + bz) & 1);
+    const int B = bx + (row << U.sh[0]);
+    const int li = i & ((1 << sh3) - 1);
+    const int l[3] = {li & (r - 1), (li >> lr) & (r - 1), li >> (2 * lr)};
+    const int Bc[3] = {bx, by, bz};
+    c[k] = (B << sh3) + li;
+    sx[i] = x[c[k]];
+    if (!liquid[c[k]]) continue;
+    act[k] = true;
+    bb[k] = b[c[k]];
+    float dgv = 0.f, exv = 0.f;
+    uint8_t m = 0;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+      const int a = d >> 1, side = d & 1;
+      int G[3] = {Bc[0] * r + l[0], Bc[1] * r + l[1], Bc[2] * r + l[2]};
+      G[a] += side ? 1 : -1;
+      if (G[a] < 0 || G[a] >= D[a] * r) continue;   // tank wall: beta 0
+      dgv += s;
+      const int nl = l[a] + (side ? 1 : -1);
+      if (nl >= 0 && nl < r) {
+        m |= (uint8_t)(1 << d);
+        continue;
+      }
+      const int q[3] = {G[0] >> lr, G[1] >> lr, G[2] >> lr};
+      const int lq[3] = {G[0] & (r - 1), G[1] & (r - 1), G[2] & (r - 1)};
+      exv += s * x[((q[0] + D[0] * (q[1] + D[1] * q[2])) << sh3) + cell_idx(lq, r)];
+    }
+    dg[k] = dgv;
+    ex[k] = exv;
+    inb[k] = m;
+    lo[k][0] = l[0];
+    lo[k][1] = l[1];
+    lo[k][2] = l[2];
+  }
+  __syncthreads();
+  const int st[3] = {1, r, r * r};
+  for (int ls = 0; ls < local_sweeps; ++ls)
+    for (int ph = 0; ph < 2; ++ph) {
+      const int cc = reverse ? 1 - ph : ph;
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        if (!act[k]) continue;
+        if (((lo[k][0] + lo[k][1] + lo[k][2]) & 1) != cc) continue;
+        const int i = threadIdx.x + k * TPB;
+        float in = 0.f;
+#pragma unroll
+        for (int d = 0; d < 6; ++d)
+          if (inb[k] & (1 << d)) in += s * sx[i + ((d & 1) ? st[d >> 1] : -st[d >> 1])];
+        const float off = ex[k] + in;
+        const float res = bb[k] - (dg[k] * sx[i] - off);
+        if (dg[k] > 0.f) sx[i] += omega * res / dg[k];
+      }
+      __syncthreads();
+    }
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+    if (act[k]) x[c[k]] = sx[threadIdx.x + k * TPB];
+}
+
 // r = b - A x (computeExplicitResidual, VCycle.cuh:81-131), 0 on air cells;
 // optionally fused with the restriction b_c = sum of children
 // (restrictResidualsKernel, MultigridTransfer.cuh:9-59) and x_c = 0.
@@ -2746,6 +2839,19 @@ void k_masked_copy(hipStream_t s, float* dst, const float* src, const uint8_t* l
 void k_smooth(hipStream_t s, const LevelDev& L, float* x, const float* b, const WGMap& wg,
               const uint32_t* blist, float omega, int local_sweeps, int reverse) {
   if (wg.n > 0) smooth_kernel<<<wg.n, TPB, 0, s>>>(L, x, b, wg, blist, omega, local_sweeps, reverse);
+}
+
+void k_smooth_uniform(hipStream_t s, const int* sh, int lr, int e, const uint8_t* liquid, float* x, const float* b,
+                      int col, float omega, int local_sweeps, int reverse) {
+  ULevel U;
+  for (int d = 0; d < 3; ++d) U.sh[d] = sh[d];
+  U.lr = lr;
+  U.e = e;
+  const int ncol = (1 << (sh[0] + sh[1] + sh[2])) >> 1;
+  const int per = 512 >> (3 * lr);
+  const int nwg = (ncol + per - 1) / per;
+  if (nwg > 0)
+    uniform_smooth_kernel<<<nwg, TPB, 0, s>>>(U, liquid, x, b, col, omega, local_sweeps, reverse);
 }
 
 void k_residual(hipStream_t s, const LevelDev& L, const float* x, const float* b, float* r, const WGMap& wg,

@@ -245,6 +245,10 @@ struct MGLevel {
   std::vector<int32_t> h_nbr;
   int nbr_dims[3] = {0, 0, 0};   // dims h_nbr was built for
   bool valid = false;            // tables and device buffers built for h_e
+  // a coarse level with one block resolution and power-of-two dims: log2
+  // dims (ush[0] < 0 otherwise), log2 resolution and cell edge
+  int ush[3] = {-1, -1, -1};
+  int ulr = 0, ue = 0;
   DevBuf e, coff, foff, nbr, liquid, beta[3], x, b, r, blist, cblk, bdesc;
   WGMap wgm[3]{};  // stencil launches over block colour 0, colour 1, both
   // algorithmic bytes of one stencil pass over colour 0, colour 1 (x read +
@@ -595,6 +599,24 @@ int upload_level(hipStream_t s, MGLevel& L, bool own_topology) {
     AW_TRY(L.cblk.ensure(cb.size() * sizeof(uint32_t)));
     AW_HIP(hipMemcpyAsync(L.cblk.p, cb.data(), cb.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
   }
+  L.ush[0] = L.ush[1] = L.ush[2] = -1;
+  if (own_topology) {
+    bool uni = true, p2 = true;
+    for (uint8_t e : L.h_e) uni = uni && e == L.h_e[0];
+    int sh[3];
+    for (int d = 0; d < 3; ++d) {
+      sh[d] = 0;
+      while ((1 << sh[d]) < L.D[d]) ++sh[d];
+      p2 = p2 && (1 << sh[d]) == L.D[d];
+    }
+    if (uni && p2 && sh[0] >= 1) {
+      int r = L.bs >> L.h_e[0], lr = 0;
+      while ((1 << lr) < r) ++lr;
+      for (int d = 0; d < 3; ++d) L.ush[d] = sh[d];
+      L.ulr = lr;
+      L.ue = L.h_e[0];
+    }
+  }
   if (!own_topology) return AW_OK;
   const std::vector<int32_t>& nbr = L.h_nbr;
   AW_TRY(L.e.ensure(L.nb));
@@ -765,7 +787,11 @@ void smooth_level(aw_solver* S, int k, float* x, const float* b, int iters, bool
     for (int ph = 0; ph < 2; ++ph) {
       const int col = reverse ? 1 - ph : ph;
       Prof pf(S->g->ctx, k == 0 ? "smooth_l0" : "smooth_coarse", L.sweep_bytes[col]);
-      k_smooth(s, V, x, b, L.wgm[col], L.bl(), S->prm.omega, S->prm.local_sweeps, reverse ? 1 : 0);
+      if (k > 0 && L.ush[0] >= 0)
+        k_smooth_uniform(s, L.ush, L.ulr, L.ue, L.d_liq, x, b, col, S->prm.omega, S->prm.local_sweeps,
+                         reverse ? 1 : 0);
+      else
+        k_smooth(s, V, x, b, L.wgm[col], L.bl(), S->prm.omega, S->prm.local_sweeps, reverse ? 1 : 0);
     }
 }
 
