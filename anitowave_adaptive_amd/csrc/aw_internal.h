@@ -131,9 +131,11 @@ void k_fill_f64(hipStream_t s, double* p, size_t n, double v);
 void k_bin(hipStream_t s, const GridDev& g, const float* pos, int64_t n, int32_t* key, uint32_t* count);
 void k_levels(hipStream_t s, const int* D, int L, const int* band, const uint32_t* count, uint8_t* surf,
               uint8_t* level, uint8_t* tmp);
-void k_scan_u32(hipStream_t s, const uint32_t* in, uint32_t* out, int n);  // exclusive, out[n] = total
+// exclusive scans, out[n] = total; `scratch` holds scan_scratch_words(n) words
+size_t scan_scratch_words(int n);
+void k_scan_u32(hipStream_t s, const uint32_t* in, uint32_t* out, int n, uint32_t* scratch);
 void k_layout_offsets(hipStream_t s, const uint8_t* level, int nb, uint32_t* coff, uint32_t* fx, uint32_t* fy,
-                      uint32_t* fz);
+                      uint32_t* fz, uint32_t* scratch);
 void k_scatter(hipStream_t s, const float* pos, const float* vel, const int32_t* ids, const int32_t* key,
                uint32_t* cursor, int64_t n, float* spos, float* svel, int32_t* sids);
 void k_init_beta(hipStream_t s, const LevelDev& L, float* bx, float* by, float* bz);

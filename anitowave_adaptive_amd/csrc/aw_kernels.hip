@@ -171,48 +171,91 @@ inline int grid_for(size_t n) {
 }
 
 // ------------------------------------------------------------- topology
-// Runs of equal keys in consecutive lanes of a wave share one atomic: the
-// run's first lane adds the run length, every lane gets base + rank.  Input
-// that is already grouped by key (particles sorted by block in the previous
-// step) then issues about one atomic per wave instead of 64 on one address.
-// All lanes of the wave must call it; inactive lanes pass active = false.
-__device__ __forceinline__ uint32_t run_atomic_add(uint32_t* counters, int key, bool active, bool want_slot) {
+// Work-group aggregation of per-block atomics for the binning and sorting
+// passes.  Particles arrive mostly grouped by block (each step leaves them
+// sorted), so the lanes of a wave form runs of one key (one atomic per run),
+// and the runs of a work-group's AGG_PPT * TPB particles fall into a few
+// keys: each key takes an LDS slot (open addressing), the runs add their
+// lengths there, and one global atomic per key and work-group reserves the
+// work-group's range.  Without the LDS step every wave of a block's segment
+// hits the same global counter, which serialises.  A key that finds no free
+// slot within AGG_PROBE probes takes the global atomic directly.
+constexpr int AGG_SLOTS = 512;
+constexpr int AGG_PROBE = 16;
+constexpr int AGG_PPT = 8;
+
+__device__ __forceinline__ int agg_slot(int* hk, int key) {
+  const uint32_t h = ((uint32_t)key * 2654435761u) >> 23;   // 9 bits
+#pragma unroll 1
+  for (int p = 0; p < AGG_PROBE; ++p) {
+    const int s = (int)((h + p) & (AGG_SLOTS - 1));
+    const int cur = __hip_atomic_load(&hk[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (cur == key) return s;
+    if (cur == -1) {
+      const int old = atomicCAS(&hk[s], -1, key);
+      if (old == -1 || old == key) return s;
+    }
+  }
+  return -1;
+}
+
+// lane's run within its wave: head flag, first lane and length of the run
+__device__ __forceinline__ void wave_run(int key, bool active, bool& head, int& start, int& len) {
   const int lane = threadIdx.x & 63;
   const int k = active ? key : -1;
   const int prev = __shfl_up(k, 1, 64);
-  const bool head = active && (lane == 0 || prev != k);
+  head = active && (lane == 0 || prev != k);
   const unsigned long long heads = __ballot(head);
   const unsigned long long act = __ballot(active);
   const unsigned long long le = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
   const unsigned long long my_heads = heads & le;
-  const int start = my_heads ? 63 - __clzll(my_heads) : lane;
+  start = my_heads ? 63 - __clzll(my_heads) : lane;
   const unsigned long long stop = (heads | ~act) & ~le;
   const int end = stop ? __ffsll(stop) - 1 : 64;
-  uint32_t base = 0;
-  if (head) base = atomicAdd(&counters[k], (uint32_t)(end - start));
-  if (!want_slot) return 0;
-  base = __shfl(base, start, 64);
-  return base + (uint32_t)(lane - start);
+  len = end - start;
 }
 
 // Block of each particle from its exact fine cell (worldToBlockCoord,
 // MSBG.cuh:151-159) and the per-block particle count.
 __global__ __launch_bounds__(TPB) void bin_kernel(GridDev g, const float* __restrict__ pos, int64_t n,
                                                   int32_t* __restrict__ key, uint32_t* __restrict__ count) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = i < n;
-  int b = 0;
-  if (active) {
-    int c[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      float u = fminf(fmaxf(pos[3 * i + a] * g.inv_h, 0.f), (float)g.N[a] - 0.0009765625f);
-      c[a] = ((int)u) >> LOGB;
-    }
-    b = block_id(g.D, c);
-    key[i] = b;
+  __shared__ int hk[AGG_SLOTS];
+  __shared__ uint32_t hc[AGG_SLOTS];
+  for (int s = threadIdx.x; s < AGG_SLOTS; s += TPB) {
+    hk[s] = -1;
+    hc[s] = 0;
   }
-  run_atomic_add(count, b, active, false);
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * (TPB * AGG_PPT);
+#pragma unroll
+  for (int j = 0; j < AGG_PPT; ++j) {
+    const int64_t i = base + j * TPB + threadIdx.x;
+    const bool active = i < n;
+    int b = 0;
+    if (active) {
+      int c[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        float u = fminf(fmaxf(pos[3 * i + a] * g.inv_h, 0.f), (float)g.N[a] - 0.0009765625f);
+        c[a] = ((int)u) >> LOGB;
+      }
+      b = block_id(g.D, c);
+      key[i] = b;
+    }
+    bool head;
+    int start, len;
+    wave_run(b, active, head, start, len);
+    if (head) {
+      const int sl = agg_slot(hk, b);
+      if (sl >= 0)
+        atomicAdd(&hc[sl], (uint32_t)len);
+      else
+        atomicAdd(&count[b], (uint32_t)len);
+    }
+  }
+  __syncthreads();
+  for (int s = threadIdx.x; s < AGG_SLOTS; s += TPB)
+    if (hk[s] >= 0) atomicAdd(&count[hk[s]], hc[s]);
 }
 
 // occupied block with an unoccupied in-domain 26-neighbour
@@ -239,28 +282,42 @@ struct Bands {
   int v[4];
 };
 
-// first level whose band (Chebyshev block distance to a surface block) holds the block
-__global__ void band_kernel(int D0, int D1, int D2, int L, Bands band, int bmax, const uint8_t* __restrict__ surf,
-                            uint8_t* __restrict__ level) {
+// Level of each block: the first level whose band holds the block's
+// Chebyshev (L-infinity) block distance to the nearest surface block.  The
+// distance within the largest band bmax is separable: one pass per axis,
+// out(b) = min over |k| <= bmax of max(|k|, in(b + k e_axis)), the first
+// pass seeded with 0 at surface blocks and CHEB_FAR elsewhere (distances
+// beyond bmax stay CHEB_FAR and fall to the last level).  The last pass
+// writes the level instead of the distance.  3 (2 bmax + 1) reads per block
+// instead of (2 bmax + 1)^3.
+constexpr int CHEB_FAR = 255;
+
+__global__ void cheb_pass_kernel(int D0, int D1, int D2, int axis, int bmax, int seed, int L, Bands band,
+                                 const uint8_t* __restrict__ in, uint8_t* __restrict__ out) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   int D[3] = {D0, D1, D2};
   if (b >= D0 * D1 * D2) return;
   int c[3];
   block_coords(D, b, c);
-  int dist = 1 << 20;
-  for (int z = max(0, c[2] - bmax); z <= min(D2 - 1, c[2] + bmax); ++z)
-    for (int y = max(0, c[1] - bmax); y <= min(D1 - 1, c[1] + bmax); ++y)
-      for (int x = max(0, c[0] - bmax); x <= min(D0 - 1, c[0] + bmax); ++x) {
-        int q[3] = {x, y, z};
-        if (surf[block_id(D, q)]) dist = min(dist, max(abs(x - c[0]), max(abs(y - c[1]), abs(z - c[2]))));
-      }
+  const int stride = axis == 0 ? 1 : (axis == 1 ? D0 : D0 * D1);
+  const int lo = max(0, c[axis] - bmax), hi = min(D[axis] - 1, c[axis] + bmax);
+  int dist = CHEB_FAR;
+  for (int q = lo; q <= hi; ++q) {
+    const int v = in[b + (q - c[axis]) * stride];
+    const int d = seed ? (v ? 0 : CHEB_FAR) : v;
+    dist = min(dist, max(abs(q - c[axis]), d));
+  }
+  if (L == 0) {
+    out[b] = (uint8_t)dist;
+    return;
+  }
   int l = L - 1;
   for (int k = 0; k < L - 1; ++k)
     if (dist <= band.v[k]) {
       l = k;
       break;
     }
-  level[b] = (uint8_t)l;
+  out[b] = (uint8_t)l;
 }
 
 // one Jacobi pass of 2:1 grading: l = min(l, min_n l_n + 1) over 26 neighbours
@@ -330,22 +387,128 @@ __global__ __launch_bounds__(SCAN_T) void scan_kernel(const uint32_t* __restrict
   if (t == 0) out[n] = carry;
 }
 
+// Many-work-group form of the same scan for long inputs: chunk totals, an
+// exclusive scan of the totals (scan_kernel), then each chunk rescans itself
+// from its base.  Chunks of SCAN_CH values, SCAN_VPT consecutive per thread.
+constexpr int SCAN_VPT = 16;
+constexpr int SCAN_CH = TPB * SCAN_VPT;
+
+__device__ __forceinline__ uint32_t scan_value(const uint32_t* in, const uint8_t* lv, int mode, int i) {
+  if (mode == 0) return in[i];
+  const int r = BASE >> lv[i];
+  return mode == 1 ? (uint32_t)(r * r * r) : (uint32_t)((r + 1) * r * r);
+}
+
+// sum over a work-group of one value per thread, returned to every thread,
+// and the exclusive prefix of the thread's value
+__device__ __forceinline__ uint32_t wg_exclusive(uint32_t v, uint32_t* wsum, uint32_t& total) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  uint32_t incl = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += y;
+  }
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  uint32_t base = 0;
+  total = 0;
+#pragma unroll
+  for (int k = 0; k < TPB / 64; ++k) {
+    base += k < w ? wsum[k] : 0u;
+    total += wsum[k];
+  }
+  return base + incl - v;
+}
+
+__global__ __launch_bounds__(TPB) void scan_reduce_kernel(const uint32_t* __restrict__ in,
+                                                          const uint8_t* __restrict__ lv, int mode, int n,
+                                                          uint32_t* __restrict__ sums) {
+  __shared__ uint32_t wsum[TPB / 64];
+  const int base = blockIdx.x * SCAN_CH;
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_VPT; ++k) {
+    const int i = base + k * TPB + threadIdx.x;
+    if (i < n) s += scan_value(in, lv, mode, i);
+  }
+  uint32_t total;
+  wg_exclusive(s, wsum, total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(TPB) void scan_final_kernel(const uint32_t* __restrict__ in,
+                                                         const uint8_t* __restrict__ lv, int mode, int n,
+                                                         const uint32_t* __restrict__ cbase, int nchunks,
+                                                         uint32_t* __restrict__ out) {
+  __shared__ uint32_t wsum[TPB / 64];
+  const int i0 = blockIdx.x * SCAN_CH + threadIdx.x * SCAN_VPT;
+  uint32_t v[SCAN_VPT], s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_VPT; ++k) {
+    v[k] = i0 + k < n ? scan_value(in, lv, mode, i0 + k) : 0u;
+    s += v[k];
+  }
+  uint32_t total;
+  uint32_t e = cbase[blockIdx.x] + wg_exclusive(s, wsum, total);
+#pragma unroll
+  for (int k = 0; k < SCAN_VPT; ++k) {
+    if (i0 + k < n) out[i0 + k] = e;
+    e += v[k];
+  }
+  if (blockIdx.x == nchunks - 1 && threadIdx.x == 0) out[n] = cbase[nchunks];
+}
+
 __global__ __launch_bounds__(TPB) void scatter_kernel(const float* __restrict__ pos, const float* __restrict__ vel,
                                                       const int32_t* __restrict__ ids,
                                                       const int32_t* __restrict__ key, uint32_t* __restrict__ cursor,
                                                       int64_t n, float* __restrict__ spos, float* __restrict__ svel,
                                                       int32_t* __restrict__ sids) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = i < n;
-  const int k = active ? key[i] : 0;
-  const uint32_t s = run_atomic_add(cursor, k, active, true);
-  if (!active) return;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    spos[3 * (int64_t)s + a] = pos[3 * i + a];
-    svel[3 * (int64_t)s + a] = vel[3 * i + a];
+  __shared__ int hk[AGG_SLOTS];
+  __shared__ uint32_t hc[AGG_SLOTS];
+  for (int s = threadIdx.x; s < AGG_SLOTS; s += TPB) {
+    hk[s] = -1;
+    hc[s] = 0;
   }
-  if (ids) sids[s] = ids[i];
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * (TPB * AGG_PPT);
+  const int lane = threadIdx.x & 63;
+  int slot[AGG_PPT];
+  uint32_t off[AGG_PPT];   // offset within the slot's range, or the global position (slot -1)
+#pragma unroll
+  for (int j = 0; j < AGG_PPT; ++j) {
+    const int64_t i = base + j * TPB + threadIdx.x;
+    const bool active = i < n;
+    const int k = active ? key[i] : 0;
+    bool head;
+    int start, len;
+    wave_run(k, active, head, start, len);
+    int sl = -1;
+    uint32_t b = 0;
+    if (head) {
+      sl = agg_slot(hk, k);
+      b = sl >= 0 ? atomicAdd(&hc[sl], (uint32_t)len) : atomicAdd(&cursor[k], (uint32_t)len);
+    }
+    slot[j] = __shfl(sl, start, 64);
+    off[j] = __shfl(b, start, 64) + (uint32_t)(lane - start);
+  }
+  __syncthreads();
+  // one global reservation per key; hc becomes the key's first position
+  for (int s = threadIdx.x; s < AGG_SLOTS; s += TPB)
+    if (hk[s] >= 0) hc[s] = atomicAdd(&cursor[hk[s]], hc[s]);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < AGG_PPT; ++j) {
+    const int64_t i = base + j * TPB + threadIdx.x;
+    if (i >= n) break;
+    const int64_t s = slot[j] >= 0 ? (int64_t)hc[slot[j]] + off[j] : (int64_t)off[j];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      spos[3 * s + a] = pos[3 * i + a];
+      svel[3 * s + a] = vel[3 * i + a];
+    }
+    if (ids) sids[s] = ids[i];
+  }
 }
 
 // beta = 1 on interior slots, 0 on tank-wall slots
@@ -2639,7 +2802,8 @@ void k_fill_f64(hipStream_t s, double* p, size_t n, double v) {
 }
 
 void k_bin(hipStream_t s, const GridDev& g, const float* pos, int64_t n, int32_t* key, uint32_t* count) {
-  if (n > 0) bin_kernel<<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, s>>>(g, pos, n, key, count);
+  if (n > 0)
+    bin_kernel<<<(unsigned)((n + TPB * AGG_PPT - 1) / (TPB * AGG_PPT)), TPB, 0, s>>>(g, pos, n, key, count);
 }
 
 void k_levels(hipStream_t s, const int* D, int L, const int* band, const uint32_t* count, uint8_t* surf,
@@ -2651,21 +2815,44 @@ void k_levels(hipStream_t s, const int* D, int L, const int* band, const uint32_
   int bmax = 0;
   for (int i = 0; i < 4; ++i) bd.v[i] = band[i];
   for (int i = 0; i < L - 1; ++i) bmax = band[i] > bmax ? band[i] : bmax;
-  band_kernel<<<gr, TPB, 0, s>>>(D[0], D[1], D[2], L, bd, bmax, surf, level);
+  // z, y, x passes: surf -> tmp -> level -> tmp (levels), then back into level
+  cheb_pass_kernel<<<gr, TPB, 0, s>>>(D[0], D[1], D[2], 2, bmax, 1, 0, bd, surf, tmp);
+  cheb_pass_kernel<<<gr, TPB, 0, s>>>(D[0], D[1], D[2], 1, bmax, 0, 0, bd, tmp, level);
+  cheb_pass_kernel<<<gr, TPB, 0, s>>>(D[0], D[1], D[2], 0, bmax, 0, L, bd, level, tmp);
+  hipMemcpyAsync(level, tmp, nb, hipMemcpyDeviceToDevice, s);
   for (int it = 0; it < L - 1; ++it) {
     grade_kernel<<<gr, TPB, 0, s>>>(D[0], D[1], D[2], level, tmp);
     hipMemcpyAsync(level, tmp, nb, hipMemcpyDeviceToDevice, s);
   }
 }
 
-void k_scan_u32(hipStream_t s, const uint32_t* in, uint32_t* out, int n) {
-  scan_kernel<<<1, SCAN_T, 0, s>>>(in, nullptr, 0, out, n);
+size_t scan_scratch_words(int n) { return 2 * (size_t)(n / SCAN_CH + 2); }
+
+// exclusive scan of n values (in, or cells / face slots from levels) into
+// out[0..n], out[n] = total; inputs beyond one chunk use `scratch`
+// (scan_scratch_words(n) words)
+static void scan_any(hipStream_t s, const uint32_t* in, const uint8_t* lv, int mode, uint32_t* out, int n,
+                     uint32_t* scratch) {
+  if (n <= SCAN_CH) {
+    scan_kernel<<<1, SCAN_T, 0, s>>>(in, lv, mode, out, n);
+    return;
+  }
+  const int nch = (n + SCAN_CH - 1) / SCAN_CH;
+  uint32_t* sums = scratch;
+  uint32_t* cbase = scratch + nch;
+  scan_reduce_kernel<<<nch, TPB, 0, s>>>(in, lv, mode, n, sums);
+  scan_kernel<<<1, SCAN_T, 0, s>>>(sums, nullptr, 0, cbase, nch);
+  scan_final_kernel<<<nch, TPB, 0, s>>>(in, lv, mode, n, cbase, nch, out);
+}
+
+void k_scan_u32(hipStream_t s, const uint32_t* in, uint32_t* out, int n, uint32_t* scratch) {
+  scan_any(s, in, nullptr, 0, out, n, scratch);
 }
 
 void k_layout_offsets(hipStream_t s, const uint8_t* level, int nb, uint32_t* coff, uint32_t* fx, uint32_t* fy,
-                      uint32_t* fz) {
-  scan_kernel<<<1, SCAN_T, 0, s>>>(nullptr, level, 1, coff, nb);
-  scan_kernel<<<1, SCAN_T, 0, s>>>(nullptr, level, 2, fx, nb);
+                      uint32_t* fz, uint32_t* scratch) {
+  scan_any(s, nullptr, level, 1, coff, nb, scratch);
+  scan_any(s, nullptr, level, 2, fx, nb, scratch);
   if (fy != fx) hipMemcpyAsync(fy, fx, (nb + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
   if (fz != fx) hipMemcpyAsync(fz, fx, (nb + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
 }
@@ -2673,7 +2860,8 @@ void k_layout_offsets(hipStream_t s, const uint8_t* level, int nb, uint32_t* cof
 void k_scatter(hipStream_t s, const float* pos, const float* vel, const int32_t* ids, const int32_t* key,
                uint32_t* cursor, int64_t n, float* spos, float* svel, int32_t* sids) {
   if (n > 0)
-    scatter_kernel<<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, s>>>(pos, vel, ids, key, cursor, n, spos, svel, sids);
+    scatter_kernel<<<(unsigned)((n + TPB * AGG_PPT - 1) / (TPB * AGG_PPT)), TPB, 0, s>>>(pos, vel, ids, key, cursor,
+                                                                                       n, spos, svel, sids);
 }
 
 void k_init_beta(hipStream_t s, const LevelDev& L, float* bx, float* by, float* bz) {

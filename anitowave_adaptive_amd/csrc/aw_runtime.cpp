@@ -195,7 +195,7 @@ struct aw_grid {
   int D[3]{}, N[3]{};
   int nb = 0;
   float h = 0.f, inv_h = 0.f;
-  DevBuf level, surf, ltmp, count, poff, cursor, coff, foff, nbr, key, halo, g2p_list;
+  DevBuf level, surf, ltmp, count, poff, cursor, coff, foff, nbr, key, halo, g2p_list, scan_tmp;
   DevBuf spos, svel, sids;
   DevBuf liquid, p, rhs, slab;
   DevBuf beta[3], Macc[3], macc[3], uold[3], ustar[3], unew[3], mass[3];
@@ -339,8 +339,9 @@ int apply_levels(aw_grid* g) {
   AW_TRY(g->coff.ensure((nb + 1) * sizeof(uint32_t)));
   AW_TRY(g->foff.ensure((nb + 1) * sizeof(uint32_t)));
   // countBlockElements + exclusive_scan on the device (MSBG.cuh:353-376, main.cu:270-273)
+  AW_TRY(g->scan_tmp.ensure(scan_scratch_words(nb) * sizeof(uint32_t)));
   k_layout_offsets(s, g->level.as<uint8_t>(), nb, g->coff.as<uint32_t>(), g->foff.as<uint32_t>(),
-                   g->foff.as<uint32_t>(), g->foff.as<uint32_t>());
+                   g->foff.as<uint32_t>(), g->foff.as<uint32_t>(), g->scan_tmp.as<uint32_t>());
   AW_TRY(g->liquid.ensure(nc));
   AW_TRY(g->p.ensure(nc * sizeof(float)));
   AW_TRY(g->rhs.ensure(nc * sizeof(float)));
@@ -388,8 +389,11 @@ int bin_and_levels(aw_grid* g, const float* d_pos, int64_t n, bool mark_levels, 
     g->occupied = (double)std::count_if(cnt.begin(), cnt.end(), [](uint32_t c) { return c > 0; });
   }
   if (!mark_levels) return launch_status("bin");
-  k_levels(s, g->D, g->desc.num_levels, g->desc.band, g->count.as<uint32_t>(), g->surf.as<uint8_t>(),
-           g->level.as<uint8_t>(), g->ltmp.as<uint8_t>());
+  {
+    Prof pf(g->ctx, "levels", 8.0 * g->nb);
+    k_levels(s, g->D, g->desc.num_levels, g->desc.band, g->count.as<uint32_t>(), g->surf.as<uint8_t>(),
+             g->level.as<uint8_t>(), g->ltmp.as<uint8_t>());
+  }
   std::vector<uint8_t> nl(g->nb);
   AW_HIP(hipMemcpyAsync(nl.data(), g->level.p, g->nb, hipMemcpyDeviceToHost, s));
   AW_HIP(hipStreamSynchronize(s));
@@ -411,7 +415,8 @@ int sort_particles(aw_grid* g, const float* d_pos, const float* d_vel, const int
   AW_TRY(g->svel.ensure((size_t)std::max<int64_t>(n, 1) * 3 * sizeof(float)));
   if (d_ids) AW_TRY(g->sids.ensure((size_t)std::max<int64_t>(n, 1) * sizeof(int32_t)));
   Prof pf(g->ctx, "sort_particles", 52.0 * n);
-  k_scan_u32(s, g->count.as<uint32_t>(), g->poff.as<uint32_t>(), g->nb);
+  AW_TRY(g->scan_tmp.ensure(scan_scratch_words(g->nb) * sizeof(uint32_t)));
+  k_scan_u32(s, g->count.as<uint32_t>(), g->poff.as<uint32_t>(), g->nb, g->scan_tmp.as<uint32_t>());
   AW_HIP(hipMemcpyAsync(g->cursor.p, g->poff.p, (size_t)g->nb * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
   k_scatter(s, d_pos, d_vel, d_ids, g->key.as<int32_t>(), g->cursor.as<uint32_t>(), n, g->spos.as<float>(),
             g->svel.as<float>(), d_ids ? g->sids.as<int32_t>() : nullptr);
@@ -786,8 +791,9 @@ void smooth_level(aw_solver* S, int k, float* x, const float* b, int iters, bool
   for (int it = 0; it < iters; ++it)
     for (int ph = 0; ph < 2; ++ph) {
       const int col = reverse ? 1 - ph : ph;
-      Prof pf(S->g->ctx, k == 0 ? "smooth_l0" : "smooth_coarse", L.sweep_bytes[col]);
-      if (k > 0 && L.ush[0] >= 0)
+      const bool uni = k > 0 && L.ush[0] >= 0;
+      Prof pf(S->g->ctx, k == 0 ? "smooth_l0" : (uni ? "smooth_uniform" : "smooth_coarse"), L.sweep_bytes[col]);
+      if (uni)
         k_smooth_uniform(s, L.ush, L.ulr, L.ue, L.d_liq, x, b, col, S->prm.omega, S->prm.local_sweeps,
                          reverse ? 1 : 0);
       else
@@ -1612,7 +1618,7 @@ int aw_grid_create(aw_ctx* ctx, const aw_grid_desc* d, aw_grid** out) {
   const float fr = std::frexp(d->h, &ex);
   if (!(d->h > 0.f) || fr != 0.5f) return fail(AW_ERR_INVALID, "h must be a positive power of two");
   for (int l = 0; l + 1 < d->num_levels; ++l)
-    if (d->band[l] < 0) return fail(AW_ERR_INVALID, "negative band");
+    if (d->band[l] < 0 || d->band[l] > 254) return fail(AW_ERR_INVALID, "band[%d] = %d outside 0..254", l, d->band[l]);
   const int64_t nb = (int64_t)d->dims[0] * d->dims[1] * d->dims[2];
   if (nb > (1 << 24)) return fail(AW_ERR_INVALID, "too many blocks");
   AW_HIP(hipSetDevice(ctx->device));

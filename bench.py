@@ -93,7 +93,8 @@ def run_gpu(args, w, pos, vel):
 
 
 # profile groups -> the kernel that runs them
-KERNEL_OF = {"smooth_l0": "smooth_kernel", "smooth_coarse": "smooth_kernel", "residual_l0": "residual_kernel",
+KERNEL_OF = {"smooth_l0": "smooth_kernel", "smooth_coarse": "smooth_kernel", "smooth_uniform": "uniform_smooth_kernel",
+             "residual_l0": "residual_kernel",
              "residual_coarse": "residual_kernel", "residual_restrict_l0": "residual_kernel",
              "residual_restrict_coarse": "residual_kernel", "g2p_advect": "g2p_advect_kernel",
              "p2g_deposit": "p2g_deposit_kernel", "p2g_gather": "p2g_gather_kernel", "to_fine": "to_fine_kernel",
@@ -333,7 +334,8 @@ def main():
             "level0_in_step": {"gbs": smoother["gbs"], "frac_hbm_peak": smoother["frac"],
                                "us_per_launch": smoother["us_per_launch"]},
             "level0_back_to_back": iso.get("smooth_l0"),
-            "coarse_levels_in_step": per.get("smooth_coarse")},
+            "coarse_levels_in_step": per.get("smooth_coarse"),
+            "uniform_levels_in_step": per.get("smooth_uniform")},
         "residual": None if resid is None else {
             "level0_restrict_in_step": {"gbs": resid["gbs"], "frac_hbm_peak": resid["frac"],
                                         "us_per_launch": resid["us_per_launch"]},

@@ -58,7 +58,8 @@ typedef struct aw_grid aw_grid;
 typedef struct aw_solver aw_solver;
 
 /* Grid: dims[] blocks of base_res^3 cells at native level 0 (world size
- * base_res*h), num_levels native levels, distance bands band[l] in blocks. */
+ * base_res*h), num_levels native levels, distance bands band[l] in blocks
+ * (0..254). */
 typedef struct {
   int dims[3];
   int base_res;        /* must be 8 */
