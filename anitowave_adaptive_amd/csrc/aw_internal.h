@@ -160,10 +160,14 @@ void k_g2p_advect(hipStream_t s, const GridDev& g, const float* fine, const floa
                   int64_t n, float alpha, float dt, float* pos_out, float* vel_out, uint32_t* list);
 
 // adjoint (fine-lattice adjoint buffers: [block][field][axis][512], stride ADJ)
-// one work-group per block over the block's sorted particles [poff[b], poff[b] + pcount[b])
-void k_g2p_advect_adjoint(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const float* svel,
-                          const int32_t* sids, const uint32_t* pcount, const uint32_t* poff, float alpha, float dt,
-                          float* xbar, float* vbar, float* fbar);
+// G2P adjoint: stage pass over the n sorted particles (partials, and one
+// scatter record of adj_record_bytes() per particle), then the scatter pass,
+// one work-group per block over its particles [poff[b], poff[b] + pcount[b])
+size_t adj_record_bytes();
+void k_g2p_adjoint_stages(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const int32_t* sids,
+                          int64_t n, float alpha, float dt, float* xbar, float* vbar, void* rec, uint32_t* list);
+void k_g2p_adjoint_scatter(hipStream_t s, const GridDev& g, const float* spos, const uint32_t* pcount,
+                           const uint32_t* poff, const void* rec, float* fbar);
 void k_pi_transpose(hipStream_t s, const GridDev& g, const float* fbar, float* const* unb, float* const* uob);
 void k_project_adjoint(hipStream_t s, const GridDev& g, float* const* beta, float* const* unb, float* const* usb,
                        float* pbar);

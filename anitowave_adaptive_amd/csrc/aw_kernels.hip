@@ -1168,6 +1168,43 @@ __device__ __forceinline__ float sample_grad(const GridDev& g, const float* __re
                 : (a == 1 ? sample_grad<1>(g, F, field, u, nr, G) : sample_grad<2>(g, F, field, u, nr, G));
 }
 
+// sample_grad on the halo path only: ok = false (and nothing sampled) where
+// the point needs the per-node path
+template <int A>
+__device__ __forceinline__ float sample_grad_fast(const GridDev& g, const float* __restrict__ F, int field,
+                                                  const float* u, const Near& nr, float* G, bool& ok) {
+  int bc[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float c = d == A ? fminf(fmaxf(u[d], 0.f), (float)g.N[d]) : fminf(fmaxf(u[d] - 0.5f, 0.f), (float)(g.N[d] - 1));
+    const int i = d == A ? min((int)floorf(c), g.N[d] - 1) : (int)floorf(c);
+    bc[d] = i >> LOGB;
+  }
+  bool fast = nr.own;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) fast = fast && abs(bc[d] - nr.bc[d]) <= 1;
+  if (!fast) fast = g.halo[bc[0] + g.D[0] * (bc[1] + g.D[1] * bc[2])] != 0;
+  if (!fast) {
+    ok = false;
+    G[0] = G[1] = G[2] = 0.f;
+    return 0.f;
+  }
+  const SG r = sample_grad_impl<A, true>(g, F, field, u);
+  G[0] = r.g0;
+  G[1] = r.g1;
+  G[2] = r.g2;
+  return r.s;
+}
+
+template <bool SLOW>
+__device__ __forceinline__ float sample_grad_sel(const GridDev& g, const float* __restrict__ F, int field, int a,
+                                                 const float* u, const Near& nr, float* G, bool& ok) {
+  if (SLOW) return sample_grad(g, F, field, a, u, nr, G);
+  return a == 0 ? sample_grad_fast<0>(g, F, field, u, nr, G, ok)
+                : (a == 1 ? sample_grad_fast<1>(g, F, field, u, nr, G, ok)
+                          : sample_grad_fast<2>(g, F, field, u, nr, G, ok));
+}
+
 // FLIP/PIC blend (G2P.cuh:87-128) and Ralston RK3 (G2P.cuh:131-173) of
 // particle i.  Without SLOW a particle with a stage point in a block without
 // halo is appended to list (list[0] counts) and nothing is written for it.
@@ -1334,24 +1371,168 @@ __device__ __forceinline__ void tile_scatter(const GridDev& g, float* tile, floa
   }
 }
 
-// Adjoint of g2p_advect_kernel, one work-group per block over the block's
-// sorted particles: recomputes the forward stages, reads dL/dx', dL/dv' of
-// each particle (by id), writes its direct partials dL/dx, dL/dv and scatters
-// the field adjoints (field 0 = Fnew-bar + Fdel-bar, field 1 = Fdel-bar) into
-// an LDS tile around the block, flushed once per node at the end.
-__global__ __launch_bounds__(TPB) void g2p_advect_adjoint_kernel(GridDev g, const float* __restrict__ F,
-                                                                 const float* __restrict__ spos,
-                                                                 const float* __restrict__ svel,
-                                                                 const int32_t* __restrict__ sids,
-                                                                 const uint32_t* __restrict__ pcount,
-                                                                 const uint32_t* __restrict__ poff, float alpha,
-                                                                 float dt, float* __restrict__ xbar,
-                                                                 float* __restrict__ vbar, float* fbar) {
+// Adjoint of g2p_advect_kernel in two passes.
+//
+// Stage pass, one thread per sorted particle (the forward kernel's shape):
+// recomputes the forward stages, reads dL/dx', dL/dv' of the particle (by
+// id), samples the stage fields' gradients, writes its direct partials dL/dx,
+// dL/dv, and leaves the field-adjoint scatters it owes as a record: the two
+// RK stage positions and the weights of the twelve trilinear scatters.
+//
+// Scatter pass, one work-group per block over the block's records: the
+// scatters go into an LDS tile around the block (field 0 = Fnew-bar +
+// Fdel-bar, field 1 = Fdel-bar), flushed once per node at the end.
+//
+// In one kernel the gradient sampling ran at the occupancy the LDS tile
+// allows, each thread walking its particles' serial RK chains one after the
+// other; split, the sampling runs at the forward kernel's parallelism.
+struct AdjRec {
+  float4 q0;   // u2.xyz, u3.x
+  float4 q1;   // u3.yz, k3.xy
+  float4 q2;   // k3.z, k2.xyz
+  float4 q3;   // s1.xyz (field 0 at u), d1.x
+  float4 q4;   // d1.yz (field 1 at u), unused
+};
+
+// Stage pass of particle i.  Without SLOW only the halo path samples: a
+// particle with a stage point in a block without halo goes to `list`
+// (list[0] counts) and nothing is written for it; the fix-up pass (SLOW)
+// redoes the listed particles with the per-node path.
+template <bool SLOW>
+__device__ __forceinline__ void g2p_adjoint_one(const GridDev& g, const float* __restrict__ F,
+                                                const float* __restrict__ spos, const int32_t* __restrict__ sids,
+                                                int64_t i, float alpha, float dt, float* __restrict__ xbar,
+                                                float* __restrict__ vbar, AdjRec* __restrict__ rec,
+                                                uint32_t* __restrict__ list) {
+  const float umax[3] = {(float)g.N[0] - 0.0009765625f, (float)g.N[1] - 0.0009765625f,
+                         (float)g.N[2] - 0.0009765625f};
+  const int64_t id = sids[i];
+  bool ok = true;
+  // live ranges kept short (the masks follow from the clamped coordinates,
+  // the RK sum accumulates as the stages finish, the record is written as
+  // its parts become final): the kernel is held to 128 VGPRs
+  float x[3], u[3], u2[3], u3[3], xn[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    x[a] = spos[3 * i + a];
+    u[a] = fminf(fmaxf(x[a] * g.inv_h, 0.f), umax[a]);
+  }
+  const Near nr = near_of(g, u);
+  // forward stages: x2 = x + 1/2 dt v1, x3 = x + 3/4 dt v2, x' = x + dt (2 v1 + 3 v2 + 4 v3) / 9
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float v1 = sample_comp<SLOW>(g, F, 0, a, u, nr, ok);
+    u2[a] = fminf(fmaxf((x[a] + 0.5f * dt * v1) * g.inv_h, -1.f), (float)g.N[a] + 1.f);
+    xn[a] = (2.f / 9.f) * v1;
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float v2 = sample_comp<SLOW>(g, F, 0, a, u2, nr, ok);
+    u3[a] = fminf(fmaxf((x[a] + 0.75f * dt * v2) * g.inv_h, -1.f), (float)g.N[a] + 1.f);
+    xn[a] += (3.f / 9.f) * v2;
+  }
+  // stage 3 gradient G[a][d] = d v3_a / d u3_d
+  float G[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float v3 = sample_grad_sel<SLOW>(g, F, 0, a, u3, nr, G[a], ok);
+    xn[a] = x[a] + dt * (xn[a] + (4.f / 9.f) * v3);
+  }
+  float k1[3], k2[3], k3[3], xacc[3], vo[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float lo = 0.25f * g.h, hi = (float)g.N[a] * g.h - 0.25f * g.h;
+    const float xb = (xn[a] > lo && xn[a] < hi) ? xbar[3 * id + a] : 0.f;
+    vo[a] = vbar[3 * id + a];
+    k1[a] = dt * (2.f / 9.f) * xb;
+    k2[a] = dt * (3.f / 9.f) * xb;
+    k3[a] = dt * (4.f / 9.f) * xb;
+    xacc[a] = xb;
+  }
+  // stage 3 (the coordinate mask: the unclamped stage coordinate lies inside
+  // (-1, N + 1), i.e. the clamped one is off both bounds)
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float m3 = (u3[d] > -1.f && u3[d] < (float)g.N[d] + 1.f) ? 1.f : 0.f;
+    const float b = (k3[0] * G[0][d] + k3[1] * G[1][d] + k3[2] * G[2][d]) * g.inv_h * m3;
+    xacc[d] += b;
+    k2[d] += 0.75f * dt * b;
+  }
+  rec[i].q0 = make_float4(u2[0], u2[1], u2[2], u3[0]);
+  rec[i].q1 = make_float4(u3[1], u3[2], k3[0], k3[1]);
+  rec[i].q2 = make_float4(k3[2], k2[0], k2[1], k2[2]);
+  // stage 2
+#pragma unroll
+  for (int a = 0; a < 3; ++a) (void)sample_grad_sel<SLOW>(g, F, 0, a, u2, nr, G[a], ok);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float m2 = (u2[d] > -1.f && u2[d] < (float)g.N[d] + 1.f) ? 1.f : 0.f;
+    const float b = (k2[0] * G[0][d] + k2[1] * G[1][d] + k2[2] * G[2][d]) * g.inv_h * m2;
+    xacc[d] += b;
+    k1[d] += 0.5f * dt * b;
+  }
+  // FLIP/PIC blend and stage 1 at x
+  float gr[3] = {0.f, 0.f, 0.f}, s1[3], d1[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float vnb = k1[a] + (1.f - alpha) * vo[a], vdb = alpha * vo[a];
+    s1[a] = vnb + vdb;
+    d1[a] = vdb;
+    float Gn[3], Gd[3];
+    (void)sample_grad_sel<SLOW>(g, F, 0, a, u, nr, Gn, ok);
+    (void)sample_grad_sel<SLOW>(g, F, 1, a, u, nr, Gd, ok);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) gr[d] += vnb * Gn[d] + vdb * Gd[d];
+  }
+  if (!SLOW && !ok) {
+    // a stage point without halo: the fix-up pass redoes the particle
+    list[1 + atomicAdd(&list[0], 1u)] = (uint32_t)i;
+    return;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float m1 = (u[d] > 0.f && u[d] < umax[d]) ? 1.f : 0.f;
+    xbar[3 * id + d] = xacc[d] + gr[d] * g.inv_h * m1;
+    vbar[3 * id + d] = d1[d];
+  }
+  rec[i].q3 = make_float4(s1[0], s1[1], s1[2], d1[0]);
+  rec[i].q4 = make_float4(d1[1], d1[2], 0.f, 0.f);
+}
+
+template <bool FIXUP>
+__global__ __launch_bounds__(TPB) void g2p_adjoint_stage_kernel(GridDev g, const float* __restrict__ F,
+                                                                const float* __restrict__ spos,
+                                                                const int32_t* __restrict__ sids, int64_t n,
+                                                                float alpha, float dt, float* __restrict__ xbar,
+                                                                float* __restrict__ vbar, AdjRec* __restrict__ rec,
+                                                                uint32_t* __restrict__ list) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (FIXUP) {
+    const int64_t m = (int64_t)list[0];
+    for (int64_t j = i; j < m; j += (int64_t)gridDim.x * blockDim.x)
+      g2p_adjoint_one<true>(g, F, spos, sids, (int64_t)list[1 + j], alpha, dt, xbar, vbar, rec, list);
+    return;
+  }
+  if (i >= n) return;
+  g2p_adjoint_one<false>(g, F, spos, sids, i, alpha, dt, xbar, vbar, rec, list);
+}
+
+// 512 threads per work-group (the LDS tile leaves room for three per CU, 24
+// waves to cover the latency of the compare-and-swap adds): measured at
+// 512^3, 16.8 ms against 24.5 ms with 256 threads and 20.4 ms with 1024.
+// (Holding both fields of a node as one pair for 64-bit compare-and-swap
+// measured 17.7 ms: two of the three stages scatter into field 0 only.)
+constexpr int ADJ_T = 512;
+
+__global__ __launch_bounds__(ADJ_T) void g2p_adjoint_scatter_kernel(GridDev g, const float* __restrict__ spos,
+                                                                  const uint32_t* __restrict__ pcount,
+                                                                  const uint32_t* __restrict__ poff,
+                                                                  const AdjRec* __restrict__ rec, float* fbar) {
   __shared__ float tile[2 * 3 * AT3];
   const int B = blockIdx.x;
   const uint32_t cnt = pcount[B];
   if (cnt == 0) return;
-  for (int k = threadIdx.x; k < 2 * 3 * AT3; k += TPB) tile[k] = 0.f;
+  for (int k = threadIdx.x; k < 2 * 3 * AT3; k += ADJ_T) tile[k] = 0.f;
   int o[3];
   block_coords(g.D, B, o);
 #pragma unroll
@@ -1362,103 +1543,32 @@ __global__ __launch_bounds__(TPB) void g2p_advect_adjoint_kernel(GridDev g, cons
   // each thread takes a contiguous run of the block's particles, so the lanes
   // of a wave hold particles far apart in the sorted order (particles of one
   // cell are often neighbours there) and their tile adds rarely meet on a word
-  const uint32_t per = (cnt + TPB - 1) / TPB;
+  const uint32_t per = (cnt + ADJ_T - 1) / ADJ_T;
   for (uint32_t it = 0; it < per; ++it) {
     const uint32_t j = threadIdx.x * per + it;
     if (j >= cnt) break;
     const int64_t i = (int64_t)poff[B] + j;
-    const int64_t id = sids[i];
-    float x[3], u[3], m1[3], v1[3], vd[3];
+    float u[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      x[a] = spos[3 * i + a];
-      const float raw = x[a] * g.inv_h;
-      u[a] = fminf(fmaxf(raw, 0.f), umax[a]);
-      m1[a] = (raw > 0.f && raw < umax[a]) ? 1.f : 0.f;
-    }
-    const Near nr = near_of(g, u);
-    // forward stages (values), then the backward pass, which samples the
-    // gradient of each stage's field (G[a][d] = d v_a / d u_d) when its
-    // partial is known
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      v1[a] = sample_comp(g, F, 0, a, u, nr);
-      vd[a] = sample_comp(g, F, 1, a, u, nr);
-    }
-    float u2[3], m2[3], v2[3], u3[3], m3[3], v3[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const float raw = (x[a] + 0.5f * dt * v1[a]) * g.inv_h;
-      u2[a] = fminf(fmaxf(raw, -1.f), (float)g.N[a] + 1.f);
-      m2[a] = (raw > -1.f && raw < (float)g.N[a] + 1.f) ? 1.f : 0.f;
-    }
-#pragma unroll
-    for (int a = 0; a < 3; ++a) v2[a] = sample_comp(g, F, 0, a, u2, nr);
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const float raw = (x[a] + 0.75f * dt * v2[a]) * g.inv_h;
-      u3[a] = fminf(fmaxf(raw, -1.f), (float)g.N[a] + 1.f);
-      m3[a] = (raw > -1.f && raw < (float)g.N[a] + 1.f) ? 1.f : 0.f;
-    }
-    float G[3][3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) v3[a] = sample_grad(g, F, 0, a, u3, nr, G[a]);
-    float k1[3], k2[3], k3[3], xacc[3], vo[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const float xn = x[a] + dt * ((2.f / 9.f) * v1[a] + (3.f / 9.f) * v2[a] + (4.f / 9.f) * v3[a]);
-      const float lo = 0.25f * g.h, hi = (float)g.N[a] * g.h - 0.25f * g.h;
-      const float xb = (xn > lo && xn < hi) ? xbar[3 * id + a] : 0.f;
-      vo[a] = vbar[3 * id + a];
-      k1[a] = dt * (2.f / 9.f) * xb;
-      k2[a] = dt * (3.f / 9.f) * xb;
-      k3[a] = dt * (4.f / 9.f) * xb;
-      xacc[a] = xb;
-    }
-    // stage 3: v3 = S(Fnew)(x3), x3 = x + 3/4 dt v2
+    for (int a = 0; a < 3; ++a) u[a] = fminf(fmaxf(spos[3 * i + a] * g.inv_h, 0.f), umax[a]);
+    const AdjRec r = rec[i];
+    const float u2[3] = {r.q0.x, r.q0.y, r.q0.z}, u3[3] = {r.q0.w, r.q1.x, r.q1.y};
+    const float k3[3] = {r.q1.z, r.q1.w, r.q2.x}, k2[3] = {r.q2.y, r.q2.z, r.q2.w};
+    const float s1[3] = {r.q3.x, r.q3.y, r.q3.z}, d1[3] = {r.q3.w, r.q4.x, r.q4.y};
 #pragma unroll
     for (int a = 0; a < 3; ++a) tile_scatter(g, tile, fbar, o, 0, a, u3, k3[a]);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const float b = (k3[0] * G[0][d] + k3[1] * G[1][d] + k3[2] * G[2][d]) * g.inv_h * m3[d];
-      xacc[d] += b;
-      k2[d] += 0.75f * dt * b;
-    }
-    // stage 2: v2 = S(Fnew)(x2), x2 = x + 1/2 dt v1
+    for (int a = 0; a < 3; ++a) tile_scatter(g, tile, fbar, o, 0, a, u2, k2[a]);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      tile_scatter(g, tile, fbar, o, 0, a, u2, k2[a]);
-      (void)sample_grad(g, F, 0, a, u2, nr, G[a]);
-    }
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const float b = (k2[0] * G[0][d] + k2[1] * G[1][d] + k2[2] * G[2][d]) * g.inv_h * m2[d];
-      xacc[d] += b;
-      k1[d] += 0.5f * dt * b;
-    }
-    // FLIP/PIC blend and stage 1 at x
-    float gr[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const float vnb = k1[a] + (1.f - alpha) * vo[a], vdb = alpha * vo[a];
-      tile_scatter(g, tile, fbar, o, 0, a, u, vnb + vdb);
-      tile_scatter(g, tile, fbar, o, 1, a, u, vdb);
-      float Gn[3], Gd[3];
-      (void)sample_grad(g, F, 0, a, u, nr, Gn);
-      (void)sample_grad(g, F, 1, a, u, nr, Gd);
-#pragma unroll
-      for (int d = 0; d < 3; ++d) gr[d] += vnb * Gn[d] + vdb * Gd[d];
-    }
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      xbar[3 * id + d] = xacc[d] + gr[d] * g.inv_h * m1[d];
-      vbar[3 * id + d] = alpha * vo[d];
+      tile_scatter(g, tile, fbar, o, 0, a, u, s1[a]);
+      tile_scatter(g, tile, fbar, o, 1, a, u, d1[a]);
     }
   }
   __syncthreads();
   // flush: tile nodes are in-domain (clamped) by construction except the
   // normal-axis ones past the +wall, which were never written
-  for (int k = threadIdx.x; k < 2 * 3 * AT3; k += TPB) {
+  for (int k = threadIdx.x; k < 2 * 3 * AT3; k += ADJ_T) {
     const float v = tile[k];
     if (v == 0.f) continue;
     const int fa = k / AT3, r = k - fa * AT3;
@@ -2913,11 +3023,22 @@ void k_g2p_advect(hipStream_t s, const GridDev& g, const float* fine, const floa
   g2p_advect_kernel<true><<<fix, TPB, 0, s>>>(g, fine, spos, svel, n, alpha, dt, pos_out, vel_out, list);
 }
 
-void k_g2p_advect_adjoint(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const float* svel,
-                          const int32_t* sids, const uint32_t* pcount, const uint32_t* poff, float alpha, float dt,
-                          float* xbar, float* vbar, float* fbar) {
-  g2p_advect_adjoint_kernel<<<g.nb, TPB, 0, s>>>(g, fine, spos, svel, sids, pcount, poff, alpha, dt, xbar, vbar,
-                                                 fbar);
+size_t adj_record_bytes() { return sizeof(AdjRec); }
+
+void k_g2p_adjoint_stages(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const int32_t* sids,
+                          int64_t n, float alpha, float dt, float* xbar, float* vbar, void* rec, uint32_t* list) {
+  if (n <= 0) return;
+  AdjRec* r = static_cast<AdjRec*>(rec);
+  fill_kernel<uint32_t><<<1, 64, 0, s>>>(list, 1, 0u);
+  const int64_t nwg = (n + TPB - 1) / TPB;
+  g2p_adjoint_stage_kernel<false><<<(unsigned)nwg, TPB, 0, s>>>(g, fine, spos, sids, n, alpha, dt, xbar, vbar, r, list);
+  const unsigned fix = (unsigned)(nwg < 1024 ? nwg : 1024);
+  g2p_adjoint_stage_kernel<true><<<fix, TPB, 0, s>>>(g, fine, spos, sids, n, alpha, dt, xbar, vbar, r, list);
+}
+
+void k_g2p_adjoint_scatter(hipStream_t s, const GridDev& g, const float* spos, const uint32_t* pcount,
+                           const uint32_t* poff, const void* rec, float* fbar) {
+  g2p_adjoint_scatter_kernel<<<g.nb, ADJ_T, 0, s>>>(g, spos, pcount, poff, static_cast<const AdjRec*>(rec), fbar);
 }
 void k_pi_transpose(hipStream_t s, const GridDev& g, const float* fbar, float* const* unb, float* const* uob) {
   pi_transpose_kernel<<<g.nb, TPB, 0, s>>>(g, fbar, fp(unb), fp(uob));

@@ -200,7 +200,7 @@ struct aw_grid {
   DevBuf liquid, p, rhs, slab;
   DevBuf beta[3], Macc[3], macc[3], uold[3], ustar[3], unew[3], mass[3];
   // adjoint fields: fine-lattice adjoints (two fields each, stride ADJ), per-slot and per-cell adjoints
-  DevBuf fbar, fmbar, pbar, rhsbar, idrange;
+  DevBuf fbar, fmbar, pbar, rhsbar, idrange, adjrec;
   DevBuf ubn[3], ubo[3], ubs[3], Mb[3], mb[3];
   bool sorted_ids = false;  // the last binning carried particle ids
   std::vector<uint8_t> h_level;
@@ -1234,13 +1234,20 @@ int g2p_adjoint(aw_grid* g, float alpha, float dt, float* xbar, float* vbar) {
   hipStream_t s = g->ctx->stream;
   const GridDev gv = g->view();
   const int64_t n = g->n_sorted;
+  AW_TRY(g->adjrec.ensure((size_t)std::max<int64_t>(n, 1) * adj_record_bytes()));
+  AW_TRY(g->g2p_list.ensure((size_t)(n + 1) * sizeof(uint32_t)));
   k_fill_f32(s, g->fbar.as<float>(), (size_t)g->nb * ADJ, 0.f);
   {
-    // particle state 28 B + adjoints read and written 48 B; forward fine lattice and its adjoint per block
-    Prof pf(g->ctx, "g2p_advect_adjoint", 76.0 * n + 2.0 * 4.0 * 3 * FINE * g->nb + 4.0 * ADJ * g->nb);
-    k_g2p_advect_adjoint(s, gv, g->slab.as<float>(), g->spos.as<float>(), g->svel.as<float>(),
-                         g->sids.as<int32_t>(), g->count.as<uint32_t>(), g->poff.as<uint32_t>(), alpha, dt, xbar,
-                         vbar, g->fbar.as<float>());
+    // particle position and id 16 B + adjoints read and written 48 B + record; forward fine lattice per block
+    Prof pf(g->ctx, "g2p_adjoint_stages", (64.0 + adj_record_bytes()) * n + 2.0 * 4.0 * 3 * FINE * g->nb);
+    k_g2p_adjoint_stages(s, gv, g->slab.as<float>(), g->spos.as<float>(), g->sids.as<int32_t>(), n, alpha, dt, xbar,
+                         vbar, g->adjrec.p, g->g2p_list.as<uint32_t>());
+  }
+  {
+    // position 12 B + record per particle; the fine-lattice adjoint per block
+    Prof pf(g->ctx, "g2p_adjoint_scatter", (12.0 + adj_record_bytes()) * n + 4.0 * ADJ * g->nb);
+    k_g2p_adjoint_scatter(s, gv, g->spos.as<float>(), g->count.as<uint32_t>(), g->poff.as<uint32_t>(), g->adjrec.p,
+                          g->fbar.as<float>());
   }
   {
     Prof pf(g->ctx, "pi_transpose", 4.0 * ADJ * g->nb + 8.0 * 3 * g->nfaces);
