@@ -49,6 +49,8 @@ struct GridDev {
   const int32_t* nbr;
   const uint8_t* halo;         // per block: the padded fine lattice's halo is filled
   const uint32_t* count;       // per block: binned particles (their neighbourhoods have halos)
+  const float* un[3];          // u_new and u_old faces (the padded lattice's sources)
+  const float* uo[3];
 };
 
 // Work-group of a stencil launch: `count` blocks of resolution 1 << lr
@@ -150,10 +152,10 @@ void k_fold(hipStream_t s, const GridDev& g, float* const* Macc, float* const* m
 void k_divergence(hipStream_t s, const GridDev& g, const uint8_t* liquid, float* const* ustar, float* rhs);
 void k_project(hipStream_t s, const GridDev& g, const float* p, float* const* ustar, float* const* beta,
                float* const* unew);
-// fine lattice: own nodes of every block, halos of the blocks within one block
-// of a particle (count: particles per block; halo: per-block flag written)
-void k_to_fine(hipStream_t s, const GridDev& g, float* const* unew, float* const* uold, float* fine,
-               const uint32_t* count, uint8_t* halo);
+// padded fine lattice of the blocks within one block of a particle, every
+// node from its owner's faces (count: particles per block; halo: per-block
+// flag written)
+void k_to_fine(hipStream_t s, const GridDev& g, float* fine, const uint32_t* count, uint8_t* halo);
 // list: n + 1 words of scratch for the particles whose stage points leave the
 // blocks with fine-lattice halos (taken in a second pass)
 void k_g2p_advect(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const float* svel,

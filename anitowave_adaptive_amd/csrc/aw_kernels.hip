@@ -534,12 +534,13 @@ __global__ void init_beta_kernel(LevelDev L, float* __restrict__ bx, float* __re
 // nodes, held in LDS as (M, m) pairs, and the node sums then go to the
 // block's slab with plain stores, so P2G issues no global float atomics
 // (P2G.cuh:16-135 scatters with global float atomics instead).  The LDS adds
-// are 64-bit compare-and-swaps of the pair (lds_add2_spread); each thread
-// takes a contiguous run of the sorted particles, so the lanes of a wave sit
-// in different cells and rarely hit the same node.  (Measured on MI355X at
-// 256^3 / 512^3: 1.58 / 5.08 ms, against 2.17 / 10.6 ms for a gather form
-// that sorts the particles by half-cell in LDS and lets each node sum its
-// support, and 1.60 / 6.22 ms for two 32-bit compare-and-swaps per node.)
+// are 64-bit compare-and-swaps of the pair (lds_add2_spread), with the lanes
+// of a wave spread over the block's particles so that they rarely hit the
+// same node.  (Measured on MI355X at 256^3 / 512^3: 1.23 / 5.31 ms; a gather
+// form that sorts the particles by half-cell in LDS and lets each node sum
+// its support took 2.17 / 10.6 ms, two 32-bit compare-and-swaps per node
+// 1.60 / 6.22 ms with one contiguous run of particles per thread, which
+// itself took 1.59 / 5.12 ms with the pair.)
 constexpr int DEP_T = 256;            // threads per work-group
 constexpr int DEP_NODES = 3 * PAD;    // padded nodes of all three axes
 
@@ -564,10 +565,20 @@ __global__ __launch_bounds__(DEP_T) void p2g_deposit_kernel(GridDev g, const flo
   for (int k = tid; k < 512; k += DEP_T) ccnt[k] = 0;
   __syncthreads();
   const uint32_t p0 = poff[B];
-  const uint32_t per = (cnt + DEP_T - 1) / DEP_T;   // particles per thread
-  for (uint32_t j = 0; j < per; ++j) {
-    const uint32_t i = tid * per + j;
-    if (i >= cnt) break;
+  // Particle order: windows of DEP_WIN consecutive particles; in a window
+  // lane l of wave v takes particles DEP_S l + DEP_SPW v + (0 .. DEP_SPW-1).
+  // Lanes sit DEP_S particles apart (particles of one cell are often
+  // neighbours in the sorted order, and lanes of one wave on one node retry
+  // their compare-and-swap), and a work-group's cache lines are those of one
+  // window, which L2 holds until the window is done (one contiguous run per
+  // thread over the whole block re-read the particle arrays from HBM about
+  // nine times over: 11 GB for 1.2 GB at 256^3).
+  constexpr uint32_t DEP_S = 16, DEP_WIN = 64 * DEP_S, DEP_SPW = DEP_WIN / DEP_T;
+  const uint32_t lane = tid & 63, wv = tid >> 6;
+  const uint32_t nit = (cnt + DEP_WIN - 1) / DEP_WIN * DEP_SPW;
+  for (uint32_t it = 0; it < nit; ++it) {
+    const uint32_t i = (it / DEP_SPW) * DEP_WIN + lane * DEP_S + wv * DEP_SPW + it % DEP_SPW;
+    if (i >= cnt) continue;
     const size_t p = p0 + i;
     float u[3], v[3];
     bool inside = true;
@@ -860,9 +871,10 @@ __global__ __launch_bounds__(TPB) void project_kernel(GridDev g, const float* __
 // floats per component, 3 PAD per field; field 0 = u_new, 1 = u_new - u_old).
 // Halo nodes hold their owner block's values, or the sampling rule's value
 // past the tank: 0 across a face-normal wall, the nearest node inside
-// transversally.  Halos are filled for the blocks within one block of a
-// particle (where RK3 stage points fall); a sample elsewhere reads the
-// owners' nodes one by one.
+// transversally.  The lattice is filled only for the blocks flagged `halo`
+// (within one block of a block holding particles: where RK3 stage points
+// fall); a sample elsewhere takes the owners' nodes one by one, from the
+// owner's lattice where it is filled and from its faces otherwise.
 
 // halo[B] = a block of the 3x3x3 neighbourhood of B holds particles
 __global__ __launch_bounds__(TPB) void halo_flags_kernel(GridDev g, const uint32_t* __restrict__ count,
@@ -882,65 +894,69 @@ __global__ __launch_bounds__(TPB) void halo_flags_kernel(GridDev g, const uint32
   halo[B] = any ? 1 : 0;
 }
 
-// Pi of u_new and of (u_new - u_old) on the block's own fine nodes
-__global__ __launch_bounds__(TPB) void to_fine_kernel(GridDev g, FacePtrs unew, FacePtrs uold, float* __restrict__ fine) {
-  const int B = blockIdx.x;
-  const int e = g.level[B];
-  float* dst = fine + (size_t)B * SLAB;
-  for (int k = threadIdx.x; k < 3 * FINE; k += blockDim.x) {
-    const int a = k >> 9, li = k & 511;
-    const int l[3] = {li & 7, (li >> 3) & 7, li >> 6};
-    int slot[8];
-    float w[8];
-    const int n = pi_weights(a, e, l, slot, w);
-    float vn = 0.f, vd = 0.f;
-    for (int j = 0; j < n; ++j) {
-      const size_t id = g.foff[a][B] + slot[j];
-      const float un = unew.p[a][id];
-      vn += w[j] * un;
-      vd += w[j] * (un - uold.p[a][id]);
-    }
-    const int pi = a * PAD + pad_idx(a, l);
-    dst[pi] = vn;
-    dst[3 * PAD + pi] = vd;
+// Pi of u_new (field 0) or of u_new - u_old (field 1) at own fine node l of
+// block B on component a: the weighted sum of the block's native slots
+__device__ __forceinline__ float node_from_faces(const GridDev& g, int field, int a, int B, const int* l) {
+  int slot[8];
+  float w[8];
+  const int n = pi_weights(a, g.level[B], l, slot, w);
+  float vn = 0.f, vd = 0.f;
+  for (int j = 0; j < n; ++j) {
+    const size_t id = g.foff[a][B] + slot[j];
+    const float un = g.un[a][id];
+    vn += w[j] * un;
+    vd += w[j] * (un - g.uo[a][id]);
   }
+  return field ? vd : vn;
 }
 
-// halo nodes of the flagged blocks' padded lattices, copied from their owners
-__global__ __launch_bounds__(TPB) void fine_halo_kernel(GridDev g, float* __restrict__ fine) {
+// Every padded node of a flagged block, own or halo, from the faces of the
+// block that owns it (the same sums for a node in every lattice that holds it)
+__global__ __launch_bounds__(TPB) void to_fine_kernel(GridDev g, float* __restrict__ fine) {
   const int B = blockIdx.x;
   if (!g.halo[B]) return;
   int Bc[3];
   block_coords(g.D, B, Bc);
   float* dst = fine + (size_t)B * SLAB;
-  for (int k = threadIdx.x; k < 3 * PAD; k += blockDim.x) {
+  for (int k = threadIdx.x; k < 3 * PAD; k += TPB) {
     const int a = k / PAD, pi = k - a * PAD;
     const int n0 = a == 0 ? 9 : 10, n1 = a == 1 ? 9 : 10;
     const int lp[3] = {pi % n0, (pi / n0) % n1, pi / (n0 * n1)};
     int q[3];
-    bool own = true, outside = false;
+    bool outside = false;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const int l = lp[d] - (d == a ? 0 : 1);   // block-local node index
-      own = own && l >= 0 && l < BASE;
-      q[d] = BASE * Bc[d] + l;
+      q[d] = BASE * Bc[d] + lp[d] - (d == a ? 0 : 1);
       if (d == a)
         outside = outside || q[d] >= g.N[d];
       else
         q[d] = iclamp(q[d], 0, g.N[d] - 1);
     }
-    if (own) continue;
     float vn = 0.f, vd = 0.f;
     if (!outside) {
       const int Bo = (q[0] >> LOGB) + g.D[0] * ((q[1] >> LOGB) + g.D[1] * (q[2] >> LOGB));
       const int l[3] = {q[0] & 7, q[1] & 7, q[2] & 7};
-      const float* src = fine + (size_t)Bo * SLAB + a * PAD + pad_idx(a, l);
-      vn = src[0];
-      vd = src[3 * PAD];
+      int slot[8];
+      float w[8];
+      const int n = pi_weights(a, g.level[Bo], l, slot, w);
+      for (int j = 0; j < n; ++j) {
+        const size_t id = g.foff[a][Bo] + slot[j];
+        const float un = g.un[a][id];
+        vn += w[j] * un;
+        vd += w[j] * (un - g.uo[a][id]);
+      }
     }
     dst[k] = vn;
     dst[3 * PAD + k] = vd;
   }
+}
+
+// node l of owner block Bo from the padded lattice where it is filled, else
+// from the owner's faces
+__device__ __forceinline__ float own_node(const GridDev& g, const float* __restrict__ F, int field, int a, int Bo,
+                                          const int* l) {
+  if (g.halo[Bo]) return F[(size_t)Bo * SLAB + field * 3 * PAD + a * PAD + pad_idx(a, l)];
+  return node_from_faces(g, field, a, Bo, l);
 }
 
 // Trilinear sample of component A of a field of the padded fine lattice at u
@@ -1021,7 +1037,7 @@ __device__ __forceinline__ float sample_comp(const GridDev& g, const float* __re
           if (d != A) q[d] = iclamp(q[d], 0, g.N[d] - 1);
         const int Bo = (q[0] >> LOGB) + g.D[0] * ((q[1] >> LOGB) + g.D[1] * (q[2] >> LOGB));
         const int l[3] = {q[0] & 7, q[1] & 7, q[2] & 7};
-        v = F[(size_t)Bo * SLAB + field * 3 * PAD + A * PAD + pad_idx(A, l)];
+        v = own_node(g, F, field, A, Bo, l);
       }
       s += w * v;
     }
@@ -1116,7 +1132,7 @@ __device__ __forceinline__ SG sample_grad_impl(const GridDev& g, const float* __
           if (d != A) q[d] = iclamp(q[d], 0, g.N[d] - 1);
         const int Bo = (q[0] >> LOGB) + g.D[0] * ((q[1] >> LOGB) + g.D[1] * (q[2] >> LOGB));
         const int l[3] = {q[0] & 7, q[1] & 7, q[2] & 7};
-        f = F[(size_t)Bo * SLAB + field * 3 * PAD + A * PAD + pad_idx(A, l)];
+        f = own_node(g, F, field, A, Bo, l);
       }
     }
     const float w0 = dd[0] ? t[0] : 1.f - t[0], w1 = dd[1] ? t[1] : 1.f - t[1], w2 = dd[2] ? t[2] : 1.f - t[2];
@@ -3005,11 +3021,9 @@ void k_project(hipStream_t s, const GridDev& g, const float* p, float* const* us
   project_kernel<<<g.nb, TPB, 0, s>>>(g, p, fp(ustar), fp(beta), fp(unew));
 }
 
-void k_to_fine(hipStream_t s, const GridDev& g, float* const* unew, float* const* uold, float* fine,
-               const uint32_t* count, uint8_t* halo) {
+void k_to_fine(hipStream_t s, const GridDev& g, float* fine, const uint32_t* count, uint8_t* halo) {
   halo_flags_kernel<<<(g.nb + TPB - 1) / TPB, TPB, 0, s>>>(g, count, halo);
-  to_fine_kernel<<<g.nb, TPB, 0, s>>>(g, fp(unew), fp(uold), fine);
-  fine_halo_kernel<<<g.nb, TPB, 0, s>>>(g, fine);
+  to_fine_kernel<<<g.nb, TPB, 0, s>>>(g, fine);
 }
 
 void k_g2p_advect(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const float* svel,

@@ -210,6 +210,7 @@ struct aw_grid {
   uint32_t ncells = 0, nfaces = 0;
   int64_t n_sorted = -1;
   double occupied = 0.0;   // blocks holding particles (for byte accounting only)
+  double lattice_blocks = 0.0, lattice_faces = 0.0;   // blocks with a filled fine lattice, their face slots (same)
 
   GridDev view() const {
     GridDev g;
@@ -217,6 +218,8 @@ struct aw_grid {
       g.D[a] = D[a];
       g.N[a] = N[a];
       g.foff[a] = foff.as<uint32_t>();
+      g.un[a] = unew[a].as<float>();
+      g.uo[a] = uold[a].as<float>();
     }
     g.nb = nb;
     g.h = h;
@@ -387,6 +390,24 @@ int bin_and_levels(aw_grid* g, const float* d_pos, int64_t n, bool mark_levels, 
     AW_HIP(hipMemcpyAsync(cnt.data(), g->count.p, g->nb * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     AW_HIP(hipStreamSynchronize(s));
     g->occupied = (double)std::count_if(cnt.begin(), cnt.end(), [](uint32_t c) { return c > 0; });
+    // blocks whose padded lattice is filled (a particle within one block) and their face slots
+    g->lattice_blocks = 0.0;
+    g->lattice_faces = 0.0;
+    for (int b = 0; b < g->nb; ++b) {
+      const int x = b % g->D[0], y = (b / g->D[0]) % g->D[1], z = b / (g->D[0] * g->D[1]);
+      bool any = false;
+      for (int dz = -1; dz <= 1 && !any; ++dz)
+        for (int dy = -1; dy <= 1 && !any; ++dy)
+          for (int dx = -1; dx <= 1 && !any; ++dx) {
+            const int qx = x + dx, qy = y + dy, qz = z + dz;
+            if (qx < 0 || qy < 0 || qz < 0 || qx >= g->D[0] || qy >= g->D[1] || qz >= g->D[2]) continue;
+            any = cnt[qx + g->D[0] * (qy + g->D[1] * qz)] > 0;
+          }
+      if (!any || !g->have_topology) continue;
+      const int r = BASE >> g->h_level[b];
+      g->lattice_blocks += 1.0;
+      g->lattice_faces += 3.0 * (r + 1) * r * r;
+    }
   }
   if (!mark_levels) return launch_status("bin");
   {
@@ -469,8 +490,9 @@ int project(aw_grid* g) {
     k_project(s, gv, g->p.as<float>(), us, bt, un);
   }
   {
-    Prof pf(g->ctx, "to_fine", 24.0 * g->nfaces + 2.0 * 4.0 * 3 * FINE * g->nb + 4.0 * g->nb);
-    k_to_fine(s, gv, un, uo, g->slab.as<float>(), g->count.as<uint32_t>(), g->halo.as<uint8_t>());
+    // u_new, u_old of the flagged blocks' faces, the padded lattice written, flags
+    Prof pf(g->ctx, "to_fine", 8.0 * g->lattice_faces + 4.0 * SLAB * g->lattice_blocks + 5.0 * g->nb);
+    k_to_fine(s, gv, g->slab.as<float>(), g->count.as<uint32_t>(), g->halo.as<uint8_t>());
   }
   return launch_status("project");
 }
@@ -1403,8 +1425,8 @@ int tape_restore(aw_tape* t, const aw_tape::Step& r) {
   g->sorted_ids = true;
   Faces un(g->unew), uo(g->uold);
   {
-    Prof pf(g->ctx, "to_fine", 24.0 * g->nfaces + 2.0 * 4.0 * 3 * FINE * g->nb + 4.0 * g->nb);
-    k_to_fine(s, g->view(), un.p, uo.p, g->slab.as<float>(), g->count.as<uint32_t>(), g->halo.as<uint8_t>());
+    Prof pf(g->ctx, "to_fine", 8.0 * g->lattice_faces + 4.0 * SLAB * g->lattice_blocks + 5.0 * g->nb);
+    k_to_fine(s, g->view(), g->slab.as<float>(), g->count.as<uint32_t>(), g->halo.as<uint8_t>());
   }
   return launch_status("tape restore");
 }
