@@ -332,7 +332,9 @@ class MSBGManager:
         upload(self.ctx, ptr, np.ascontiguousarray(vals, np.float32))
 
     def fine(self):
-        """Virtual fine lattice [block][field][axis][512] (field 0 = u_new, 1 = u_new - u_old)."""
+        """Padded fine lattice [block][field][axis][900] (field 0 = u_new, 1 = u_new - u_old): each
+        block's 8^3 nodes and a one-node halo, filled for the blocks within one block of a block
+        holding particles (other blocks' entries are not current)."""
         L = self.layout()
         return download(self.ctx, L.d_fine, np.float32, L.num_blocks * 5400).reshape(L.num_blocks, 5400)
 

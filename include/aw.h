@@ -87,7 +87,9 @@ typedef struct {
   float* d_u_star[3];           /* + body force */
   float* d_u_new[3];            /* after projection */
   float* d_mass[3];             /* folded particle weight per face */
-  float* d_fine;                /* virtual fine lattice: [block][field 0 new, 1 delta][axis][512] */
+  float* d_fine;                /* padded fine lattice [block][field 0 new, 1 delta][axis][900]: each
+                                   block's 8^3 nodes plus a one-node halo; filled for the blocks within
+                                   one block of a block holding particles, others not current */
 } aw_layout;
 
 typedef struct {
