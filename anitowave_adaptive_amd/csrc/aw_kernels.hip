@@ -656,10 +656,32 @@ __global__ __launch_bounds__(TPB) void p2g_gather_kernel(GridDev g, const uint32
                                                          float* Mz, float* mx, float* my, float* mz) {
   __shared__ float fM[3 * FINE], fm[3 * FINE];
   __shared__ float nM[3 * 576], nm[3 * 576];
+  __shared__ int occ[27];   // occupied 3x3x3 neighbours, (dx + 1) + 3 (dy + 1) + 9 (dz + 1)
+  __shared__ int any_occ;
   const int B = blockIdx.x, tid = threadIdx.x;
   const int e = g.level[B], r = BASE >> e, nf = (r + 1) * r * r;
   int Bc[3];
   block_coords(g.D, B, Bc);
+  if (tid == 0) any_occ = 0;
+  __syncthreads();
+  if (tid < 27) {
+    const int q[3] = {Bc[0] + tid % 3 - 1, Bc[1] + (tid / 3) % 3 - 1, Bc[2] + tid / 9 - 1};
+    const bool in = q[0] >= 0 && q[1] >= 0 && q[2] >= 0 && q[0] < g.D[0] && q[1] < g.D[1] && q[2] < g.D[2];
+    occ[tid] = in && pcount[block_id(g.D, q)] > 0;
+    if (occ[tid]) any_occ = 1;
+  }
+  __syncthreads();
+  float* dM[3] = {Mx, My, Mz};
+  float* dm[3] = {mx, my, mz};
+  if (!any_occ) {
+    // no particle within one block: every slot of the block is empty
+    for (int k = tid; k < 3 * nf; k += TPB) {
+      const int a = k / nf, f = k - a * nf;
+      dM[a][g.foff[a][B] + f] = 0.f;
+      dm[a][g.foff[a][B] + f] = 0.f;
+    }
+    return;
+  }
   for (int k = tid; k < 3 * FINE; k += TPB) {
     const int a = k >> 9, li = k & 511;
     const int l[3] = {li & 7, (li >> 3) & 7, li >> 6};
@@ -673,9 +695,9 @@ __global__ __launch_bounds__(TPB) void p2g_gather_kernel(GridDev g, const uint32
     for (int dz = lo[2]; dz <= hi[2]; ++dz)
       for (int dy = lo[1]; dy <= hi[1]; ++dy)
         for (int dx = lo[0]; dx <= hi[0]; ++dx) {
+          if (!occ[(dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)]) continue;
           const int q[3] = {Bc[0] + dx, Bc[1] + dy, Bc[2] + dz};
           const int N = block_id(g.D, q);
-          if (!pcount[N]) continue;
           const int lp[3] = {l[0] - BASE * dx, l[1] - BASE * dy, l[2] - BASE * dz};
           const float* src = slab + (size_t)N * SLAB + a * PAD + pad_idx(a, lp);
           sM += src[0];
@@ -709,8 +731,6 @@ __global__ __launch_bounds__(TPB) void p2g_gather_kernel(GridDev g, const uint32
     }
   }
   __syncthreads();
-  float* dM[3] = {Mx, My, Mz};
-  float* dm[3] = {mx, my, mz};
   for (int k = tid; k < 3 * nf; k += TPB) {
     const int a = k / nf, f = k - a * nf;
     dM[a][g.foff[a][B] + f] = nM[a * 576 + f];
