@@ -1293,6 +1293,19 @@ __device__ __forceinline__ void g2p_advect_one(const GridDev& g, const float* __
   }
 }
 
+// Work-groups are dealt to the 8 XCDs round-robin (group w on XCD w mod 8),
+// and each XCD has its own L2.  Particle kernels sample the padded lattice of
+// the particle's block, and a block's particles are consecutive after the
+// binning sort: with the plain mapping a block's groups run on all 8 XCDs and
+// each XCD's L2 fetches the block's lattice.  xcd_group gives XCD x the
+// consecutive groups [x q + min(x, r), ...) of a launch of q * 8 + r groups,
+// so that a block's particles stay on one XCD.
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int64_t xcd_group(int64_t w, int64_t nwg) {
+  const int64_t q = nwg / NUM_XCD, r = nwg % NUM_XCD, x = w % NUM_XCD, j = w / NUM_XCD;
+  return x * q + (x < r ? x : r) + j;
+}
+
 // FIXUP = false: every particle; FIXUP = true: the particles listed by the
 // first pass (a grid-stride loop over list[0] entries, so the host needs no
 // count)
@@ -1302,7 +1315,7 @@ __global__ __launch_bounds__(TPB) void g2p_advect_kernel(GridDev g, const float*
                                                          const float* __restrict__ svel, int64_t n, float alpha,
                                                          float dt, float* __restrict__ pos_out,
                                                          float* __restrict__ vel_out, uint32_t* __restrict__ list) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = (FIXUP ? (int64_t)blockIdx.x : xcd_group(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
   if (FIXUP) {
     const int64_t m = (int64_t)list[0];
     for (int64_t j = i; j < m; j += (int64_t)gridDim.x * blockDim.x)
@@ -1542,7 +1555,7 @@ __global__ __launch_bounds__(TPB) void g2p_adjoint_stage_kernel(GridDev g, const
                                                                 float alpha, float dt, float* __restrict__ xbar,
                                                                 float* __restrict__ vbar, AdjRec* __restrict__ rec,
                                                                 uint32_t* __restrict__ list) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = (FIXUP ? (int64_t)blockIdx.x : xcd_group(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
   if (FIXUP) {
     const int64_t m = (int64_t)list[0];
     for (int64_t j = i; j < m; j += (int64_t)gridDim.x * blockDim.x)
@@ -1847,7 +1860,7 @@ __global__ __launch_bounds__(TPB) void p2g_adjoint_kernel(GridDev g, const float
                                                           const float* __restrict__ svel,
                                                           const int32_t* __restrict__ sids, int64_t n,
                                                           float* __restrict__ xbar, float* __restrict__ vbar) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = xcd_group(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t id = sids[i];
   float u[3], m1[3], gr[3] = {0.f, 0.f, 0.f};
