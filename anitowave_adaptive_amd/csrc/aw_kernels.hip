@@ -120,6 +120,94 @@ __device__ __forceinline__ int pi_weights(int a, int e, const int* l, int* slot,
   return n;
 }
 
+// One axis factor of pi_weights: fine index l on the normal (face) axis or a
+// transverse axis of a level-e block weighs 1 - t on slot i0 and t on i0 + 1.
+__device__ __forceinline__ float pi_w1(int e, bool normal, int l, int I) {
+  const int s = 1 << e, r = BASE >> e;
+  const float inv_s = 1.0f / (float)s;
+  int i0;
+  float t;
+  if (normal) {
+    i0 = l >> e;
+    t = (float)(l & (s - 1)) * inv_s;
+  } else {
+    float eta = ((float)l + 0.5f) * inv_s - 0.5f;
+    eta = fminf(fmaxf(eta, 0.f), (float)(r - 1));
+    int j = (int)floorf(eta);
+    j = r >= 2 ? min(j, r - 2) : 0;
+    i0 = j;
+    t = eta - (float)j;
+  }
+  return I == i0 ? 1.f - t : (I == i0 + 1 ? t : 0.f);
+}
+
+// Pi^T of a block of level e > 0 for two fields: slot (I0, I1, I2) of axis a
+// (face_idx order) = sum over the block's own fine nodes l of the product of
+// the three axis factors times F_a(l).  The product separates, so it runs as
+// three 1-D passes (x, then y, then z; at most 8 terms per output) through
+// LDS scratch instead of scattering each node's 8 weights with atomics (the
+// LDS float atomic is slow on gfx950 and neighbouring nodes share slots).
+// in(f, a, li) reads field f, axis a, own node li = l0 + 8 (l1 + 8 l2);
+// out(a, slot, v0, v1) receives both fields of a slot.  tA*: 3 * 576 floats,
+// tB*: 3 * 512 floats; the caller synchronises before reusing them.
+template <class In, class Out>
+__device__ __forceinline__ void pi_transpose_sep(int e, In in, float* tA0, float* tA1, float* tB0, float* tB1,
+                                                 Out out) {
+  const int r = BASE >> e, tid = threadIdx.x;
+  const int cA = (3 * r + 1) * 64;   // pass x outputs: n0(a) x 8 x 8 over the three axes
+  for (int q = tid; q < cA; q += blockDim.x) {
+    const int a = q < (r + 1) * 64 ? 0 : (q < (2 * r + 1) * 64 ? 1 : 2);
+    const int qa = q - (a == 0 ? 0 : (a == 1 ? (r + 1) * 64 : (2 * r + 1) * 64));
+    const int n0 = a == 0 ? r + 1 : r;
+    const int I0 = qa % n0, l12 = qa / n0;
+    float v0 = 0.f, v1 = 0.f;
+    for (int l0 = 0; l0 < BASE; ++l0) {
+      const float w = pi_w1(e, a == 0, l0, I0);
+      if (w == 0.f) continue;
+      v0 += w * in(0, a, l0 + BASE * l12);
+      v1 += w * in(1, a, l0 + BASE * l12);
+    }
+    tA0[a * 576 + qa] = v0;
+    tA1[a * 576 + qa] = v1;
+  }
+  __syncthreads();
+  // pass y: n0 x n1 x 8 per axis
+  const int cB = ((r + 1) * r + r * (r + 1) + r * r) * 8;
+  for (int q = tid; q < cB; q += blockDim.x) {
+    const int c0 = (r + 1) * r * 8, c1 = c0 + r * (r + 1) * 8;
+    const int a = q < c0 ? 0 : (q < c1 ? 1 : 2);
+    const int qa = q - (a == 0 ? 0 : (a == 1 ? c0 : c1));
+    const int n0 = a == 0 ? r + 1 : r, n1 = a == 1 ? r + 1 : r;
+    const int I0 = qa % n0, I1 = (qa / n0) % n1, l2 = qa / (n0 * n1);
+    float v0 = 0.f, v1 = 0.f;
+    for (int l1 = 0; l1 < BASE; ++l1) {
+      const float w = pi_w1(e, a == 1, l1, I1);
+      if (w == 0.f) continue;
+      v0 += w * tA0[a * 576 + I0 + n0 * (l1 + BASE * l2)];
+      v1 += w * tA1[a * 576 + I0 + n0 * (l1 + BASE * l2)];
+    }
+    tB0[a * 512 + qa] = v0;
+    tB1[a * 512 + qa] = v1;
+  }
+  __syncthreads();
+  // pass z: the slots
+  const int nfa = (r + 1) * r * r;
+  for (int q = tid; q < 3 * nfa; q += blockDim.x) {
+    const int a = q / nfa, f = q - a * nfa;
+    const int n0 = a == 0 ? r + 1 : r, n1 = a == 1 ? r + 1 : r, n2 = a == 2 ? r + 1 : r;
+    const int I0 = f % n0, I1 = (f / n0) % n1, I2 = f / (n0 * n1);
+    (void)n2;
+    float v0 = 0.f, v1 = 0.f;
+    for (int l2 = 0; l2 < BASE; ++l2) {
+      const float w = pi_w1(e, a == 2, l2, I2);
+      if (w == 0.f) continue;
+      v0 += w * tB0[a * 512 + I0 + n0 * (I1 + n1 * l2)];
+      v1 += w * tB1[a * 512 + I0 + n0 * (I1 + n1 * l2)];
+    }
+    out(a, f, v0, v1);
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ T block_reduce_sum(T v) {
   __shared__ T red[TPB / 64];
@@ -706,6 +794,17 @@ __global__ __launch_bounds__(TPB) void p2g_gather_kernel(GridDev g, const uint32
     fM[k] = sM;
     fm[k] = sm;
   }
+  __syncthreads();
+  if (e > 0) {
+    pi_transpose_sep(
+        e, [&](int f, int a, int li) { return f ? fm[a * FINE + li] : fM[a * FINE + li]; }, nM, nm, fM, fm,
+        [&](int a, int f, float v0, float v1) {
+          dM[a][g.foff[a][B] + f] = v0;
+          dm[a][g.foff[a][B] + f] = v1;
+        });
+    return;
+  }
+  // a level-0 block's fine nodes are its native slots, one to one
   for (int k = tid; k < 3 * 576; k += TPB) {
     nM[k] = 0.f;
     nm[k] = 0.f;
@@ -718,17 +817,9 @@ __global__ __launch_bounds__(TPB) void p2g_gather_kernel(GridDev g, const uint32
     const int l[3] = {li & 7, (li >> 3) & 7, li >> 6};
     int slot[8];
     float w[8];
-    const int n = pi_weights(a, e, l, slot, w);
-    if (e == 0) {
-      // a level-0 block's fine nodes are its native slots, one to one
-      nM[a * 576 + slot[0]] += w[0] * M;
-      nm[a * 576 + slot[0]] += w[0] * m;
-      continue;
-    }
-    for (int j = 0; j < n; ++j) {
-      atomicAdd(&nM[a * 576 + slot[j]], w[j] * M);
-      atomicAdd(&nm[a * 576 + slot[j]], w[j] * m);
-    }
+    (void)pi_weights(a, e, l, slot, w);
+    nM[a * 576 + slot[0]] += w[0] * M;
+    nm[a * 576 + slot[0]] += w[0] * m;
   }
   __syncthreads();
   for (int k = tid; k < 3 * nf; k += TPB) {
@@ -1634,14 +1725,25 @@ __global__ __launch_bounds__(ADJ_T) void g2p_adjoint_scatter_kernel(GridDev g, c
 __global__ __launch_bounds__(TPB) void pi_transpose_kernel(GridDev g, const float* __restrict__ fbar, FacePtrs unb,
                                                            FacePtrs uob) {
   __shared__ float nA[3 * 576], nB[3 * 576];
+  __shared__ float tA[3 * 512], tB[3 * 512];
   const int B = blockIdx.x, tid = threadIdx.x;
   const int e = g.level[B], r = BASE >> e, nf = (r + 1) * r * r;
+  const float* src = fbar + (size_t)B * ADJ;
+  if (e > 0) {
+    pi_transpose_sep(
+        e, [&](int f, int a, int li) { return src[f * 3 * FINE + a * FINE + li]; }, nA, nB, tA, tB,
+        [&](int a, int f, float v0, float v1) {
+          unb.p[a][g.foff[a][B] + f] = v0;
+          uob.p[a][g.foff[a][B] + f] = -v1;
+        });
+    return;
+  }
+  // level 0: own fine nodes are the native slots, one to one
   for (int k = tid; k < 3 * 576; k += TPB) {
     nA[k] = 0.f;
     nB[k] = 0.f;
   }
   __syncthreads();
-  const float* src = fbar + (size_t)B * ADJ;
   for (int k = tid; k < 3 * FINE; k += TPB) {
     const float fa = src[k], fb = src[3 * FINE + k];
     if (fa == 0.f && fb == 0.f) continue;
@@ -1649,16 +1751,9 @@ __global__ __launch_bounds__(TPB) void pi_transpose_kernel(GridDev g, const floa
     const int l[3] = {li & 7, (li >> 3) & 7, li >> 6};
     int slot[8];
     float w[8];
-    const int nw = pi_weights(a, e, l, slot, w);
-    if (e == 0) {
-      nA[a * 576 + slot[0]] += w[0] * fa;
-      nB[a * 576 + slot[0]] += w[0] * fb;
-      continue;
-    }
-    for (int j = 0; j < nw; ++j) {
-      atomicAdd(&nA[a * 576 + slot[j]], w[j] * fa);
-      atomicAdd(&nB[a * 576 + slot[j]], w[j] * fb);
-    }
+    (void)pi_weights(a, e, l, slot, w);
+    nA[a * 576 + slot[0]] += w[0] * fa;
+    nB[a * 576 + slot[0]] += w[0] * fb;
   }
   __syncthreads();
   for (int k = tid; k < 3 * nf; k += TPB) {
