@@ -208,9 +208,9 @@ __device__ __forceinline__ void pi_transpose_sep(int e, In in, float* tA0, float
   }
 }
 
-template <typename T>
+template <typename T, int NT = TPB>
 __device__ __forceinline__ T block_reduce_sum(T v) {
-  __shared__ T red[TPB / 64];
+  __shared__ T red[NT / 64];
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -218,15 +218,16 @@ __device__ __forceinline__ T block_reduce_sum(T v) {
   __syncthreads();
   T s = 0;
   if (threadIdx.x == 0)
-    for (int i = 0; i < TPB / 64; ++i) s += red[i];
+    for (int i = 0; i < NT / 64; ++i) s += red[i];
   return s;
 }
 
 // Grid-wide sums go to SC_SLOTS accumulators per scalar (work-group b adds
 // into slot b % SC_SLOTS), so that thousands of work-groups do not serialise
 // on one address; readers add the slots.  Call from every thread.
+template <int NT = TPB>
 __device__ __forceinline__ void grid_sum(double v, double* acc) {
-  const double s = block_reduce_sum(v);
+  const double s = block_reduce_sum<double, NT>(v);
   if (threadIdx.x == 0) atomicAdd(acc + (blockIdx.x & (SC_SLOTS - 1)), s);
 }
 
@@ -2247,18 +2248,23 @@ __device__ __forceinline__ uint32_t parent_cell(const LevelDev& F, const LevelDe
 // The stencil kernels are latency-bound (one round of work-groups): they are
 // held to 64 VGPRs for eight waves per SIMD.
 #define AW_STENCIL_OCCUPANCY __attribute__((amdgpu_waves_per_eu(8, 8)))
+// A stencil work-group covers 512 cells with ST_T threads, ST_CPT cells each.
+// One cell per thread: each thread's chain of dependent loads is as short as
+// it gets (measured at 256^3: level-1 smoother phase 11.7 us against 15.9 us
+// with 256 threads x 2 cells, level 0 even; 4 cells per thread: 33 us).
+constexpr int ST_T = 512, ST_CPT = 512 / ST_T;
 
-__global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void smooth_kernel(
+__global__ __launch_bounds__(ST_T) AW_STENCIL_OCCUPANCY void smooth_kernel(
     LevelDev L, float* __restrict__ x, const float* __restrict__ b, WGMap wgm,
     const uint32_t* __restrict__ blist, float omega, int local_sweeps, int reverse) {
   __shared__ float sx[512];
   const WG w = wg_at(wgm, blockIdx.x);
-  CellRef cr[2];
-  bool act[2];
-  float bb[2], dg[2], ex[2], cw[2][6];
+  CellRef cr[ST_CPT];
+  bool act[ST_CPT];
+  float bb[ST_CPT], dg[ST_CPT], ex[ST_CPT], cw[ST_CPT][6];
 #pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const int i = threadIdx.x + k * TPB;
+  for (int k = 0; k < ST_CPT; ++k) {
+    const int i = threadIdx.x + k * ST_T;
     act[k] = false;
     const uint32_t* bd;
     if (!decode_bd(L, w, i, cr[k], bd)) continue;
@@ -2274,11 +2280,11 @@ __global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void smooth_kernel(
     for (int ph = 0; ph < 2; ++ph) {
       const int col = reverse ? 1 - ph : ph;
 #pragma unroll
-      for (int k = 0; k < 2; ++k) {
+      for (int k = 0; k < ST_CPT; ++k) {
         if (!act[k]) continue;
         const CellRef& c = cr[k];
         if (((c.l[0] + c.l[1] + c.l[2]) & 1) != col) continue;
-        const int i = threadIdx.x + k * TPB;
+        const int i = threadIdx.x + k * ST_T;
         const int base = i - c.li;
         const float off = ex[k] + inblock(cw[k], c.li, c.r, [&](int li) { return sx[base + li]; });
         const float res = bb[k] - (dg[k] * sx[i] - off);
@@ -2287,8 +2293,8 @@ __global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void smooth_kernel(
       __syncthreads();
     }
 #pragma unroll
-  for (int k = 0; k < 2; ++k)
-    if (act[k]) x[cr[k].c] = sx[threadIdx.x + k * TPB];
+  for (int k = 0; k < ST_CPT; ++k)
+    if (act[k]) x[cr[k].c] = sx[threadIdx.x + k * ST_T];
 }
 
 
@@ -2388,26 +2394,26 @@ __global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void uniform_smooth_kerne
 // r = b - A x (computeExplicitResidual, VCycle.cuh:81-131), 0 on air cells;
 // optionally fused with the restriction b_c = sum of children
 // (restrictResidualsKernel, MultigridTransfer.cuh:9-59) and x_c = 0.
-__global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void residual_kernel(
+__global__ __launch_bounds__(ST_T) AW_STENCIL_OCCUPANCY void residual_kernel(
     LevelDev L, const float* __restrict__ x, const float* __restrict__ b, float* __restrict__ r,
     WGMap wgm, const uint32_t* __restrict__ blist, Restr rs) {
   __shared__ float sx[512];
   __shared__ float sr[512];
   const WG w = wg_at(wgm, blockIdx.x);
-  CellRef cr[2];
-  bool ok[2];
-  const uint32_t* bd[2];
+  CellRef cr[ST_CPT];
+  bool ok[ST_CPT];
+  const uint32_t* bd[ST_CPT];
 #pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const int i = threadIdx.x + k * TPB;
+  for (int k = 0; k < ST_CPT; ++k) {
+    const int i = threadIdx.x + k * ST_T;
     ok[k] = decode_bd(L, w, i, cr[k], bd[k]);
     if (ok[k]) sx[i] = x[cr[k].c];
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < ST_CPT; ++k) {
     if (!ok[k]) continue;
-    const int i = threadIdx.x + k * TPB;
+    const int i = threadIdx.x + k * ST_T;
     const uint32_t c = cr[k].c;
     float rv = 0.f;
     if (L.liquid[c]) {
@@ -2428,7 +2434,7 @@ __global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void residual_kernel(
   const bool halves = rs.C.e[B0] != L.e[B0];
   const int rC = halves ? rF >> 1 : rF;
   const int cpbC = rC * rC * rC, cpbF = rF * rF * rF;
-  for (int j = threadIdx.x; j < w.count * cpbC; j += TPB) {
+  for (int j = threadIdx.x; j < w.count * cpbC; j += ST_T) {
     const int q = j / cpbC, lj = j - q * cpbC;
     const int B = blist[w.start + q];
     const int l[3] = {lj % rC, (lj / rC) % rC, lj / (rC * rC)};
@@ -2448,26 +2454,26 @@ __global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void residual_kernel(
 }
 
 // y = A x with sum.acc += x . y
-__global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void apply_dot_kernel(
+__global__ __launch_bounds__(ST_T) AW_STENCIL_OCCUPANCY void apply_dot_kernel(
     LevelDev L, const float* __restrict__ x, float* __restrict__ y, WGMap wgm,
     const uint32_t* __restrict__ blist, double* acc) {
   __shared__ float sx[512];
   const WG w = wg_at(wgm, blockIdx.x);
-  CellRef cr[2];
-  bool ok[2];
-  const uint32_t* bd[2];
+  CellRef cr[ST_CPT];
+  bool ok[ST_CPT];
+  const uint32_t* bd[ST_CPT];
 #pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const int i = threadIdx.x + k * TPB;
+  for (int k = 0; k < ST_CPT; ++k) {
+    const int i = threadIdx.x + k * ST_T;
     ok[k] = decode_bd(L, w, i, cr[k], bd[k]);
     if (ok[k]) sx[i] = x[cr[k].c];
   }
   __syncthreads();
   double part = 0.0;
 #pragma unroll
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < ST_CPT; ++k) {
     if (!ok[k]) continue;
-    const int i = threadIdx.x + k * TPB;
+    const int i = threadIdx.x + k * ST_T;
     const uint32_t c = cr[k].c;
     if (!L.liquid[c]) {
       y[c] = 0.f;
@@ -2481,7 +2487,7 @@ __global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void apply_dot_kernel(
     y[c] = v;
     part += (double)sx[i] * (double)v;
   }
-  if (acc) grid_sum(part, acc);
+  if (acc) grid_sum<ST_T>(part, acc);
 }
 
 // children sum of one coarse cell (halving / keeping block, or aggregation)
@@ -3216,7 +3222,7 @@ void k_masked_copy(hipStream_t s, float* dst, const float* src, const uint8_t* l
 
 void k_smooth(hipStream_t s, const LevelDev& L, float* x, const float* b, const WGMap& wg,
               const uint32_t* blist, float omega, int local_sweeps, int reverse) {
-  if (wg.n > 0) smooth_kernel<<<wg.n, TPB, 0, s>>>(L, x, b, wg, blist, omega, local_sweeps, reverse);
+  if (wg.n > 0) smooth_kernel<<<wg.n, ST_T, 0, s>>>(L, x, b, wg, blist, omega, local_sweeps, reverse);
 }
 
 void k_smooth_uniform(hipStream_t s, const int* sh, int lr, int e, const uint8_t* liquid, float* x, const float* b,
@@ -3234,12 +3240,12 @@ void k_smooth_uniform(hipStream_t s, const int* sh, int lr, int e, const uint8_t
 
 void k_residual(hipStream_t s, const LevelDev& L, const float* x, const float* b, float* r, const WGMap& wg,
                 const uint32_t* blist, const Restr& rs) {
-  if (wg.n > 0) residual_kernel<<<wg.n, TPB, 0, s>>>(L, x, b, r, wg, blist, rs);
+  if (wg.n > 0) residual_kernel<<<wg.n, ST_T, 0, s>>>(L, x, b, r, wg, blist, rs);
 }
 
 void k_apply_dot(hipStream_t s, const LevelDev& L, const float* x, float* y, const WGMap& wg,
                  const uint32_t* blist, double* acc) {
-  if (wg.n > 0) apply_dot_kernel<<<wg.n, TPB, 0, s>>>(L, x, y, wg, blist, acc);
+  if (wg.n > 0) apply_dot_kernel<<<wg.n, ST_T, 0, s>>>(L, x, y, wg, blist, acc);
 }
 
 void k_restrict(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, const float* rf, float* bc,
