@@ -2308,7 +2308,7 @@ struct ULevel {
   int lr, e;   // log2 block resolution, log2 cell edge
 };
 
-__global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void uniform_smooth_kernel(
+__global__ __launch_bounds__(ST_T) AW_STENCIL_OCCUPANCY void uniform_smooth_kernel(
     ULevel U, const uint8_t* __restrict__ liquid, float* __restrict__ x, const float* __restrict__ b, int col,
     float omega, int local_sweeps, int reverse) {
   __shared__ float sx[512];
@@ -2317,13 +2317,13 @@ __global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void uniform_smooth_kerne
   const int ncol = (D[0] * D[1] * D[2]) >> 1;       // blocks of one colour
   const int per = 512 >> sh3;                        // blocks per work-group
   const float s = (float)(1 << U.e);
-  int c[2], lo[2][3];
-  bool act[2];
-  float bb[2], dg[2], ex[2];
-  uint8_t inb[2];   // bit d: the face-d neighbour is in the block
+  int c[ST_CPT], lo[ST_CPT][3];
+  bool act[ST_CPT];
+  float bb[ST_CPT], dg[ST_CPT], ex[ST_CPT];
+  uint8_t inb[ST_CPT];   // bit d: the face-d neighbour is in the block
 #pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const int i = threadIdx.x + k * TPB;
+  for (int k = 0; k < ST_CPT; ++k) {
+    const int i = threadIdx.x + k * ST_T;
     act[k] = false;
     c[k] = -1;
     const int kb = blockIdx.x * per + (i >> sh3);
@@ -2372,10 +2372,10 @@ __global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void uniform_smooth_kerne
     for (int ph = 0; ph < 2; ++ph) {
       const int cc = reverse ? 1 - ph : ph;
 #pragma unroll
-      for (int k = 0; k < 2; ++k) {
+      for (int k = 0; k < ST_CPT; ++k) {
         if (!act[k]) continue;
         if (((lo[k][0] + lo[k][1] + lo[k][2]) & 1) != cc) continue;
-        const int i = threadIdx.x + k * TPB;
+        const int i = threadIdx.x + k * ST_T;
         float in = 0.f;
 #pragma unroll
         for (int d = 0; d < 6; ++d)
@@ -2387,8 +2387,8 @@ __global__ __launch_bounds__(TPB) AW_STENCIL_OCCUPANCY void uniform_smooth_kerne
       __syncthreads();
     }
 #pragma unroll
-  for (int k = 0; k < 2; ++k)
-    if (act[k]) x[c[k]] = sx[threadIdx.x + k * TPB];
+  for (int k = 0; k < ST_CPT; ++k)
+    if (act[k]) x[c[k]] = sx[threadIdx.x + k * ST_T];
 }
 
 // r = b - A x (computeExplicitResidual, VCycle.cuh:81-131), 0 on air cells;
@@ -2547,15 +2547,15 @@ __global__ __launch_bounds__(TPB) void coarsen_liquid_kernel(LevelDev F, LevelDe
 
 // x_f += scale * x_c[parent] on liquid cells (prolongateCorrection,
 // MultigridTransfer.cuh:63-102)
-__global__ __launch_bounds__(TPB) void prolong_kernel(LevelDev F, LevelDev C, int aggregated, float* __restrict__ xf,
-                                                      const float* __restrict__ xc, float scale,
-                                                      WGMap wgm,
-                                                      const uint32_t* __restrict__ blist) {
+__global__ __launch_bounds__(ST_T) void prolong_kernel(LevelDev F, LevelDev C, int aggregated, float* __restrict__ xf,
+                                                       const float* __restrict__ xc, float scale,
+                                                       WGMap wgm,
+                                                       const uint32_t* __restrict__ blist) {
   const WG w = wg_at(wgm, blockIdx.x);
 #pragma unroll
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < ST_CPT; ++k) {
     CellRef c;
-    if (!decode(F, w, blist, threadIdx.x + k * TPB, c)) continue;
+    if (!decode(F, w, blist, threadIdx.x + k * ST_T, c)) continue;
     if (!F.liquid[c.c]) continue;
     xf[c.c] += scale * xc[parent_cell(F, C, aggregated, c.B, c.l, c.r, c.li)];
   }
@@ -3235,7 +3235,7 @@ void k_smooth_uniform(hipStream_t s, const int* sh, int lr, int e, const uint8_t
   const int per = 512 >> (3 * lr);
   const int nwg = (ncol + per - 1) / per;
   if (nwg > 0)
-    uniform_smooth_kernel<<<nwg, TPB, 0, s>>>(U, liquid, x, b, col, omega, local_sweeps, reverse);
+    uniform_smooth_kernel<<<nwg, ST_T, 0, s>>>(U, liquid, x, b, col, omega, local_sweeps, reverse);
 }
 
 void k_residual(hipStream_t s, const LevelDev& L, const float* x, const float* b, float* r, const WGMap& wg,
@@ -3255,7 +3255,7 @@ void k_restrict(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggrega
 
 void k_prolong(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, float* xf, const float* xc,
                float scale, const WGMap& wgf, const uint32_t* blistf) {
-  if (wgf.n > 0) prolong_kernel<<<wgf.n, TPB, 0, s>>>(F, C, aggregated, xf, xc, scale, wgf, blistf);
+  if (wgf.n > 0) prolong_kernel<<<wgf.n, ST_T, 0, s>>>(F, C, aggregated, xf, xc, scale, wgf, blistf);
 }
 
 void k_coarsen_liquid(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, uint8_t* liquid_c,
