@@ -1318,7 +1318,7 @@ int p2g_adjoint(aw_grid* g, float* xbar, float* vbar) {
     k_fold_adjoint(s, gv, uo.p, ms.p, ubo.p, ubs.p, Mb.p, mb.p);
   }
   {
-    Prof pf(g->ctx, "pi_forward", 8.0 * 3 * g->nfaces + 4.0 * ADJ * g->nb);
+    Prof pf(g->ctx, "pi_forward", 8.0 * g->lattice_faces + 4.0 * ADJ * g->lattice_blocks);
     k_pi_forward2(s, gv, Mb.p, mb.p, g->fmbar.as<float>());
   }
   {

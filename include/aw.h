@@ -235,7 +235,8 @@ typedef struct {
   float* d_pbar;          /* dL/dp on liquid cells */
   float* d_rhsbar;        /* A^{-1} pbar */
   float* d_fine_bar;      /* fine lattice [block][field][axis][512]: field 0 Fnew-bar + Fdel-bar, field 1 Fdel-bar */
-  float* d_fine_mbar;     /* fine lattice: field 0 Pi Mbar, field 1 Pi mbar */
+  float* d_fine_mbar;     /* fine lattice: field 0 Pi Mbar, field 1 Pi mbar; current for the blocks
+                             within one block of a block holding particles */
   const int32_t* d_sorted_ids; /* particle id of each sorted slot of the last binning */
   int64_t num_sorted;
 } aw_adjoint_layout;
