@@ -118,6 +118,11 @@ def test_residual_and_smoother_parity(ctx, name):
             og.smooth(k, 1, rev)
             sv.smooth(k, 1, rev)
             assert rel_err(sv.level_get(k, "x"), og.mg_get(k, "x")) < TOL, ("smooth", k, rev)
+        # two iterations per call, as the V-cycle smooths
+        for rev in (False, True):
+            og.smooth(k, 2, rev)
+            sv.smooth(k, 2, rev)
+            assert rel_err(sv.level_get(k, "x"), og.mg_get(k, "x")) < TOL, ("smooth x2", k, rev)
 
 
 @pytest.mark.parametrize("name", NAMES)
