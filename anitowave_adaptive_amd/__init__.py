@@ -48,7 +48,8 @@ class SolveStats(ctypes.Structure):
 
 class StepParams(ctypes.Structure):
     _fields_ = [("dt", ctypes.c_float), ("gravity", ctypes.c_float * 3), ("flip_alpha", ctypes.c_float),
-                ("rel_tol", ctypes.c_double), ("max_iters", ctypes.c_int), ("rebuild_topology", ctypes.c_int)]
+                ("rel_tol", ctypes.c_double), ("max_iters", ctypes.c_int), ("rebuild_topology", ctypes.c_int),
+                ("warm_start", ctypes.c_int)]
 
 
 class StepStats(ctypes.Structure):
@@ -481,7 +482,9 @@ class FlipStep:
     """The whole step (aw_step) on device particle buffers."""
 
     def __init__(self, grid, solver, dt=1.0 / 240.0, gravity=(0.0, -9.81, 0.0), alpha=0.95, tol=1e-6,
-                 max_iters=200, rebuild_topology=True):
+                 max_iters=200, rebuild_topology=True, warm_start=False):
+        """warm_start: start each pressure solve from the previous step's
+        pressure (remapped across topology changes) instead of zero."""
         self.grid, self.solver = grid, solver
         p = StepParams()
         p.dt = dt
@@ -490,6 +493,7 @@ class FlipStep:
         p.rel_tol = tol
         p.max_iters = max_iters
         p.rebuild_topology = int(rebuild_topology)
+        p.warm_start = int(warm_start)
         self.params = p
 
     def __call__(self, d_pos, d_vel, d_ids, n, allow_not_converged=False):

@@ -156,6 +156,9 @@ void k_project(hipStream_t s, const GridDev& g, const float* p, float* const* us
 // node from its owner's faces (count: particles per block; halo: per-block
 // flag written)
 void k_to_fine(hipStream_t s, const GridDev& g, float* fine, const uint32_t* count, uint8_t* halo);
+// p (current layout) from p_prev of the previous layout (its per-block levels and cell offsets)
+void k_pressure_remap(hipStream_t s, const GridDev& g, const uint8_t* level_prev, const uint32_t* coff_prev,
+                      const float* p_prev, float* p);
 // list: n + 1 words of scratch for the particles whose stage points leave the
 // blocks with fine-lattice halos (taken in a second pass)
 void k_g2p_advect(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const float* svel,

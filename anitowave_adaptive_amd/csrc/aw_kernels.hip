@@ -988,6 +988,27 @@ __global__ __launch_bounds__(TPB) void project_kernel(GridDev g, const float* __
 // fall); a sample elsewhere takes the owners' nodes one by one, from the
 // owner's lattice where it is filled and from its faces otherwise.
 
+// Warm start of the pressure solve: each cell of the current layout takes the
+// previous step's pressure of the previous layout's cell (same block, that
+// block's previous resolution) holding its centre.  Blocks do not move, so a
+// block whose level did not change copies its cells one to one.
+__global__ __launch_bounds__(TPB) void pressure_remap_kernel(GridDev g, const uint8_t* __restrict__ level_prev,
+                                                             const uint32_t* __restrict__ coff_prev,
+                                                             const float* __restrict__ p_prev,
+                                                             float* __restrict__ p) {
+  const int B = blockIdx.x;
+  const int e = g.level[B], r = BASE >> e, s = 1 << e;
+  const int eo = level_prev[B], ro = BASE >> eo;
+  const uint32_t co = g.coff[B], cp = coff_prev[B];
+  for (int li = threadIdx.x; li < r * r * r; li += TPB) {
+    const int l[3] = {li & (r - 1), (li / r) & (r - 1), li / (r * r)};
+    int lo[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) lo[d] = min((l[d] * s + (s >> 1)) >> eo, ro - 1);
+    p[co + li] = p_prev[cp + cell_idx(lo, ro)];
+  }
+}
+
 // halo[B] = a block of the 3x3x3 neighbourhood of B holds particles
 __global__ __launch_bounds__(TPB) void halo_flags_kernel(GridDev g, const uint32_t* __restrict__ count,
                                                          uint8_t* __restrict__ halo) {
@@ -3162,6 +3183,11 @@ void k_divergence(hipStream_t s, const GridDev& g, const uint8_t* liquid, float*
 void k_project(hipStream_t s, const GridDev& g, const float* p, float* const* ustar, float* const* beta,
                float* const* unew) {
   project_kernel<<<g.nb, TPB, 0, s>>>(g, p, fp(ustar), fp(beta), fp(unew));
+}
+
+void k_pressure_remap(hipStream_t s, const GridDev& g, const uint8_t* level_prev, const uint32_t* coff_prev,
+                      const float* p_prev, float* p) {
+  pressure_remap_kernel<<<g.nb, TPB, 0, s>>>(g, level_prev, coff_prev, p_prev, p);
 }
 
 void k_to_fine(hipStream_t s, const GridDev& g, float* fine, const uint32_t* count, uint8_t* halo) {

@@ -207,6 +207,11 @@ struct aw_grid {
   std::vector<uint32_t> h_coff;
   bool have_topology = false;
   int topo_version = 0;
+  // warm start: the pressure of the last step's solve, its layout version,
+  // and that layout's levels and cell offsets once the topology moves on
+  bool p_valid = false;
+  int p_version = -1;
+  DevBuf p_prev, level_prev, coff_prev;
   uint32_t ncells = 0, nfaces = 0;
   int64_t n_sorted = -1;
   double occupied = 0.0;   // blocks holding particles (for byte accounting only)
@@ -1902,6 +1907,19 @@ int aw_step(aw_grid* g, aw_solver* S, float* d_pos, float* d_vel, int32_t* d_ids
   if (n > 0x7FFFFFFF) return fail(AW_ERR_INVALID, "too many particles");
   hipStream_t s = g->ctx->stream;
   int changed = 0;
+  // warm start from the last step's pressure, if the layout it was solved in
+  // is still the grid's (nothing else rebuilt the topology since)
+  const bool warm = prm->warm_start && g->p_valid && g->have_topology && g->p_version == g->topo_version;
+  if (warm) {
+    AW_TRY(g->p_prev.ensure((size_t)g->ncells * sizeof(float)));
+    AW_TRY(g->level_prev.ensure(g->nb));
+    AW_TRY(g->coff_prev.ensure((size_t)(g->nb + 1) * sizeof(uint32_t)));
+    AW_HIP(hipMemcpyAsync(g->p_prev.p, g->p.p, (size_t)g->ncells * sizeof(float), hipMemcpyDeviceToDevice, s));
+    AW_HIP(hipMemcpyAsync(g->level_prev.p, g->level.p, g->nb, hipMemcpyDeviceToDevice, s));
+    AW_HIP(hipMemcpyAsync(g->coff_prev.p, g->coff.p, (size_t)(g->nb + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                          s));
+  }
+  g->p_valid = false;
   if (prm->rebuild_topology || !g->have_topology) {
     AW_TRY(bin_and_levels(g, d_pos, n, true, &changed));
   } else {
@@ -1911,10 +1929,18 @@ int aw_step(aw_grid* g, aw_solver* S, float* d_pos, float* d_vel, int32_t* d_ids
   AW_TRY(p2g_sorted(g, g->spos.as<float>(), g->svel.as<float>(), prm->dt, prm->gravity));
   AW_TRY(divergence(g));
   AW_TRY(ensure_setup(S));
-  k_fill_f32(s, g->p.as<float>(), g->ncells, 0.f);
+  if (warm) {
+    Prof pf(g->ctx, "pressure_remap", 8.0 * g->ncells + 9.0 * g->nb);
+    k_pressure_remap(s, g->view(), g->level_prev.as<uint8_t>(), g->coff_prev.as<uint32_t>(), g->p_prev.as<float>(),
+                     g->p.as<float>());
+  } else {
+    k_fill_f32(s, g->p.as<float>(), g->ncells, 0.f);
+  }
   aw_solve_stats sst{};
   int st = solve(S, prm->rel_tol, prm->max_iters, &sst);
   if (st != AW_OK && st != AW_ERR_NOT_CONVERGED) return st;
+  g->p_valid = true;
+  g->p_version = g->topo_version;
   std::string keep = g_err;
   AW_TRY(project(g));
   {
@@ -2031,6 +2057,7 @@ int aw_rollout_forward(aw_tape* t, float* d_pos, float* d_vel, int32_t* d_ids, c
   hipStream_t s = g->ctx->stream;
   const int64_t n = t->n;
   t->prm = *prm;
+  t->prm.warm_start = 0;   // every step of a rollout starts its solve from zero, as its recomputation does
   t->have_forward = false;
   t->mask_level.clear();
   t->mask_liquid.clear();
@@ -2047,7 +2074,7 @@ int aw_rollout_forward(aw_tape* t, float* d_pos, float* d_vel, int32_t* d_ids, c
       AW_TRY(copy_d2d(s, c.ids, d_ids, (size_t)n * sizeof(int32_t)));
     }
     aw_step_stats ss{};
-    int rc = aw_step(g, t->S, d_pos, d_vel, d_ids, n, prm, &ss);
+    int rc = aw_step(g, t->S, d_pos, d_vel, d_ids, n, &t->prm, &ss);
     if (rc != AW_OK && rc != AW_ERR_NOT_CONVERGED) return rc;
     if (rc != AW_OK) worst = rc;
     rs.forward_iterations += ss.solve.iterations;

@@ -117,6 +117,9 @@ typedef struct {
   double rel_tol;
   int max_iters;
   int rebuild_topology; /* 1 = mark/grade from the particles every step */
+  int warm_start;       /* 1 = start the pressure solve from the previous step's pressure, remapped
+                           cell by cell when the topology changed (no counterpart in the reference,
+                           whose p = 0 start main.cu:300 is the default 0); ignored by rollouts */
 } aw_step_params;
 
 typedef struct {
