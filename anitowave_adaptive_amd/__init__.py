@@ -99,6 +99,7 @@ SIGNATURES = {
     "aw_grid_create": (_I, [_V, _P(GridDesc), _P(_V)]),
     "aw_grid_destroy": (_I, [_V]),
     "aw_grid_build_topology": (_I, [_V, _V, _I64, _P(_I)]),
+    "aw_grid_build_topology_radius": (_I, [_V, _V, _V, _I64, _P(_I)]),
     "aw_grid_set_levels": (_I, [_V, _V]),
     "aw_grid_layout": (_I, [_V, _P(Layout)]),
     "aw_bin_particles": (_I, [_V, _V, _V, _V, _I64]),
@@ -282,6 +283,15 @@ class MSBGManager:
     def build_topology(self, d_pos, n):
         ch = ctypes.c_int(0)
         check(lib().aw_grid_build_topology(self._h, d_pos, int(n), ctypes.byref(ch)), "aw_grid_build_topology")
+        return bool(ch.value)
+
+    def build_topology_radius(self, d_pos, d_radius, n):
+        """Levels from per-particle radii (the reference's markActiveBlocks
+        rule, MSBG.cuh:271-314, with grading to a fixed point); d_radius is
+        a device float[n] in d_pos's order."""
+        ch = ctypes.c_int(0)
+        check(lib().aw_grid_build_topology_radius(self._h, d_pos, d_radius, int(n), ctypes.byref(ch)),
+              "aw_grid_build_topology_radius")
         return bool(ch.value)
 
     def set_levels(self, levels):

@@ -428,6 +428,65 @@ __global__ void grade_kernel(int D0, int D1, int D2, const uint8_t* __restrict__
   out[b] = (uint8_t)l;
 }
 
+// ---- alternative level rule: particle radii (markActiveBlocks, MSBG.cuh:271-314)
+// Each particle marks its block's neighbourhood of `support` blocks with its
+// level plev = floor(max(0, log2(h / radius))) clamped to [0, L-1]
+// (MSBG.cuh:286-291); the map starts at the coarsest level everywhere
+// (initRefinementMap with MAX_LEVELS - 1, main.cu:158-162), so every block
+// exists.  plev is evaluated exactly, not through log2f: with radius = m 2^E
+// (m in [0.5, 1)) and h = 2^H, log2(h / radius) = H - E + log2(1/m), and
+// log2(1/m) lies in (0, 1], reaching 1 only for m = 0.5.
+__device__ __forceinline__ int radius_level(float r, int logh, int L) {
+  int E;
+  const float m = frexpf(r, &E);
+  const int k = logh - E + (m == 0.5f ? 1 : 0);
+  return min(max(k, 0), L - 1);
+}
+
+// support = ceil(radius * cellSize / blockSize) = ceil(radius / res) at the
+// particle's level (MSBG.cuh:294-296, one for every radius up to res world
+// units); capped at the largest dimension, beyond which the box covers the
+// domain anyway.  The block is the particle's bin (clamped into the domain).
+// A radius that is not positive and finite sets *bad.  Lanes that repeat the
+// previous lane's (block, level, support), the common case for block-sorted
+// particles of one radius, skip their atomics.
+__global__ __launch_bounds__(TPB) void radius_mark_kernel(GridDev g, int logh, int L, int smax,
+                                                          const float* __restrict__ pos,
+                                                          const float* __restrict__ rad, int64_t n,
+                                                          uint32_t* __restrict__ rmap, uint32_t* __restrict__ bad) {
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  long long key = -1;
+  int c[3] = {0, 0, 0}, lev = 0, sup = 0;
+  if (i < n) {
+    const float r = rad[i];
+    if (!(r > 0.f) || isinf(r)) {
+      atomicOr(bad, 1u);
+    } else {
+      lev = radius_level(r, logh, L);
+      sup = (int)fminf(ceilf(r / (float)(BASE >> lev)), (float)smax);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const float u = fminf(fmaxf(pos[3 * i + a] * g.inv_h, 0.f), (float)g.N[a] - 0.0009765625f);
+        c[a] = ((int)u) >> LOGB;
+      }
+      key = ((long long)block_id(g.D, c) << 32) | ((long long)lev << 24) | sup;
+    }
+  }
+  const long long prev = __shfl_up(key, 1, 64);
+  if (key < 0 || ((threadIdx.x & 63) != 0 && prev == key)) return;
+  const int lo0 = max(0, c[0] - sup), hi0 = min(g.D[0] - 1, c[0] + sup);
+  const int lo1 = max(0, c[1] - sup), hi1 = min(g.D[1] - 1, c[1] + sup);
+  const int lo2 = max(0, c[2] - sup), hi2 = min(g.D[2] - 1, c[2] + sup);
+  for (int z = lo2; z <= hi2; ++z)
+    for (int y = lo1; y <= hi1; ++y)
+      for (int x = lo0; x <= hi0; ++x) atomicMin(&rmap[x + g.D[0] * (y + g.D[1] * z)], (uint32_t)lev);
+}
+
+__global__ void u32_to_u8_kernel(const uint32_t* __restrict__ in, uint8_t* __restrict__ out, int n) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < n) out[b] = (uint8_t)in[b];
+}
+
 // Single-work-group exclusive scan (the reference's thrust::exclusive_scan,
 // main.cu:270-273).  mode 0: values; 1: cells r^3 from levels; 2: face
 // slots (r+1) r^2 from levels.  out[n] receives the total.
@@ -3096,6 +3155,18 @@ void k_bin(hipStream_t s, const GridDev& g, const float* pos, int64_t n, int32_t
     bin_kernel<<<(unsigned)((n + TPB * AGG_PPT - 1) / (TPB * AGG_PPT)), TPB, 0, s>>>(g, pos, n, key, count);
 }
 
+// 2:1 grading to its fixed point: level l_b ends as min over blocks q of
+// l_q + (Chebyshev distance b..q), and terms with distance >= L - 1 cannot
+// go below l_b <= L - 1, so L - 1 Jacobi passes reach it.
+static void grade_passes(hipStream_t s, const int* D, int L, uint8_t* level, uint8_t* tmp) {
+  const int nb = D[0] * D[1] * D[2];
+  const int gr = (nb + TPB - 1) / TPB;
+  for (int it = 0; it < L - 1; ++it) {
+    grade_kernel<<<gr, TPB, 0, s>>>(D[0], D[1], D[2], level, tmp);
+    hipMemcpyAsync(level, tmp, nb, hipMemcpyDeviceToDevice, s);
+  }
+}
+
 void k_levels(hipStream_t s, const int* D, int L, const int* band, const uint32_t* count, uint8_t* surf,
               uint8_t* level, uint8_t* tmp) {
   const int nb = D[0] * D[1] * D[2];
@@ -3110,10 +3181,20 @@ void k_levels(hipStream_t s, const int* D, int L, const int* band, const uint32_
   cheb_pass_kernel<<<gr, TPB, 0, s>>>(D[0], D[1], D[2], 1, bmax, 0, 0, bd, tmp, level);
   cheb_pass_kernel<<<gr, TPB, 0, s>>>(D[0], D[1], D[2], 0, bmax, 0, L, bd, level, tmp);
   hipMemcpyAsync(level, tmp, nb, hipMemcpyDeviceToDevice, s);
-  for (int it = 0; it < L - 1; ++it) {
-    grade_kernel<<<gr, TPB, 0, s>>>(D[0], D[1], D[2], level, tmp);
-    hipMemcpyAsync(level, tmp, nb, hipMemcpyDeviceToDevice, s);
-  }
+  grade_passes(s, D, L, level, tmp);
+}
+
+void k_radius_levels(hipStream_t s, const GridDev& g, int logh, int L, const float* pos, const float* rad, int64_t n,
+                     uint32_t* rmap, uint8_t* level, uint8_t* tmp) {
+  const int nb = g.nb;
+  k_fill_u32(s, rmap, nb, (uint32_t)(L - 1));
+  k_fill_u32(s, rmap + nb, 1, 0u);
+  const int smax = max(g.D[0], max(g.D[1], g.D[2]));
+  if (n > 0)
+    radius_mark_kernel<<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, s>>>(g, logh, L, smax, pos, rad, n, rmap,
+                                                                       rmap + nb);
+  u32_to_u8_kernel<<<(nb + TPB - 1) / TPB, TPB, 0, s>>>(rmap, level, nb);
+  grade_passes(s, g.D, L, level, tmp);
 }
 
 size_t scan_scratch_words(int n) { return 2 * (size_t)(n / SCAN_CH + 2); }

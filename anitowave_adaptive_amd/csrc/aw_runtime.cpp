@@ -195,7 +195,7 @@ struct aw_grid {
   int D[3]{}, N[3]{};
   int nb = 0;
   float h = 0.f, inv_h = 0.f;
-  DevBuf level, surf, ltmp, count, poff, cursor, coff, foff, nbr, key, halo, g2p_list, scan_tmp;
+  DevBuf level, surf, ltmp, count, poff, cursor, coff, foff, nbr, key, halo, g2p_list, scan_tmp, rmap;
   DevBuf spos, svel, sids;
   DevBuf liquid, p, rhs, slab;
   DevBuf beta[3], Macc[3], macc[3], uold[3], ustar[3], unew[3], mass[3];
@@ -380,7 +380,10 @@ int apply_levels(aw_grid* g) {
   return launch_status("apply_levels");
 }
 
-int bin_and_levels(aw_grid* g, const float* d_pos, int64_t n, bool mark_levels, int* changed) {
+// radius_rule: levels from the particle radii d_radius instead of the
+// distance bands
+int bin_and_levels(aw_grid* g, const float* d_pos, int64_t n, bool mark_levels, int* changed,
+                   bool radius_rule = false, const float* d_radius = nullptr) {
   hipStream_t s = g->ctx->stream;
   AW_TRY(g->key.ensure((size_t)std::max<int64_t>(n, 1) * sizeof(int32_t)));
   k_fill_u32(s, g->count.as<uint32_t>(), g->nb, 0u);
@@ -415,15 +418,32 @@ int bin_and_levels(aw_grid* g, const float* d_pos, int64_t n, bool mark_levels, 
     }
   }
   if (!mark_levels) return launch_status("bin");
-  {
-    Prof pf(g->ctx, "levels", 8.0 * g->nb);
-    k_levels(s, g->D, g->desc.num_levels, g->desc.band, g->count.as<uint32_t>(), g->surf.as<uint8_t>(),
-             g->level.as<uint8_t>(), g->ltmp.as<uint8_t>());
-  }
   std::vector<uint8_t> nl(g->nb);
-  AW_HIP(hipMemcpyAsync(nl.data(), g->level.p, g->nb, hipMemcpyDeviceToHost, s));
-  AW_HIP(hipStreamSynchronize(s));
-  AW_TRY(launch_status("levels"));
+  if (radius_rule) {
+    // graded into surf (scratch here), checked, then made the level map
+    AW_TRY(g->rmap.ensure((size_t)(g->nb + 1) * sizeof(uint32_t)));
+    uint32_t bad = 0;
+    {
+      Prof pf(g->ctx, "levels", 16.0 * n + 12.0 * g->nb);
+      k_radius_levels(s, gv, std::ilogb(g->h), g->desc.num_levels, d_pos, d_radius, n, g->rmap.as<uint32_t>(),
+                      g->surf.as<uint8_t>(), g->ltmp.as<uint8_t>());
+    }
+    AW_HIP(hipMemcpyAsync(&bad, g->rmap.as<uint32_t>() + g->nb, sizeof bad, hipMemcpyDeviceToHost, s));
+    AW_HIP(hipMemcpyAsync(nl.data(), g->surf.p, g->nb, hipMemcpyDeviceToHost, s));
+    AW_HIP(hipStreamSynchronize(s));
+    AW_TRY(launch_status("levels"));
+    if (bad) return fail(AW_ERR_INVALID, "particle radius not positive and finite");
+    AW_HIP(hipMemcpyAsync(g->level.p, g->surf.p, g->nb, hipMemcpyDeviceToDevice, s));
+  } else {
+    {
+      Prof pf(g->ctx, "levels", 8.0 * g->nb);
+      k_levels(s, g->D, g->desc.num_levels, g->desc.band, g->count.as<uint32_t>(), g->surf.as<uint8_t>(),
+               g->level.as<uint8_t>(), g->ltmp.as<uint8_t>());
+    }
+    AW_HIP(hipMemcpyAsync(nl.data(), g->level.p, g->nb, hipMemcpyDeviceToHost, s));
+    AW_HIP(hipStreamSynchronize(s));
+    AW_TRY(launch_status("levels"));
+  }
   bool ch = !g->have_topology || nl != g->h_level;
   if (ch) {
     g->h_level.swap(nl);
@@ -1696,6 +1716,14 @@ int aw_grid_build_topology(aw_grid* g, const float* d_pos, int64_t n, int* chang
   AW_TRY(bin_and_levels(g, d_pos, n, true, changed));
   AW_HIP(hipStreamSynchronize(g->ctx->stream));
   return launch_status("build_topology");
+}
+
+int aw_grid_build_topology_radius(aw_grid* g, const float* d_pos, const float* d_radius, int64_t n, int* changed) {
+  if (!g || (n > 0 && (!d_pos || !d_radius)) || n < 0) return fail(AW_ERR_INVALID, "bad arguments");
+  if (n > 0x7FFFFFFF) return fail(AW_ERR_INVALID, "too many particles");
+  AW_TRY(bin_and_levels(g, d_pos, n, true, changed, true, d_radius));
+  AW_HIP(hipStreamSynchronize(g->ctx->stream));
+  return launch_status("build_topology_radius");
 }
 
 int aw_grid_set_levels(aw_grid* g, const uint8_t* h_levels) {

@@ -171,6 +171,37 @@ def run_mixed(args, w, pos, vel):
             "peak_hbm_gib": round(aw.memory_stats()[1] / 2**30, 3)}
 
 
+def run_warm(args, w, pos, vel):
+    """The headline workload with the pressure solve warm-started from the
+    previous step's pressure (remapped onto each new block layout), same
+    tolerance.  The reference starts every solve from zero, as the headline does."""
+    import anitowave_adaptive_amd as aw
+    ctx = aw.Context(0)
+    n = len(pos)
+    dpos = aw.DeviceArray.from_numpy(ctx, pos)
+    dvel = aw.DeviceArray.from_numpy(ctx, vel)
+    dids = aw.DeviceArray.from_numpy(ctx, np.arange(n, dtype=np.int32))
+    grid = aw.MSBGManager(ctx, w["dims"], 1.0 / w["n"], num_levels=w["levels"], band=w["band"])
+    solver = aw.VCycleSolver(grid)
+    step = aw.FlipStep(grid, solver, dt=1.0 / 240.0, gravity=(0.0, -9.81, 0.0), alpha=0.95, tol=args.tol,
+                       max_iters=200, rebuild_topology=True, warm_start=True)
+    for _ in range(args.warmup):
+        step(dpos.ptr, dvel.ptr, dids.ptr, n, allow_not_converged=True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    it, changed, rel = [], 0, 0.0
+    for _ in range(args.warm_steps):
+        st = step(dpos.ptr, dvel.ptr, dids.ptr, n, allow_not_converged=True)
+        it.append(st.solve.iterations)
+        changed += st.topology_changed
+        rel = max(rel, st.solve.rel_residual)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    return {"steps_per_s": round(args.warm_steps / dt, 3), "ms_per_step": round(1e3 * dt / args.warm_steps, 3),
+            "rel_tol": args.tol, "worst_rel_residual": rel, "steps": args.warm_steps,
+            "iterations_mean": round(float(np.mean(it)), 2), "topology_changes": changed}
+
+
 def run_adjoint(args, name=None, steps=None, scene=None):
     """Differentiable rollout, by default slosh128: 128^3 effective, 2 native
     levels, sloshing pool (y < 0.4 plus a seeded sinusoidal perturbation of
@@ -276,6 +307,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", dest="cpu", action="store_false")
     ap.add_argument("--no-adjoint", dest="adjoint", action="store_false")
     ap.add_argument("--mixed-steps", type=int, default=5, help="steps of the mixed-precision 1e-8 leg (0: skip)")
+    ap.add_argument("--warm-steps", type=int, default=20,
+                    help="steps of the warm-started leg (pressure from the previous step; 0: skip)")
     ap.add_argument("--adjoint-workload", default="slosh128")
     ap.add_argument("--adjoint-steps", type=int, default=0, help="rollout length (0: the workload's own)")
     ap.add_argument("--no-large", dest="large", action="store_false",
@@ -296,6 +329,10 @@ def main():
     if args.mixed_steps > 0:
         mixed = run_mixed(args, w, pos, vel)
         log(f"[bench] mixed precision 1e-8: {mixed}")
+    warm = None
+    if args.warm_steps > 0:
+        warm = run_warm(args, w, pos, vel)
+        log(f"[bench] warm-started pressure: {warm}")
     adj = run_adjoint(args) if args.adjoint else None
     large = run_large(args) if args.large else None
     cpu = None
@@ -341,6 +378,7 @@ def main():
                                         "us_per_launch": resid["us_per_launch"]},
             "level0_back_to_back": iso.get("residual_l0")},
         "mixed_precision_1e-8": mixed,
+        "warm_start": warm,
         "adjoint_forward_ratio": None if adj is None else adj["ratio"],
         "adjoint": adj,
         "kernels": per,

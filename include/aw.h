@@ -16,6 +16,8 @@
  *   aw_grid_build_topology               initRefinementMap, markActiveBlocks, enforceGrading,
  *                                        countBlockElements, exclusive_scan, allocateChannels
  *                                                                            src/main.cu:156-302, include/MSBG.cuh:249-376,599-659
+ *   aw_grid_build_topology_radius        the same, with markActiveBlocks's particle-radius levels
+ *                                                                            src/main.cu:156-196, include/MSBG.cuh:271-351
  *   aw_grid_layout                       MSBG POD fields                     include/MSBG.cuh:71-106
  *   aw_p2g                               P2GTransfer::run                    include/P2G.cuh:173-196
  *   aw_divergence, aw_project_velocity   (no counterpart: projection is missing upstream)
@@ -178,6 +180,19 @@ int aw_grid_destroy(aw_grid* grid);
  * *changed (may be NULL) is set to 1 when the level map differs from the
  * previous one. */
 int aw_grid_build_topology(aw_grid* grid, const float* d_pos, int64_t n, int* changed);
+/* The reference's level rule instead of the distance bands: every block
+ * starts at the coarsest level; each particle lowers the blocks within
+ * `support` blocks of its own to plev = floor(max(0, log2(h / radius))),
+ * clamped to num_levels - 1, with support = ceil(radius / res) at that level
+ * (res = 8 >> plev; one block for every radius up to res world units);
+ * then 2:1 grading to its fixed point.  d_radius is float[n] in d_pos's
+ * order; every radius must be positive and finite (else AW_ERR_INVALID and
+ * the topology is left as it was).  Otherwise as aw_grid_build_topology.
+ * aw_step with rebuild_topology = 1 rebuilds with the distance bands: keep
+ * rebuild_topology = 0 to step on this topology, as the reference does,
+ * whose topology is fixed at start-up. */
+int aw_grid_build_topology_radius(aw_grid* grid, const float* d_pos, const float* d_radius, int64_t n,
+                                  int* changed);
 /* Explicit native levels (host array, num_blocks) instead of the band rule. */
 int aw_grid_set_levels(aw_grid* grid, const uint8_t* h_levels);
 int aw_grid_layout(aw_grid* grid, aw_layout* out);

@@ -18,7 +18,8 @@
 // aggregation case analysis the HIP transfer kernels use.
 //
 // Parity pins (tests/test_oracle_*.py): closed-form layout and Morton values,
-// brute-force topology on tiny grids, A = A^T and A SPD, a dense solve of the
+// brute-force topology on tiny grids (distance bands; particle radii against
+// exact rational arithmetic), A = A^T and A SPD, a dense solve of the
 // single-level box problem, partition of unity / linear-field reproduction of
 // the transfers, mass conservation of P2G, hydrostatic column, divergence of
 // the projected field equal to the solver residual, RK3 exact translation;
@@ -177,6 +178,30 @@ void set_topology(OGrid& g, const uint8_t* level) {
   g.mg.clear();
 }
 
+// 2:1 grading over the 26-neighbourhood, double-buffered, to the fixed point
+// (L - 1 passes: the result is min over q of l_q + Chebyshev distance, and
+// terms at distance >= L - 1 cannot go below any level) (enforceGrading,
+// MSBG.cuh:316-351, ran 3 racy in-place passes).
+void grade(const OGrid& g, std::vector<uint8_t>& lv) {
+  std::vector<uint8_t> tmp(g.nb);
+  for (int it = 0; it < g.L - 1; ++it) {
+    for (int b = 0; b < g.nb; ++b) {
+      int c[3];
+      block_coords(g.D, b, c);
+      int l = lv[b];
+      for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dx = -1; dx <= 1; ++dx) {
+            int x = c[0] + dx, y = c[1] + dy, z = c[2] + dz;
+            if (x < 0 || y < 0 || z < 0 || x >= g.D[0] || y >= g.D[1] || z >= g.D[2]) continue;
+            l = std::min(l, (int)lv[block_id(g.D, x, y, z)] + 1);
+          }
+      tmp[b] = (uint8_t)l;
+    }
+    lv.swap(tmp);
+  }
+}
+
 // Level rule (replaces the particle-radius rule of markActiveBlocks,
 // MSBG.cuh:271-314, by the distance bands the workload specifies):
 //   occupied  = block holds >= 1 particle
@@ -203,7 +228,7 @@ void levels_from_counts(const OGrid& g, const uint32_t* count, uint8_t* out) {
   }
   int bmax = 0;
   for (int l = 0; l < g.L - 1; ++l) bmax = std::max(bmax, g.band[l]);
-  std::vector<uint8_t> lv(g.nb), tmp(g.nb);
+  std::vector<uint8_t> lv(g.nb);
   for (int b = 0; b < g.nb; ++b) {
     int c[3];
     block_coords(g.D, b, c);
@@ -220,23 +245,43 @@ void levels_from_counts(const OGrid& g, const uint32_t* count, uint8_t* out) {
       if (dist <= g.band[k]) { l = k; break; }
     lv[b] = (uint8_t)l;
   }
-  for (int it = 0; it < g.L - 1; ++it) {
-    for (int b = 0; b < g.nb; ++b) {
-      int c[3];
-      block_coords(g.D, b, c);
-      int l = lv[b];
-      for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-          for (int dx = -1; dx <= 1; ++dx) {
-            int x = c[0] + dx, y = c[1] + dy, z = c[2] + dz;
-            if (x < 0 || y < 0 || z < 0 || x >= g.D[0] || y >= g.D[1] || z >= g.D[2]) continue;
-            l = std::min(l, (int)lv[block_id(g.D, x, y, z)] + 1);
-          }
-      tmp[b] = (uint8_t)l;
-    }
-    lv.swap(tmp);
-  }
+  grade(g, lv);
   std::copy(lv.begin(), lv.end(), out);
+}
+
+// The reference's own level rule (markActiveBlocks, MSBG.cuh:271-314, after
+// initRefinementMap to MAX_LEVELS - 1, main.cu:158-162): every block starts
+// at the coarsest level; each particle lowers the blocks within `support`
+// blocks of its bin to its level
+//   plev    = floor(max(0, log2(h / radius))), clamped to [0, L-1]
+//   support = ceil(radius * cellSize / blockSize) = ceil(radius / res(plev))
+// then the same graded fixed point.  plev is the largest k <= L-1 with
+// radius 2^k <= h, found with exact power-of-two scaling in FP64.  Returns 1
+// (out untouched) if a radius is not positive and finite.
+int levels_from_radii(const OGrid& g, const float* pos, const float* radius, int64_t n, uint8_t* out) {
+  for (int64_t i = 0; i < n; ++i)
+    if (!(radius[i] > 0.f) || std::isinf(radius[i])) return 1;
+  std::vector<uint8_t> lv(g.nb, (uint8_t)(g.L - 1));
+  const int smax = std::max(g.D[0], std::max(g.D[1], g.D[2]));
+  for (int64_t i = 0; i < n; ++i) {
+    const double r = radius[i];
+    int plev = 0;
+    while (plev < g.L - 1 && std::ldexp(r, plev + 1) <= g.h) ++plev;
+    const int res = B >> plev;
+    const int support = (int)std::min<double>(std::ceil(r / res), smax);
+    int gc[3];
+    fine_cell(g, pos + 3 * i, gc);
+    const int c[3] = {gc[0] >> LOGB, gc[1] >> LOGB, gc[2] >> LOGB};
+    for (int z = std::max(0, c[2] - support); z <= std::min(g.D[2] - 1, c[2] + support); ++z)
+      for (int y = std::max(0, c[1] - support); y <= std::min(g.D[1] - 1, c[1] + support); ++y)
+        for (int x = std::max(0, c[0] - support); x <= std::min(g.D[0] - 1, c[0] + support); ++x) {
+          uint8_t& l = lv[block_id(g.D, x, y, z)];
+          l = std::min<uint8_t>(l, (uint8_t)plev);
+        }
+  }
+  grade(g, lv);
+  std::copy(lv.begin(), lv.end(), out);
+  return 0;
 }
 
 // ----------------------------------------------------- multigrid levels
@@ -1511,6 +1556,10 @@ int og_bin(void* p, const float* pos, int64_t n, int32_t* key, uint32_t* count) 
 int og_levels_from_counts(void* p, const uint32_t* count, uint8_t* out) {
   levels_from_counts(*static_cast<OGrid*>(p), count, out);
   return 0;
+}
+
+int og_levels_from_radii(void* p, const float* pos, const float* radius, int64_t n, uint8_t* out) {
+  return levels_from_radii(*static_cast<OGrid*>(p), pos, radius, n, out);
 }
 
 int og_set_levels(void* p, const uint8_t* level) {

@@ -40,6 +40,7 @@ def lib():
             "og_morton3d": (ctypes.c_uint32, [ctypes.c_uint32] * 3),
             "og_bin": (ctypes.c_int, [_V, _F, ctypes.c_int64, _I32, _U32]),
             "og_levels_from_counts": (ctypes.c_int, [_V, _U32, _U8]),
+            "og_levels_from_radii": (ctypes.c_int, [_V, _F, _F, ctypes.c_int64, _U8]),
             "og_set_levels": (ctypes.c_int, [_V, _U8]),
             "og_get_levels": (ctypes.c_int, [_V, _U8]),
             "og_get_offsets": (ctypes.c_int, [_V, _U32, _U32, _U32, _U32]),
@@ -140,6 +141,19 @@ class OracleGrid:
         out = np.zeros(self.nb, np.uint8)
         lib().og_levels_from_counts(self._h, _p(cnt, _U32), _p(out, _U8))
         return out
+
+    def levels_from_radii(self, pos, radius):
+        pos = np.ascontiguousarray(pos, np.float32)
+        radius = np.ascontiguousarray(radius, np.float32)
+        out = np.zeros(self.nb, np.uint8)
+        if lib().og_levels_from_radii(self._h, _p(pos, _F), _p(radius, _F), len(radius), _p(out, _U8)) != 0:
+            raise ValueError("particle radius not positive and finite")
+        return out
+
+    def build_topology_radius(self, pos, radius):
+        lv = self.levels_from_radii(pos, radius)
+        self.set_levels(lv)
+        return lv
 
     def set_levels(self, lv):
         lv = np.ascontiguousarray(lv, np.uint8)

@@ -1,7 +1,8 @@
 """Pins of the FP64 oracle against closed forms, brute force and invariants.
 
 None of these re-types the oracle's own formulas: layouts are checked against
-the reference's index definitions, topology against a brute-force Python rule,
+the reference's index definitions, topology against a brute-force Python rule
+(the particle-radius rule in exact rational arithmetic),
 the operator against a hand-assembled 7-point Laplacian and against the
 coarse-fine coefficients fixed by the geometry, transfers against partition of
 unity / linear reproduction / mass conservation, the projection against the
@@ -113,6 +114,102 @@ def test_levels_brute_force(seed):
         g = O.OracleGrid(D, num_levels=L, h=1.0 / 64, band=band)
         got = g.levels_from_counts(count)
         np.testing.assert_array_equal(got, brute_levels(D, L, band, count))
+
+
+def brute_grade(lv, D):
+    """2:1 grading over the 26-neighbourhood, iterated until nothing changes."""
+    coords = list(itertools.product(range(D[2]), range(D[1]), range(D[0])))
+    while True:
+        new = lv.copy()
+        for z, y, x in coords:
+            for dz, dy, dx in itertools.product((-1, 0, 1), repeat=3):
+                q = (z + dz, y + dy, x + dx)
+                if all(0 <= q[i] < (D[2], D[1], D[0])[i] for i in range(3)):
+                    new[z, y, x] = min(new[z, y, x], lv[q] + 1)
+        if np.array_equal(new, lv):
+            return lv
+        lv = new
+
+
+def brute_radius_levels(D, L, h, pos, radius):
+    """The reference's particle-radius rule (MSBG.cuh:286-298) in exact
+    rational arithmetic: plev = floor(max(0, log2(h / r))) as the largest k
+    with 2^k <= h / r, support = ceil(r / res), block = floor(x / h) // 8
+    clamped into the domain."""
+    from fractions import Fraction
+    import math
+    lv = np.full((D[2], D[1], D[0]), L - 1, int)
+    smax = max(D)
+    H = Fraction(h)
+    for p, r in zip(pos, radius):
+        R = Fraction(float(r))
+        k = 0
+        while k < L - 1 and R * 2 ** (k + 1) <= H:
+            k += 1
+        res = 8 >> k
+        sup = min(math.ceil(R / res), smax)
+        c = [min(max(math.floor(Fraction(float(p[a])) / H), 0), 8 * D[a] - 1) // 8 for a in range(3)]
+        for z in range(max(0, c[2] - sup), min(D[2] - 1, c[2] + sup) + 1):
+            for y in range(max(0, c[1] - sup), min(D[1] - 1, c[1] + sup) + 1):
+                for x in range(max(0, c[0] - sup), min(D[0] - 1, c[0] + sup) + 1):
+                    lv[z, y, x] = min(lv[z, y, x], k)
+    return brute_grade(lv, D).reshape(-1).astype(np.uint8)
+
+
+def radius_cases(rng, m, h, D):
+    """Positions over the grid (some outside it) and radii at and around the
+    powers of two that bound each level, plus a few large enough for a
+    support of two blocks."""
+    ext = np.array([8 * d * h for d in D])
+    pos = (rng.random((m, 3)) * 1.2 - 0.1) * ext
+    k = rng.integers(-1, 5, m)
+    f = rng.choice([1.0, 0.5, 0.75, 1.5, np.nextafter(1.0, 0.0), np.nextafter(1.0, 2.0), 0.999, 1.001], m)
+    r = h * np.ldexp(f, -k)
+    big = [9.0, 8.5, 16.0 + 1e-3]
+    r[: min(m, 3)] = big[: min(m, 3)]
+    return pos.astype(np.float32), r.astype(np.float32)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_radius_levels_brute_force(seed):
+    D = (6, 5, 4)
+    rng = np.random.default_rng(seed)
+    for L in (1, 2, 3, 4):
+        g = O.OracleGrid(D, num_levels=L, h=1.0 / 64)
+        for m in (0, 1, 7, 60):
+            pos, r = radius_cases(rng, m, 1.0 / 64, D)
+            got = g.levels_from_radii(pos, r)
+            np.testing.assert_array_equal(got, brute_radius_levels(D, L, 1.0 / 64, pos, r), err_msg=f"L={L} m={m}")
+
+
+def test_radius_levels_known_answers():
+    """One particle in block (2, 3, 1) of a 6^3-block grid, 4 levels: radius
+    h/4 marks its 27 blocks with level 2 and leaves level 3 elsewhere; radius
+    h marks them with level 0, and grading then sets every block to
+    min(3, max(0, d - 1)) with d its Chebyshev block distance to (2, 3, 1)."""
+    h = 1.0 / 64
+    D = (6, 6, 6)
+    g = O.OracleGrid(D, num_levels=4, h=h)
+    pos = np.array([[(2 * 8 + 3) * h, (3 * 8 + 5) * h, (1 * 8 + 7.5) * h]], np.float32)
+    z, y, x = np.meshgrid(range(6), range(6), range(6), indexing="ij")
+    d = np.maximum(np.maximum(np.abs(x - 2), np.abs(y - 3)), np.abs(z - 1)).reshape(-1)
+    got = g.levels_from_radii(pos, np.array([h / 4], np.float32))
+    np.testing.assert_array_equal(got, np.where(d <= 1, 2, 3))
+    got = g.levels_from_radii(pos, np.array([h], np.float32))
+    np.testing.assert_array_equal(got, np.minimum(3, np.maximum(0, d - 1)))
+    # just above h / 4 the level is 1, just below h / 8 it is 3
+    got = g.levels_from_radii(pos, np.array([np.nextafter(np.float32(h / 4), np.float32(1))], np.float32))
+    np.testing.assert_array_equal(got, np.where(d <= 1, 1, np.minimum(3, d)))
+    got = g.levels_from_radii(pos, np.array([np.nextafter(np.float32(h / 8), np.float32(0))], np.float32))
+    np.testing.assert_array_equal(got, np.full(216, 3))
+
+
+@pytest.mark.parametrize("bad", [0.0, -1.0, np.nan, np.inf])
+def test_radius_levels_reject_bad_radius(bad):
+    g = O.OracleGrid((4, 4, 4), num_levels=3, h=1.0 / 32)
+    pos = np.full((2, 3), 0.5, np.float32)
+    with pytest.raises(ValueError):
+        g.levels_from_radii(pos, np.array([1.0 / 64, bad], np.float32))
 
 
 def test_levels_pool_scene_graded():
