@@ -447,16 +447,21 @@ __device__ __forceinline__ int radius_level(float r, int logh, int L) {
 // particle's level (MSBG.cuh:294-296, one for every radius up to res world
 // units); capped at the largest dimension, beyond which the box covers the
 // domain anyway.  The block is the particle's bin (clamped into the domain).
-// A radius that is not positive and finite sets *bad.  Lanes that repeat the
-// previous lane's (block, level, support), the common case for block-sorted
-// particles of one radius, skip their atomics.
+// A radius that is not positive and finite sets *bad.
+// Marking is split in two so that the same addresses are not hit 27 times
+// per wave: a particle of support 1 lowers only its own block's minimum
+// own[b] (radius_dilate_kernel then spreads each minimum over the 26
+// neighbours); a wider support marks its whole box in direct[] at once.
+// Lanes that repeat the previous lane's (block, level, support), the common
+// case for block-sorted particles of one radius, skip their atomics.
 __global__ __launch_bounds__(TPB) void radius_mark_kernel(GridDev g, int logh, int L, int smax,
                                                           const float* __restrict__ pos,
                                                           const float* __restrict__ rad, int64_t n,
-                                                          uint32_t* __restrict__ rmap, uint32_t* __restrict__ bad) {
+                                                          uint32_t* __restrict__ direct, uint32_t* __restrict__ own,
+                                                          uint32_t* __restrict__ bad) {
   const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
   long long key = -1;
-  int c[3] = {0, 0, 0}, lev = 0, sup = 0;
+  int c[3] = {0, 0, 0}, lev = 0, sup = 0, b = 0;
   if (i < n) {
     const float r = rad[i];
     if (!(r > 0.f) || isinf(r)) {
@@ -469,22 +474,41 @@ __global__ __launch_bounds__(TPB) void radius_mark_kernel(GridDev g, int logh, i
         const float u = fminf(fmaxf(pos[3 * i + a] * g.inv_h, 0.f), (float)g.N[a] - 0.0009765625f);
         c[a] = ((int)u) >> LOGB;
       }
-      key = ((long long)block_id(g.D, c) << 32) | ((long long)lev << 24) | sup;
+      b = block_id(g.D, c);
+      key = ((long long)b << 32) | ((long long)lev << 24) | sup;
     }
   }
   const long long prev = __shfl_up(key, 1, 64);
   if (key < 0 || ((threadIdx.x & 63) != 0 && prev == key)) return;
+  if (sup == 1) {
+    atomicMin(&own[b], (uint32_t)lev);
+    return;
+  }
   const int lo0 = max(0, c[0] - sup), hi0 = min(g.D[0] - 1, c[0] + sup);
   const int lo1 = max(0, c[1] - sup), hi1 = min(g.D[1] - 1, c[1] + sup);
   const int lo2 = max(0, c[2] - sup), hi2 = min(g.D[2] - 1, c[2] + sup);
   for (int z = lo2; z <= hi2; ++z)
     for (int y = lo1; y <= hi1; ++y)
-      for (int x = lo0; x <= hi0; ++x) atomicMin(&rmap[x + g.D[0] * (y + g.D[1] * z)], (uint32_t)lev);
+      for (int x = lo0; x <= hi0; ++x) atomicMin(&direct[x + g.D[0] * (y + g.D[1] * z)], (uint32_t)lev);
 }
 
-__global__ void u32_to_u8_kernel(const uint32_t* __restrict__ in, uint8_t* __restrict__ out, int n) {
+// level = min(direct[b], min over b and its 26 neighbours q of own[q])
+__global__ void radius_dilate_kernel(int D0, int D1, int D2, const uint32_t* __restrict__ direct,
+                                     const uint32_t* __restrict__ own, uint8_t* __restrict__ level) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < n) out[b] = (uint8_t)in[b];
+  int D[3] = {D0, D1, D2};
+  if (b >= D0 * D1 * D2) return;
+  int c[3];
+  block_coords(D, b, c);
+  uint32_t l = direct[b];
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        int q[3] = {c[0] + dx, c[1] + dy, c[2] + dz};
+        if (q[0] < 0 || q[1] < 0 || q[2] < 0 || q[0] >= D0 || q[1] >= D1 || q[2] >= D2) continue;
+        l = min(l, own[block_id(D, q)]);
+      }
+  level[b] = (uint8_t)l;
 }
 
 // Single-work-group exclusive scan (the reference's thrust::exclusive_scan,
@@ -3187,13 +3211,14 @@ void k_levels(hipStream_t s, const int* D, int L, const int* band, const uint32_
 void k_radius_levels(hipStream_t s, const GridDev& g, int logh, int L, const float* pos, const float* rad, int64_t n,
                      uint32_t* rmap, uint8_t* level, uint8_t* tmp) {
   const int nb = g.nb;
-  k_fill_u32(s, rmap, nb, (uint32_t)(L - 1));
-  k_fill_u32(s, rmap + nb, 1, 0u);
+  uint32_t *direct = rmap, *own = rmap + nb, *bad = rmap + 2 * (size_t)nb;
+  k_fill_u32(s, rmap, 2 * (size_t)nb, (uint32_t)(L - 1));
+  k_fill_u32(s, bad, 1, 0u);
   const int smax = max(g.D[0], max(g.D[1], g.D[2]));
   if (n > 0)
-    radius_mark_kernel<<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, s>>>(g, logh, L, smax, pos, rad, n, rmap,
-                                                                       rmap + nb);
-  u32_to_u8_kernel<<<(nb + TPB - 1) / TPB, TPB, 0, s>>>(rmap, level, nb);
+    radius_mark_kernel<<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, s>>>(g, logh, L, smax, pos, rad, n, direct, own,
+                                                                       bad);
+  radius_dilate_kernel<<<(nb + TPB - 1) / TPB, TPB, 0, s>>>(g.D[0], g.D[1], g.D[2], direct, own, level);
   grade_passes(s, g.D, L, level, tmp);
 }
 

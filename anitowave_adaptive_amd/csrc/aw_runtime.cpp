@@ -421,14 +421,14 @@ int bin_and_levels(aw_grid* g, const float* d_pos, int64_t n, bool mark_levels, 
   std::vector<uint8_t> nl(g->nb);
   if (radius_rule) {
     // graded into surf (scratch here), checked, then made the level map
-    AW_TRY(g->rmap.ensure((size_t)(g->nb + 1) * sizeof(uint32_t)));
+    AW_TRY(g->rmap.ensure((size_t)(2 * g->nb + 1) * sizeof(uint32_t)));
     uint32_t bad = 0;
     {
       Prof pf(g->ctx, "levels", 16.0 * n + 12.0 * g->nb);
       k_radius_levels(s, gv, std::ilogb(g->h), g->desc.num_levels, d_pos, d_radius, n, g->rmap.as<uint32_t>(),
                       g->surf.as<uint8_t>(), g->ltmp.as<uint8_t>());
     }
-    AW_HIP(hipMemcpyAsync(&bad, g->rmap.as<uint32_t>() + g->nb, sizeof bad, hipMemcpyDeviceToHost, s));
+    AW_HIP(hipMemcpyAsync(&bad, g->rmap.as<uint32_t>() + 2 * (size_t)g->nb, sizeof bad, hipMemcpyDeviceToHost, s));
     AW_HIP(hipMemcpyAsync(nl.data(), g->surf.p, g->nb, hipMemcpyDeviceToHost, s));
     AW_HIP(hipStreamSynchronize(s));
     AW_TRY(launch_status("levels"));

@@ -86,7 +86,27 @@ def run_gpu(args, w, pos, vel):
         iso = ctx.profile_report()
         ctx.profile(False)
     lv = list(st.num_blocks_by_level)
-    res = dict(seconds=t1 - t0, iters=iters, topology_changes=changed, blocks_by_level=lv,
+    # the particle-radius level rule on the same particles (block-sorted by
+    # the steps), radius h / 2 everywhere: the first call changes the
+    # topology, the timed ones find it unchanged
+    radius = None
+    if args.radius_calls > 0:
+        drad = aw.DeviceArray.from_numpy(ctx, np.full(n, 0.5 / w["n"], np.float32))
+        grid.build_topology_radius(dpos.ptr, drad.ptr, n)
+        ctx.synchronize()
+        ta = time.perf_counter()
+        for _ in range(args.radius_calls):
+            grid.build_topology_radius(dpos.ptr, drad.ptr, n)
+        tb = time.perf_counter()
+        ctx.profile(True, runs=True)
+        grid.build_topology_radius(dpos.ptr, drad.ptr, n)
+        rp = ctx.profile_report()
+        ctx.profile(False)
+        radius = {"ms_per_call": round(1e3 * (tb - ta) / args.radius_calls, 3), "particles": n,
+                  "radius": "h/2", "levels_ms": round(rp.get("levels", {}).get("ms", 0.0), 4),
+                  "bin_ms": round(rp.get("bin", {}).get("ms", 0.0), 4),
+                  "blocks_by_level": [int(v) for v in np.bincount(grid.levels(), minlength=w["levels"])]}
+    res = dict(seconds=t1 - t0, iters=iters, topology_changes=changed, blocks_by_level=lv, radius_rule=radius,
                total_cells=int(st.total_cells), rel_residual=st.solve.rel_residual, profile=prof, isolated=iso,
                peak_bytes=aw.memory_stats()[1])
     return res
@@ -268,7 +288,7 @@ def run_large(args):
     pos, vel = w["scene"](n=w["n"])
     log(f"[bench] large: seeded {len(pos)} particles in {time.perf_counter() - t:.1f}s")
     a = copy.copy(args)
-    a.steps, a.warmup, a.profile_steps, a.mixed_steps = 5, 2, 2, 3
+    a.steps, a.warmup, a.profile_steps, a.mixed_steps, a.radius_calls = 5, 2, 2, 3, 0
     gpu = run_gpu(a, w, pos, vel)
     rl, per = roofline(gpu["profile"])
     log(f"[bench] large: {a.steps / gpu['seconds']:.3f} steps/s")
@@ -309,6 +329,8 @@ def main():
     ap.add_argument("--mixed-steps", type=int, default=5, help="steps of the mixed-precision 1e-8 leg (0: skip)")
     ap.add_argument("--warm-steps", type=int, default=20,
                     help="steps of the warm-started leg (pressure from the previous step; 0: skip)")
+    ap.add_argument("--radius-calls", type=int, default=5,
+                    help="timed calls of the particle-radius level rule on the workload's particles (0: skip)")
     ap.add_argument("--adjoint-workload", default="slosh128")
     ap.add_argument("--adjoint-steps", type=int, default=0, help="rollout length (0: the workload's own)")
     ap.add_argument("--no-large", dest="large", action="store_false",
@@ -379,6 +401,7 @@ def main():
             "level0_back_to_back": iso.get("residual_l0")},
         "mixed_precision_1e-8": mixed,
         "warm_start": warm,
+        "topology_radius_rule": gpu["radius_rule"],
         "adjoint_forward_ratio": None if adj is None else adj["ratio"],
         "adjoint": adj,
         "kernels": per,

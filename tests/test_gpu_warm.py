@@ -9,7 +9,9 @@ within 2e-5 absolute and velocities within 1e-3 of max |v| after 6 steps of
 the 64^3 pool with a sphere falling at 8 m/s, whose block levels change at
 steps 2 and 3 (the oracle's topology of the same rollout, run on the CPU,
 changes there), so two warm starts go through the remap onto a new layout.
-The warm-started solves must take fewer PCG iterations in total.
+The warm-started solves must take fewer PCG iterations in total; the same
+holds for mixed precision to 1e-8, whose FP64 iterative refinement starts from
+the remapped pressure.
 """
 import numpy as np
 import pytest
@@ -23,12 +25,12 @@ pytestmark = pytest.mark.gpu
 STEPS = 6
 
 
-def _run(ctx, warm):
+def _run(ctx, warm, precision=0, tol=1e-6):
     pos, vel = scenes.pool_sphere(n=64, seed=1234, vy=-8.0)
     gg = gpu_grid(ctx, "pool64")
-    sv = aw.VCycleSolver(gg)
+    sv = aw.VCycleSolver(gg, precision=precision)
     gp = GpuParticles(ctx, pos, vel)
-    step = aw.FlipStep(gg, sv, dt=DT, gravity=GRAV, alpha=0.95, tol=1e-6, warm_start=warm)
+    step = aw.FlipStep(gg, sv, dt=DT, gravity=GRAV, alpha=0.95, tol=tol, warm_start=warm, max_iters=400)
     iters, changed = [], []
     for _ in range(STEPS):
         st = step(gp.pos.ptr, gp.vel.ptr, gp.ids.ptr, gp.n)
@@ -44,6 +46,19 @@ def test_warm_start_tracks_zero_start(ctx):
     print(f"[warm] PCG iterations zero start {it0}, warm start {it1}; topology changes {ch0}", flush=True)
     assert ch0 == ch1 and sum(ch0[1:]) >= 2, (ch0, ch1)
     assert it1[0] == it0[0], "the first step has no previous pressure"
+    assert sum(it1[1:]) < sum(it0[1:]), (it0, it1)
+    assert np.abs(P1 - P0).max() < 2e-5
+    vmax = max(np.abs(V0).max(), 1e-6)
+    assert np.abs(V1 - V0).max() / vmax < 1e-3
+
+
+def test_warm_start_mixed_precision(ctx):
+    """The FP64 iterative refinement starts from the remapped pressure too:
+    to 1e-8, fewer inner iterations in total and the same particles."""
+    P0, V0, it0, ch0 = _run(ctx, False, precision=1, tol=1e-8)
+    P1, V1, it1, ch1 = _run(ctx, True, precision=1, tol=1e-8)
+    print(f"[warm] mixed 1e-8 inner iterations zero start {it0}, warm start {it1}", flush=True)
+    assert ch0 == ch1, (ch0, ch1)
     assert sum(it1[1:]) < sum(it0[1:]), (it0, it1)
     assert np.abs(P1 - P0).max() < 2e-5
     vmax = max(np.abs(V0).max(), 1e-6)
