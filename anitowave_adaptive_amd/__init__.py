@@ -130,6 +130,12 @@ SIGNATURES = {
     "aw_rollout_forward": (_I, [_V, _V, _V, _V, _P(StepParams), _P(RolloutStats)]),
     "aw_rollout_backward": (_I, [_V, _V, _V, _P(RolloutStats)]),
     "aw_tape_masks": (_I, [_V, _I, _V, _V, ctypes.c_uint32, _P(ctypes.c_uint32)]),
+    "aw_resample_cells": (_I, [_V, _V, _P(_I), _V]),
+    "aw_write_vtk_structured": (_I, [ctypes.c_char_p, ctypes.c_char_p, _V, _P(_I), _P(_F), _P(_F)]),
+    "aw_write_vtk_blocks": (_I, [ctypes.c_char_p, _P(_I), _F, _V]),
+    "aw_write_vtk_particles": (_I, [ctypes.c_char_p, _V, _V, _V, _I64]),
+    "aw_export_cells_vtk": (_I, [_V, _V, _P(_I), ctypes.c_char_p, ctypes.c_char_p]),
+    "aw_export_blocks_vtk": (_I, [_V, ctypes.c_char_p]),
 }
 
 _lib = None
@@ -255,6 +261,45 @@ def upload(ctx, ptr, arr):
         check(lib().aw_copy(ctx.handle, ptr, arr.ctypes.data_as(_V), arr.nbytes), "aw_copy")
 
 
+def _i3(v):
+    return (ctypes.c_int * 3)(*[int(x) for x in v])
+
+
+def _f3(v):
+    return (ctypes.c_float * 3)(*[float(x) for x in v])
+
+
+def write_vtk_structured(path, name, data, dims, origin, spacing):
+    """Binary legacy VTK STRUCTURED_POINTS scalar field; data has dims[0] *
+    dims[1] * dims[2] float32 values, x fastest (host only)."""
+    data = np.ascontiguousarray(data, np.float32).reshape(-1)
+    if data.size != int(dims[0]) * int(dims[1]) * int(dims[2]):
+        raise ValueError("data size does not match dims")
+    check(lib().aw_write_vtk_structured(os.fsencode(path), name.encode(), data.ctypes.data_as(_V), _i3(dims),
+                                        _f3(origin), _f3(spacing)), "aw_write_vtk_structured")
+
+
+def write_vtk_blocks(path, dims, block_size, levels):
+    """One hexahedron per block with its level as refineLevel (host only)."""
+    lv = np.ascontiguousarray(levels, np.uint8).reshape(-1)
+    if lv.size != int(dims[0]) * int(dims[1]) * int(dims[2]):
+        raise ValueError("levels size does not match dims")
+    check(lib().aw_write_vtk_blocks(os.fsencode(path), _i3(dims), float(block_size), lv.ctypes.data_as(_V)),
+          "aw_write_vtk_blocks")
+
+
+def write_vtk_particles(path, pos, radius=None, vel=None):
+    """Particles as POLYDATA points with one vertex cell each, optional
+    radius and velocity (host only)."""
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    n = len(pos)
+    r = None if radius is None else np.ascontiguousarray(radius, np.float32).reshape(n)
+    v = None if vel is None else np.ascontiguousarray(vel, np.float32).reshape(n, 3)
+    check(lib().aw_write_vtk_particles(os.fsencode(path), pos.ctypes.data_as(_V),
+                                       None if r is None else r.ctypes.data_as(_V),
+                                       None if v is None else v.ctypes.data_as(_V), n), "aw_write_vtk_particles")
+
+
 class MSBGManager:
     """The block grid (MSBGManager, reference include/MSBG.cuh:502-677).
 
@@ -293,6 +338,21 @@ class MSBGManager:
         check(lib().aw_grid_build_topology_radius(self._h, d_pos, d_radius, int(n), ctypes.byref(ch)),
               "aw_grid_build_topology_radius")
         return bool(ch.value)
+
+    def resample(self, d_field, out_dims, d_out):
+        """Nearest-cell resample of a device cell field onto out_dims voxel
+        centres, x fastest (sampleMSBG, vtx_visualization.cuh:8-62)."""
+        check(lib().aw_resample_cells(self._h, d_field, _i3(out_dims), d_out), "aw_resample_cells")
+
+    def export_cells_vtk(self, d_field, out_dims, path, name="pressure"):
+        """Resample and write a binary legacy VTK STRUCTURED_POINTS file
+        (exportUniformPressure_VTK, vtx_visualization.cuh:64-126)."""
+        check(lib().aw_export_cells_vtk(self._h, d_field, _i3(out_dims), name.encode(), os.fsencode(path)),
+              "aw_export_cells_vtk")
+
+    def export_blocks_vtk(self, path):
+        """Blocks as hexahedra with their level (exportMultigrid_VTK, vtx_visualization.cuh:128-190)."""
+        check(lib().aw_export_blocks_vtk(self._h, os.fsencode(path)), "aw_export_blocks_vtk")
 
     def set_levels(self, levels):
         lv = np.ascontiguousarray(levels, np.uint8)

@@ -138,6 +138,9 @@ void k_levels(hipStream_t s, const int* D, int L, const int* band, const uint32_
 // tmp is scratch
 void k_radius_levels(hipStream_t s, const GridDev& g, int logh, int L, const float* pos, const float* rad, int64_t n,
                      uint32_t* rmap, uint8_t* level, uint8_t* tmp);
+// nearest-cell resample of a cell field onto an o[0] x o[1] x o[2] voxel grid
+// (x fastest); o[1], o[2] <= 65535 and (2 o - 1) N < 2^32 per axis
+void k_resample(hipStream_t s, const GridDev& g, const float* field, const int* o, float* out);
 // exclusive scans, out[n] = total; `scratch` holds scan_scratch_words(n) words
 size_t scan_scratch_words(int n);
 void k_scan_u32(hipStream_t s, const uint32_t* in, uint32_t* out, int n, uint32_t* scratch);

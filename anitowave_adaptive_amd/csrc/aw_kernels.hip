@@ -511,6 +511,60 @@ __global__ void radius_dilate_kernel(int D0, int D1, int D2, const uint32_t* __r
   level[b] = (uint8_t)l;
 }
 
+// ---- uniform resample of a cell field (sampleMSBG, vtx_visualization.cuh:8-62)
+// Voxel (x, y, z) of an o0 x o1 x o2 grid over the domain samples the cell
+// that contains its centre.  Along each axis that centre lies in fine cell
+// floor((2 q + 1) N / (2 o)), evaluated in exact integer arithmetic (the
+// reference's FP32 position can land on either side of a cell face; the
+// host keeps (2 o - 1) N below 2^32); its block and the native cell holding
+// it follow from the block's level.  A thread writes RS_V consecutive voxels
+// of one row (one float4 store when rows are multiples of 4), stepping the
+// fine cell by the quotient and remainder of 2N / 2o instead of dividing
+// again; a work-group covers RS_Y rows, blockIdx.z is z.
+constexpr int RS_X = 128, RS_Y = 2, RS_V = 4;
+
+__global__ __launch_bounds__(RS_X * RS_Y) void resample_kernel(GridDev g, const float* __restrict__ field, int o0,
+                                                               int o1, int o2, float* __restrict__ out) {
+  const int tx = threadIdx.x % RS_X, ty = threadIdx.x / RS_X;
+  const uint32_t x0 = ((uint32_t)blockIdx.x * RS_X + tx) * RS_V;
+  const uint32_t y = blockIdx.y * RS_Y + ty, z = blockIdx.z;
+  if (x0 >= (uint32_t)o0 || y >= (uint32_t)o1) return;
+  const uint32_t fy = ((2u * y + 1u) * (uint32_t)g.N[1]) / (2u * (uint32_t)o1);
+  const uint32_t fz = ((2u * z + 1u) * (uint32_t)g.N[2]) / (2u * (uint32_t)o2);
+  const uint32_t d = 2u * (uint32_t)o0, step = 2u * (uint32_t)g.N[0];
+  const uint32_t kq = step / d, kr = step % d;
+  const uint32_t num = (2u * x0 + 1u) * (uint32_t)g.N[0];
+  uint32_t fx = num / d, rem = num % d;
+  float v[RS_V];
+#pragma unroll
+  for (int j = 0; j < RS_V; ++j) {
+    if (j) {
+      fx += kq;
+      rem += kr;
+      if (rem >= d) {
+        rem -= d;
+        ++fx;
+      }
+    }
+    v[j] = 0.f;
+    if (x0 + j < (uint32_t)o0) {
+      const int c[3] = {(int)(fx >> LOGB), (int)(fy >> LOGB), (int)(fz >> LOGB)};
+      const int B = block_id(g.D, c);
+      const int e = g.level[B];
+      const int l[3] = {(int)(fx & (BASE - 1)) >> e, (int)(fy & (BASE - 1)) >> e, (int)(fz & (BASE - 1)) >> e};
+      v[j] = field[g.coff[B] + cell_idx(l, BASE >> e)];
+    }
+  }
+  float* row = out + (size_t)o0 * (y + (size_t)o1 * z);
+  if ((o0 & 3) == 0) {
+    *reinterpret_cast<float4*>(row + x0) = make_float4(v[0], v[1], v[2], v[3]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < RS_V; ++j)
+      if (x0 + j < (uint32_t)o0) row[x0 + j] = v[j];
+  }
+}
+
 // Single-work-group exclusive scan (the reference's thrust::exclusive_scan,
 // main.cu:270-273).  mode 0: values; 1: cells r^3 from levels; 2: face
 // slots (r+1) r^2 from levels.  out[n] receives the total.
@@ -3206,6 +3260,11 @@ void k_levels(hipStream_t s, const int* D, int L, const int* band, const uint32_
   cheb_pass_kernel<<<gr, TPB, 0, s>>>(D[0], D[1], D[2], 0, bmax, 0, L, bd, level, tmp);
   hipMemcpyAsync(level, tmp, nb, hipMemcpyDeviceToDevice, s);
   grade_passes(s, D, L, level, tmp);
+}
+
+void k_resample(hipStream_t s, const GridDev& g, const float* field, const int* o, float* out) {
+  resample_kernel<<<dim3((o[0] + RS_X * RS_V - 1) / (RS_X * RS_V), (o[1] + RS_Y - 1) / RS_Y, o[2]), RS_X * RS_Y, 0,
+                    s>>>(g, field, o[0], o[1], o[2], out);
 }
 
 void k_radius_levels(hipStream_t s, const GridDev& g, int logh, int L, const float* pos, const float* rad, int64_t n,

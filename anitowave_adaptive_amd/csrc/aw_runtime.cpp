@@ -2188,4 +2188,168 @@ int aw_tape_masks(aw_tape* t, int step, uint8_t* h_levels, uint8_t* h_liquid, ui
   return AW_OK;
 }
 
+// ---- uniform resample and VTK export (vtx_visualization.cuh) ----
+
+int aw_resample_cells(aw_grid* g, const float* d_field, const int out_dims[3], float* d_out) {
+  if (!g || !d_field || !out_dims || !d_out) return fail(AW_ERR_INVALID, "NULL argument");
+  if (!g->have_topology) return fail(AW_ERR_INVALID, "no topology");
+  for (int a = 0; a < 3; ++a) {
+    const int o = out_dims[a];
+    if (o < 1 || o > 65535 || (2ull * o - 1) * (uint64_t)(BASE * g->D[a]) >= (1ull << 32))
+      return fail(AW_ERR_INVALID, "output size %d along axis %d out of range", o, a);
+  }
+  // algorithmic bytes: the output once and the field once
+  Prof pf(g->ctx, "resample", 4.0 * out_dims[0] * out_dims[1] * out_dims[2] + 4.0 * g->ncells);
+  k_resample(g->ctx->stream, g->view(), d_field, out_dims, d_out);
+  return launch_status("resample");
+}
+
+namespace {
+
+bool one_token(const char* s) {
+  if (!s || !*s || std::strlen(s) > 200) return false;
+  for (const char* c = s; *c; ++c)
+    if (*c == ' ' || *c == '\n' || *c == '\r' || *c == '\t') return false;
+  return true;
+}
+
+// legacy VTK binary sections are big-endian 32-bit words
+void put_be32(FILE* f, const void* src, size_t words) {
+  const uint32_t* w = static_cast<const uint32_t*>(src);
+  std::vector<uint32_t> tmp(std::min<size_t>(std::max<size_t>(words, 1), 1 << 16));
+  for (size_t i = 0; i < words; i += tmp.size()) {
+    const size_t k = std::min(tmp.size(), words - i);
+    for (size_t j = 0; j < k; ++j) tmp[j] = __builtin_bswap32(w[i + j]);
+    std::fwrite(tmp.data(), 4, k, f);
+  }
+}
+
+struct OutFile {
+  FILE* f = nullptr;
+  explicit OutFile(const char* path) : f(std::fopen(path, "wb")) {}
+  ~OutFile() {
+    if (f) std::fclose(f);
+  }
+  int close(const char* path) {
+    const bool bad = std::ferror(f) != 0;
+    const int rc = std::fclose(f);
+    f = nullptr;
+    return bad || rc ? fail(AW_ERR_INVALID, "writing %s failed", path) : AW_OK;
+  }
+};
+
+}  // namespace
+
+int aw_write_vtk_structured(const char* path, const char* name, const float* h_data, const int dims[3],
+                            const float origin[3], const float spacing[3]) {
+  if (!path || !h_data || !dims || !origin || !spacing) return fail(AW_ERR_INVALID, "NULL argument");
+  if (!one_token(name)) return fail(AW_ERR_INVALID, "field name must be one word");
+  if (dims[0] < 1 || dims[1] < 1 || dims[2] < 1) return fail(AW_ERR_INVALID, "bad dimensions");
+  const size_t n = (size_t)dims[0] * dims[1] * dims[2];
+  OutFile o(path);
+  if (!o.f) return fail(AW_ERR_INVALID, "cannot open %s", path);
+  std::fprintf(o.f, "# vtk DataFile Version 3.0\nMSBG %s resample\nBINARY\nDATASET STRUCTURED_POINTS\n", name);
+  std::fprintf(o.f, "DIMENSIONS %d %d %d\nORIGIN %.9g %.9g %.9g\nSPACING %.9g %.9g %.9g\n", dims[0], dims[1], dims[2],
+               origin[0], origin[1], origin[2], spacing[0], spacing[1], spacing[2]);
+  std::fprintf(o.f, "POINT_DATA %zu\nSCALARS %s float 1\nLOOKUP_TABLE default\n", n, name);
+  put_be32(o.f, h_data, n);
+  std::fputc('\n', o.f);
+  return o.close(path);
+}
+
+int aw_write_vtk_blocks(const char* path, const int dims[3], float block_size, const uint8_t* h_levels) {
+  if (!path || !dims || !h_levels) return fail(AW_ERR_INVALID, "NULL argument");
+  if (dims[0] < 1 || dims[1] < 1 || dims[2] < 1 || (int64_t)dims[0] * dims[1] * dims[2] > (1 << 26))
+    return fail(AW_ERR_INVALID, "bad dimensions");
+  if (!(block_size > 0.f)) return fail(AW_ERR_INVALID, "block size must be positive");
+  const uint32_t nb = (uint32_t)dims[0] * dims[1] * dims[2];
+  OutFile o(path);
+  if (!o.f) return fail(AW_ERR_INVALID, "cannot open %s", path);
+  std::fprintf(o.f, "# vtk DataFile Version 3.0\nMSBG blocks\nBINARY\nDATASET UNSTRUCTURED_GRID\n");
+  std::fprintf(o.f, "POINTS %u float\n", 8 * nb);
+  // the eight corners of each block in VTK hexahedron order, blocks x fastest
+  static const int corner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                   {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+  std::vector<float> pts((size_t)24 * nb);
+  for (uint32_t b = 0; b < nb; ++b) {
+    const uint32_t c[3] = {b % dims[0], (b / dims[0]) % dims[1], b / (dims[0] * dims[1])};
+    for (int k = 0; k < 8; ++k)
+      for (int a = 0; a < 3; ++a) pts[24 * (size_t)b + 3 * k + a] = (float)(c[a] + corner[k][a]) * block_size;
+  }
+  put_be32(o.f, pts.data(), pts.size());
+  std::fprintf(o.f, "\nCELLS %u %u\n", nb, 9 * nb);
+  std::vector<int32_t> cells((size_t)9 * nb);
+  for (uint32_t b = 0; b < nb; ++b) {
+    cells[9 * (size_t)b] = 8;
+    for (int k = 0; k < 8; ++k) cells[9 * (size_t)b + 1 + k] = (int32_t)(8 * b + k);
+  }
+  put_be32(o.f, cells.data(), cells.size());
+  std::fprintf(o.f, "\nCELL_TYPES %u\n", nb);
+  std::vector<int32_t> v(nb, 12);  // VTK_HEXAHEDRON
+  put_be32(o.f, v.data(), nb);
+  std::fprintf(o.f, "\nCELL_DATA %u\nSCALARS refineLevel int 1\nLOOKUP_TABLE default\n", nb);
+  for (uint32_t b = 0; b < nb; ++b) v[b] = h_levels[b];
+  put_be32(o.f, v.data(), nb);
+  std::fputc('\n', o.f);
+  return o.close(path);
+}
+
+int aw_write_vtk_particles(const char* path, const float* h_pos, const float* h_radius, const float* h_vel,
+                           int64_t n) {
+  if (!path || (n > 0 && !h_pos) || n < 0) return fail(AW_ERR_INVALID, "bad arguments");
+  if (n > (1 << 30)) return fail(AW_ERR_INVALID, "too many particles for one legacy VTK file");
+  OutFile o(path);
+  if (!o.f) return fail(AW_ERR_INVALID, "cannot open %s", path);
+  std::fprintf(o.f, "# vtk DataFile Version 3.0\nParticle Cloud\nBINARY\nDATASET POLYDATA\nPOINTS %lld float\n",
+               (long long)n);
+  put_be32(o.f, h_pos, 3 * (size_t)n);
+  // one vertex cell per particle, so that the points render
+  std::fprintf(o.f, "\nVERTICES %lld %lld\n", (long long)n, 2 * (long long)n);
+  std::vector<int32_t> vc(2 * (size_t)std::min<int64_t>(n, 1 << 16));
+  for (int64_t i = 0; i < n; i += (int64_t)vc.size() / 2) {
+    const int64_t k = std::min<int64_t>((int64_t)vc.size() / 2, n - i);
+    for (int64_t j = 0; j < k; ++j) {
+      vc[2 * j] = 1;
+      vc[2 * j + 1] = (int32_t)(i + j);
+    }
+    put_be32(o.f, vc.data(), 2 * (size_t)k);
+  }
+  if (h_radius || h_vel) std::fprintf(o.f, "\nPOINT_DATA %lld", (long long)n);
+  if (h_radius) {
+    std::fprintf(o.f, "\nSCALARS radius float 1\nLOOKUP_TABLE default\n");
+    put_be32(o.f, h_radius, (size_t)n);
+  }
+  if (h_vel) {
+    std::fprintf(o.f, "\nVECTORS velocity float\n");
+    put_be32(o.f, h_vel, 3 * (size_t)n);
+  }
+  std::fputc('\n', o.f);
+  return o.close(path);
+}
+
+int aw_export_cells_vtk(aw_grid* g, const float* d_field, const int out_dims[3], const char* name, const char* path) {
+  if (!g || !d_field || !out_dims || !path) return fail(AW_ERR_INVALID, "NULL argument");
+  if (!one_token(name)) return fail(AW_ERR_INVALID, "field name must be one word");
+  const size_t n = (size_t)std::max(out_dims[0], 0) * std::max(out_dims[1], 0) * std::max(out_dims[2], 0);
+  DevBuf tmp;
+  AW_TRY(tmp.ensure(std::max<size_t>(n, 1) * sizeof(float)));
+  AW_TRY(aw_resample_cells(g, d_field, out_dims, tmp.as<float>()));
+  std::vector<float> h(n);
+  AW_HIP(hipMemcpyAsync(h.data(), tmp.p, n * sizeof(float), hipMemcpyDeviceToHost, g->ctx->stream));
+  AW_HIP(hipStreamSynchronize(g->ctx->stream));
+  float spacing[3], origin[3];
+  for (int a = 0; a < 3; ++a) {
+    // samples sit at voxel centres, half a spacing in from the domain's corner
+    spacing[a] = (float)((double)BASE * g->D[a] * g->h / out_dims[a]);
+    origin[a] = 0.5f * spacing[a];
+  }
+  return aw_write_vtk_structured(path, name, h.data(), out_dims, origin, spacing);
+}
+
+int aw_export_blocks_vtk(aw_grid* g, const char* path) {
+  if (!g || !path) return fail(AW_ERR_INVALID, "NULL argument");
+  if (!g->have_topology) return fail(AW_ERR_INVALID, "no topology");
+  return aw_write_vtk_blocks(path, g->D, (float)(BASE * g->h), g->h_level.data());
+}
+
 }  // extern "C"

@@ -86,6 +86,26 @@ def run_gpu(args, w, pos, vel):
         iso = ctx.profile_report()
         ctx.profile(False)
     lv = list(st.num_blocks_by_level)
+    # nearest-cell resample of the last pressure onto a uniform grid
+    resample = None
+    if args.resample > 0:
+        o = (args.resample,) * 3
+        nvox = int(np.prod(o))
+        dout = aw.DeviceArray(ctx, 4 * nvox)
+        lay = grid.layout()
+        grid.resample(lay.d_pressure, o, dout.ptr)
+        ctx.synchronize()
+        ctx.profile(True, runs=False)
+        for _ in range(5):
+            grid.resample(lay.d_pressure, o, dout.ptr)
+        ctx.synchronize()
+        g = ctx.profile_report()["resample"]
+        ctx.profile(False)
+        dout.free()
+        gbs = g["bytes"] / (g["ms"] * 1e-3) / 1e9
+        resample = {"out_dims": list(o), "us_per_call": round(1e3 * g["ms"] / g["launches"], 2),
+                    "bytes_per_call": round(g["bytes"] / g["launches"]), "gbs": round(gbs, 1),
+                    "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
     # the particle-radius level rule on the same particles (block-sorted by
     # the steps), radius h / 2 everywhere: the first call changes the
     # topology, the timed ones find it unchanged
@@ -107,6 +127,7 @@ def run_gpu(args, w, pos, vel):
                   "bin_ms": round(rp.get("bin", {}).get("ms", 0.0), 4),
                   "blocks_by_level": [int(v) for v in np.bincount(grid.levels(), minlength=w["levels"])]}
     res = dict(seconds=t1 - t0, iters=iters, topology_changes=changed, blocks_by_level=lv, radius_rule=radius,
+               resample=resample,
                total_cells=int(st.total_cells), rel_residual=st.solve.rel_residual, profile=prof, isolated=iso,
                peak_bytes=aw.memory_stats()[1])
     return res
@@ -288,7 +309,7 @@ def run_large(args):
     pos, vel = w["scene"](n=w["n"])
     log(f"[bench] large: seeded {len(pos)} particles in {time.perf_counter() - t:.1f}s")
     a = copy.copy(args)
-    a.steps, a.warmup, a.profile_steps, a.mixed_steps, a.radius_calls = 5, 2, 2, 3, 0
+    a.steps, a.warmup, a.profile_steps, a.mixed_steps, a.radius_calls, a.resample = 5, 2, 2, 3, 0, 0
     gpu = run_gpu(a, w, pos, vel)
     rl, per = roofline(gpu["profile"])
     log(f"[bench] large: {a.steps / gpu['seconds']:.3f} steps/s")
@@ -331,6 +352,8 @@ def main():
                     help="steps of the warm-started leg (pressure from the previous step; 0: skip)")
     ap.add_argument("--radius-calls", type=int, default=5,
                     help="timed calls of the particle-radius level rule on the workload's particles (0: skip)")
+    ap.add_argument("--resample", type=int, default=512,
+                    help="edge of the uniform grid the last pressure is resampled onto (0: skip)")
     ap.add_argument("--adjoint-workload", default="slosh128")
     ap.add_argument("--adjoint-steps", type=int, default=0, help="rollout length (0: the workload's own)")
     ap.add_argument("--no-large", dest="large", action="store_false",
@@ -402,6 +425,7 @@ def main():
         "mixed_precision_1e-8": mixed,
         "warm_start": warm,
         "topology_radius_rule": gpu["radius_rule"],
+        "resample": gpu["resample"],
         "adjoint_forward_ratio": None if adj is None else adj["ratio"],
         "adjoint": adj,
         "kernels": per,

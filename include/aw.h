@@ -30,6 +30,9 @@
  *                                                                            include/MultigridTransfer.cuh:107-149
  *   aw_solver_solve                      main.cu's fixed V-cycle loop        src/main.cu:314-317
  *   aw_step                              the main.cu step sequence, completed with the projection
+ *   aw_resample_cells                    sampleMSBG                          include/vtx_visualization.cuh:8-62
+ *   aw_write_vtk_*, aw_export_*_vtk      exportUniformPressure_VTK, exportMultigrid_VTK, exportParticles_VTK
+ *                                                                            include/vtx_visualization.cuh:64-216
  *   aw_*_adjoint, aw_tape_*, aw_rollout_* (no counterpart: the reference has no gradient code; these
  *                                        are the reverse-mode derivative of aw_step and of a rollout)
  */
@@ -306,6 +309,33 @@ int aw_rollout_backward(aw_tape* tape, float* d_xbar, float* d_vbar, aw_rollout_
  * forward step `step` used; h_liquid may be NULL to query *ncells. */
 int aw_tape_masks(aw_tape* tape, int step, uint8_t* h_levels, uint8_t* h_liquid, uint32_t liquid_cap,
                   uint32_t* ncells);
+
+/* ---- Uniform resample and VTK export (include/vtx_visualization.cuh) ----
+ * aw_resample_cells: nearest-cell resample of a cell field (float[total_cells]
+ * of the current topology, e.g. aw_layout.d_pressure) onto an out_dims[0] x
+ * out_dims[1] x out_dims[2] grid of voxel centres over the domain, x fastest,
+ * into d_out (sampleMSBG, vtx_visualization.cuh:8-62).  A centre on a cell
+ * face samples the cell above it (exact integer arithmetic).  Each size
+ * 1..65535 with (2 size - 1) * 8 * dims below 2^32.  Asynchronous.
+ * The writers are host-only and write legacy VTK with BINARY (big-endian)
+ * sections: aw_write_vtk_structured a STRUCTURED_POINTS scalar field (the
+ * reference's exportUniformPressure_VTK, vtx_visualization.cuh:64-126, wrote
+ * ASCII), aw_write_vtk_blocks one hexahedron per block with its level as
+ * refineLevel (exportMultigrid_VTK, :128-190), aw_write_vtk_particles
+ * positions as POLYDATA with one vertex cell each and optional radius and
+ * velocity (exportParticles_VTK, :192-216; h_radius, h_vel may be NULL).
+ * aw_export_cells_vtk resamples and writes (ORIGIN at the first voxel
+ * centre, where the reference put 0); aw_export_blocks_vtk writes the grid's
+ * blocks.  Synchronous; AW_ERR_INVALID if a file cannot be written. */
+int aw_resample_cells(aw_grid* grid, const float* d_field, const int out_dims[3], float* d_out);
+int aw_write_vtk_structured(const char* path, const char* name, const float* h_data, const int dims[3],
+                            const float origin[3], const float spacing[3]);
+int aw_write_vtk_blocks(const char* path, const int dims[3], float block_size, const uint8_t* h_levels);
+int aw_write_vtk_particles(const char* path, const float* h_pos, const float* h_radius, const float* h_vel,
+                           int64_t n);
+int aw_export_cells_vtk(aw_grid* grid, const float* d_field, const int out_dims[3], const char* name,
+                        const char* path);
+int aw_export_blocks_vtk(aw_grid* grid, const char* path);
 
 #ifdef __cplusplus
 }

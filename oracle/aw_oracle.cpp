@@ -1562,6 +1562,30 @@ int og_levels_from_radii(void* p, const float* pos, const float* radius, int64_t
   return levels_from_radii(*static_cast<OGrid*>(p), pos, radius, n, out);
 }
 
+// Nearest-cell resample onto an ox x oy x oz grid of voxel centres, x fastest
+// (sampleMSBG, vtx_visualization.cuh:8-62): the centre of voxel q along an
+// axis of N fine cells lies at (q + 1/2) N / o fine cells, in fine cell
+// floor((2q + 1) N / (2o)) (exact; a centre on a face goes to the cell above),
+// which names the block and the native cell holding it.
+int og_resample(void* p, const double* field, int ox, int oy, int oz, double* out) {
+  OGrid& g = *static_cast<OGrid*>(p);
+  const int o[3] = {ox, oy, oz};
+  for (int a = 0; a < 3; ++a)
+    if (o[a] < 1) return 1;
+  for (int z = 0; z < oz; ++z)
+    for (int y = 0; y < oy; ++y)
+      for (int x = 0; x < ox; ++x) {
+        const int q[3] = {x, y, z};
+        int gc[3];
+        for (int a = 0; a < 3; ++a) gc[a] = (int)(((2 * (int64_t)q[a] + 1) * g.N[a]) / (2 * (int64_t)o[a]));
+        const int b = block_id(g.D, gc[0] >> LOGB, gc[1] >> LOGB, gc[2] >> LOGB);
+        const int e = g.level[b], r = B >> e;
+        const int l[3] = {(gc[0] & (B - 1)) >> e, (gc[1] & (B - 1)) >> e, (gc[2] & (B - 1)) >> e};
+        out[x + (size_t)ox * (y + (size_t)oy * z)] = field[g.coff[b] + cell_index(l[0], l[1], l[2], r)];
+      }
+  return 0;
+}
+
 int og_set_levels(void* p, const uint8_t* level) {
   OGrid& g = *static_cast<OGrid*>(p);
   for (int b = 0; b < g.nb; ++b)

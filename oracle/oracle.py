@@ -42,6 +42,7 @@ def lib():
             "og_levels_from_counts": (ctypes.c_int, [_V, _U32, _U8]),
             "og_levels_from_radii": (ctypes.c_int, [_V, _F, _F, ctypes.c_int64, _U8]),
             "og_set_levels": (ctypes.c_int, [_V, _U8]),
+            "og_resample": (ctypes.c_int, [_V, _D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]),
             "og_get_levels": (ctypes.c_int, [_V, _U8]),
             "og_get_offsets": (ctypes.c_int, [_V, _U32, _U32, _U32, _U32]),
             "og_totals": (ctypes.c_int, [_V, _I64]),
@@ -154,6 +155,15 @@ class OracleGrid:
         lv = self.levels_from_radii(pos, radius)
         self.set_levels(lv)
         return lv
+
+    def resample(self, field, out_dims):
+        """Nearest-cell resample of a cell field onto out_dims voxel centres (x fastest)."""
+        field = np.ascontiguousarray(field, np.float64)
+        assert len(field) == int(self.totals()[0])
+        out = np.zeros(int(np.prod(out_dims)), np.float64)
+        if lib().og_resample(self._h, _p(field, _D), *[int(v) for v in out_dims], _p(out, _D)) != 0:
+            raise ValueError("bad output size")
+        return out
 
     def set_levels(self, lv):
         lv = np.ascontiguousarray(lv, np.uint8)
