@@ -454,42 +454,48 @@ __device__ __forceinline__ int radius_level(float r, int logh, int L) {
 // neighbours); a wider support marks its whole box in direct[] at once.
 // Lanes that repeat the previous lane's (block, level, support), the common
 // case for block-sorted particles of one radius, skip their atomics.
+constexpr int RM_PPT = 4;  // particles per thread, TPB apart (coalesced)
+
 __global__ __launch_bounds__(TPB) void radius_mark_kernel(GridDev g, int logh, int L, int smax,
                                                           const float* __restrict__ pos,
                                                           const float* __restrict__ rad, int64_t n,
                                                           uint32_t* __restrict__ direct, uint32_t* __restrict__ own,
                                                           uint32_t* __restrict__ bad) {
-  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
-  long long key = -1;
-  int c[3] = {0, 0, 0}, lev = 0, sup = 0, b = 0;
-  if (i < n) {
-    const float r = rad[i];
-    if (!(r > 0.f) || isinf(r)) {
-      atomicOr(bad, 1u);
-    } else {
-      lev = radius_level(r, logh, L);
-      sup = (int)fminf(ceilf(r / (float)(BASE >> lev)), (float)smax);
+  const int64_t base = (int64_t)blockIdx.x * (TPB * RM_PPT);
+#pragma unroll 1
+  for (int it = 0; it < RM_PPT; ++it) {
+    const int64_t i = base + it * TPB + threadIdx.x;
+    long long key = -1;
+    int c[3] = {0, 0, 0}, lev = 0, sup = 0, b = 0;
+    if (i < n) {
+      const float r = rad[i];
+      if (!(r > 0.f) || isinf(r)) {
+        atomicOr(bad, 1u);
+      } else {
+        lev = radius_level(r, logh, L);
+        sup = (int)fminf(ceilf(r / (float)(BASE >> lev)), (float)smax);
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const float u = fminf(fmaxf(pos[3 * i + a] * g.inv_h, 0.f), (float)g.N[a] - 0.0009765625f);
-        c[a] = ((int)u) >> LOGB;
+        for (int a = 0; a < 3; ++a) {
+          const float u = fminf(fmaxf(pos[3 * i + a] * g.inv_h, 0.f), (float)g.N[a] - 0.0009765625f);
+          c[a] = ((int)u) >> LOGB;
+        }
+        b = block_id(g.D, c);
+        key = ((long long)b << 32) | ((long long)lev << 24) | sup;
       }
-      b = block_id(g.D, c);
-      key = ((long long)b << 32) | ((long long)lev << 24) | sup;
     }
+    const long long prev = __shfl_up(key, 1, 64);
+    if (key < 0 || ((threadIdx.x & 63) != 0 && prev == key)) continue;
+    if (sup == 1) {
+      atomicMin(&own[b], (uint32_t)lev);
+      continue;
+    }
+    const int lo0 = max(0, c[0] - sup), hi0 = min(g.D[0] - 1, c[0] + sup);
+    const int lo1 = max(0, c[1] - sup), hi1 = min(g.D[1] - 1, c[1] + sup);
+    const int lo2 = max(0, c[2] - sup), hi2 = min(g.D[2] - 1, c[2] + sup);
+    for (int z = lo2; z <= hi2; ++z)
+      for (int y = lo1; y <= hi1; ++y)
+        for (int x = lo0; x <= hi0; ++x) atomicMin(&direct[x + g.D[0] * (y + g.D[1] * z)], (uint32_t)lev);
   }
-  const long long prev = __shfl_up(key, 1, 64);
-  if (key < 0 || ((threadIdx.x & 63) != 0 && prev == key)) return;
-  if (sup == 1) {
-    atomicMin(&own[b], (uint32_t)lev);
-    return;
-  }
-  const int lo0 = max(0, c[0] - sup), hi0 = min(g.D[0] - 1, c[0] + sup);
-  const int lo1 = max(0, c[1] - sup), hi1 = min(g.D[1] - 1, c[1] + sup);
-  const int lo2 = max(0, c[2] - sup), hi2 = min(g.D[2] - 1, c[2] + sup);
-  for (int z = lo2; z <= hi2; ++z)
-    for (int y = lo1; y <= hi1; ++y)
-      for (int x = lo0; x <= hi0; ++x) atomicMin(&direct[x + g.D[0] * (y + g.D[1] * z)], (uint32_t)lev);
 }
 
 // level = min(direct[b], min over b and its 26 neighbours q of own[q])
@@ -3275,8 +3281,8 @@ void k_radius_levels(hipStream_t s, const GridDev& g, int logh, int L, const flo
   k_fill_u32(s, bad, 1, 0u);
   const int smax = max(g.D[0], max(g.D[1], g.D[2]));
   if (n > 0)
-    radius_mark_kernel<<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, s>>>(g, logh, L, smax, pos, rad, n, direct, own,
-                                                                       bad);
+    radius_mark_kernel<<<(unsigned)((n + TPB * RM_PPT - 1) / (TPB * RM_PPT)), TPB, 0, s>>>(g, logh, L, smax, pos, rad,
+                                                                                         n, direct, own, bad);
   radius_dilate_kernel<<<(nb + TPB - 1) / TPB, TPB, 0, s>>>(g.D[0], g.D[1], g.D[2], direct, own, level);
   grade_passes(s, g.D, L, level, tmp);
 }
