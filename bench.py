@@ -86,6 +86,8 @@ def run_gpu(args, w, pos, vel):
         iso = ctx.profile_report()
         ctx.profile(False)
     lv = list(st.num_blocks_by_level)
+    # the step's own high-water mark, before the legs below allocate more
+    step_peak = aw.memory_stats()[1]
     # nearest-cell resample of the last pressure onto a uniform grid
     resample = None
     if args.resample > 0:
@@ -129,7 +131,7 @@ def run_gpu(args, w, pos, vel):
     res = dict(seconds=t1 - t0, iters=iters, topology_changes=changed, blocks_by_level=lv, radius_rule=radius,
                resample=resample,
                total_cells=int(st.total_cells), rel_residual=st.solve.rel_residual, profile=prof, isolated=iso,
-               peak_bytes=aw.memory_stats()[1])
+               peak_bytes=step_peak)
     return res
 
 
