@@ -744,19 +744,28 @@ __global__ __launch_bounds__(TPB) void scatter_kernel(const float* __restrict__ 
 }
 
 // beta = 1 on interior slots, 0 on tank-wall slots
-__global__ void init_beta_kernel(LevelDev L, float* __restrict__ bx, float* __restrict__ by, float* __restrict__ bz) {
-  const int B = blockIdx.x;
+// beta = 1 on every face slot, 0 on the tank's walls.  One wave per block,
+// the axes in turn (no division by the slot count); four blocks per
+// work-group, so that the many small coarse blocks do not each take one.
+__global__ __launch_bounds__(TPB) void init_beta_kernel(LevelDev L, float* __restrict__ bx, float* __restrict__ by,
+                                                        float* __restrict__ bz) {
+  const int B = blockIdx.x * (TPB / 64) + (int)(threadIdx.x >> 6);
+  if (B >= L.nb) return;
+  const int lane = threadIdx.x & 63;
   const int r = L.bs >> L.e[B];
   int Bc[3];
   block_coords(L.D, B, Bc);
   const int nf = (r + 1) * r * r;
   float* dst[3] = {bx, by, bz};
-  for (int k = threadIdx.x; k < 3 * nf; k += blockDim.x) {
-    const int a = k / nf, f = k - a * nf;
-    int c[3];
-    face_decode(a, f, r, c);
-    const bool wall = (c[a] == 0 && Bc[a] == 0) || (c[a] == r && Bc[a] == L.D[a] - 1);
-    dst[a][L.foff[a][B] + f] = wall ? 0.f : 1.f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float* d = dst[a] + L.foff[a][B];
+    for (int f = lane; f < nf; f += 64) {
+      int c[3];
+      face_decode(a, f, r, c);
+      const bool wall = (c[a] == 0 && Bc[a] == 0) || (c[a] == r && Bc[a] == L.D[a] - 1);
+      d[f] = wall ? 0.f : 1.f;
+    }
   }
 }
 
@@ -3326,7 +3335,7 @@ void k_scatter(hipStream_t s, const float* pos, const float* vel, const int32_t*
 }
 
 void k_init_beta(hipStream_t s, const LevelDev& L, float* bx, float* by, float* bz) {
-  init_beta_kernel<<<L.nb, TPB, 0, s>>>(L, bx, by, bz);
+  init_beta_kernel<<<(L.nb + TPB / 64 - 1) / (TPB / 64), TPB, 0, s>>>(L, bx, by, bz);
 }
 
 void k_p2g_deposit(hipStream_t s, const GridDev& g, const float* spos, const float* svel, const uint32_t* pcount,
