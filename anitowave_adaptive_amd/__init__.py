@@ -49,7 +49,7 @@ class SolveStats(ctypes.Structure):
 class StepParams(ctypes.Structure):
     _fields_ = [("dt", ctypes.c_float), ("gravity", ctypes.c_float * 3), ("flip_alpha", ctypes.c_float),
                 ("rel_tol", ctypes.c_double), ("max_iters", ctypes.c_int), ("rebuild_topology", ctypes.c_int),
-                ("warm_start", ctypes.c_int)]
+                ("warm_start", ctypes.c_int), ("d_radius", _V)]
 
 
 class StepStats(ctypes.Structure):
@@ -552,9 +552,12 @@ class FlipStep:
     """The whole step (aw_step) on device particle buffers."""
 
     def __init__(self, grid, solver, dt=1.0 / 240.0, gravity=(0.0, -9.81, 0.0), alpha=0.95, tol=1e-6,
-                 max_iters=200, rebuild_topology=True, warm_start=False):
-        """warm_start: start each pressure solve from the previous step's
-        pressure (remapped across topology changes) instead of zero."""
+                 max_iters=200, rebuild_topology=True, warm_start=False, d_radius=None):
+        """rebuild_topology: False keeps the topology, True (1) rebuilds it
+        from the distance bands every step, 2 from the particle radii
+        d_radius (device float[n] indexed by particle id; the step then needs
+        d_ids).  warm_start: start each pressure solve from the previous
+        step's pressure (remapped across topology changes) instead of zero."""
         self.grid, self.solver = grid, solver
         p = StepParams()
         p.dt = dt
@@ -564,6 +567,7 @@ class FlipStep:
         p.max_iters = max_iters
         p.rebuild_topology = int(rebuild_topology)
         p.warm_start = int(warm_start)
+        p.d_radius = d_radius
         self.params = p
 
     def __call__(self, d_pos, d_vel, d_ids, n, allow_not_converged=False):

@@ -133,11 +133,11 @@ void k_fill_f64(hipStream_t s, double* p, size_t n, double v);
 void k_bin(hipStream_t s, const GridDev& g, const float* pos, int64_t n, int32_t* key, uint32_t* count);
 void k_levels(hipStream_t s, const int* D, int L, const int* band, const uint32_t* count, uint8_t* surf,
               uint8_t* level, uint8_t* tmp);
-// levels from particle radii (rmap: 2 num_blocks + 1 words, the direct and
-// per-block marks and the invalid-radius flag), graded; result in level,
-// tmp is scratch
-void k_radius_levels(hipStream_t s, const GridDev& g, int logh, int L, const float* pos, const float* rad, int64_t n,
-                     uint32_t* rmap, uint8_t* level, uint8_t* tmp);
+// levels from particle radii (rad[ids[i]] for particle slot i, rad[i] if ids
+// is null; rmap: 2 num_blocks + 1 words, the direct and per-block marks and
+// the invalid-radius flag), graded; result in level, tmp is scratch
+void k_radius_levels(hipStream_t s, const GridDev& g, int logh, int L, const float* pos, const float* rad,
+                     const int32_t* ids, int64_t n, uint32_t* rmap, uint8_t* level, uint8_t* tmp);
 // nearest-cell resample of a cell field onto an o[0] x o[1] x o[2] voxel grid
 // (x fastest); o[1], o[2] <= 65535 and (2 o - 1) N < 2^32 per axis
 void k_resample(hipStream_t s, const GridDev& g, const float* field, const int* o, float* out);

@@ -458,7 +458,8 @@ constexpr int RM_PPT = 4;  // particles per thread, TPB apart (coalesced)
 
 __global__ __launch_bounds__(TPB) void radius_mark_kernel(GridDev g, int logh, int L, int smax,
                                                           const float* __restrict__ pos,
-                                                          const float* __restrict__ rad, int64_t n,
+                                                          const float* __restrict__ rad,
+                                                          const int32_t* __restrict__ ids, int64_t n,
                                                           uint32_t* __restrict__ direct, uint32_t* __restrict__ own,
                                                           uint32_t* __restrict__ bad) {
   const int64_t base = (int64_t)blockIdx.x * (TPB * RM_PPT);
@@ -468,7 +469,7 @@ __global__ __launch_bounds__(TPB) void radius_mark_kernel(GridDev g, int logh, i
     long long key = -1;
     int c[3] = {0, 0, 0}, lev = 0, sup = 0, b = 0;
     if (i < n) {
-      const float r = rad[i];
+      const float r = rad[ids ? ids[i] : i];
       if (!(r > 0.f) || isinf(r)) {
         atomicOr(bad, 1u);
       } else {
@@ -3282,8 +3283,8 @@ void k_resample(hipStream_t s, const GridDev& g, const float* field, const int* 
                     s>>>(g, field, o[0], o[1], o[2], out);
 }
 
-void k_radius_levels(hipStream_t s, const GridDev& g, int logh, int L, const float* pos, const float* rad, int64_t n,
-                     uint32_t* rmap, uint8_t* level, uint8_t* tmp) {
+void k_radius_levels(hipStream_t s, const GridDev& g, int logh, int L, const float* pos, const float* rad,
+                     const int32_t* ids, int64_t n, uint32_t* rmap, uint8_t* level, uint8_t* tmp) {
   const int nb = g.nb;
   uint32_t *direct = rmap, *own = rmap + nb, *bad = rmap + 2 * (size_t)nb;
   k_fill_u32(s, rmap, 2 * (size_t)nb, (uint32_t)(L - 1));
@@ -3291,7 +3292,7 @@ void k_radius_levels(hipStream_t s, const GridDev& g, int logh, int L, const flo
   const int smax = max(g.D[0], max(g.D[1], g.D[2]));
   if (n > 0)
     radius_mark_kernel<<<(unsigned)((n + TPB * RM_PPT - 1) / (TPB * RM_PPT)), TPB, 0, s>>>(g, logh, L, smax, pos, rad,
-                                                                                         n, direct, own, bad);
+                                                                                         ids, n, direct, own, bad);
   radius_dilate_kernel<<<(nb + TPB - 1) / TPB, TPB, 0, s>>>(g.D[0], g.D[1], g.D[2], direct, own, level);
   grade_passes(s, g.D, L, level, tmp);
 }

@@ -380,10 +380,10 @@ int apply_levels(aw_grid* g) {
   return launch_status("apply_levels");
 }
 
-// radius_rule: levels from the particle radii d_radius instead of the
-// distance bands
+// radius_rule: levels from the particle radii d_radius (by particle id
+// through d_rids when given, else by slot) instead of the distance bands
 int bin_and_levels(aw_grid* g, const float* d_pos, int64_t n, bool mark_levels, int* changed,
-                   bool radius_rule = false, const float* d_radius = nullptr) {
+                   bool radius_rule = false, const float* d_radius = nullptr, const int32_t* d_rids = nullptr) {
   hipStream_t s = g->ctx->stream;
   AW_TRY(g->key.ensure((size_t)std::max<int64_t>(n, 1) * sizeof(int32_t)));
   k_fill_u32(s, g->count.as<uint32_t>(), g->nb, 0u);
@@ -425,8 +425,8 @@ int bin_and_levels(aw_grid* g, const float* d_pos, int64_t n, bool mark_levels, 
     uint32_t bad = 0;
     {
       Prof pf(g->ctx, "levels", 16.0 * n + 12.0 * g->nb);
-      k_radius_levels(s, gv, std::ilogb(g->h), g->desc.num_levels, d_pos, d_radius, n, g->rmap.as<uint32_t>(),
-                      g->surf.as<uint8_t>(), g->ltmp.as<uint8_t>());
+      k_radius_levels(s, gv, std::ilogb(g->h), g->desc.num_levels, d_pos, d_radius, d_rids, n,
+                      g->rmap.as<uint32_t>(), g->surf.as<uint8_t>(), g->ltmp.as<uint8_t>());
     }
     AW_HIP(hipMemcpyAsync(&bad, g->rmap.as<uint32_t>() + 2 * (size_t)g->nb, sizeof bad, hipMemcpyDeviceToHost, s));
     AW_HIP(hipMemcpyAsync(nl.data(), g->surf.p, g->nb, hipMemcpyDeviceToHost, s));
@@ -1933,6 +1933,12 @@ int aw_step(aw_grid* g, aw_solver* S, float* d_pos, float* d_vel, int32_t* d_ids
   if (!g || !S || !prm || S->g != g || n < 0 || (n > 0 && (!d_pos || !d_vel)))
     return fail(AW_ERR_INVALID, "bad arguments");
   if (n > 0x7FFFFFFF) return fail(AW_ERR_INVALID, "too many particles");
+  if (prm->rebuild_topology < 0 || prm->rebuild_topology > 2) return fail(AW_ERR_INVALID, "rebuild_topology 0, 1 or 2");
+  if (prm->rebuild_topology == 2) {
+    if (!prm->d_radius || (n > 0 && !d_ids))
+      return fail(AW_ERR_INVALID, "rebuild_topology 2 needs d_radius and d_ids");
+    AW_TRY(validate_ids(g, d_ids, n));  // radii are read through the ids
+  }
   hipStream_t s = g->ctx->stream;
   int changed = 0;
   // warm start from the last step's pressure, if the layout it was solved in
@@ -1948,7 +1954,9 @@ int aw_step(aw_grid* g, aw_solver* S, float* d_pos, float* d_vel, int32_t* d_ids
                           s));
   }
   g->p_valid = false;
-  if (prm->rebuild_topology || !g->have_topology) {
+  if (prm->rebuild_topology == 2) {
+    AW_TRY(bin_and_levels(g, d_pos, n, true, &changed, true, prm->d_radius, d_ids));
+  } else if (prm->rebuild_topology || !g->have_topology) {
     AW_TRY(bin_and_levels(g, d_pos, n, true, &changed));
   } else {
     AW_TRY(bin_and_levels(g, d_pos, n, false, nullptr));

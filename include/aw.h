@@ -121,10 +121,13 @@ typedef struct {
   float flip_alpha;
   double rel_tol;
   int max_iters;
-  int rebuild_topology; /* 1 = mark/grade from the particles every step */
+  int rebuild_topology; /* 1 = levels from the distance bands every step; 2 = from the particle
+                           radii d_radius every step (aw_grid_build_topology_radius's rule) */
   int warm_start;       /* 1 = start the pressure solve from the previous step's pressure, remapped
                            cell by cell when the topology changed (no counterpart in the reference,
                            whose p = 0 start main.cu:300 is the default 0); ignored by rollouts */
+  const float* d_radius; /* rebuild_topology 2: radius of each particle, indexed by particle id;
+                            the step needs d_ids (ids in [0, n)) to find them as it reorders */
 } aw_step_params;
 
 typedef struct {
@@ -193,7 +196,8 @@ int aw_grid_build_topology(aw_grid* grid, const float* d_pos, int64_t n, int* ch
  * the topology is left as it was).  Otherwise as aw_grid_build_topology.
  * aw_step with rebuild_topology = 1 rebuilds with the distance bands: keep
  * rebuild_topology = 0 to step on this topology, as the reference does,
- * whose topology is fixed at start-up. */
+ * whose topology is fixed at start-up, or set rebuild_topology = 2 with
+ * aw_step_params.d_radius to re-mark it from the radii every step. */
 int aw_grid_build_topology_radius(aw_grid* grid, const float* d_pos, const float* d_radius, int64_t n,
                                   int* changed);
 /* Explicit native levels (host array, num_blocks) instead of the band rule. */

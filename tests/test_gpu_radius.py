@@ -136,3 +136,46 @@ def test_step_on_radius_topology(ctx):
     assert np.abs(P - ref["pos"]).max() < 1e-5
     vmax = max(np.abs(ref["vel"]).max(), 1e-6)
     assert np.abs(V - ref["vel"]).max() / vmax < 1e-3
+
+
+def test_steps_rebuilding_with_radius_rule(ctx):
+    """rebuild_topology = 2 re-marks the levels from the radii every step,
+    finding each particle's radius by its id as the step reorders the
+    particles: the levels equal the oracle's rule on the radii in the
+    current order, and the rollout equals building the radius topology
+    before each step and stepping with the topology kept (particles within
+    1e-6 absolute: P2G's LDS float adds are not order-deterministic).  A
+    step without ids is rejected."""
+    n, L = 64, 3
+    h = 1.0 / n
+    pos, vel = scenes.pool_sphere(n=n, seed=1234, vy=-8.0)
+    r = np.where(pos[:, 1] < 0.3, h / 2, h).astype(np.float32)
+    dr = _Radius(ctx, r)
+    og = O.OracleGrid((n // 8,) * 3, num_levels=L, h=h)
+    runs = []
+    for mode in (2, 0):
+        gg = aw.MSBGManager(ctx, (n // 8,) * 3, h, num_levels=L)
+        sv = aw.VCycleSolver(gg)
+        gp = GpuParticles(ctx, pos, vel)
+        step = aw.FlipStep(gg, sv, dt=DT, gravity=GRAV, alpha=0.95, tol=1e-6, rebuild_topology=mode,
+                           d_radius=dr.d.ptr if mode == 2 else None)
+        runs.append((gg, gp, step))
+    (ga, pa, sa), (gb, pb, sb) = runs
+    changes = []
+    for k in range(6):
+        ids = pb.ids.download(np.int32, pb.n)
+        pcur = pb.pos.download(np.float32, 3 * pb.n).reshape(-1, 3)
+        rslot = _Radius(ctx, r[ids])
+        gb.build_topology_radius(pb.pos.ptr, rslot.d.ptr, pb.n)
+        sb(pb.pos.ptr, pb.vel.ptr, pb.ids.ptr, pb.n)
+        st = sa(pa.pos.ptr, pa.vel.ptr, pa.ids.ptr, pa.n)
+        changes.append(st.topology_changed)
+        np.testing.assert_array_equal(ga.levels(), og.levels_from_radii(pcur, r[ids]), err_msg=f"step {k}")
+        np.testing.assert_array_equal(ga.levels(), gb.levels())
+    print(f"[radius] rebuild 2: topology changes {changes}", flush=True)
+    PA, VA = pa.unpermuted()
+    PB, VB = pb.unpermuted()
+    assert np.abs(PA - PB).max() < 1e-6
+    assert np.abs(VA - VB).max() / max(np.abs(VB).max(), 1e-6) < 1e-4
+    with pytest.raises(aw.AWError):
+        sa(pa.pos.ptr, pa.vel.ptr, None, pa.n)
