@@ -43,7 +43,7 @@ class SolverParams(ctypes.Structure):
 
 class SolveStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("outer_iterations", ctypes.c_int), ("rel_residual", ctypes.c_double),
-                ("b_norm", ctypes.c_double)]
+                ("b_norm", ctypes.c_double), ("recurrence_rel_residual", ctypes.c_double)]
 
 
 class StepParams(ctypes.Structure):
@@ -530,8 +530,11 @@ class VCycleSolver:
         check(lib().aw_solver_vcycle(self._h, int(k)), "aw_solver_vcycle")
 
     def solve(self, tol=1e-6, max_iters=200, allow_not_converged=False):
+        """Solve to true relative residual `tol`; AW_ERR_NOT_CONVERGED raises
+        unless allowed (then `self.status` says which it was)."""
         st = SolveStats()
         s = lib().aw_solver_solve(self._h, float(tol), int(max_iters), ctypes.byref(st))
+        self.status = s
         if not (allow_not_converged and s == AW_ERR_NOT_CONVERGED):
             check(s, "aw_solver_solve")
         return st
@@ -571,9 +574,13 @@ class FlipStep:
         self.params = p
 
     def __call__(self, d_pos, d_vel, d_ids, n, allow_not_converged=False):
+        """One step; a pressure solve above its tolerance (AW_ERR_NOT_CONVERGED,
+        e.g. an FP32 solve at its floor) raises unless allowed, and
+        `self.status` keeps the step's status either way."""
         st = StepStats()
         s = lib().aw_step(self.grid.handle, self.solver.handle, d_pos, d_vel, d_ids, int(n), ctypes.byref(self.params),
                           ctypes.byref(st))
+        self.status = s
         if not (allow_not_converged and s == AW_ERR_NOT_CONVERGED):
             check(s, "aw_step")
         return st

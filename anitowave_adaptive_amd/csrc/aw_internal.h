@@ -221,6 +221,8 @@ void k_pcg_p(hipStream_t s, const float* z, float* p, int n, const double* sc, i
 void k_pcg_xr(hipStream_t s, float* x, float* r, const float* p, const float* q, const uint8_t* liquid, int n,
               double* sc);
 void k_mask(hipStream_t s, float* x, const uint8_t* liquid, int n);
+// *out = number of nonzero bytes of m[0..n) (cleared by the launcher)
+void k_count_nonzero(hipStream_t s, const uint8_t* m, uint32_t n, uint32_t* out);
 // FP64 iterative refinement
 void k_residual64(hipStream_t s, const LevelDev& L, const double* x, const float* b, float* r32, const WGMap& wg,
                   const uint32_t* blist, double* rr);

@@ -447,7 +447,8 @@ __device__ __forceinline__ int radius_level(float r, int logh, int L) {
 // particle's level (MSBG.cuh:294-296, one for every radius up to res world
 // units); capped at the largest dimension, beyond which the box covers the
 // domain anyway.  The block is the particle's bin (clamped into the domain).
-// A radius that is not positive and finite sets *bad.
+// A radius that is not positive and finite sets bit 0 of *bad, a particle id
+// outside [0, n) bit 1.
 // Marking is split in two so that the same addresses are not hit 27 times
 // per wave: a particle of support 1 lowers only its own block's minimum
 // own[b] (radius_dilate_kernel then spreads each minimum over the 26
@@ -469,8 +470,13 @@ __global__ __launch_bounds__(TPB) void radius_mark_kernel(GridDev g, int logh, i
     long long key = -1;
     int c[3] = {0, 0, 0}, lev = 0, sup = 0, b = 0;
     if (i < n) {
-      const float r = rad[ids ? ids[i] : i];
-      if (!(r > 0.f) || isinf(r)) {
+      // radii by particle id: an id outside [0, n) is an invalid input (bit 1)
+      const int32_t id = ids ? ids[i] : (int32_t)i;
+      const bool id_ok = id >= 0 && (int64_t)id < n;
+      const float r = id_ok ? rad[id] : 0.f;
+      if (!id_ok) {
+        atomicOr(bad, 2u);
+      } else if (!(r > 0.f) || isinf(r)) {
         atomicOr(bad, 1u);
       } else {
         lev = radius_level(r, logh, L);
@@ -3216,6 +3222,14 @@ __global__ __launch_bounds__(TPB) void pcg_xr_kernel(float* __restrict__ x, floa
   grid_sum(s, sc + SC_RR * SC_SLOTS);
 }
 
+__global__ __launch_bounds__(TPB) void count_nonzero_kernel(const uint8_t* __restrict__ m, uint32_t n,
+                                                            uint32_t* __restrict__ out) {
+  uint32_t c = 0;
+  for (uint32_t i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) c += m[i] != 0;
+  const uint32_t s = block_reduce_sum<uint32_t>(c);
+  if (threadIdx.x == 0 && s) atomicAdd(out, s);
+}
+
 __global__ void mask_kernel(float* __restrict__ x, const uint8_t* __restrict__ liq, int n) {
   for (int i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB)
     if (!liq[i]) x[i] = 0.f;
@@ -3515,6 +3529,11 @@ void k_pcg_p(hipStream_t s, const float* z, float* p, int n, const double* sc, i
 void k_pcg_xr(hipStream_t s, float* x, float* r, const float* p, const float* q, const uint8_t* liquid, int n,
               double* sc) {
   pcg_xr_kernel<<<red_grid(n), TPB, 0, s>>>(x, r, p, q, liquid, n, sc);
+}
+
+void k_count_nonzero(hipStream_t s, const uint8_t* m, uint32_t n, uint32_t* out) {
+  k_fill_u32(s, out, 1, 0u);
+  if (n > 0) count_nonzero_kernel<<<grid_for(n) < 1024 ? grid_for(n) : 1024, TPB, 0, s>>>(m, n, out);
 }
 
 void k_mask(hipStream_t s, float* x, const uint8_t* liquid, int n) {

@@ -201,6 +201,8 @@ struct aw_grid {
   DevBuf beta[3], Macc[3], macc[3], uold[3], ustar[3], unew[3], mass[3];
   // adjoint fields: fine-lattice adjoints (two fields each, stride ADJ), per-slot and per-cell adjoints
   DevBuf fbar, fmbar, pbar, rhsbar, idrange, adjrec;
+  DevBuf liq_count;                  // liquid cells of the last step, read back with the step's final sync
+  uint32_t* h_liq_count = nullptr;   // pinned host copy
   DevBuf ubn[3], ubo[3], ubs[3], Mb[3], mb[3];
   bool sorted_ids = false;  // the last binning carried particle ids
   std::vector<uint8_t> h_level;
@@ -373,7 +375,7 @@ int apply_levels(aw_grid* g) {
   k_init_beta(s, L, g->beta[0].as<float>(), g->beta[1].as<float>(), g->beta[2].as<float>());
   k_fill_f32(s, g->p.as<float>(), nc, 0.f);
   k_fill_f32(s, g->rhs.as<float>(), nc, 0.f);
-  hipMemsetAsync(g->liquid.p, 0, nc, s);
+  AW_HIP(hipMemsetAsync(g->liquid.p, 0, nc, s));
   g->have_topology = true;
   g->topo_version++;
   g->n_sorted = -1;
@@ -385,6 +387,11 @@ int apply_levels(aw_grid* g) {
 int bin_and_levels(aw_grid* g, const float* d_pos, int64_t n, bool mark_levels, int* changed,
                    bool radius_rule = false, const float* d_radius = nullptr, const int32_t* d_rids = nullptr) {
   hipStream_t s = g->ctx->stream;
+  // the counts below are rewritten for these positions: the sorted particles
+  // of an earlier binning no longer match them, whatever happens next (a
+  // failure below leaves no sorted state the adjoint could pair with them)
+  g->n_sorted = -1;
+  g->sorted_ids = false;
   AW_TRY(g->key.ensure((size_t)std::max<int64_t>(n, 1) * sizeof(int32_t)));
   k_fill_u32(s, g->count.as<uint32_t>(), g->nb, 0u);
   GridDev gv = g->view();
@@ -432,6 +439,7 @@ int bin_and_levels(aw_grid* g, const float* d_pos, int64_t n, bool mark_levels, 
     AW_HIP(hipMemcpyAsync(nl.data(), g->surf.p, g->nb, hipMemcpyDeviceToHost, s));
     AW_HIP(hipStreamSynchronize(s));
     AW_TRY(launch_status("levels"));
+    if (bad & 2u) return fail(AW_ERR_INVALID, "particle ids outside [0, %lld)", (long long)n);
     if (bad) return fail(AW_ERR_INVALID, "particle radius not positive and finite");
     AW_HIP(hipMemcpyAsync(g->level.p, g->surf.p, g->nb, hipMemcpyDeviceToDevice, s));
   } else {
@@ -1089,6 +1097,7 @@ int solve_fp32(aw_solver* S, float* x, const float* b, double tol, int max_iters
   if (st) {
     st->b_norm = bn;
     st->outer_iterations = 1;
+    st->recurrence_rel_residual = 0.0;
   }
   if (!(bn > 0.0)) {
     k_fill_f32(s, x, n, 0.f);
@@ -1112,22 +1121,26 @@ int solve_fp32(aw_solver* S, float* x, const float* b, double tol, int max_iters
       ++it;
       if (!std::isfinite(rn)) return fail(AW_ERR_NAN, "V-cycle iteration diverged to non-finite values");
     }
+    if (st) st->recurrence_rel_residual = rn / bn;
   } else {
     // Conjugate gradients preconditioned by one V-cycle.  The recursively
     // updated FP32 residual drifts from b - A x; when it reports convergence
     // the true residual is recomputed and, if it is still above the
     // tolerance, the iteration restarts once from it.  The FP32 pressure
     // itself bounds the attainable true residual (rounding p to FP32 leaves
-    // a residual of order eps32 |A| |p|), so after the restart the
-    // recurrence criterion is accepted and the recomputed residual reported.
+    // a residual of order eps32 |A| |p|): when the recurrence converges a
+    // second time and the true residual is still above the tolerance, the
+    // solve stops there and reports AW_ERR_NOT_CONVERGED with both
+    // residuals (tolerances below that floor need precision 1).
     int restarts = 0;
-    bool recurrence_ok = false;
     int first = 1;
+    double rec = rn;
     while (it < max_iters) {
       AW_TRY(run_pcg_body(S, x, first));
       first = 0;
       AW_TRY(read_scalars(S));
       rn = std::sqrt(S->h_sc[SC_RR]);
+      rec = rn;
       ++it;
       if (!std::isfinite(rn)) return fail(AW_ERR_NAN, "PCG produced non-finite residual");
       if (rn > tol * bn) continue;
@@ -1136,19 +1149,20 @@ int solve_fp32(aw_solver* S, float* x, const float* b, double tol, int max_iters
       k_dot(s, r, r, liq, n, scp(S, SC_RR));
       AW_TRY(read_scalars(S));
       rn = std::sqrt(S->h_sc[SC_RR]);
-      if (rn <= tol * bn || restarts == 1) {
-        recurrence_ok = true;
-        break;
-      }
+      if (rn <= tol * bn || restarts == 1) break;
       ++restarts;
       first = 1;
     }
-    if (recurrence_ok) {
+    if (st) st->recurrence_rel_residual = rec / bn;
+    if (rec <= tol * bn && rn > tol * bn) {
       if (st) {
         st->iterations = it;
         st->rel_residual = rn / bn;
       }
-      return launch_status("solve");
+      AW_TRY(launch_status("solve"));
+      return fail(AW_ERR_NOT_CONVERGED,
+                  "FP32 floor: true relative residual %.3e above %.1e after %d iterations (recurrence %.3e)", rn / bn,
+                  tol, it, rec / bn);
     }
   }
   if (st) {
@@ -1186,6 +1200,7 @@ int solve_mixed(aw_solver* S, double tol, int max_iters, aw_solve_stats* st) {
     st->iterations = 0;
     st->outer_iterations = 0;
     st->rel_residual = 0.0;
+    st->recurrence_rel_residual = 0.0;
   }
   if (!(bn > 0.0)) {
     k_fill_f32(s, g->p.as<float>(), n, 0.f);
@@ -1218,6 +1233,7 @@ int solve_mixed(aw_solver* S, double tol, int max_iters, aw_solve_stats* st) {
     st->iterations = inner_total;
     st->outer_iterations = outer;
     st->rel_residual = rn / bn;
+    st->recurrence_rel_residual = rn / bn;  // the FP64 residual is evaluated directly every refinement
   }
   AW_TRY(launch_status("solve"));
   return rn <= tol * bn ? AW_OK
@@ -1394,6 +1410,7 @@ struct aw_tape {
   std::vector<std::unique_ptr<Ckpt>> ck;
   std::vector<std::unique_ptr<Step>> rec;
   DevBuf wpos, wvel, wids;
+  DevBuf radius;  // rebuild_topology 2: the tape's own copy of the radii (the recompute re-reads them)
   std::vector<std::vector<uint8_t>> mask_level, mask_liquid;
 };
 
@@ -1472,6 +1489,7 @@ void ctx_release(aw_ctx* c) {
 void grid_free(aw_grid* g) {
   aw_ctx* c = g->ctx;
   (void)hipStreamSynchronize(c->stream);
+  if (g->h_liq_count) (void)hipHostFree(g->h_liq_count);
   delete g;
   ctx_release(c);
 }
@@ -1693,6 +1711,9 @@ int aw_grid_create(aw_ctx* ctx, const aw_grid_desc* d, aw_grid** out) {
   AW_TRY(g->count.ensure(nb * sizeof(uint32_t)));
   AW_TRY(g->poff.ensure((nb + 1) * sizeof(uint32_t)));
   AW_TRY(g->cursor.ensure(nb * sizeof(uint32_t)));
+  AW_TRY(g->liq_count.ensure(sizeof(uint32_t)));
+  AW_HIP(hipHostMalloc((void**)&g->h_liq_count, sizeof(uint32_t), hipHostMallocDefault));
+  *g->h_liq_count = 0;
   std::vector<int32_t> nbr;
   neighbour_table(g->D, nbr);
   AW_TRY(g->nbr.ensure(nbr.size() * sizeof(int32_t)));
@@ -1934,11 +1955,10 @@ int aw_step(aw_grid* g, aw_solver* S, float* d_pos, float* d_vel, int32_t* d_ids
     return fail(AW_ERR_INVALID, "bad arguments");
   if (n > 0x7FFFFFFF) return fail(AW_ERR_INVALID, "too many particles");
   if (prm->rebuild_topology < 0 || prm->rebuild_topology > 2) return fail(AW_ERR_INVALID, "rebuild_topology 0, 1 or 2");
-  if (prm->rebuild_topology == 2) {
-    if (!prm->d_radius || (n > 0 && !d_ids))
-      return fail(AW_ERR_INVALID, "rebuild_topology 2 needs d_radius and d_ids");
-    AW_TRY(validate_ids(g, d_ids, n));  // radii are read through the ids
-  }
+  if (prm->rebuild_topology == 2 && (!prm->d_radius || (n > 0 && !d_ids)))
+    return fail(AW_ERR_INVALID, "rebuild_topology 2 needs d_radius and d_ids");
+  // (rebuild_topology 2 reads the radii through the ids: the marking kernel
+  // range-checks each id and flags one outside [0, n) as an invalid input)
   hipStream_t s = g->ctx->stream;
   int changed = 0;
   // warm start from the last step's pressure, if the layout it was solved in
@@ -1986,6 +2006,10 @@ int aw_step(aw_grid* g, aw_solver* S, float* d_pos, float* d_vel, int32_t* d_ids
                  prm->dt, d_pos, d_vel, g->g2p_list.as<uint32_t>());
   }
   if (d_ids && n > 0) AW_HIP(hipMemcpyAsync(d_ids, g->sids.p, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+  if (stats) {
+    k_count_nonzero(s, g->liquid.as<uint8_t>(), g->ncells, g->liq_count.as<uint32_t>());
+    AW_HIP(hipMemcpyAsync(g->h_liq_count, g->liq_count.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  }
   AW_HIP(hipStreamSynchronize(s));
   AW_TRY(launch_status("step"));
   if (stats) {
@@ -1994,6 +2018,7 @@ int aw_step(aw_grid* g, aw_solver* S, float* d_pos, float* d_vel, int32_t* d_ids
     stats->solve = sst;
     for (uint8_t l : g->h_level) stats->num_blocks_by_level[l]++;
     stats->total_cells = g->ncells;
+    stats->liquid_cells = *g->h_liq_count;
   }
   if (st != AW_OK) g_err = keep;
   return st;
@@ -2094,6 +2119,12 @@ int aw_rollout_forward(aw_tape* t, float* d_pos, float* d_vel, int32_t* d_ids, c
   const int64_t n = t->n;
   t->prm = *prm;
   t->prm.warm_start = 0;   // every step of a rollout starts its solve from zero, as its recomputation does
+  if (prm->rebuild_topology == 2 && prm->d_radius) {
+    // the backward pass re-marks the levels of every recomputed step: keep the
+    // radii as they were, whatever the caller does with its buffer meanwhile
+    AW_TRY(copy_d2d(s, t->radius, prm->d_radius, (size_t)n * sizeof(float)));
+    t->prm.d_radius = t->radius.as<float>();
+  }
   t->have_forward = false;
   t->mask_level.clear();
   t->mask_liquid.clear();

@@ -108,11 +108,22 @@ typedef struct {
   int precision;       /* 0 = FP32; 1 = FP32 V-cycles inside FP64 iterative refinement */
 } aw_solver_params;
 
+/* Convergence contract of every solve: AW_OK only when the true relative
+ * residual ||b - A p||_2 / ||b||_2, recomputed from the returned p, is at or
+ * below rel_tol; otherwise AW_ERR_NOT_CONVERGED with these statistics
+ * filled and p left at the last iterate.  With precision 0 the FP32 pressure
+ * bounds the attainable true residual (about eps32 |A| |p| / |b|, 1e-5 to
+ * 2e-5 relative on free-surface scenes at 256^3): PCG then stops when its
+ * recurrence residual has met rel_tol twice (one restart from the true
+ * residual in between) and reports the true residual above it as
+ * AW_ERR_NOT_CONVERGED.  Tolerances below that floor need precision 1. */
 typedef struct {
   int iterations;      /* PCG iterations or V-cycles (outer+inner for mixed precision) */
   int outer_iterations;
-  double rel_residual; /* ||b - A p||_2 / ||b||_2 as tracked by the solver */
+  double rel_residual; /* true ||b - A p||_2 / ||b||_2 of the returned p (the recurrence residual only when
+                          the iteration cap ended an FP32 PCG solve) */
   double b_norm;
+  double recurrence_rel_residual; /* FP32 PCG: the last recurrence residual; otherwise rel_residual */
 } aw_solve_stats;
 
 typedef struct {
@@ -127,7 +138,9 @@ typedef struct {
                            cell by cell when the topology changed (no counterpart in the reference,
                            whose p = 0 start main.cu:300 is the default 0); ignored by rollouts */
   const float* d_radius; /* rebuild_topology 2: radius of each particle, indexed by particle id;
-                            the step needs d_ids (ids in [0, n)) to find them as it reorders */
+                            the step needs d_ids (ids in [0, n); another id fails the step with
+                            AW_ERR_INVALID) to find them as it reorders.  Read during aw_step only;
+                            aw_rollout_forward copies the radii into its tape for the backward pass */
 } aw_step_params;
 
 typedef struct {

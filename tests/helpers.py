@@ -62,3 +62,30 @@ def rel_err(a, b):
     b = np.asarray(b, np.float64)
     den = max(np.abs(b).max(), 1e-30)
     return np.abs(a - b).max() / den
+
+
+# Bound on the true relative residual an FP32 pressure reaches on these scenes
+# (rounding p to FP32 alone leaves about eps32 |A| |p| / |b|).
+FP32_FLOOR = 5e-5
+
+
+def check_solve_contract(status, st, tol, precision=0):
+    """The solver's convergence contract: AW_OK only with the true relative
+    residual at or below tol; an FP32 solve may stop at its floor instead,
+    with AW_ERR_NOT_CONVERGED, its recurrence residual at or below tol and
+    the true one above tol but below FP32_FLOOR."""
+    if status == aw.AW_OK:
+        assert st.rel_residual <= tol, (st.rel_residual, tol)
+        return
+    assert status == aw.AW_ERR_NOT_CONVERGED, status
+    assert precision == 0, "a mixed-precision solve must reach its tolerance"
+    assert st.rel_residual > tol, (st.rel_residual, tol)
+    assert st.recurrence_rel_residual <= tol, st.recurrence_rel_residual
+    assert st.rel_residual < FP32_FLOOR, st.rel_residual
+
+
+def step_checked(stepper, gp, precision=0):
+    """One aw_step on GpuParticles under the convergence contract."""
+    st = stepper(gp.pos.ptr, gp.vel.ptr, gp.ids.ptr, gp.n, allow_not_converged=True)
+    check_solve_contract(stepper.status, st.solve, stepper.params.rel_tol, precision)
+    return st

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import anitowave_adaptive_amd as aw
-from helpers import DT, GRAV, GpuParticles, gpu_grid, oracle_grid, particles
+from helpers import DT, GRAV, GpuParticles, gpu_grid, oracle_grid, particles, step_checked
 
 pytestmark = pytest.mark.gpu
 
@@ -46,7 +46,7 @@ def test_step_edge_cases(ctx, name, precision):
     sv = aw.VCycleSolver(gg, precision=precision)
     gp = GpuParticles(ctx, pos, vel)
     step = aw.FlipStep(gg, sv, dt=DT, gravity=GRAV, alpha=0.95, tol=1e-6)
-    st = step(gp.pos.ptr, gp.vel.ptr, gp.ids.ptr, gp.n)
+    st = step_checked(step, gp, precision)
     print(f"[edges] {name}: iters gpu {st.solve.iterations} oracle {ref['iters']}", flush=True)
     np.testing.assert_array_equal(gg.levels(), og.levels())
     _, cnt = og.bin(pos)

@@ -17,7 +17,7 @@ import pytest
 import anitowave_adaptive_amd as aw
 import oracle as O
 from anitowave_adaptive_amd import scenes
-from helpers import DT, GRAV, GpuParticles
+from helpers import DT, GRAV, GpuParticles, step_checked
 
 pytestmark = pytest.mark.gpu
 
@@ -128,7 +128,7 @@ def test_step_on_radius_topology(ctx):
     gg.build_topology_radius(gp.pos.ptr, dr.d.ptr, gp.n)
     np.testing.assert_array_equal(gg.levels(), lv)
     step = aw.FlipStep(gg, sv, dt=DT, gravity=GRAV, alpha=0.95, tol=1e-6, rebuild_topology=False)
-    st = step(gp.pos.ptr, gp.vel.ptr, gp.ids.ptr, gp.n)
+    st = step_checked(step, gp)
     print(f"[radius] step iters gpu {st.solve.iterations} oracle {ref['iters']}", flush=True)
     assert not st.topology_changed
     np.testing.assert_array_equal(gg.levels(), lv)
@@ -163,14 +163,16 @@ def test_steps_rebuilding_with_radius_rule(ctx):
     (ga, pa, sa), (gb, pb, sb) = runs
     changes = []
     for k in range(6):
+        # the oracle sees exactly the input run A's rebuild-2 step sees
+        ida = pa.ids.download(np.int32, pa.n)
+        pcur = pa.pos.download(np.float32, 3 * pa.n).reshape(-1, 3)
         ids = pb.ids.download(np.int32, pb.n)
-        pcur = pb.pos.download(np.float32, 3 * pb.n).reshape(-1, 3)
         rslot = _Radius(ctx, r[ids])
         gb.build_topology_radius(pb.pos.ptr, rslot.d.ptr, pb.n)
-        sb(pb.pos.ptr, pb.vel.ptr, pb.ids.ptr, pb.n)
-        st = sa(pa.pos.ptr, pa.vel.ptr, pa.ids.ptr, pa.n)
+        step_checked(sb, pb)
+        st = step_checked(sa, pa)
         changes.append(st.topology_changed)
-        np.testing.assert_array_equal(ga.levels(), og.levels_from_radii(pcur, r[ids]), err_msg=f"step {k}")
+        np.testing.assert_array_equal(ga.levels(), og.levels_from_radii(pcur, r[ida]), err_msg=f"step {k}")
         np.testing.assert_array_equal(ga.levels(), gb.levels())
     print(f"[radius] rebuild 2: topology changes {changes}", flush=True)
     PA, VA = pa.unpermuted()
@@ -179,3 +181,34 @@ def test_steps_rebuilding_with_radius_rule(ctx):
     assert np.abs(VA - VB).max() / max(np.abs(VB).max(), 1e-6) < 1e-4
     with pytest.raises(aw.AWError):
         sa(pa.pos.ptr, pa.vel.ptr, None, pa.n)
+
+
+def test_failed_radius_rebuild_invalidates_the_adjoint_state(ctx):
+    """A rebuild_topology = 2 step that fails on a bad radius rebins the
+    particles before it fails: the previous step's sorted state no longer
+    matches the per-block counts, so the step adjoint must refuse to run
+    (AW_ERR_INVALID) instead of reading the stale particle ranges."""
+    n, L = 32, 3
+    h = 1.0 / n
+    pos, vel = scenes.pool_sphere(n=n, seed=7, radius=0.15)
+    r = np.full(len(pos), h, np.float32)
+    gg = aw.MSBGManager(ctx, (n // 8,) * 3, h, num_levels=L)
+    sv = aw.VCycleSolver(gg)
+    gp = GpuParticles(ctx, pos, vel)
+    good = _Radius(ctx, r)
+    step = aw.FlipStep(gg, sv, dt=DT, gravity=GRAV, alpha=0.95, tol=1e-6, rebuild_topology=2, d_radius=good.d.ptr)
+    step_checked(step, gp)
+    adj = aw.Adjoint(gg, sv)
+    xb = aw.DeviceArray.from_numpy(ctx, np.zeros((gp.n, 3), np.float32))
+    vb = aw.DeviceArray.from_numpy(ctx, np.zeros((gp.n, 3), np.float32))
+    adj.step(step.params, xb.ptr, vb.ptr)   # valid after a good step
+    bad_r = r.copy()
+    bad_r[len(r) // 2] = np.nan
+    bad = _Radius(ctx, bad_r)
+    step.params.d_radius = bad.d.ptr
+    with pytest.raises(aw.AWError) as e:
+        step(gp.pos.ptr, gp.vel.ptr, gp.ids.ptr, gp.n)
+    assert e.value.status == aw.AW_ERR_INVALID
+    with pytest.raises(aw.AWError) as e:
+        adj.step(step.params, xb.ptr, vb.ptr)
+    assert e.value.status == aw.AW_ERR_INVALID

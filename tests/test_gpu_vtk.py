@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import anitowave_adaptive_amd as aw
-from helpers import DT, GRAV, GpuParticles, gpu_grid, oracle_grid, particles
+from helpers import DT, GRAV, GpuParticles, gpu_grid, oracle_grid, particles, step_checked
 from test_vtk import _read_vtk
 
 pytestmark = pytest.mark.gpu
@@ -23,7 +23,7 @@ def test_resample_bit_exact(ctx, name, tmp_path):
     gg = gpu_grid(ctx, name)
     sv = aw.VCycleSolver(gg)
     gp = GpuParticles(ctx, pos, vel)
-    aw.FlipStep(gg, sv, dt=DT, gravity=GRAV, alpha=0.95, tol=1e-6)(gp.pos.ptr, gp.vel.ptr, gp.ids.ptr, gp.n)
+    step_checked(aw.FlipStep(gg, sv, dt=DT, gravity=GRAV, alpha=0.95, tol=1e-6), gp)
     p = gg.cell("p")
     assert np.any(p)
     og = oracle_grid(name)

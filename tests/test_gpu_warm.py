@@ -18,7 +18,7 @@ import pytest
 
 import anitowave_adaptive_amd as aw
 from anitowave_adaptive_amd import scenes
-from helpers import DT, GRAV, GpuParticles, gpu_grid
+from helpers import DT, GRAV, GpuParticles, gpu_grid, step_checked
 
 pytestmark = pytest.mark.gpu
 
@@ -33,7 +33,7 @@ def _run(ctx, warm, precision=0, tol=1e-6):
     step = aw.FlipStep(gg, sv, dt=DT, gravity=GRAV, alpha=0.95, tol=tol, warm_start=warm, max_iters=400)
     iters, changed = [], []
     for _ in range(STEPS):
-        st = step(gp.pos.ptr, gp.vel.ptr, gp.ids.ptr, gp.n)
+        st = step_checked(step, gp, precision)
         iters.append(st.solve.iterations)
         changed.append(st.topology_changed)
     P, V = gp.unpermuted()
