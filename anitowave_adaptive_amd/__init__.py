@@ -131,6 +131,7 @@ SIGNATURES = {
     "aw_rollout_backward": (_I, [_V, _V, _V, _P(RolloutStats)]),
     "aw_tape_masks": (_I, [_V, _I, _V, _V, ctypes.c_uint32, _P(ctypes.c_uint32)]),
     "aw_resample_cells": (_I, [_V, _V, _P(_I), _V]),
+    "aw_remap_cells": (_I, [_V, _V, _V, _V]),
     "aw_write_vtk_structured": (_I, [ctypes.c_char_p, ctypes.c_char_p, _V, _P(_I), _P(_F), _P(_F)]),
     "aw_write_vtk_blocks": (_I, [ctypes.c_char_p, _P(_I), _F, _V]),
     "aw_write_vtk_particles": (_I, [ctypes.c_char_p, _V, _V, _V, _I64]),
@@ -343,6 +344,12 @@ class MSBGManager:
         """Nearest-cell resample of a device cell field onto out_dims voxel
         centres, x fastest (sampleMSBG, vtx_visualization.cuh:8-62)."""
         check(lib().aw_resample_cells(self._h, d_field, _i3(out_dims), d_out), "aw_resample_cells")
+
+    def remap_cells(self, d_level_prev, d_src, d_dst):
+        """Cell field of the layout with native levels d_level_prev (device
+        uint8[num_blocks]) onto the current layout: each cell takes the value
+        of the previous cell containing its centre."""
+        check(lib().aw_remap_cells(self._h, d_level_prev, d_src, d_dst), "aw_remap_cells")
 
     def export_cells_vtk(self, d_field, out_dims, path, name="pressure"):
         """Resample and write a binary legacy VTK STRUCTURED_POINTS file

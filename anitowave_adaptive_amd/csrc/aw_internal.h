@@ -144,6 +144,7 @@ void k_resample(hipStream_t s, const GridDev& g, const float* field, const int* 
 // exclusive scans, out[n] = total; `scratch` holds scan_scratch_words(n) words
 size_t scan_scratch_words(int n);
 void k_scan_u32(hipStream_t s, const uint32_t* in, uint32_t* out, int n, uint32_t* scratch);
+void k_cell_offsets(hipStream_t s, const uint8_t* level, int nb, uint32_t* coff, uint32_t* scratch);
 void k_layout_offsets(hipStream_t s, const uint8_t* level, int nb, uint32_t* coff, uint32_t* fx, uint32_t* fy,
                       uint32_t* fz, uint32_t* scratch);
 void k_scatter(hipStream_t s, const float* pos, const float* vel, const int32_t* ids, const int32_t* key,

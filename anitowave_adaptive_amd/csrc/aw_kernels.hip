@@ -3334,6 +3334,10 @@ void k_scan_u32(hipStream_t s, const uint32_t* in, uint32_t* out, int n, uint32_
   scan_any(s, in, nullptr, 0, out, n, scratch);
 }
 
+void k_cell_offsets(hipStream_t s, const uint8_t* level, int nb, uint32_t* coff, uint32_t* scratch) {
+  scan_any(s, nullptr, level, 1, coff, nb, scratch);
+}
+
 void k_layout_offsets(hipStream_t s, const uint8_t* level, int nb, uint32_t* coff, uint32_t* fx, uint32_t* fy,
                       uint32_t* fz, uint32_t* scratch) {
   scan_any(s, nullptr, level, 1, coff, nb, scratch);

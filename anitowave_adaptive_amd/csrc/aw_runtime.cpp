@@ -213,7 +213,7 @@ struct aw_grid {
   // and that layout's levels and cell offsets once the topology moves on
   bool p_valid = false;
   int p_version = -1;
-  DevBuf p_prev, level_prev, coff_prev;
+  DevBuf p_prev, level_prev, coff_prev, remap_coff;
   uint32_t ncells = 0, nfaces = 0;
   int64_t n_sorted = -1;
   double occupied = 0.0;   // blocks holding particles (for byte accounting only)
@@ -2228,6 +2228,20 @@ int aw_tape_masks(aw_tape* t, int step, uint8_t* h_levels, uint8_t* h_liquid, ui
 }
 
 // ---- uniform resample and VTK export (vtx_visualization.cuh) ----
+
+int aw_remap_cells(aw_grid* g, const uint8_t* d_level_prev, const float* d_src, float* d_dst) {
+  if (!g || !d_level_prev || !d_src || !d_dst) return fail(AW_ERR_INVALID, "NULL argument");
+  if (!g->have_topology) return fail(AW_ERR_INVALID, "no topology");
+  if (d_src == d_dst) return fail(AW_ERR_INVALID, "d_src and d_dst must not alias");
+  hipStream_t s = g->ctx->stream;
+  // the previous layout's cell offsets from its levels (an exclusive scan of r^3)
+  AW_TRY(g->remap_coff.ensure((size_t)(g->nb + 1) * sizeof(uint32_t)));
+  AW_TRY(g->scan_tmp.ensure(scan_scratch_words(g->nb) * sizeof(uint32_t)));
+  k_cell_offsets(s, d_level_prev, g->nb, g->remap_coff.as<uint32_t>(), g->scan_tmp.as<uint32_t>());
+  Prof pf(g->ctx, "pressure_remap", 8.0 * g->ncells + 9.0 * g->nb);
+  k_pressure_remap(s, g->view(), d_level_prev, g->remap_coff.as<uint32_t>(), d_src, d_dst);
+  return launch_status("remap_cells");
+}
 
 int aw_resample_cells(aw_grid* g, const float* d_field, const int out_dims[3], float* d_out) {
   if (!g || !d_field || !out_dims || !d_out) return fail(AW_ERR_INVALID, "NULL argument");

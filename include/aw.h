@@ -31,6 +31,7 @@
  *   aw_solver_solve                      main.cu's fixed V-cycle loop        src/main.cu:314-317
  *   aw_step                              the main.cu step sequence, completed with the projection
  *   aw_resample_cells                    sampleMSBG                          include/vtx_visualization.cuh:8-62
+ *   aw_remap_cells                       (no counterpart: warm start across topology changes)
  *   aw_write_vtk_*, aw_export_*_vtk      exportUniformPressure_VTK, exportMultigrid_VTK, exportParticles_VTK
  *                                                                            include/vtx_visualization.cuh:64-216
  *   aw_*_adjoint, aw_tape_*, aw_rollout_* (no counterpart: the reference has no gradient code; these
@@ -326,6 +327,17 @@ int aw_rollout_backward(aw_tape* tape, float* d_xbar, float* d_vbar, aw_rollout_
  * forward step `step` used; h_liquid may be NULL to query *ncells. */
 int aw_tape_masks(aw_tape* tape, int step, uint8_t* h_levels, uint8_t* h_liquid, uint32_t liquid_cap,
                   uint32_t* ncells);
+
+/* Remap of a cell field across a topology change of this grid (no reference
+ * counterpart; the reference builds its topology once and starts every solve
+ * from p = 0, src/main.cu:300): d_src holds a field of the layout whose native
+ * levels were d_level_prev (device uint8[num_blocks], e.g. copied aside before
+ * aw_grid_build_topology), d_dst (float[total_cells]) receives it on the
+ * current layout, each cell taking the value of the previous cell that
+ * contains its centre (a centre on a face of the previous cells takes the
+ * cell above).  The warm start of aw_step remaps the pressure this way.
+ * Asynchronous; d_src and d_dst must not alias. */
+int aw_remap_cells(aw_grid* grid, const uint8_t* d_level_prev, const float* d_src, float* d_dst);
 
 /* ---- Uniform resample and VTK export (include/vtx_visualization.cuh) ----
  * aw_resample_cells: nearest-cell resample of a cell field (float[total_cells]

@@ -1586,6 +1586,40 @@ int og_resample(void* p, const double* field, int ox, int oy, int oz, double* ou
   return 0;
 }
 
+// Remap of a cell field from a previous layout (native levels level_prev, same
+// blocks) onto the current one: every current cell takes the value of the
+// previous layout's cell that contains its centre.  The centre of a cell of
+// edge s at local index l lies at fine coordinate 8 B + s l + s / 2; the
+// point is located by containment in a level built for the previous levels
+// (cell_of), at the fine cell just above the centre, so a centre that falls on
+// a face of the previous cells takes the cell above it (the warm start of the
+// pressure solve across topology changes; the reference has no counterpart:
+// main.cu:300 starts every solve from p = 0).
+int og_remap_cells(void* p, const uint8_t* level_prev, const double* src, double* dst) {
+  OGrid& g = *static_cast<OGrid*>(p);
+  Level P;
+  for (int a = 0; a < 3; ++a) P.D[a] = g.D[a];
+  P.bs = B;
+  P.e.resize(g.nb);
+  for (int b = 0; b < g.nb; ++b) {
+    if (level_prev[b] > LOGB) return 1;
+    P.e[b] = level_prev[b];
+  }
+  finish_level_cells(P);
+  for (int b = 0; b < g.nb; ++b) {
+    int Bc[3];
+    block_coords(g.D, b, Bc);
+    const int e = g.level[b], r = B >> e, s = 1 << e;
+    for (int li = 0; li < r * r * r; ++li) {
+      const int l[3] = {li % r, (li / r) % r, li / (r * r)};
+      int q[3];
+      for (int a = 0; a < 3; ++a) q[a] = Bc[a] * B + l[a] * s + s / 2;   // s = 1: the cell itself
+      dst[g.coff[b] + li] = src[cell_of(P, q)];
+    }
+  }
+  return 0;
+}
+
 int og_set_levels(void* p, const uint8_t* level) {
   OGrid& g = *static_cast<OGrid*>(p);
   for (int b = 0; b < g.nb; ++b)

@@ -43,6 +43,7 @@ def lib():
             "og_levels_from_radii": (ctypes.c_int, [_V, _F, _F, ctypes.c_int64, _U8]),
             "og_set_levels": (ctypes.c_int, [_V, _U8]),
             "og_resample": (ctypes.c_int, [_V, _D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]),
+            "og_remap_cells": (ctypes.c_int, [_V, _U8, _D, _D]),
             "og_get_levels": (ctypes.c_int, [_V, _U8]),
             "og_get_offsets": (ctypes.c_int, [_V, _U32, _U32, _U32, _U32]),
             "og_totals": (ctypes.c_int, [_V, _I64]),
@@ -163,6 +164,18 @@ class OracleGrid:
         out = np.zeros(int(np.prod(out_dims)), np.float64)
         if lib().og_resample(self._h, _p(field, _D), *[int(v) for v in out_dims], _p(out, _D)) != 0:
             raise ValueError("bad output size")
+        return out
+
+    def remap_cells(self, level_prev, src):
+        """Cell field of the layout with native levels level_prev onto the current layout (each cell
+        takes the previous cell containing its centre; a centre on a face takes the cell above)."""
+        lp = np.ascontiguousarray(level_prev, np.uint8)
+        src = np.ascontiguousarray(src, np.float64)
+        r = 8 >> lp.astype(np.int64)
+        assert len(src) == int((r ** 3).sum())
+        out = np.zeros(int(self.totals()[0]), np.float64)
+        if lib().og_remap_cells(self._h, _p(lp, _U8), _p(src, _D), _p(out, _D)) != 0:
+            raise ValueError("level out of range")
         return out
 
     def set_levels(self, lv):

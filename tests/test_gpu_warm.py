@@ -63,3 +63,46 @@ def test_warm_start_mixed_precision(ctx):
     assert np.abs(P1 - P0).max() < 2e-5
     vmax = max(np.abs(V0).max(), 1e-6)
     assert np.abs(V1 - V0).max() / vmax < 1e-3
+
+
+def _graded(rng, D, L):
+    """Random native levels with 2:1 grading over the 26-neighbourhood."""
+    from scipy.ndimage import minimum_filter
+    lv = rng.integers(0, L, (D[2], D[1], D[0]))
+    while True:
+        new = np.minimum(lv, minimum_filter(lv, size=3, mode="nearest") + 1)
+        if np.array_equal(new, lv):
+            return lv.reshape(-1).astype(np.uint8)
+        lv = new
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_remap_cells_bit_exact(ctx, seed):
+    """aw_remap_cells (the warm start's pressure_remap_kernel) against the
+    oracle's remap by containment, bit-exact, between random graded layouts
+    with all four native levels; an unchanged layout copies the field."""
+    import oracle as O
+    D = (6, 5, 4)
+    rng = np.random.default_rng(seed)
+    prev, cur = _graded(rng, D, 4), _graded(rng, D, 4)
+    print(f"[warm] remap seed {seed}: levels {np.bincount(prev, minlength=4)} -> {np.bincount(cur, minlength=4)}",
+          flush=True)
+    gg = aw.MSBGManager(ctx, D, 1.0 / 64, num_levels=4)
+    og = O.OracleGrid(D, num_levels=4, h=1.0 / 64)
+    gg.set_levels(prev)
+    nprev = int(gg.layout().total_cells)
+    src = rng.standard_normal(nprev).astype(np.float32)
+    dsrc = aw.DeviceArray.from_numpy(ctx, src)
+    dlev = aw.DeviceArray.from_numpy(ctx, prev)
+    gg.set_levels(cur)
+    ncur = int(gg.layout().total_cells)
+    dout = aw.DeviceArray(ctx, 4 * ncur)
+    gg.remap_cells(dlev.ptr, dsrc.ptr, dout.ptr)
+    got = dout.download(np.float32, ncur)
+    og.set_levels(cur)
+    want = og.remap_cells(prev, src.astype(np.float64)).astype(np.float32)
+    np.testing.assert_array_equal(got, want)
+    gg.set_levels(prev)
+    dcopy = aw.DeviceArray(ctx, 4 * nprev)
+    gg.remap_cells(dlev.ptr, dsrc.ptr, dcopy.ptr)
+    np.testing.assert_array_equal(dcopy.download(np.float32, nprev), src)

@@ -1,0 +1,348 @@
+"""Pins of the oracle's solver and particle-update functions that the GPU
+parity tests alone cannot pin (they compare the HIP path with the oracle).
+
+* Smoother (Smoother.cuh:105-202): each (block colour, cell colour) class is
+  an independent set of the assembled operator, and one forward sweep equals
+  the textbook SOR iteration (D + w L) x' = w b + ((1 - w) D - w U) x in the
+  class order, solved as a dense triangular system; with several local
+  sweeps, the composition of the class updates.
+* V-cycle (VCycle.cuh:251-304): symmetric and positive definite as a
+  preconditioner, equal to a dense numpy V-cycle built from the assembled
+  matrices, the parents and the pinned smoother, and contracting as a
+  stationary iteration with the coarse-correction scale of method 1.
+* RK3 (G2P.cuh:131-173): on an affine velocity field the step is the third
+  order Taylor polynomial of the exact flow (1 + z + z^2/2 + z^3/6 for any
+  three-stage third-order Runge-Kutta); forward Euler or a second-order
+  scheme fails it.
+* FLIP (G2P.cuh:87-128): v' = a (v_p + u_new - u_old) + (1 - a) u_new with
+  u_new != u_old and v_p != 0, on affine fields sampled at the particle.
+* Warm-start remap: the oracle's remap against exact containment of each new
+  cell centre in the previous layout's cells.
+"""
+import itertools
+from fractions import Fraction
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse as sp
+
+import oracle as O
+from anitowave_adaptive_amd import scenes
+
+OMEGA = float(np.float32(0.85714))
+
+
+def dense(g, k):
+    rp, col, val, dg = g.matrix(k)
+    n = len(dg)
+    A = (sp.csr_matrix((val, col, rp), shape=(n, n)) + sp.diags(dg)).toarray()
+    liq = g.mg_get(k, "liquid") > 0
+    return A, liq
+
+
+def classes(g, k):
+    """(block colour, cell colour) of every cell of MG level k, from the
+    level's block dims, per-block cell edge and cell offsets."""
+    info = g.mg_info(k)
+    D = info["dims"]
+    e, coff = g.mg_e(k)
+    bs = info["bs"]
+    bc = np.zeros(info["ncells"], int)
+    cc = np.zeros(info["ncells"], int)
+    for b in range(info["nb"]):
+        B = (b % D[0], (b // D[0]) % D[1], b // (D[0] * D[1]))
+        r = bs >> int(e[b])
+        for li in range(r ** 3):
+            l = (li % r, (li // r) % r, li // (r * r))
+            bc[coff[b] + li] = sum(B) & 1
+            cc[coff[b] + li] = sum(l) & 1
+    return bc, cc
+
+
+def class_sequence(local_sweeps, reverse):
+    order = (1, 0) if reverse else (0, 1)
+    return [(b, c) for b in order for _ in range(local_sweeps) for c in order]
+
+
+def smooth_by_classes(A, liq, bc, cc, x, b, iters, reverse, local_sweeps, omega=OMEGA):
+    x = x.copy()
+    d = np.diag(A)
+    for _ in range(iters):
+        for cb, cl in class_sequence(local_sweeps, reverse):
+            S = liq & (bc == cb) & (cc == cl) & (d > 0)
+            r = b - A @ x
+            x[S] += omega * r[S] / d[S]
+    return x
+
+
+def sor_sweep(A, liq, bc, cc, x, b, reverse, omega=OMEGA):
+    """One SOR sweep over the liquid unknowns in the class order (one local
+    sweep), in the classical matrix-splitting form."""
+    idx = []
+    for cb, cl in class_sequence(1, reverse):
+        idx.extend(np.nonzero(liq & (bc == cb) & (cc == cl))[0])
+    idx = np.array(idx)
+    Ap = A[np.ix_(idx, idx)]
+    D = np.diag(np.diag(Ap))
+    Lo = np.tril(Ap, -1)
+    Up = np.triu(Ap, 1)
+    rhs = omega * b[idx] + ((1 - omega) * D - omega * Up) @ x[idx]
+    out = x.copy()
+    out[idx] = sla.solve_triangular(D + omega * Lo, rhs, lower=True)
+    return out
+
+
+def two_level_grid():
+    """2 x 2 x 1 blocks: one level-0 block (8^3) beside three level-1 blocks
+    (4^3), coarse-fine interfaces on two sides, random air cells."""
+    g = O.OracleGrid((2, 2, 1), num_levels=2, h=1.0 / 16)
+    g.set_levels(np.array([0, 1, 1, 1], np.uint8))
+    rng = np.random.default_rng(11)
+    n = int(g.totals()[0])
+    g.set_cell("liquid", (rng.random(n) < 0.85).astype(np.float64))
+    return g, rng
+
+
+@pytest.mark.parametrize("local_sweeps", [1, 2])
+@pytest.mark.parametrize("k", [0, 1])
+def test_smoother_is_class_ordered_sor(k, local_sweeps):
+    g, rng = two_level_grid()
+    g.set_params(local_sweeps=local_sweeps)
+    g.mg_build()
+    A, liq = dense(g, k)
+    bc, cc = classes(g, k)
+    # every class is an independent set: no coupling inside it
+    for cb, cl in itertools.product((0, 1), repeat=2):
+        S = np.nonzero(liq & (bc == cb) & (cc == cl))[0]
+        sub = A[np.ix_(S, S)]
+        assert np.count_nonzero(sub - np.diag(np.diag(sub))) == 0, (cb, cl)
+    n = len(liq)
+    for reverse in (False, True):
+        x0 = rng.standard_normal(n)
+        b = rng.standard_normal(n)
+        g.mg_set(k, "x", x0)
+        g.mg_set(k, "b", b)
+        g.smooth(k, 2, reverse)
+        got = g.mg_get(k, "x")
+        want = smooth_by_classes(A, liq, bc, cc, x0, b, 2, reverse, local_sweeps)
+        np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * np.abs(want).max())
+        if local_sweeps == 1:
+            sor = sor_sweep(A, liq, bc, cc, sor_sweep(A, liq, bc, cc, x0, b, reverse), b, reverse)
+            np.testing.assert_allclose(got[liq], sor[liq], rtol=0, atol=1e-12 * np.abs(sor).max())
+        # air cells are Dirichlet: the smoother leaves them alone
+        np.testing.assert_array_equal(got[~liq], x0[~liq])
+
+
+def pool_grid(n=32, prolong_scale=1.0, coarse_sweeps=16):
+    pos, vel = scenes.pool_sphere(n=n, radius=0.15)
+    g = O.OracleGrid((n // 8,) * 3, num_levels=3, h=1.0 / n, band=(0, 1, 0, 0))
+    g.build_topology(pos)
+    g.p2g(pos, vel, 1 / 240, (0, -9.81, 0))
+    g.set_params(prolong_scale=prolong_scale, coarse_sweeps=coarse_sweeps)
+    g.mg_build()
+    return g
+
+
+def vcycle(g, r):
+    g.mg_set(0, "b", r)
+    g.mg_set(0, "x", np.zeros_like(r))
+    g.vcycle(0)
+    return g.mg_get(0, "x")
+
+
+def test_vcycle_symmetric_positive_definite():
+    g = pool_grid()
+    liq = g.mg_get(0, "liquid") > 0
+    rng = np.random.default_rng(3)
+    n = g.mg_info(0)["ncells"]
+    rs = []
+    for _ in range(4):
+        r = rng.standard_normal(n)
+        r[~liq] = 0.0
+        rs.append(r)
+    Vs = [vcycle(g, r) for r in rs]
+    for i, j in itertools.combinations(range(4), 2):
+        a, b = Vs[i] @ rs[j], rs[i] @ Vs[j]
+        assert abs(a - b) <= 1e-11 * (abs(a) + abs(b) + np.linalg.norm(Vs[i]) * np.linalg.norm(rs[j])), (a, b)
+    for r, v in zip(rs, Vs):
+        assert r @ v > 0
+        # air cells stay zero
+        assert not np.any(v[~liq])
+    # linear: V(2 r1 - r2) = 2 V r1 - V r2
+    comb = vcycle(g, 2 * rs[0] - rs[1])
+    np.testing.assert_allclose(comb, 2 * Vs[0] - Vs[1], rtol=0, atol=1e-11 * np.abs(comb).max())
+
+
+def numpy_vcycle(g, k, b, pre=2, post=2, local_sweeps=1, scale=1.0):
+    """VCycle.cuh:251-304 on dense matrices: pre-smooth, residual, restrict
+    (sum of children), zero coarse guess, recurse, prolong (injection add on
+    liquid cells), post-smooth in reverse order; the coarsest level (<= 64
+    cells) solved exactly on its liquid cells."""
+    A, liq = dense(g, k)
+    if k + 1 == g.num_mg_levels():
+        assert len(b) <= 64
+        x = np.zeros_like(b)
+        if liq.any():
+            x[liq] = np.linalg.solve(A[np.ix_(liq, liq)], b[liq])
+        return x
+    bc, cc = classes(g, k)
+    x = smooth_by_classes(A, liq, bc, cc, np.zeros_like(b), b, pre, False, local_sweeps)
+    r = np.where(liq, b - A @ x, 0.0)
+    par = g.mg_get(k, "parent").astype(int)
+    nc = g.mg_info(k + 1)["ncells"]
+    xc = numpy_vcycle(g, k + 1, np.bincount(par, weights=r, minlength=nc), pre, post, local_sweeps, scale)
+    x = x + np.where(liq, scale * xc[par], 0.0)
+    return smooth_by_classes(A, liq, bc, cc, x, b, post, True, local_sweeps)
+
+
+@pytest.mark.parametrize("scale", [1.0, 0.75])
+def test_vcycle_matches_dense_restatement(scale):
+    # a small pool, 16^3 cells in 2^3 blocks of native levels 0 and 1: a
+    # hierarchy of four levels (cell edges up to 8) down to 8 cells
+    pos, vel = scenes.pool_sphere(n=16, radius=0.2, depth=0.4)
+    g = O.OracleGrid((2, 2, 2), num_levels=2, h=1.0 / 16)
+    g.set_levels(np.array([0, 0, 1, 1, 0, 1, 1, 1], np.uint8))
+    g.p2g(pos, vel)
+    g.set_params(prolong_scale=scale)
+    g.mg_build()
+    assert g.num_mg_levels() == 4
+    assert g.mg_info(g.num_mg_levels() - 1)["ncells"] <= 64
+    rng = np.random.default_rng(7)
+    liq = g.mg_get(0, "liquid") > 0
+    b = np.where(liq, rng.standard_normal(len(liq)), 0.0)
+    got = vcycle(g, b)
+    want = numpy_vcycle(g, 0, b, scale=scale)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-11 * np.abs(want).max())
+
+
+def test_stationary_vcycle_contracts():
+    """x <- x + V (b - A x) with the coarse-correction scale 0.75 of the
+    stationary method reduces the error in the A-norm by a factor below 1
+    (power iteration on the error propagator I - V A)."""
+    g = pool_grid(prolong_scale=0.75)
+    rp, col, val, dg = g.matrix(0)
+    n = len(dg)
+    A = sp.csr_matrix((val, col, rp), shape=(n, n)) + sp.diags(dg)
+    liq = g.mg_get(0, "liquid") > 0
+    rng = np.random.default_rng(9)
+    e = np.where(liq, rng.standard_normal(n), 0.0)
+    anorm = lambda v: np.sqrt(v @ (A @ v))
+    ratios = []
+    for _ in range(12):
+        e2 = e - vcycle(g, np.where(liq, A @ e, 0.0))
+        ratios.append(anorm(e2) / anorm(e))
+        e = e2 / anorm(e2)
+    assert max(ratios[4:]) < 0.9, ratios
+    assert min(ratios) > 0.0
+
+
+def affine_faces(g, coef):
+    """Native face values of u_a(x) = c0 + c1 x + c2 y + c3 z at the face
+    centres (fine units), every block of the grid."""
+    lv = g.levels()
+    c, f, _, _ = g.offsets()
+    D = g.dims
+    out = [np.zeros(int(g.totals()[1 + a])) for a in range(3)]
+    for b in range(g.nb):
+        B = np.array([b % D[0], (b // D[0]) % D[1], b // (D[0] * D[1])])
+        e = int(lv[b])
+        s, r = 1 << e, 8 >> e
+        for a in range(3):
+            dims = [r, r, r]
+            dims[a] = r + 1
+            for z, y, x in itertools.product(range(dims[2]), range(dims[1]), range(dims[0])):
+                sl = np.array([x, y, z], float)
+                posf = 8 * B + s * (sl + 0.5)
+                posf[a] = 8 * B[a] + s * sl[a]
+                k = x + (dims[0]) * (y + dims[1] * z)
+                out[a][f[b] + k] = coef[a][0] + coef[a][1:] @ posf
+    return out
+
+
+def affine_grid(coef_new, coef_old):
+    n = 32
+    g = O.OracleGrid((4, 4, 4), num_levels=1, h=1.0 / n)
+    g.set_levels(np.zeros(64, np.uint8))
+    g.p2g(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.float32))
+    fn = affine_faces(g, coef_new)
+    fo = affine_faces(g, coef_old)
+    for a in range(3):
+        g.set_face("u_star", a, fn[a])
+        g.set_face("u_old", a, fo[a])
+    g.set_cell("p", np.zeros(int(g.totals()[0])))
+    g.project()   # p = 0: u_new = u_star away from the tank walls
+    return g, n
+
+
+def test_rk3_is_third_order_on_an_affine_field():
+    # u(x) = c + M x / h (world velocity, x in world units); the exact flow's
+    # Taylor polynomial to third order is what any 3-stage 3rd-order RK gives
+    coef = [np.array([0.30, 0.020, -0.015, 0.010]), np.array([-0.20, 0.010, 0.025, -0.020]),
+            np.array([0.25, -0.015, 0.010, 0.020])]
+    g, n = affine_grid(coef, coef)
+    h = 1.0 / n
+    c = np.array([cf[0] for cf in coef])
+    M = np.array([cf[1:] for cf in coef]) / h
+    rng = np.random.default_rng(21)
+    p0 = rng.uniform(0.3, 0.7, (300, 3)).astype(np.float32)
+    v0 = np.zeros_like(p0)
+    dt = 0.02
+    po, _ = g.g2p_advect(p0, v0, 0.95, dt)
+    x = p0.astype(np.float64)
+    f = c + x @ M.T
+    Z = dt * M
+    taylor = x + dt * f + (dt * f) @ (Z / 2).T + (dt * f) @ (Z @ Z / 6).T
+    np.testing.assert_allclose(po, taylor, rtol=0, atol=1e-12)
+    # the terms are resolved: dropping the third-order term misses by far more
+    second = x + dt * f + (dt * f) @ (Z / 2).T
+    assert np.abs(second - taylor).max() > 1e-7
+    assert np.abs(x + dt * f - taylor).max() > 1e-5
+
+
+def test_flip_blend_with_a_velocity_change():
+    coef_new = [np.array([0.30, 0.010, -0.02, 0.0]), np.array([-0.40, 0.0, 0.015, 0.01]),
+                np.array([0.10, -0.01, 0.0, 0.02])]
+    coef_old = [np.array([0.10, 0.0, 0.01, -0.01]), np.array([0.20, 0.02, 0.0, 0.0]),
+                np.array([-0.30, 0.0, -0.01, 0.0])]
+    g, n = affine_grid(coef_new, coef_old)
+    rng = np.random.default_rng(5)
+    p0 = rng.uniform(0.25, 0.75, (200, 3)).astype(np.float32)
+    v0 = rng.standard_normal((200, 3)).astype(np.float32)
+    units = p0.astype(np.float64) * n
+    un = np.column_stack([coef_new[a][0] + units @ coef_new[a][1:] for a in range(3)])
+    uo = np.column_stack([coef_old[a][0] + units @ coef_old[a][1:] for a in range(3)])
+    for alpha in (0.95, 1.0, 0.0):
+        _, vo = g.g2p_advect(p0, v0, alpha, 1.0 / 240)
+        want = alpha * (v0.astype(np.float64) + un - uo) + (1 - alpha) * un
+        np.testing.assert_allclose(vo, want, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_remap_cells_by_exact_containment(seed):
+    D = (3, 2, 2)
+    rng = np.random.default_rng(seed)
+    g = O.OracleGrid(D, num_levels=4, h=1.0 / 32)
+    prev = rng.integers(0, 4, g.nb).astype(np.uint8)
+    cur = rng.integers(0, 4, g.nb).astype(np.uint8)
+    r_prev = 8 >> prev.astype(np.int64)
+    coff_prev = np.concatenate([[0], np.cumsum(r_prev ** 3)])
+    src = rng.standard_normal(int(coff_prev[-1]))
+    g.set_levels(cur)
+    got = g.remap_cells(prev, src)
+    r_cur = 8 >> cur.astype(np.int64)
+    want = []
+    for b in range(g.nb):
+        s, r = 8 // int(r_cur[b]), int(r_cur[b])
+        sp_ = 8 // int(r_prev[b])
+        for li in range(r ** 3):
+            l = (li % r, (li // r) % r, li // (r * r))
+            # centre of the new cell in fine units inside the block (exact), and
+            # the previous cell containing it (floor: a centre on a face goes above)
+            q = [int((Fraction(l[a]) + Fraction(1, 2)) * s // sp_) for a in range(3)]
+            rp = int(r_prev[b])
+            want.append(src[coff_prev[b] + q[0] + rp * (q[1] + rp * q[2])])
+    np.testing.assert_array_equal(got, np.array(want))
+    # an unchanged layout copies the field
+    g.set_levels(prev)
+    np.testing.assert_array_equal(g.remap_cells(prev, src), src)
