@@ -54,14 +54,21 @@ def run_gpu(args, w, pos, vel):
     iters = []
     for i in range(args.warmup):
         st = step(dpos.ptr, dvel.ptr, dids.ptr, n, allow_not_converged=True)
-        log(f"[bench] warmup step {i}: pcg iters {st.solve.iterations} rel {st.solve.rel_residual:.2e}")
+        log(f"[bench] warmup step {i}: pcg iters {st.solve.iterations} true rel. residual "
+            f"{st.solve.rel_residual:.2e} (recurrence {st.solve.recurrence_rel_residual:.2e})")
     ctx.synchronize()
     t0 = time.perf_counter()
     changed = 0
+    # the solver's contract: a step whose true residual stays above the
+    # tolerance (the FP32 floor) returns AW_ERR_NOT_CONVERGED; count them
+    floors, worst_true, worst_rec = 0, 0.0, 0.0
     for i in range(args.steps):
         st = step(dpos.ptr, dvel.ptr, dids.ptr, n, allow_not_converged=True)
         iters.append(st.solve.iterations)
         changed += st.topology_changed
+        floors += step.status != aw.AW_OK
+        worst_true = max(worst_true, st.solve.rel_residual)
+        worst_rec = max(worst_rec, st.solve.recurrence_rel_residual)
     ctx.synchronize()
     t1 = time.perf_counter()
     # per-kernel HIP-event timings from separate steps, so that the events
@@ -129,7 +136,8 @@ def run_gpu(args, w, pos, vel):
                   "bin_ms": round(rp.get("bin", {}).get("ms", 0.0), 4),
                   "blocks_by_level": [int(v) for v in np.bincount(grid.levels(), minlength=w["levels"])]}
     res = dict(seconds=t1 - t0, iters=iters, topology_changes=changed, blocks_by_level=lv, radius_rule=radius,
-               resample=resample,
+               resample=resample, steps_above_tol=floors, worst_true_rel_residual=worst_true,
+               worst_recurrence_rel_residual=worst_rec, liquid_cells=int(st.liquid_cells),
                total_cells=int(st.total_cells), rel_residual=st.solve.rel_residual, profile=prof, isolated=iso,
                peak_bytes=step_peak)
     return res
@@ -142,6 +150,27 @@ KERNEL_OF = {"smooth_l0": "smooth_kernel", "smooth_coarse": "smooth_kernel", "sm
              "residual_restrict_coarse": "residual_kernel", "g2p_advect": "g2p_advect_kernel",
              "p2g_deposit": "p2g_deposit_kernel", "p2g_gather": "p2g_gather_kernel", "to_fine": "to_fine_kernel",
              "apply_l0": "apply_dot_kernel", "sort_particles": "scatter_kernel", "vcycle_tail": "tail_kernel"}
+
+
+def kernel_source_sha():
+    import hashlib
+    src = os.path.join(ROOT, "anitowave_adaptive_amd", "csrc", "aw_kernels.hip")
+    return hashlib.sha256(open(src, "rb").read()).hexdigest()[:16]
+
+
+def traffic_for(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (profiles/traffic.json), only while the kernel source is the one
+    it was measured on (its sha is recorded there); otherwise null."""
+    tf = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        t = json.load(open(tf))
+    except Exception:
+        return None, "no traffic measurement"
+    if t.get("kernel_source_sha") != kernel_source_sha():
+        return None, "traffic measurement is for another kernel source: not used"
+    v = t.get("bytes_per_launch", {}).get(kernel)
+    return v, (t.get("measured_by", "profiles/traffic.json") if v is not None else "kernel not measured")
 
 
 def roofline(prof):
@@ -169,23 +198,19 @@ def roofline(prof):
     dom = max(kern, key=lambda k: kern[k]["ms"])
     d = kern[dom]
     gbs = d["bytes"] / (d["ms"] * 1e-3) / 1e9
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tf):
-        try:
-            traffic = json.load(open(tf)).get(dom)
-        except Exception:
-            traffic = None
+    traffic, traffic_note = traffic_for(dom)
     rl = {"kernel": dom, "groups": d["groups"], "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
-          "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+          "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_note,
           "bytes_per_launch": round(d["bytes"] / d["launches"]), "us_per_launch": round(1e3 * d["ms"] / d["launches"], 3),
           "launches": d["launches"]}
     return rl, per
 
 
-def run_mixed(args, w, pos, vel):
+def run_mixed(args, w, pos, vel, tol=1e-8, steps=None):
     """The same workload with mixed precision (FP32 V-cycle PCG inside FP64
-    iterative refinement) to relative residual 1e-8."""
+    iterative refinement) to relative residual `tol`.  The first step (from
+    the initial particles, untimed) is kept for the error against the FP64
+    oracle's step: its FP64 pressure, u_new and the positions by id."""
     import anitowave_adaptive_amd as aw
     aw.memory_stats(reset_peak=True)
     ctx = aw.Context(0)
@@ -195,23 +220,30 @@ def run_mixed(args, w, pos, vel):
     dids = aw.DeviceArray.from_numpy(ctx, np.arange(n, dtype=np.int32))
     grid = aw.MSBGManager(ctx, w["dims"], 1.0 / w["n"], num_levels=w["levels"], band=w["band"])
     solver = aw.VCycleSolver(grid, precision=1)
-    step = aw.FlipStep(grid, solver, dt=1.0 / 240.0, gravity=(0.0, -9.81, 0.0), alpha=0.95, tol=1e-8,
+    step = aw.FlipStep(grid, solver, dt=1.0 / 240.0, gravity=(0.0, -9.81, 0.0), alpha=0.95, tol=tol,
                        max_iters=400, rebuild_topology=True)
+    steps = steps or args.mixed_steps
     step(dpos.ptr, dvel.ptr, dids.ptr, n)
+    first = None
+    if args.cpu:
+        P = np.empty((n, 3), np.float32)
+        P[dids.download(np.int32, n)] = dpos.download(np.float32, 3 * n).reshape(-1, 3)
+        first = {"p": solver.level_get(0, "x64"), "u_new": [grid.face("u_new", a) for a in range(3)], "pos": P,
+                 "liquid": grid.cell("liquid")}
     ctx.synchronize()
     t0 = time.perf_counter()
     it, outer, rel = [], [], 0.0
-    for _ in range(args.mixed_steps):
+    for _ in range(steps):
         st = step(dpos.ptr, dvel.ptr, dids.ptr, n)
         it.append(st.solve.iterations)
         outer.append(st.solve.outer_iterations)
         rel = max(rel, st.solve.rel_residual)
     ctx.synchronize()
     dt = time.perf_counter() - t0
-    return {"steps_per_s": round(args.mixed_steps / dt, 3), "ms_per_step": round(1e3 * dt / args.mixed_steps, 3),
-            "rel_tol": 1e-8, "worst_rel_residual": rel, "steps": args.mixed_steps,
+    return {"steps_per_s": round(steps / dt, 3), "ms_per_step": round(1e3 * dt / steps, 3),
+            "rel_tol": tol, "worst_rel_residual": rel, "steps": steps,
             "iterations_mean": round(float(np.mean(it)), 2), "refinements_mean": round(float(np.mean(outer)), 2),
-            "peak_hbm_gib": round(aw.memory_stats()[1] / 2**30, 3)}
+            "peak_hbm_gib": round(aw.memory_stats()[1] / 2**30, 3)}, first
 
 
 def run_warm(args, w, pos, vel):
@@ -312,30 +344,127 @@ def run_large(args):
     log(f"[bench] large: seeded {len(pos)} particles in {time.perf_counter() - t:.1f}s")
     a = copy.copy(args)
     a.steps, a.warmup, a.profile_steps, a.mixed_steps, a.radius_calls, a.resample = 5, 2, 2, 3, 0, 0
+    a.cpu = False
     gpu = run_gpu(a, w, pos, vel)
     rl, per = roofline(gpu["profile"])
     log(f"[bench] large: {a.steps / gpu['seconds']:.3f} steps/s")
-    mixed = run_mixed(a, w, pos, vel)
+    mixed, _ = run_mixed(a, w, pos, vel)
     log(f"[bench] large mixed precision 1e-8: {mixed}")
     adj = run_adjoint(a, name="double512", steps=8, scene=(pos, vel))
     return {"workload": f"double512: 512^3 effective, 4 levels, {len(pos)} particles",
             "steps_per_s": round(a.steps / gpu["seconds"], 3), "ms_per_step": round(1e3 * gpu["seconds"] / a.steps, 3),
             "pcg_iters_mean": round(float(np.mean(gpu["iters"])), 2), "peak_hbm_gib": round(gpu["peak_bytes"] / 2**30, 3),
             "blocks_by_level": gpu["blocks_by_level"][: w["levels"]], "roofline": rl, "kernels": per,
-            "mixed_precision_1e-8": mixed, "adjoint": adj}
+            "mixed_precision_1e-8": mixed,
+        "mixed_precision_1e-10": mixed10,
+        "whole_scene": scene, "adjoint": adj}
 
 
-def run_cpu_baseline(args, w, pos, vel):
+def run_scene(args, w, pos, vel):
+    """The whole scene from its initial state: every step of the workload
+    (100 for the pool, so the sphere's impact on the pool and the splash are
+    inside), timed as one run, fresh grid and solver (allocations included)."""
+    import anitowave_adaptive_amd as aw
+    ctx = aw.Context(0)
+    n = len(pos)
+    dpos = aw.DeviceArray.from_numpy(ctx, pos)
+    dvel = aw.DeviceArray.from_numpy(ctx, vel)
+    dids = aw.DeviceArray.from_numpy(ctx, np.arange(n, dtype=np.int32))
+    grid = aw.MSBGManager(ctx, w["dims"], 1.0 / w["n"], num_levels=w["levels"], band=w["band"])
+    solver = aw.VCycleSolver(grid)
+    step = aw.FlipStep(grid, solver, dt=1.0 / 240.0, gravity=(0.0, -9.81, 0.0), alpha=0.95, tol=args.tol,
+                       max_iters=200, rebuild_topology=True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    it, floors, worst, level0 = [], 0, 0.0, []
+    for k in range(args.scene_steps):
+        st = step(dpos.ptr, dvel.ptr, dids.ptr, n, allow_not_converged=True)
+        it.append(st.solve.iterations)
+        floors += step.status != aw.AW_OK
+        worst = max(worst, st.solve.rel_residual)
+        level0.append(int(st.num_blocks_by_level[0]))
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    return {"steps": args.scene_steps, "steps_per_s": round(args.scene_steps / dt, 3),
+            "ms_per_step": round(1e3 * dt / args.scene_steps, 3), "pcg_iters_mean": round(float(np.mean(it)), 2),
+            "pcg_iters_max": int(max(it)), "steps_above_tol": floors, "worst_true_rel_residual": worst,
+            "level0_blocks_min_max": [min(level0), max(level0)]}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def rel_max(a, b, mask=None):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    if mask is not None:
+        a, b = a[mask], b[mask]
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def run_cpu_baseline(args, w, pos, vel, firsts):
+    """FP64 C++/OpenMP oracle on the host cores, at relative residual 1e-8
+    (the tolerance the GPU's mixed-precision leg is compared at): two steps
+    of the same workload from the same particles, all threads; the first
+    step's pressure, u_new and positions give the error of the GPU's mixed
+    1e-8 step, a 1e-10 step (not timed into the value) that of the 1e-10 leg;
+    one step of the 64^3 dam break on one thread and on all, for scaling."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
+    nt = O.num_threads()
+    O.set_threads(nt)
     g = O.OracleGrid(w["dims"], num_levels=w["levels"], h=1.0 / w["n"], band=w["band"])
     g.set_params()
+    grav = (0.0, -9.81, 0.0)
+    errors = {}
+
+    def errs(tol, r):
+        f = firsts.get(tol)
+        if f is None:
+            return None
+        liq = f["liquid"] > 0
+        return {"pressure_rel": rel_max(f["p"], g.cell("p"), liq),
+                "u_new_rel": max(rel_max(f["u_new"][a], g.face("u_new", a)) for a in range(3)),
+                "position_abs": float(np.abs(f["pos"] - r["pos"]).max())}
+
     t0 = time.perf_counter()
-    r = g.step(pos, vel, 1.0 / 240.0, (0.0, -9.81, 0.0), 0.95, tol=args.tol)
-    dt = time.perf_counter() - t0
-    return {"value": round(1.0 / dt, 5), "unit": "steps/s", "cores": O.num_threads(), "kind": "port",
-            "sample": f"1 full step of the same workload ({len(pos)} particles) through the FP64 C++/OpenMP "
-                      f"oracle, {dt:.2f} s, stage ms {[round(x, 1) for x in r['stage_ms']]}"}
+    r1 = g.step(pos, vel, 1.0 / 240.0, grav, 0.95, tol=1e-8, max_iters=400)
+    t1 = time.perf_counter()
+    errors["1e-08"] = errs(1e-8, r1)
+    r2 = g.step64(r1["pos"], r1["vel"], 1.0 / 240.0, grav, 0.95, tol=1e-8, max_iters=400)
+    t2 = time.perf_counter()
+    dt = (t1 - t0) + (t2 - t1)
+    r3 = g.step(pos, vel, 1.0 / 240.0, grav, 0.95, tol=1e-10, max_iters=400)
+    t3 = time.perf_counter()
+    errors["1e-10"] = errs(1e-10, r3)
+    # thread scaling on the 64^3 dam break
+    wd = workload("dam64")
+    pd, vd = wd["scene"](n=wd["n"])
+    gd = O.OracleGrid(wd["dims"], num_levels=wd["levels"], h=1.0 / wd["n"], band=wd["band"])
+    gd.set_params()
+    gd.step(pd, vd, 1.0 / 240.0, grav, 0.95, tol=1e-8)
+    ta = time.perf_counter()
+    gd.step(pd, vd, 1.0 / 240.0, grav, 0.95, tol=1e-8)
+    tb = time.perf_counter()
+    O.set_threads(1)
+    gd.step(pd, vd, 1.0 / 240.0, grav, 0.95, tol=1e-8)
+    tc = time.perf_counter()
+    O.set_threads(nt)
+    return {"value": round(2.0 / dt, 5), "unit": "steps/s", "cores": nt, "kind": "port",
+            "cpu_model": cpu_model(), "rel_tol": 1e-8,
+            "sample": f"2 full steps of the same workload ({len(pos)} particles) through the FP64 C++/OpenMP "
+                      f"oracle at relative residual 1e-8 on {nt} threads: {t1 - t0:.2f} s + {t2 - t1:.2f} s, "
+                      f"{r1['iters']} + {r2['iters']} PCG iterations",
+            "step_1e-10_s": round(t3 - t2, 3),
+            "dam64_steps_per_s": {"threads_all": round(1.0 / (tb - ta), 3), "threads_1": round(1.0 / (tc - tb), 3)},
+            "errors_vs_gpu_mixed": errors}
 
 
 def main():
@@ -349,7 +478,10 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=5, help="extra steps timed per kernel with HIP events")
     ap.add_argument("--no-cpu-baseline", dest="cpu", action="store_false")
     ap.add_argument("--no-adjoint", dest="adjoint", action="store_false")
-    ap.add_argument("--mixed-steps", type=int, default=5, help="steps of the mixed-precision 1e-8 leg (0: skip)")
+    ap.add_argument("--mixed-steps", type=int, default=5,
+                    help="steps of each mixed-precision leg, 1e-8 and 1e-10 (0: skip)")
+    ap.add_argument("--scene-steps", type=int, default=100,
+                    help="steps of the whole-scene leg from the initial state (0: skip)")
     ap.add_argument("--warm-steps", type=int, default=20,
                     help="steps of the warm-started leg (pressure from the previous step; 0: skip)")
     ap.add_argument("--radius-calls", type=int, default=5,
@@ -372,10 +504,16 @@ def main():
     gpu = run_gpu(args, w, pos, vel)
     log(f"[bench] gpu {args.steps} steps in {gpu['seconds']:.3f}s, pcg iters {gpu['iters']}")
     rl, per = roofline(gpu["profile"])
-    mixed = None
+    mixed, mixed10, firsts = None, None, {}
     if args.mixed_steps > 0:
-        mixed = run_mixed(args, w, pos, vel)
+        mixed, firsts[1e-8] = run_mixed(args, w, pos, vel, 1e-8)
         log(f"[bench] mixed precision 1e-8: {mixed}")
+        mixed10, firsts[1e-10] = run_mixed(args, w, pos, vel, 1e-10)
+        log(f"[bench] mixed precision 1e-10: {mixed10}")
+    scene = None
+    if args.scene_steps > 0:
+        scene = run_scene(args, w, pos, vel)
+        log(f"[bench] whole scene: {scene}")
     warm = None
     if args.warm_steps > 0:
         warm = run_warm(args, w, pos, vel)
@@ -384,8 +522,9 @@ def main():
     large = run_large(args) if args.large else None
     cpu = None
     if args.cpu:
-        log("[bench] cpu baseline: one oracle step")
-        cpu = run_cpu_baseline(args, w, pos, vel)
+        log("[bench] cpu baseline: oracle steps at 1e-8 and 1e-10, dam64 thread scaling")
+        cpu = run_cpu_baseline(args, w, pos, vel, firsts)
+        firsts.clear()
         log(f"[bench] cpu baseline {cpu['value']} steps/s on {cpu['cores']} threads")
     sps = args.steps / gpu["seconds"]
     smoother = per.get("smooth_l0")
@@ -406,7 +545,16 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"{args.workload}: {w['n']}^3 effective, {w['levels']} levels, 8^3-cell blocks, "
-                        f"{len(pos)} particles, MGPCG V(2,2) to rel. residual {args.tol:g}, FLIP 0.95, RK3",
+                        f"{len(pos)} particles, FP32 MGPCG V(2,2) stopped at recurrence residual {args.tol:g} "
+                        f"(true residual: worst {gpu['worst_true_rel_residual']:.2e}, "
+                        f"{gpu['steps_above_tol']}/{args.steps} steps above {args.tol:g}, the FP32 floor), "
+                        f"FLIP 0.95, RK3",
+            "solve": {"stopping": f"PCG recurrence residual <= {args.tol:g} |b|, then the true residual "
+                                  f"recomputed; one restart from it; AW_ERR_NOT_CONVERGED when it stays above",
+                      "steps_above_tol": gpu["steps_above_tol"],
+                      "worst_true_rel_residual": gpu["worst_true_rel_residual"],
+                      "worst_recurrence_rel_residual": gpu["worst_recurrence_rel_residual"]},
+            "liquid_cells": gpu["liquid_cells"],
             "blocks_by_level": gpu["blocks_by_level"][: w["levels"]],
             "mg0_cells": gpu["total_cells"],
             "pcg_iters_mean": round(float(np.mean(gpu["iters"])), 2),

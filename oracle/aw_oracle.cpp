@@ -1918,6 +1918,11 @@ int og_step(void* p, const float* pos, const float* vel, int64_t n, double dt, d
 }
 
 int og_num_threads(void) { return omp_get_max_threads(); }
+int og_set_threads(int n) {
+  if (n < 1) return 1;
+  omp_set_num_threads(n);
+  return 0;
+}
 
 static std::vector<double>* fine_field(OGrid& g, int which) {
   switch (which) {

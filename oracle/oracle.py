@@ -74,6 +74,7 @@ def lib():
             "og_step": (ctypes.c_int, [_V, _F, _F, ctypes.c_int64] + [ctypes.c_double] * 5
                         + [ctypes.c_int, ctypes.c_double, ctypes.c_int, _D, _D, _I, _D, _D]),
             "og_num_threads": (ctypes.c_int, []),
+            "og_set_threads": (ctypes.c_int, [ctypes.c_int]),
             "og_get_fine": (ctypes.c_int, [_V, ctypes.c_int, _D]),
             "og_set_fine": (ctypes.c_int, [_V, ctypes.c_int, _D]),
             "og_pi": (ctypes.c_int, [_V, ctypes.c_int, ctypes.c_int]),
@@ -444,3 +445,9 @@ def morton3d(x, y, z):
 
 def num_threads():
     return lib().og_num_threads()
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's parallel loops from here on."""
+    if lib().og_set_threads(int(n)) != 0:
+        raise ValueError(n)

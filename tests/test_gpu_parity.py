@@ -182,8 +182,10 @@ def test_pcg_solve(ctx, name):
     # FP32 floor (rounding p alone leaves ~eps32 |A||p|); bar 2e-5 relative
     assert true_rel < 2e-5, true_rel
     assert gs.rel_residual < 2e-5
-    # the reported residual is the true one (FP32 arithmetic on the same p)
-    assert abs(gs.rel_residual - true_rel) < 0.25 * true_rel + 1e-7, (gs.rel_residual, true_rel)
+    # the reported residual is the true one of the returned p, evaluated in
+    # FP32 arithmetic: at the floor that evaluation's own rounding is of the
+    # same order, so it agrees with the FP64 evaluation within a factor of 2
+    assert 0.5 * true_rel <= gs.rel_residual <= 2.0 * true_rel + 1e-7, (gs.rel_residual, true_rel)
     assert rel_err(p_gpu, og.cell("p")) < 1e-4
 
 
