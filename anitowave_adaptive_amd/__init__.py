@@ -93,6 +93,7 @@ SIGNATURES = {
     "aw_malloc": (_I, [_V, ctypes.c_uint64, _P(_V)]),
     "aw_free": (_I, [_V, _V]),
     "aw_copy": (_I, [_V, _V, _V, ctypes.c_uint64]),
+    "aw_fill_f32": (_I, [_V, _V, ctypes.c_uint64, _F]),
     "aw_memory_stats": (_I, [_P(ctypes.c_uint64), _P(ctypes.c_uint64), _I]),
     "aw_profile_enable": (_I, [_V, _I]),
     "aw_profile_report": (_I, [_V, ctypes.c_char_p, ctypes.c_uint64]),
@@ -107,6 +108,9 @@ SIGNATURES = {
     "aw_divergence": (_I, [_V]),
     "aw_project_velocity": (_I, [_V]),
     "aw_g2p_advect": (_I, [_V, _V, _V, _I64, _F, _F]),
+    "aw_p2g_particles": (_I, [_V, _V, _V, _I64, _F, _P(_F)]),
+    "aw_g2p": (_I, [_V, _V, _V, _I64, _F]),
+    "aw_advect": (_I, [_V, _V, _I64, _F]),
     "aw_solver_create": (_I, [_V, _P(SolverParams), _P(_V)]),
     "aw_solver_destroy": (_I, [_V]),
     "aw_solver_set_graph": (_I, [_V, _I]),
@@ -437,8 +441,15 @@ class P2GTransfer:
 
     @staticmethod
     def run(grid, d_pos, d_vel, n, dt=0.0, gravity=(0.0, 0.0, 0.0)):
+        """P2G of particles binned by `bin` for the current topology."""
         g = (ctypes.c_float * 3)(*gravity)
         check(lib().aw_p2g(grid.handle, d_pos, d_vel, int(n), float(dt), g), "aw_p2g")
+
+    @staticmethod
+    def run_unsorted(grid, d_pos, d_vel, n, dt=0.0, gravity=(0.0, 0.0, 0.0)):
+        """P2G of particles in any order; the caller's arrays are not reordered."""
+        g = (ctypes.c_float * 3)(*gravity)
+        check(lib().aw_p2g_particles(grid.handle, d_pos, d_vel, int(n), float(dt), g), "aw_p2g_particles")
 
 
 class G2PManager:
@@ -455,6 +466,16 @@ class G2PManager:
     @staticmethod
     def g2p_advect(grid, d_pos, d_vel, n, alpha=0.95, dt=1.0 / 240.0):
         check(lib().aw_g2p_advect(grid.handle, d_pos, d_vel, int(n), float(alpha), float(dt)), "aw_g2p_advect")
+
+    @staticmethod
+    def perform_g2p(grid, d_vel, d_pos, n, alpha=0.97):
+        """FLIP/PIC velocity update in place (performG2P, G2P.cuh:192-214)."""
+        check(lib().aw_g2p(grid.handle, d_pos, d_vel, int(n), float(alpha)), "aw_g2p")
+
+    @staticmethod
+    def advect(grid, d_pos, n, dt):
+        """RK3 advection through u_new in place (G2PManager::advect, G2P.cuh:216-227)."""
+        check(lib().aw_advect(grid.handle, d_pos, int(n), float(dt)), "aw_advect")
 
 
 def solver_params(method=0, pre=2, post=2, coarse=16, local=1, omega=0.85714, prolong_scale=1.0, precision=0):

@@ -33,6 +33,27 @@ def _compile(src):
     return obj
 
 
+EXAMPLE_SRC = os.path.join(HERE, "examples", "msbg_main.cpp")
+EXAMPLE = os.path.join(HERE, "examples", "msbg_main")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+
+def build_example(verbose=False):
+    """The C++ caller written against include/msbg_compat.hpp, compiled by the
+    host compiler alone (no HIP headers) and linked to libanitowave.so."""
+    cxx = os.environ.get("CXX", "g++")
+    tmp = EXAMPLE + ".tmp"
+    cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", INCLUDE, EXAMPLE_SRC, "-o", tmp, "-L", HERE,
+           "-lanitowave", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"example build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, EXAMPLE)
+    if verbose:
+        print("built", EXAMPLE)
+    return EXAMPLE
+
+
 def build(verbose=False):
     with ThreadPoolExecutor(len(SOURCES)) as ex:
         objs = list(ex.map(_compile, SOURCES))
@@ -44,6 +65,7 @@ def build(verbose=False):
     os.replace(tmp, OUT)
     if verbose:
         print("built", OUT)
+    build_example(verbose)
     return OUT
 
 

@@ -169,9 +169,10 @@ void k_to_fine(hipStream_t s, const GridDev& g, float* fine, const uint32_t* cou
 void k_pressure_remap(hipStream_t s, const GridDev& g, const uint8_t* level_prev, const uint32_t* coff_prev,
                       const float* p_prev, float* p);
 // list: n + 1 words of scratch for the particles whose stage points leave the
-// blocks with fine-lattice halos (taken in a second pass)
+// blocks with fine-lattice halos (taken in a second pass).  mode 0: FLIP blend
+// and RK3 advection; 1: the blend only (vel_out); 2: the advection only (pos_out)
 void k_g2p_advect(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const float* svel,
-                  int64_t n, float alpha, float dt, float* pos_out, float* vel_out, uint32_t* list);
+                  int64_t n, float alpha, float dt, float* pos_out, float* vel_out, uint32_t* list, int mode = 0);
 
 // adjoint (fine-lattice adjoint buffers: [block][field][axis][512], stride ADJ)
 // G2P adjoint: stage pass over the n sorted particles (partials, and one

@@ -1516,7 +1516,14 @@ __device__ __forceinline__ float sample_grad_sel(const GridDev& g, const float* 
 // FLIP/PIC blend (G2P.cuh:87-128) and Ralston RK3 (G2P.cuh:131-173) of
 // particle i.  Without SLOW a particle with a stage point in a block without
 // halo is appended to list (list[0] counts) and nothing is written for it.
-template <bool SLOW>
+// MODE: G2P_BOTH the blend and the advection (the step's fused pass);
+// G2P_VEL the blend only (G2PManager::performG2P, G2P.cuh:192-214), positions
+// untouched; G2P_POS the advection only (G2PManager::advect, G2P.cuh:216-227),
+// velocities untouched.  The advection samples u_new only, so the two halves
+// in either order give the fused pass's result.
+enum { G2P_BOTH = 0, G2P_VEL = 1, G2P_POS = 2 };
+
+template <bool SLOW, int MODE>
 __device__ __forceinline__ void g2p_advect_one(const GridDev& g, const float* __restrict__ F,
                                                const float* __restrict__ spos, const float* __restrict__ svel,
                                                int64_t i, float alpha, float dt, float* __restrict__ pos_out,
@@ -1532,11 +1539,20 @@ __device__ __forceinline__ void g2p_advect_one(const GridDev& g, const float* __
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     v1[a] = sample_comp<SLOW>(g, F, 0, a, u, nr, ok);
-    vd[a] = sample_comp<SLOW>(g, F, 1, a, u, nr, ok);
+    vd[a] = MODE == G2P_POS ? 0.f : sample_comp<SLOW>(g, F, 1, a, u, nr, ok);
   }
   float vn[3];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) vn[a] = alpha * (svel[3 * i + a] + vd[a]) + (1.f - alpha) * v1[a];
+  for (int a = 0; a < 3; ++a) vn[a] = MODE == G2P_POS ? 0.f : alpha * (svel[3 * i + a] + vd[a]) + (1.f - alpha) * v1[a];
+  if (MODE == G2P_VEL) {
+    if (!SLOW && !ok) {
+      list[1 + atomicAdd(&list[0], 1u)] = (uint32_t)i;
+      return;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) vel_out[3 * i + a] = vn[a];
+    return;
+  }
   float xx[3], v2[3], v3[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -1560,7 +1576,7 @@ __device__ __forceinline__ void g2p_advect_one(const GridDev& g, const float* __
   for (int a = 0; a < 3; ++a) {
     const float xn = x[a] + dt * ((2.f / 9.f) * v1[a] + (3.f / 9.f) * v2[a] + (4.f / 9.f) * v3[a]);
     const float lo = 0.25f * g.h, hi = (float)g.N[a] * g.h - 0.25f * g.h;
-    vel_out[3 * i + a] = vn[a];
+    if (MODE == G2P_BOTH) vel_out[3 * i + a] = vn[a];
     pos_out[3 * i + a] = fminf(fmaxf(xn, lo), hi);
   }
 }
@@ -1581,7 +1597,7 @@ __device__ __forceinline__ int64_t xcd_group(int64_t w, int64_t nwg) {
 // FIXUP = false: every particle; FIXUP = true: the particles listed by the
 // first pass (a grid-stride loop over list[0] entries, so the host needs no
 // count)
-template <bool FIXUP>
+template <bool FIXUP, int MODE>
 __global__ __launch_bounds__(TPB) void g2p_advect_kernel(GridDev g, const float* __restrict__ F,
                                                          const float* __restrict__ spos,
                                                          const float* __restrict__ svel, int64_t n, float alpha,
@@ -1591,11 +1607,11 @@ __global__ __launch_bounds__(TPB) void g2p_advect_kernel(GridDev g, const float*
   if (FIXUP) {
     const int64_t m = (int64_t)list[0];
     for (int64_t j = i; j < m; j += (int64_t)gridDim.x * blockDim.x)
-      g2p_advect_one<true>(g, F, spos, svel, (int64_t)list[1 + j], alpha, dt, pos_out, vel_out, list);
+      g2p_advect_one<true, MODE>(g, F, spos, svel, (int64_t)list[1 + j], alpha, dt, pos_out, vel_out, list);
     return;
   }
   if (i >= n) return;
-  g2p_advect_one<false>(g, F, spos, svel, i, alpha, dt, pos_out, vel_out, list);
+  g2p_advect_one<false, MODE>(g, F, spos, svel, i, alpha, dt, pos_out, vel_out, list);
 }
 
 // ------------------------------------------------------------- adjoint
@@ -3394,15 +3410,26 @@ void k_to_fine(hipStream_t s, const GridDev& g, float* fine, const uint32_t* cou
   to_fine_kernel<<<g.nb, TPB, 0, s>>>(g, fine);
 }
 
-void k_g2p_advect(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const float* svel,
-                  int64_t n, float alpha, float dt, float* pos_out, float* vel_out, uint32_t* list) {
-  if (n <= 0) return;
+template <int MODE>
+static void g2p_launch(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const float* svel,
+                       int64_t n, float alpha, float dt, float* pos_out, float* vel_out, uint32_t* list) {
   fill_kernel<uint32_t><<<1, 64, 0, s>>>(list, 1, 0u);
-  g2p_advect_kernel<false><<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, s>>>(g, fine, spos, svel, n, alpha, dt,
-                                                                            pos_out, vel_out, list);
+  g2p_advect_kernel<false, MODE><<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, s>>>(g, fine, spos, svel, n, alpha, dt,
+                                                                                  pos_out, vel_out, list);
   const int64_t nwg = (n + TPB - 1) / TPB;
   const unsigned fix = (unsigned)(nwg < 1024 ? nwg : 1024);
-  g2p_advect_kernel<true><<<fix, TPB, 0, s>>>(g, fine, spos, svel, n, alpha, dt, pos_out, vel_out, list);
+  g2p_advect_kernel<true, MODE><<<fix, TPB, 0, s>>>(g, fine, spos, svel, n, alpha, dt, pos_out, vel_out, list);
+}
+
+void k_g2p_advect(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const float* svel,
+                  int64_t n, float alpha, float dt, float* pos_out, float* vel_out, uint32_t* list, int mode) {
+  if (n <= 0) return;
+  if (mode == G2P_VEL)
+    g2p_launch<G2P_VEL>(s, g, fine, spos, svel, n, alpha, dt, pos_out, vel_out, list);
+  else if (mode == G2P_POS)
+    g2p_launch<G2P_POS>(s, g, fine, spos, svel, n, alpha, dt, pos_out, vel_out, list);
+  else
+    g2p_launch<G2P_BOTH>(s, g, fine, spos, svel, n, alpha, dt, pos_out, vel_out, list);
 }
 
 size_t adj_record_bytes() { return sizeof(AdjRec); }

@@ -1676,6 +1676,12 @@ int aw_copy(aw_ctx* c, void* dst, const void* src, uint64_t bytes) {
   return launch_status("copy");
 }
 
+int aw_fill_f32(aw_ctx* c, float* d, uint64_t n, float value) {
+  if (!c || (n && !d)) return fail(AW_ERR_INVALID, "NULL argument");
+  if (n) k_fill_f32(c->stream, d, n, value);
+  return launch_status("fill");
+}
+
 int aw_grid_create(aw_ctx* ctx, const aw_grid_desc* d, aw_grid** out) {
   if (!ctx || !d || !out) return fail(AW_ERR_INVALID, "NULL argument");
   *out = nullptr;
@@ -1835,6 +1841,38 @@ int aw_g2p_advect(aw_grid* g, float* d_pos, float* d_vel, int64_t n, float alpha
   k_g2p_advect(g->ctx->stream, g->view(), g->slab.as<float>(), d_pos, d_vel, n, alpha, dt, d_pos, d_vel,
                g->g2p_list.as<uint32_t>());
   return launch_status("g2p_advect");
+}
+
+int aw_p2g_particles(aw_grid* g, const float* d_pos, const float* d_vel, int64_t n, float dt,
+                     const float gravity[3]) {
+  if (!g || n < 0 || (n > 0 && (!d_pos || !d_vel))) return fail(AW_ERR_INVALID, "bad arguments");
+  if (n > 0x7FFFFFFF) return fail(AW_ERR_INVALID, "too many particles");
+  if (!g->have_topology) return fail(AW_ERR_INVALID, "grid has no topology");
+  // binned and sorted into the library's own buffers: the caller's arrays keep their order
+  AW_TRY(bin_and_levels(g, d_pos, n, false, nullptr));
+  AW_TRY(sort_particles(g, d_pos, d_vel, nullptr, n));
+  const float zero[3] = {0.f, 0.f, 0.f};
+  return p2g_sorted(g, g->spos.as<float>(), g->svel.as<float>(), dt, gravity ? gravity : zero);
+}
+
+int aw_g2p(aw_grid* g, const float* d_pos, float* d_vel, int64_t n, float alpha) {
+  if (!g || n < 0 || (n > 0 && (!d_pos || !d_vel))) return fail(AW_ERR_INVALID, "bad arguments");
+  if (!g->have_topology) return fail(AW_ERR_INVALID, "grid has no topology");
+  AW_TRY(g->g2p_list.ensure((size_t)(n + 1) * sizeof(uint32_t)));
+  Prof pf(g->ctx, "g2p", 36.0 * n);
+  k_g2p_advect(g->ctx->stream, g->view(), g->slab.as<float>(), d_pos, d_vel, n, alpha, 0.f, nullptr, d_vel,
+               g->g2p_list.as<uint32_t>(), 1);
+  return launch_status("g2p");
+}
+
+int aw_advect(aw_grid* g, float* d_pos, int64_t n, float dt) {
+  if (!g || n < 0 || (n > 0 && !d_pos)) return fail(AW_ERR_INVALID, "bad arguments");
+  if (!g->have_topology) return fail(AW_ERR_INVALID, "grid has no topology");
+  AW_TRY(g->g2p_list.ensure((size_t)(n + 1) * sizeof(uint32_t)));
+  Prof pf(g->ctx, "advect", 24.0 * n);
+  k_g2p_advect(g->ctx->stream, g->view(), g->slab.as<float>(), d_pos, nullptr, n, 0.f, dt, d_pos, nullptr,
+               g->g2p_list.as<uint32_t>(), 2);
+  return launch_status("advect");
 }
 
 int aw_solver_create(aw_grid* g, const aw_solver_params* p, aw_solver** out) {

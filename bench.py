@@ -355,9 +355,7 @@ def run_large(args):
             "steps_per_s": round(a.steps / gpu["seconds"], 3), "ms_per_step": round(1e3 * gpu["seconds"] / a.steps, 3),
             "pcg_iters_mean": round(float(np.mean(gpu["iters"])), 2), "peak_hbm_gib": round(gpu["peak_bytes"] / 2**30, 3),
             "blocks_by_level": gpu["blocks_by_level"][: w["levels"]], "roofline": rl, "kernels": per,
-            "mixed_precision_1e-8": mixed,
-        "mixed_precision_1e-10": mixed10,
-        "whole_scene": scene, "adjoint": adj}
+            "mixed_precision_1e-8": mixed, "adjoint": adj}
 
 
 def run_scene(args, w, pos, vel):
@@ -573,6 +571,8 @@ def main():
                                         "us_per_launch": resid["us_per_launch"]},
             "level0_back_to_back": iso.get("residual_l0")},
         "mixed_precision_1e-8": mixed,
+        "mixed_precision_1e-10": mixed10,
+        "whole_scene": scene,
         "warm_start": warm,
         "topology_radius_rule": gpu["radius_rule"],
         "resample": gpu["resample"],

@@ -21,7 +21,9 @@
  *   aw_grid_layout                       MSBG POD fields                     include/MSBG.cuh:71-106
  *   aw_p2g                               P2GTransfer::run                    include/P2G.cuh:173-196
  *   aw_divergence, aw_project_velocity   (no counterpart: projection is missing upstream)
+ *   aw_p2g_particles                     P2GTransfer::run (caller's order kept) include/P2G.cuh:173-196
  *   aw_g2p_advect                        G2PManager::performG2P + advect     include/G2P.cuh:198-227
+ *   aw_g2p, aw_advect                    G2PManager::performG2P, ::advect    include/G2P.cuh:192-227
  *   aw_solver_create / _destroy          VCycleSolver ctor / init / dtor     include/VCycle.cuh:161-249
  *   aw_solver_vcycle                     VCycleSolver::runCycle              include/VCycle.cuh:251-304
  *   aw_solver_smooth                     AdaptiveSmoother::solveLevel        include/Smoother.cuh:249-291
@@ -179,6 +181,9 @@ int aw_ctx_synchronize(aw_ctx* ctx);
 int aw_malloc(aw_ctx* ctx, uint64_t bytes, void** out);
 int aw_free(aw_ctx* ctx, void* p);
 int aw_copy(aw_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* d[0..n) = value on the context's stream (asynchronous; a fill kernel, no
+ * memset node, so that it can sit in a captured graph). */
+int aw_fill_f32(aw_ctx* ctx, float* d, uint64_t n, float value);
 /* Device memory the library holds in this process (grids, solvers, tapes,
  * work buffers and aw_malloc buffers, over all contexts) and its high-water
  * mark since load or the last reset (no reference counterpart).  reset_peak
@@ -229,6 +234,21 @@ int aw_p2g(aw_grid* grid, const float* d_pos, const float* d_vel, int64_t n, flo
 int aw_divergence(aw_grid* grid);                 /* rhs = -D u_star on liquid cells */
 int aw_project_velocity(aw_grid* grid);           /* u_new = u_star - beta grad p; fine lattice */
 int aw_g2p_advect(aw_grid* grid, float* d_pos, float* d_vel, int64_t n, float alpha, float dt);
+/* P2G of particles in any order (P2GTransfer::run, include/P2G.cuh:173-196):
+ * bins and sorts them into the library's own buffers, so the caller's arrays
+ * keep their order and contents, then deposits as aw_p2g.  Needs a topology
+ * (aw_grid_build_topology); asynchronous. */
+int aw_p2g_particles(aw_grid* grid, const float* d_pos, const float* d_vel, int64_t n, float dt,
+                     const float gravity[3]);
+/* The two halves of aw_g2p_advect on particles in any order, in place:
+ * aw_g2p the FLIP/PIC velocity update at each particle's position
+ * (G2PManager::performG2P, include/G2P.cuh:192-214; positions untouched),
+ * aw_advect the RK3 advection through u_new (G2PManager::advect,
+ * include/G2P.cuh:216-227; velocities untouched).  Both read the fine lattice
+ * aw_project_velocity leaves; the advection does not read the particle
+ * velocities, so either order gives aw_g2p_advect's result.  Asynchronous. */
+int aw_g2p(aw_grid* grid, const float* d_pos, float* d_vel, int64_t n, float alpha);
+int aw_advect(aw_grid* grid, float* d_pos, int64_t n, float dt);
 
 int aw_solver_create(aw_grid* grid, const aw_solver_params* params, aw_solver** out);
 int aw_solver_destroy(aw_solver* s);
