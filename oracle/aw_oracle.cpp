@@ -89,11 +89,13 @@ struct OGrid {
   int64_t ncells = 0, nfaces[3] = {0, 0, 0};
   // native fields
   std::vector<double> Macc[3], macc[3], uold[3], ustar[3], unew[3], mt[3], beta[3];
+  std::vector<double> Dacc[3];          // density-weighted deposit weight per slot (two-phase beta)
   std::vector<double> p, rhs;
   std::vector<uint8_t> liquid;
   std::vector<uint32_t> pcount;
   // virtual fine lattice: [block][axis][512]
   std::vector<double> Fm, FM, Fnew, Fdel;
+  std::vector<double> FD;               // fine lattice: density-weighted deposit weight
   // adjoint fields (bar = derivative of the loss with respect to the field)
   std::vector<double> Fbar_nd, Fbar_d;   // fine lattice: Fnew-bar + Fdel-bar, Fdel-bar
   std::vector<double> FMbar, Fmbar;      // fine lattice: deposit accumulators
@@ -342,13 +344,36 @@ void finish_level_cells(Level& L) {
   L.liquid.assign(L.ncells, 0);
 }
 
+// beta of the native face that holds the unit fine sub-face on the plane
+// between fine cells P - 1 and P along axis a, at fine transverse coordinates
+// q[t1], q[t2]: the slot of the finer of the two native cells there (a level
+// change's coarse side is not an unknown; equal cells hold equal duplicates).
+double beta_native(const OGrid& g, int a, int P, const int* q) {
+  int lo[3] = {q[0], q[1], q[2]}, hi[3] = {q[0], q[1], q[2]};
+  lo[a] = P - 1;
+  hi[a] = P;
+  const int bl = block_id(g.D, lo[0] >> LOGB, lo[1] >> LOGB, lo[2] >> LOGB);
+  const int bu = block_id(g.D, hi[0] >> LOGB, hi[1] >> LOGB, hi[2] >> LOGB);
+  const int X = g.level[bu] < g.level[bl] ? bu : bl;
+  const int e = g.level[X], r = B >> e;
+  int Xc[3];
+  block_coords(g.D, X, Xc);
+  int l[3];
+  for (int d = 0; d < 3; ++d) l[d] = ((d == a ? P : q[d]) - B * Xc[d]) >> e;
+  return g.beta[a][g.foff[a][X] + face_index(a, l[0], l[1], l[2], r)];
+}
+
 // Geometric assembly.  For a cell pair (c, c') sharing area A (fine units)
-// the face coefficient is  w = beta * A / d,  d = (s_c + s_c') / 2 the centre
+// the face coefficient is  w = beta A / d,  d = (s_c + s_c') / 2 the centre
 // distance along the axis — h and 2h for equal cells, 1.5 h_f across a
-// coarse–fine (T-junction) interface.  A = sum over faces of w (e_c - e_c')
-// (e_c - e_c')^T is symmetric by construction, unlike the reference's
-// point-sampled ghosts (Smoother.cuh:70-91, VCycle.cuh:56-75).  Air cells are
-// Dirichlet p = 0, tank walls have beta = 0.  Rows exist for liquid cells.
+// coarse–fine (T-junction) interface — where beta A is the sum of the native
+// beta over the unit fine sub-faces of the shared area: on every level the
+// coefficient is the area integral of the finest beta (the reference's
+// galerkinCoarsen averages the four fine faces under each coarse face,
+// MSBG.cuh:401-498).  A = sum over faces of w (e_c - e_c') (e_c - e_c')^T is
+// symmetric by construction, unlike the reference's point-sampled ghosts
+// (Smoother.cuh:70-91, VCycle.cuh:56-75).  Air cells are Dirichlet p = 0, tank
+// walls have beta = 0.  Rows exist for liquid cells.
 void assemble(const OGrid& g, Level& L) {
   L.rowptr.assign(L.ncells + 1, 0);
   L.diag.assign(L.ncells, 0.0);
@@ -358,13 +383,14 @@ void assemble(const OGrid& g, Level& L) {
     if (!L.liquid[c]) continue;
     int g0[3], s;
     cell_geom(L, c, g0, s);
-    std::vector<std::pair<int, int>> hits;   // (neighbour, fine sub-faces)
+    std::vector<std::pair<int, double>> hits;   // (neighbour, sum of beta over its fine sub-faces)
     double dsum = 0.0;
     for (int a = 0; a < 3; ++a) {
       int t1 = (a + 1) % 3, t2 = (a + 2) % 3;
       for (int side = 0; side < 2; ++side) {
         int ga = side ? g0[a] + s : g0[a] - 1;
         if (ga < 0 || ga >= g.N[a]) continue;           // tank wall: beta = 0
+        const int P = side ? g0[a] + s : g0[a];         // the plane of the face
         hits.clear();
         for (int j = 0; j < s; ++j)
           for (int k = 0; k < s; ++k) {
@@ -373,15 +399,16 @@ void assemble(const OGrid& g, Level& L) {
             q[t1] = g0[t1] + j;
             q[t2] = g0[t2] + k;
             int cn = cell_of(L, q);
+            const double bt = beta_native(g, a, P, q);
             bool found = false;
             for (auto& hcnt : hits)
-              if (hcnt.first == cn) { hcnt.second++; found = true; break; }
-            if (!found) hits.push_back({cn, 1});
+              if (hcnt.first == cn) { hcnt.second += bt; found = true; break; }
+            if (!found) hits.push_back({cn, bt});
           }
         for (auto& hcnt : hits) {
           int gn[3], sn;
           cell_geom(L, hcnt.first, gn, sn);
-          double w = 1.0 * hcnt.second / (0.5 * (s + sn));
+          double w = hcnt.second / (0.5 * (s + sn));
           dsum += w;
           if (L.liquid[hcnt.first]) rows[c].push_back({hcnt.first, -w});
         }
@@ -781,11 +808,16 @@ inline void to_units(const OGrid& g, const double* x, double* u) {
   for (int a = 0; a < 3; ++a) u[a] = std::min(std::max(x[a] / g.h, 0.0), (double)g.N[a] - 0.0009765625);
 }
 
+// dens (may be null): relative density of each particle (input order).  With
+// it the fold also sets beta = 1 / rho_face, rho_face = sum w rho / sum w (the
+// density channels the reference reserves, MSBG.cuh:21-34, and its commented
+// beta deposit, P2G.cuh:93-97).
 template <class T>
-void p2g(OGrid& g, const T* pos, const T* vel, int64_t n, double dt, const double* grav) {
+void p2g(OGrid& g, const T* pos, const T* vel, int64_t n, double dt, const double* grav, const double* dens = nullptr) {
   const size_t fl = (size_t)g.nb * 3 * 512;
   g.FM.assign(fl, 0.0);
   g.Fm.assign(fl, 0.0);
+  g.FD.assign(dens ? fl : 0, 0.0);
   std::fill(g.pcount.begin(), g.pcount.end(), 0u);
   // bin by block; deposit in 8 parity classes so blocks of one class touch
   // disjoint nodes (a block's stencils reach one node past its faces)
@@ -819,6 +851,7 @@ void p2g(OGrid& g, const T* pos, const T* vel, int64_t n, double dt, const doubl
             if (ad[k] < 0) continue;
             g.FM[ad[k]] += w[k] * (double)vel[3 * p + a];
             g.Fm[ad[k]] += w[k];
+            if (dens) g.FD[ad[k]] += w[k] * dens[p];
           }
         }
       }
@@ -828,6 +861,7 @@ void p2g(OGrid& g, const T* pos, const T* vel, int64_t n, double dt, const doubl
   for (int a = 0; a < 3; ++a) {
     std::fill(g.Macc[a].begin(), g.Macc[a].end(), 0.0);
     std::fill(g.macc[a].begin(), g.macc[a].end(), 0.0);
+    g.Dacc[a].assign(dens ? g.nfaces[a] : 0, 0.0);
   }
 #pragma omp parallel for schedule(dynamic, 8)
   for (int b = 0; b < g.nb; ++b) {
@@ -843,6 +877,7 @@ void p2g(OGrid& g, const T* pos, const T* vel, int64_t n, double dt, const doubl
         for (int k = 0; k < pw.n; ++k) {
           g.Macc[a][g.foff[a][b] + pw.slot[k]] += pw.w[k] * M;
           g.macc[a][g.foff[a][b] + pw.slot[k]] += pw.w[k] * m;
+          if (dens) g.Dacc[a][g.foff[a][b] + pw.slot[k]] += pw.w[k] * g.FD[ad];
         }
       }
   }
@@ -855,6 +890,10 @@ void p2g(OGrid& g, const T* pos, const T* vel, int64_t n, double dt, const doubl
   // split 1/4 to each fine sub-face; wall slots are 0.
   const double eps = 1e-6;
   auto norm = [&](double M, double m) { return m > eps ? M / m : 0.0; };
+  // beta = 1 / rho_face = m / D on a face with mass, the reference density's
+  // 1 on one without; folded like the velocity (duplicates summed, the coarse
+  // side of a level change the mean of its four fine sub-faces); walls 0
+  auto binv = [&](double D, double m) { return m > eps ? m / D : 1.0; };
 #pragma omp parallel for schedule(dynamic, 8)
   for (int b = 0; b < g.nb; ++b) {
     int c[3];
@@ -877,13 +916,17 @@ void p2g(OGrid& g, const T* pos, const T* vel, int64_t n, double dt, const doubl
               cc[a] += sl[a] == 0 ? -1 : 1;
               if (cc[a] >= 0 && cc[a] < g.D[a]) nbb = block_id(g.D, cc[0], cc[1], cc[2]);
             }
+            const double D = dens ? g.Dacc[a][id] : 0.0;
+            double bt = 1.0;
             if (bnd && nbb < 0) {
               uo = us = 0.0;
               mtot = 0.0;
+              bt = 0.0;
             } else if (!bnd) {
               mtot = m;
               uo = norm(M, m);
               us = uo + (m > eps ? dt * grav[a] : 0.0);
+              if (dens) bt = binv(D, m);
             } else {
               int en = g.level[nbb], rn = B >> en, e = g.level[b];
               int pa = sl[a] == 0 ? rn : 0;    // partner slot along the axis
@@ -895,6 +938,7 @@ void p2g(OGrid& g, const T* pos, const T* vel, int64_t n, double dt, const doubl
                 mtot = m + g.macc[a][pid];
                 uo = norm(M + g.Macc[a][pid], mtot);
                 us = uo + (mtot > eps ? dt * grav[a] : 0.0);
+                if (dens) bt = binv(D + g.Dacc[a][pid], mtot);
               } else if (en == e + 1) {
                 int ps[3];
                 ps[a] = pa;
@@ -904,8 +948,10 @@ void p2g(OGrid& g, const T* pos, const T* vel, int64_t n, double dt, const doubl
                 mtot = m + 0.25 * g.macc[a][pid];
                 uo = norm(M + 0.25 * g.Macc[a][pid], mtot);
                 us = uo + (mtot > eps ? dt * grav[a] : 0.0);
+                if (dens) bt = binv(D + 0.25 * g.Dacc[a][pid], mtot);
               } else {
                 uo = us = mtot = 0.0;
+                bt = 0.0;
                 for (int d1 = 0; d1 < 2; ++d1)
                   for (int d2 = 0; d2 < 2; ++d2) {
                     int ps[3];
@@ -918,12 +964,15 @@ void p2g(OGrid& g, const T* pos, const T* vel, int64_t n, double dt, const doubl
                     uo += 0.25 * u0;
                     us += 0.25 * (u0 + (ms > eps ? dt * grav[a] : 0.0));
                     mtot += ms;
+                    if (dens) bt += 0.25 * binv(g.Dacc[a][pid] + 0.25 * D, ms);
                   }
+                if (!dens) bt = 1.0;
               }
             }
             g.uold[a][id] = uo;
             g.ustar[a][id] = us;
             g.mt[a][id] = mtot;
+            if (dens) g.beta[a][id] = bt;
           }
     }
   }
@@ -1652,9 +1701,10 @@ int og_totals(void* p, int64_t* out) {
   return 0;
 }
 
-int og_p2g(void* p, const float* pos, const float* vel, int64_t n, double dt, double gx, double gy, double gz) {
+int og_p2g(void* p, const float* pos, const float* vel, int64_t n, double dt, double gx, double gy, double gz,
+           const double* density) {
   double gr[3] = {gx, gy, gz};
-  p2g(*static_cast<OGrid*>(p), pos, vel, n, dt, gr);
+  p2g(*static_cast<OGrid*>(p), pos, vel, n, dt, gr, density);
   return 0;
 }
 
@@ -1881,10 +1931,11 @@ int og_set_cell(void* p, int which, const double* in) {
 
 // The whole FLIP step on FP32 particles: topology from the particles, P2G with
 // the body force, divergence, pressure solve, projection, G2P + RK3.  Outputs
-// are FP64; the caller rounds them to FP32 for the next step.
+// are FP64; the caller rounds them to FP32 for the next step.  density (may
+// be null): relative density per particle (input order), beta = 1 / rho.
 int og_step(void* p, const float* pos, const float* vel, int64_t n, double dt, double gx, double gy, double gz,
             double alpha, int method, double tol, int max_iters, double* pos_out, double* vel_out, int* iters,
-            double* rel, double* stage_ms) {
+            double* rel, double* stage_ms, const double* density) {
   OGrid& g = *static_cast<OGrid*>(p);
   auto now = [] { return omp_get_wtime(); };
   double t0 = now();
@@ -1896,7 +1947,7 @@ int og_step(void* p, const float* pos, const float* vel, int64_t n, double dt, d
   set_topology(g, lv.data());
   double t1 = now();
   double gr[3] = {gx, gy, gz};
-  p2g(g, pos, vel, n, dt, gr);
+  p2g(g, pos, vel, n, dt, gr, density);
   divergence(g);
   double t2 = now();
   mg_build(g);

@@ -47,7 +47,7 @@ def lib():
             "og_get_levels": (ctypes.c_int, [_V, _U8]),
             "og_get_offsets": (ctypes.c_int, [_V, _U32, _U32, _U32, _U32]),
             "og_totals": (ctypes.c_int, [_V, _I64]),
-            "og_p2g": (ctypes.c_int, [_V, _F, _F, ctypes.c_int64] + [ctypes.c_double] * 4),
+            "og_p2g": (ctypes.c_int, [_V, _F, _F, ctypes.c_int64] + [ctypes.c_double] * 4 + [_D]),
             "og_divergence": (ctypes.c_int, [_V]),
             "og_set_params": (ctypes.c_int, [_V, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
             "og_mg_build": (ctypes.c_int, [_V]),
@@ -72,7 +72,7 @@ def lib():
             "og_get_cell": (ctypes.c_int, [_V, ctypes.c_int, _D]),
             "og_set_cell": (ctypes.c_int, [_V, ctypes.c_int, _D]),
             "og_step": (ctypes.c_int, [_V, _F, _F, ctypes.c_int64] + [ctypes.c_double] * 5
-                        + [ctypes.c_int, ctypes.c_double, ctypes.c_int, _D, _D, _I, _D, _D]),
+                        + [ctypes.c_int, ctypes.c_double, ctypes.c_int, _D, _D, _I, _D, _D, _D]),
             "og_num_threads": (ctypes.c_int, []),
             "og_set_threads": (ctypes.c_int, [ctypes.c_int]),
             "og_get_fine": (ctypes.c_int, [_V, ctypes.c_int, _D]),
@@ -206,10 +206,13 @@ class OracleGrid:
         return t
 
     # transfers
-    def p2g(self, pos, vel, dt=0.0, gravity=(0.0, 0.0, 0.0)):
+    def p2g(self, pos, vel, dt=0.0, gravity=(0.0, 0.0, 0.0), density=None):
+        """density: relative density per particle (beta = 1 / rho on faces with mass), or None (beta = 1)."""
         pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
         vel = np.ascontiguousarray(vel, np.float32).reshape(-1, 3)
-        lib().og_p2g(self._h, _p(pos, _F), _p(vel, _F), len(pos), float(dt), *[float(g) for g in gravity])
+        dens = None if density is None else np.ascontiguousarray(density, np.float64).reshape(len(pos))
+        lib().og_p2g(self._h, _p(pos, _F), _p(vel, _F), len(pos), float(dt), *[float(g) for g in gravity],
+                     None if dens is None else _p(dens, _D))
 
     def divergence(self):
         lib().og_divergence(self._h)
@@ -321,9 +324,11 @@ class OracleGrid:
         st = lib().og_solve(self._h, int(method), float(tol), int(max_iters), ctypes.byref(it), ctypes.byref(rel))
         return st, it.value, rel.value
 
-    def step(self, pos, vel, dt, gravity, alpha, method=0, tol=1e-6, max_iters=200):
+    def step(self, pos, vel, dt, gravity, alpha, method=0, tol=1e-6, max_iters=200, density=None):
+        """One step; density: relative density per particle (input order) or None."""
         pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
         vel = np.ascontiguousarray(vel, np.float32).reshape(-1, 3)
+        dens = None if density is None else np.ascontiguousarray(density, np.float64).reshape(len(pos))
         po = np.zeros(pos.shape, np.float64)
         vo = np.zeros(vel.shape, np.float64)
         it = ctypes.c_int(0)
@@ -331,7 +336,7 @@ class OracleGrid:
         ms = np.zeros(4, np.float64)
         st = lib().og_step(self._h, _p(pos, _F), _p(vel, _F), len(pos), float(dt), *[float(g) for g in gravity],
                            float(alpha), int(method), float(tol), int(max_iters), _p(po, _D), _p(vo, _D),
-                           ctypes.byref(it), ctypes.byref(rel), _p(ms, _D))
+                           ctypes.byref(it), ctypes.byref(rel), _p(ms, _D), None if dens is None else _p(dens, _D))
         return dict(status=st, pos=po, vel=vo, iters=it.value, relres=rel.value, stage_ms=ms)
 
 
