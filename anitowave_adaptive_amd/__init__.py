@@ -49,7 +49,7 @@ class SolveStats(ctypes.Structure):
 class StepParams(ctypes.Structure):
     _fields_ = [("dt", ctypes.c_float), ("gravity", ctypes.c_float * 3), ("flip_alpha", ctypes.c_float),
                 ("rel_tol", ctypes.c_double), ("max_iters", ctypes.c_int), ("rebuild_topology", ctypes.c_int),
-                ("warm_start", ctypes.c_int), ("d_radius", _V)]
+                ("warm_start", ctypes.c_int), ("d_radius", _V), ("d_density", _V)]
 
 
 class StepStats(ctypes.Structure):
@@ -102,6 +102,7 @@ SIGNATURES = {
     "aw_grid_build_topology": (_I, [_V, _V, _I64, _P(_I)]),
     "aw_grid_build_topology_radius": (_I, [_V, _V, _V, _I64, _P(_I)]),
     "aw_grid_set_levels": (_I, [_V, _V]),
+    "aw_grid_beta_changed": (_I, [_V]),
     "aw_grid_layout": (_I, [_V, _P(Layout)]),
     "aw_bin_particles": (_I, [_V, _V, _V, _V, _I64]),
     "aw_p2g": (_I, [_V, _V, _V, _I64, _F, _P(_F)]),
@@ -365,6 +366,10 @@ class MSBGManager:
         """Blocks as hexahedra with their level (exportMultigrid_VTK, vtx_visualization.cuh:128-190)."""
         check(lib().aw_export_blocks_vtk(self._h, os.fsencode(path)), "aw_export_blocks_vtk")
 
+    def beta_changed(self):
+        """The face coefficients d_beta were written directly: coarsen them at the next solver setup."""
+        check(lib().aw_grid_beta_changed(self._h), "aw_grid_beta_changed")
+
     def set_levels(self, levels):
         lv = np.ascontiguousarray(levels, np.uint8)
         check(lib().aw_grid_set_levels(self._h, lv.ctypes.data_as(_V)), "aw_grid_set_levels")
@@ -583,12 +588,14 @@ class FlipStep:
     """The whole step (aw_step) on device particle buffers."""
 
     def __init__(self, grid, solver, dt=1.0 / 240.0, gravity=(0.0, -9.81, 0.0), alpha=0.95, tol=1e-6,
-                 max_iters=200, rebuild_topology=True, warm_start=False, d_radius=None):
+                 max_iters=200, rebuild_topology=True, warm_start=False, d_radius=None, d_density=None):
         """rebuild_topology: False keeps the topology, True (1) rebuilds it
         from the distance bands every step, 2 from the particle radii
         d_radius (device float[n] indexed by particle id; the step then needs
         d_ids).  warm_start: start each pressure solve from the previous
-        step's pressure (remapped across topology changes) instead of zero."""
+        step's pressure (remapped across topology changes) instead of zero.
+        d_density: device float[n] relative density per particle id (two-phase
+        beta = 1 / rho; the step then needs d_ids), or None for beta = 1."""
         self.grid, self.solver = grid, solver
         p = StepParams()
         p.dt = dt
@@ -599,6 +606,7 @@ class FlipStep:
         p.rebuild_topology = int(rebuild_topology)
         p.warm_start = int(warm_start)
         p.d_radius = d_radius
+        p.d_density = d_density
         self.params = p
 
     def __call__(self, d_pos, d_vel, d_ids, n, allow_not_converged=False):

@@ -147,13 +147,19 @@ void k_scan_u32(hipStream_t s, const uint32_t* in, uint32_t* out, int n, uint32_
 void k_cell_offsets(hipStream_t s, const uint8_t* level, int nb, uint32_t* coff, uint32_t* scratch);
 void k_layout_offsets(hipStream_t s, const uint8_t* level, int nb, uint32_t* coff, uint32_t* fx, uint32_t* fy,
                       uint32_t* fz, uint32_t* scratch);
+// dens (needs ids): per-particle density by particle id into sdens (sorted order), or null
 void k_scatter(hipStream_t s, const float* pos, const float* vel, const int32_t* ids, const int32_t* key,
-               uint32_t* cursor, int64_t n, float* spos, float* svel, int32_t* sids);
+               uint32_t* cursor, int64_t n, float* spos, float* svel, int32_t* sids, const float* dens = nullptr,
+               float* sdens = nullptr);
 void k_init_beta(hipStream_t s, const LevelDev& L, float* bx, float* by, float* bz);
 
 // transfers
+// sdens (sorted): deposit the particles' density instead of their velocity (the
+// slab's first field then holds the density-weighted weights)
 void k_p2g_deposit(hipStream_t s, const GridDev& g, const float* spos, const float* svel, const uint32_t* pcount,
-                   const uint32_t* poff, float* slab, uint8_t* liquid);
+                   const uint32_t* poff, float* slab, uint8_t* liquid, const float* sdens = nullptr);
+// beta = m / D folded (two-phase coefficients), 0 on walls
+void k_beta_fold(hipStream_t s, const GridDev& g, float* const* macc, float* const* Dacc, float* const* beta);
 void k_p2g_gather(hipStream_t s, const GridDev& g, const uint32_t* pcount, const float* slab, float* const* Macc,
                   float* const* macc);
 void k_fold(hipStream_t s, const GridDev& g, float* const* Macc, float* const* macc, float* const* uold,
@@ -213,6 +219,9 @@ void k_prolong(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregat
                float scale, const WGMap& wgf, const uint32_t* blistf);
 void k_coarsen_liquid(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, uint8_t* liquid_c,
                       const WGMap& wgc, const uint32_t* blistc);
+// beta of level C from level F (area means of the finer side's faces)
+void k_coarsen_beta(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, float* bx, float* by,
+                    float* bz);
 void k_tail(hipStream_t s, const TailArgs& args, size_t lds_bytes);
 void k_coarse_factor(hipStream_t s, const LevelDev& L, int n, double* inv);
 

@@ -698,11 +698,14 @@ __global__ __launch_bounds__(TPB) void scan_final_kernel(const uint32_t* __restr
   if (blockIdx.x == nchunks - 1 && threadIdx.x == 0) out[n] = cbase[nchunks];
 }
 
+// dens (with ids): per-particle density by particle id, gathered into sdens
+// in the sorted order
 __global__ __launch_bounds__(TPB) void scatter_kernel(const float* __restrict__ pos, const float* __restrict__ vel,
                                                       const int32_t* __restrict__ ids,
                                                       const int32_t* __restrict__ key, uint32_t* __restrict__ cursor,
                                                       int64_t n, float* __restrict__ spos, float* __restrict__ svel,
-                                                      int32_t* __restrict__ sids) {
+                                                      int32_t* __restrict__ sids, const float* __restrict__ dens,
+                                                      float* __restrict__ sdens) {
   __shared__ int hk[AGG_SLOTS];
   __shared__ uint32_t hc[AGG_SLOTS];
   for (int s = threadIdx.x; s < AGG_SLOTS; s += TPB) {
@@ -747,6 +750,7 @@ __global__ __launch_bounds__(TPB) void scatter_kernel(const float* __restrict__ 
       svel[3 * s + a] = vel[3 * i + a];
     }
     if (ids) sids[s] = ids[i];
+    if (dens) sdens[s] = dens[ids[i]];
   }
 }
 
@@ -792,6 +796,10 @@ __global__ __launch_bounds__(TPB) void init_beta_kernel(LevelDev L, float* __res
 constexpr int DEP_T = 256;            // threads per work-group
 constexpr int DEP_NODES = 3 * PAD;    // padded nodes of all three axes
 
+// DENS: the deposited value is the particle's density sdens[p] on every
+// axis instead of its velocity (the density-weighted weight of the
+// two-phase coefficients), with the same weights
+template <bool DENS>
 __global__ __launch_bounds__(DEP_T) void p2g_deposit_kernel(GridDev g, const float* __restrict__ spos,
                                                           const float* __restrict__ svel,
                                                           const uint32_t* __restrict__ pcount,
@@ -834,7 +842,7 @@ __global__ __launch_bounds__(DEP_T) void p2g_deposit_kernel(GridDev g, const flo
     for (int a = 0; a < 3; ++a) {
       const float ua = fminf(fmaxf(spos[3 * p + a] * g.inv_h, 0.f), (float)g.N[a] - 0.0009765625f);
       u[a] = ua - (float)(BASE * Bc[a]);
-      v[a] = svel[3 * p + a];
+      v[a] = DENS ? svel[p] : svel[3 * p + a];
       inside = inside && ((int)ua >> LOGB) == Bc[a];
     }
     // a particle outside this block (unsorted input) is skipped
@@ -1053,6 +1061,60 @@ __global__ __launch_bounds__(TPB) void fold_kernel(GridDev g, FacePtrs Macc, Fac
     uold.p[a][id] = uo;
     ustar.p[a][id] = us;
     mass.p[a][id] = mt;
+  }
+}
+
+// Two-phase coefficients: beta = 1 / rho_face = m / D on a face with mass
+// (m > 1e-6), the reference density's 1 on one without, 0 on the tank walls;
+// duplicates and coarse-fine slots folded as fold_kernel folds the velocity
+// (the coarse side of a level change, not an unknown of the operator, gets
+// the mean of its four fine sub-faces).  The reference reserves BETA = 1/rho
+// and MASS_DENSITY channels for this (MSBG.cuh:21-34, P2G.cuh:93-97).
+__device__ __forceinline__ float beta_of(float D, float m) { return m > 1e-6f ? m / D : 1.f; }
+
+__global__ __launch_bounds__(TPB) void beta_fold_kernel(GridDev g, FacePtrs macc, FacePtrs Dacc, FacePtrs beta) {
+  const int B = blockIdx.x;
+  const int e = g.level[B], r = BASE >> e, nf = (r + 1) * r * r;
+  for (int k = threadIdx.x; k < 3 * nf; k += blockDim.x) {
+    const int a = k / nf, f = k - a * nf;
+    const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+    int sl[3];
+    face_decode(a, f, r, sl);
+    const size_t id = g.foff[a][B] + f;
+    const float m = macc.p[a][id], D = Dacc.p[a][id];
+    float bt;
+    if (sl[a] > 0 && sl[a] < r) {
+      bt = beta_of(D, m);
+    } else {
+      const int side = sl[a] == r;
+      const int N = g.nbr[B * 6 + 2 * a + side];
+      if (N < 0) {
+        bt = 0.f;
+      } else {
+        const int en = g.level[N], rn = BASE >> en;
+        int ps[3] = {sl[0], sl[1], sl[2]};
+        ps[a] = side ? 0 : rn;
+        if (en == e) {
+          const size_t pid = g.foff[a][N] + face_idx(a, ps, rn);
+          bt = beta_of(D + Dacc.p[a][pid], m + macc.p[a][pid]);
+        } else if (en == e + 1) {
+          ps[t1] = sl[t1] >> 1;
+          ps[t2] = sl[t2] >> 1;
+          const size_t pid = g.foff[a][N] + face_idx(a, ps, rn);
+          bt = beta_of(D + 0.25f * Dacc.p[a][pid], m + 0.25f * macc.p[a][pid]);
+        } else {
+          bt = 0.f;
+          for (int d1 = 0; d1 < 2; ++d1)
+            for (int d2 = 0; d2 < 2; ++d2) {
+              ps[t1] = 2 * sl[t1] + d1;
+              ps[t2] = 2 * sl[t2] + d2;
+              const size_t pid = g.foff[a][N] + face_idx(a, ps, rn);
+              bt += 0.25f * beta_of(Dacc.p[a][pid] + 0.25f * D, macc.p[a][pid] + 0.25f * m);
+            }
+        }
+      }
+    }
+    beta.p[a][id] = bt;
   }
 }
 
@@ -2736,6 +2798,75 @@ __global__ __launch_bounds__(TPB) void restrict_kernel(LevelDev F, LevelDev C, i
   }
 }
 
+// Coefficient coarsening (galerkinCoarsen / runGalerkin, MSBG.cuh:401-498,
+// 661-676): every face slot of the coarser level C is the area mean of the
+// finer level F's beta over the slot's square, taken from the side that holds
+// the finer faces there (the block itself, or across a block face its
+// neighbour when that was finer on F: a level change's coarse-side slot is
+// not an unknown and is never read).  A block that kept its resolution copies
+// its slots.  Across a 2x2x2 aggregation a slot is the mean of the four fine
+// blocks' slots on its side.  One work-group per block of C.
+__global__ __launch_bounds__(TPB) void coarsen_beta_kernel(LevelDev F, LevelDev C, int aggregated, float* __restrict__ bx,
+                                                           float* __restrict__ by, float* __restrict__ bz) {
+  const int B = blockIdx.x;
+  float* out[3] = {bx, by, bz};
+  const int eC = C.e[B], rC = C.bs >> eC, nf = (rC + 1) * rC * rC;
+  int Bc[3];
+  block_coords(C.D, B, Bc);
+  for (int k = threadIdx.x; k < 3 * nf; k += TPB) {
+    const int a = k / nf, f = k - a * nf;
+    const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+    int sl[3];
+    face_decode(a, f, rC, sl);
+    float v = 0.f;
+    if (aggregated) {
+      // one-cell blocks: slot sl[a] (0 or 1) is the face on that side
+#pragma unroll
+      for (int d1 = 0; d1 < 2; ++d1)
+#pragma unroll
+        for (int d2 = 0; d2 < 2; ++d2) {
+          int q[3];
+          q[a] = 2 * Bc[a] + sl[a];
+          q[t1] = 2 * Bc[t1] + d1;
+          q[t2] = 2 * Bc[t2] + d2;
+          const int X = block_id(F.D, q);
+          v += F.beta[a][F.foff[a][X] + sl[a]];
+        }
+      v *= 0.25f;
+    } else {
+      int X = B;
+      bool wall = false;
+      if (sl[a] == 0 || sl[a] == rC) {
+        const int N = C.nbr[B * 6 + 2 * a + (sl[a] == rC ? 1 : 0)];
+        if (N < 0)
+          wall = true;
+        else if (F.e[N] < F.e[B])
+          X = N;
+      }
+      if (!wall) {
+        const int S = 1 << eC;                        // the slot's edge in fine cells
+        const int P = C.bs * Bc[a] + sl[a] * S;       // its plane along a
+        const int eX = F.e[X], rX = F.bs >> eX, m = S >> eX;
+        int Xc[3];
+        block_coords(F.D, X, Xc);
+        int l[3];
+        l[a] = (P - F.bs * Xc[a]) >> eX;
+        const int o1 = (C.bs * Bc[t1] + sl[t1] * S - F.bs * Xc[t1]) >> eX;
+        const int o2 = (C.bs * Bc[t2] + sl[t2] * S - F.bs * Xc[t2]) >> eX;
+        const uint32_t fo = F.foff[a][X];
+        for (int i = 0; i < m; ++i)
+          for (int j = 0; j < m; ++j) {
+            l[t1] = o1 + i;
+            l[t2] = o2 + j;
+            v += F.beta[a][fo + face_idx(a, l, rX)];
+          }
+        v /= (float)(m * m);
+      }
+    }
+    out[a][C.foff[a][B] + f] = v;
+  }
+}
+
 // liquid_c = all children liquid (Dirichlet air dominates on coarse levels)
 __global__ __launch_bounds__(TPB) void coarsen_liquid_kernel(LevelDev F, LevelDev C, int aggregated,
                                                              uint8_t* __restrict__ liq, WGMap wgm,
@@ -3363,10 +3494,10 @@ void k_layout_offsets(hipStream_t s, const uint8_t* level, int nb, uint32_t* cof
 }
 
 void k_scatter(hipStream_t s, const float* pos, const float* vel, const int32_t* ids, const int32_t* key,
-               uint32_t* cursor, int64_t n, float* spos, float* svel, int32_t* sids) {
+               uint32_t* cursor, int64_t n, float* spos, float* svel, int32_t* sids, const float* dens, float* sdens) {
   if (n > 0)
-    scatter_kernel<<<(unsigned)((n + TPB * AGG_PPT - 1) / (TPB * AGG_PPT)), TPB, 0, s>>>(pos, vel, ids, key, cursor,
-                                                                                       n, spos, svel, sids);
+    scatter_kernel<<<(unsigned)((n + TPB * AGG_PPT - 1) / (TPB * AGG_PPT)), TPB, 0, s>>>(
+        pos, vel, ids, key, cursor, n, spos, svel, sids, dens, sdens);
 }
 
 void k_init_beta(hipStream_t s, const LevelDev& L, float* bx, float* by, float* bz) {
@@ -3374,8 +3505,11 @@ void k_init_beta(hipStream_t s, const LevelDev& L, float* bx, float* by, float* 
 }
 
 void k_p2g_deposit(hipStream_t s, const GridDev& g, const float* spos, const float* svel, const uint32_t* pcount,
-                   const uint32_t* poff, float* slab, uint8_t* liquid) {
-  p2g_deposit_kernel<<<g.nb, DEP_T, 0, s>>>(g, spos, svel, pcount, poff, slab, liquid);
+                   const uint32_t* poff, float* slab, uint8_t* liquid, const float* sdens) {
+  if (sdens)
+    p2g_deposit_kernel<true><<<g.nb, DEP_T, 0, s>>>(g, spos, sdens, pcount, poff, slab, liquid);
+  else
+    p2g_deposit_kernel<false><<<g.nb, DEP_T, 0, s>>>(g, spos, svel, pcount, poff, slab, liquid);
 }
 
 void k_p2g_gather(hipStream_t s, const GridDev& g, const uint32_t* pcount, const float* slab, float* const* Macc,
@@ -3389,6 +3523,10 @@ void k_fold(hipStream_t s, const GridDev& g, float* const* Macc, float* const* m
             float* const* ustar, float* const* mass, float dt, const float* grav) {
   fold_kernel<<<g.nb, TPB, 0, s>>>(g, fp(Macc), fp(macc), fp(uold), fp(ustar), fp(mass), dt, grav[0], grav[1],
                                     grav[2]);
+}
+
+void k_beta_fold(hipStream_t s, const GridDev& g, float* const* macc, float* const* Dacc, float* const* beta) {
+  beta_fold_kernel<<<g.nb, TPB, 0, s>>>(g, fp(macc), fp(Dacc), fp(beta));
 }
 
 void k_divergence(hipStream_t s, const GridDev& g, const uint8_t* liquid, float* const* ustar, float* rhs) {
@@ -3522,6 +3660,11 @@ void k_prolong(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregat
 void k_coarsen_liquid(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, uint8_t* liquid_c,
                       const WGMap& wgc, const uint32_t* blistc) {
   if (wgc.n > 0) coarsen_liquid_kernel<<<wgc.n, TPB, 0, s>>>(F, C, aggregated, liquid_c, wgc, blistc);
+}
+
+void k_coarsen_beta(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, float* bx, float* by,
+                    float* bz) {
+  coarsen_beta_kernel<<<C.nb, TPB, 0, s>>>(F, C, aggregated, bx, by, bz);
 }
 
 void k_tail(hipStream_t s, const TailArgs& args, size_t lds_bytes) {

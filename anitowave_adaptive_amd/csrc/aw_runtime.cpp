@@ -202,6 +202,10 @@ struct aw_grid {
   // adjoint fields: fine-lattice adjoints (two fields each, stride ADJ), per-slot and per-cell adjoints
   DevBuf fbar, fmbar, pbar, rhsbar, idrange, adjrec;
   DevBuf liq_count;                  // liquid cells of the last step, read back with the step's final sync
+  // two-phase coefficients: beta = 1 / rho from a per-particle density (the
+  // last step deposited one), its density-weighted deposit and sorted densities
+  bool beta_variable = false;
+  DevBuf Dacc[3], sdens;
   uint32_t* h_liq_count = nullptr;   // pinned host copy
   DevBuf ubn[3], ubo[3], ubs[3], Mb[3], mb[3];
   bool sorted_ids = false;  // the last binning carried particle ids
@@ -327,6 +331,9 @@ struct aw_solver {
   hipEvent_t upload_event = nullptr;  // after the last hierarchy build's table uploads
   bool upload_event_set = false;
   bool use_graph = false;  // aw_solver_set_graph
+  // the coarse levels' beta arrays hold coarsened variable coefficients (they
+  // are coarsened again, to the uniform 1 / 0, once the grid's beta is uniform)
+  bool coarse_beta_variable = false;
   bool capture_ok = true;  // false once capture failed (e.g. on the legacy null stream): run eagerly
 };
 
@@ -373,6 +380,7 @@ int apply_levels(aw_grid* g) {
   L.nbr = g->nbr.as<int32_t>();
   L.liquid = g->liquid.as<uint8_t>();
   k_init_beta(s, L, g->beta[0].as<float>(), g->beta[1].as<float>(), g->beta[2].as<float>());
+  g->beta_variable = false;
   k_fill_f32(s, g->p.as<float>(), nc, 0.f);
   k_fill_f32(s, g->rhs.as<float>(), nc, 0.f);
   AW_HIP(hipMemsetAsync(g->liquid.p, 0, nc, s));
@@ -463,26 +471,66 @@ int bin_and_levels(aw_grid* g, const float* d_pos, int64_t n, bool mark_levels, 
   return AW_OK;
 }
 
-int sort_particles(aw_grid* g, const float* d_pos, const float* d_vel, const int32_t* d_ids, int64_t n) {
+int sort_particles(aw_grid* g, const float* d_pos, const float* d_vel, const int32_t* d_ids, int64_t n,
+                   const float* d_density = nullptr) {
   hipStream_t s = g->ctx->stream;
   AW_TRY(g->spos.ensure((size_t)std::max<int64_t>(n, 1) * 3 * sizeof(float)));
   AW_TRY(g->svel.ensure((size_t)std::max<int64_t>(n, 1) * 3 * sizeof(float)));
   if (d_ids) AW_TRY(g->sids.ensure((size_t)std::max<int64_t>(n, 1) * sizeof(int32_t)));
+  if (d_density) AW_TRY(g->sdens.ensure((size_t)std::max<int64_t>(n, 1) * sizeof(float)));
   Prof pf(g->ctx, "sort_particles", 52.0 * n);
   AW_TRY(g->scan_tmp.ensure(scan_scratch_words(g->nb) * sizeof(uint32_t)));
   k_scan_u32(s, g->count.as<uint32_t>(), g->poff.as<uint32_t>(), g->nb, g->scan_tmp.as<uint32_t>());
   AW_HIP(hipMemcpyAsync(g->cursor.p, g->poff.p, (size_t)g->nb * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
   k_scatter(s, d_pos, d_vel, d_ids, g->key.as<int32_t>(), g->cursor.as<uint32_t>(), n, g->spos.as<float>(),
-            g->svel.as<float>(), d_ids ? g->sids.as<int32_t>() : nullptr);
+            g->svel.as<float>(), d_ids ? g->sids.as<int32_t>() : nullptr, d_density,
+            d_density ? g->sdens.as<float>() : nullptr);
   g->n_sorted = n;
   g->sorted_ids = d_ids != nullptr;
   return launch_status("sort_particles");
 }
 
-int p2g_sorted(aw_grid* g, const float* spos, const float* svel, float dt, const float* grav) {
+// sdens (sorted densities, or null): also deposit the density-weighted
+// weights and set beta = 1 / rho_face (two-phase); without it beta keeps the
+// uniform 1 (reset here if the last step had variable beta)
+int p2g_sorted(aw_grid* g, const float* spos, const float* svel, float dt, const float* grav,
+               const float* sdens = nullptr) {
   hipStream_t s = g->ctx->stream;
   GridDev gv = g->view();
   const double nf3 = 3.0 * g->nfaces;
+  float* mm[3] = {g->macc[0].as<float>(), g->macc[1].as<float>(), g->macc[2].as<float>()};
+  if (sdens) {
+    for (int a = 0; a < 3; ++a) AW_TRY(g->Dacc[a].ensure((size_t)g->nfaces * sizeof(float)));
+    float* D[3] = {g->Dacc[0].as<float>(), g->Dacc[1].as<float>(), g->Dacc[2].as<float>()};
+    {
+      Prof pf(g->ctx, "p2g_deposit_density", 8.0 * g->n_sorted + 4.0 * SLAB * g->occupied + g->ncells);
+      k_p2g_deposit(s, gv, spos, nullptr, g->count.as<uint32_t>(), g->poff.as<uint32_t>(), g->slab.as<float>(),
+                    g->liquid.as<uint8_t>(), sdens);
+    }
+    {
+      // the weights go to macc here and are deposited again with the velocities below
+      Prof pf(g->ctx, "p2g_gather_density", 4.0 * SLAB * g->occupied + 8.0 * nf3);
+      k_p2g_gather(s, gv, g->count.as<uint32_t>(), g->slab.as<float>(), D, mm);
+    }
+    Prof pf(g->ctx, "beta_fold", 8.0 * nf3 + 4.0 * nf3);
+    float* bt[3] = {g->beta[0].as<float>(), g->beta[1].as<float>(), g->beta[2].as<float>()};
+    k_beta_fold(s, gv, mm, D, bt);
+    g->beta_variable = true;
+  } else if (g->beta_variable) {
+    LevelDev L;
+    for (int a = 0; a < 3; ++a) {
+      L.D[a] = g->D[a];
+      L.foff[a] = g->foff.as<uint32_t>();
+      L.beta[a] = g->beta[a].as<float>();
+    }
+    L.nb = g->nb;
+    L.bs = BASE;
+    L.e = g->level.as<uint8_t>();
+    L.coff = g->coff.as<uint32_t>();
+    L.nbr = g->nbr.as<int32_t>();
+    k_init_beta(s, L, g->beta[0].as<float>(), g->beta[1].as<float>(), g->beta[2].as<float>());
+    g->beta_variable = false;
+  }
   {
     Prof pf(g->ctx, "p2g_deposit", 24.0 * g->n_sorted + 4.0 * SLAB * g->occupied + g->ncells);
     k_p2g_deposit(s, gv, spos, svel, g->count.as<uint32_t>(), g->poff.as<uint32_t>(), g->slab.as<float>(),
@@ -827,9 +875,25 @@ void coarsen_masks(aw_solver* S) {
   }
 }
 
+// coefficient coarsening level by level (runGalerkin, MSBG.cuh:661-676), when
+// the grid's beta is variable or the coarse levels still hold variable values
+void coarsen_betas(aw_solver* S) {
+  aw_grid* g = S->g;
+  if (!g->beta_variable && !S->coarse_beta_variable) return;
+  hipStream_t s = g->ctx->stream;
+  for (size_t k = 0; k + 1 < S->lv.size(); ++k) {
+    const MGLevel& F = *S->lv[k];
+    MGLevel& C = *S->lv[k + 1];
+    Prof pf(g->ctx, "coarsen_beta", 12.0 * F.nfaces + 12.0 * C.nfaces);
+    k_coarsen_beta(s, F.view(), C.view(), C.aggregated, C.d_beta[0], C.d_beta[1], C.d_beta[2]);
+  }
+  S->coarse_beta_variable = g->beta_variable;
+}
+
 int ensure_setup(aw_solver* S) {
   if (!S->g->have_topology) return fail(AW_ERR_INVALID, "grid has no topology");
   if (S->built_version != S->g->topo_version) AW_TRY(build_hierarchy(S));
+  coarsen_betas(S);
   coarsen_masks(S);
   if (S->direct) {
     const MGLevel& C = *S->lv.back();
@@ -846,7 +910,8 @@ void smooth_level(aw_solver* S, int k, float* x, const float* b, int iters, bool
   for (int it = 0; it < iters; ++it)
     for (int ph = 0; ph < 2; ++ph) {
       const int col = reverse ? 1 - ph : ph;
-      const bool uni = k > 0 && L.ush[0] >= 0;
+      // the arithmetic stencil of uniform levels assumes beta = 1 inside the tank
+      const bool uni = k > 0 && L.ush[0] >= 0 && !S->g->beta_variable;
       Prof pf(S->g->ctx, k == 0 ? "smooth_l0" : (uni ? "smooth_uniform" : "smooth_coarse"), L.sweep_bytes[col]);
       if (uni)
         k_smooth_uniform(s, L.ush, L.ulr, L.ue, L.d_liq, x, b, col, S->prm.omega, S->prm.local_sweeps,
@@ -877,7 +942,7 @@ void tail_vcycle(aw_solver* S, int t0, float* x, const float* b) {
     // the tank); level 0 keeps the general one
     T.ulr = -1;
     T.ue = 0;
-    if (t0 + i > 0) {
+    if (t0 + i > 0 && !S->g->beta_variable) {
       bool uni = true;
       for (uint8_t e : L.h_e) uni = uni && e == L.h_e[0];
       if (uni) {
@@ -1286,6 +1351,8 @@ int validate_ids(aw_grid* g, const int32_t* ids, int64_t n) {
 
 int check_adjoint_state(aw_grid* g) {
   if (!g->have_topology) return fail(AW_ERR_INVALID, "grid has no topology");
+  if (g->beta_variable)
+    return fail(AW_ERR_UNSUPPORTED, "the adjoint holds beta = 1: the last step had variable density");
   if (g->n_sorted < 0 || !g->sorted_ids)
     return fail(AW_ERR_INVALID, "the adjoint needs a forward step binned with particle ids");
   return ensure_adjoint(g);
@@ -1753,6 +1820,13 @@ int aw_grid_build_topology_radius(aw_grid* g, const float* d_pos, const float* d
   return launch_status("build_topology_radius");
 }
 
+int aw_grid_beta_changed(aw_grid* g) {
+  if (!g) return fail(AW_ERR_INVALID, "grid is NULL");
+  if (!g->have_topology) return fail(AW_ERR_INVALID, "grid has no topology");
+  g->beta_variable = true;
+  return AW_OK;
+}
+
 int aw_grid_set_levels(aw_grid* g, const uint8_t* h_levels) {
   if (!g || !h_levels) return fail(AW_ERR_INVALID, "NULL argument");
   for (int b = 0; b < g->nb; ++b)
@@ -1995,6 +2069,7 @@ int aw_step(aw_grid* g, aw_solver* S, float* d_pos, float* d_vel, int32_t* d_ids
   if (prm->rebuild_topology < 0 || prm->rebuild_topology > 2) return fail(AW_ERR_INVALID, "rebuild_topology 0, 1 or 2");
   if (prm->rebuild_topology == 2 && (!prm->d_radius || (n > 0 && !d_ids)))
     return fail(AW_ERR_INVALID, "rebuild_topology 2 needs d_radius and d_ids");
+  if (prm->d_density && n > 0 && !d_ids) return fail(AW_ERR_INVALID, "d_density needs d_ids");
   // (rebuild_topology 2 reads the radii through the ids: the marking kernel
   // range-checks each id and flags one outside [0, n) as an invalid input)
   hipStream_t s = g->ctx->stream;
@@ -2019,8 +2094,10 @@ int aw_step(aw_grid* g, aw_solver* S, float* d_pos, float* d_vel, int32_t* d_ids
   } else {
     AW_TRY(bin_and_levels(g, d_pos, n, false, nullptr));
   }
-  AW_TRY(sort_particles(g, d_pos, d_vel, d_ids, n));
-  AW_TRY(p2g_sorted(g, g->spos.as<float>(), g->svel.as<float>(), prm->dt, prm->gravity));
+  const float* dens = n > 0 ? prm->d_density : nullptr;
+  AW_TRY(sort_particles(g, d_pos, d_vel, d_ids, n, dens));
+  AW_TRY(p2g_sorted(g, g->spos.as<float>(), g->svel.as<float>(), prm->dt, prm->gravity,
+                    dens ? g->sdens.as<float>() : nullptr));
   AW_TRY(divergence(g));
   AW_TRY(ensure_setup(S));
   if (warm) {
@@ -2152,6 +2229,7 @@ int aw_tape_destroy(aw_tape* t) {
 int aw_rollout_forward(aw_tape* t, float* d_pos, float* d_vel, int32_t* d_ids, const aw_step_params* prm,
                        aw_rollout_stats* stats) {
   if (!t || !d_pos || !d_vel || !d_ids || !prm) return fail(AW_ERR_INVALID, "NULL argument");
+  if (prm->d_density) return fail(AW_ERR_UNSUPPORTED, "rollouts differentiate with beta = 1 (no d_density)");
   aw_grid* g = t->g;
   hipStream_t s = g->ctx->stream;
   const int64_t n = t->n;

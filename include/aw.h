@@ -90,7 +90,9 @@ typedef struct {
   float* d_pressure;            /* solver units: p * dt / (rho h) */
   float* d_rhs;                 /* -divergence of u* (h^2 velocity) */
   uint8_t* d_liquid;            /* 1 = liquid cell */
-  float* d_beta[3];
+  float* d_beta[3];             /* face coefficients 1 / rho (0 on the tank walls): the uniform 1, or from
+                                   aw_step_params.d_density; written by P2G.  A caller writing them
+                                   directly calls aw_grid_beta_changed before the next solve */
   float* d_u_old[3];            /* P2G velocity */
   float* d_u_star[3];           /* + body force */
   float* d_u_new[3];            /* after projection */
@@ -144,6 +146,13 @@ typedef struct {
                             the step needs d_ids (ids in [0, n); another id fails the step with
                             AW_ERR_INVALID) to find them as it reorders.  Read during aw_step only;
                             aw_rollout_forward copies the radii into its tape for the backward pass */
+  const float* d_density; /* two-phase / variable density: relative density of each particle (the
+                            reference density is 1), indexed by particle id (needs d_ids), or NULL for
+                            beta = 1.  P2G also deposits it, beta = 1 / rho_face on faces with mass
+                            (rho_face = sum w rho / sum w; 1 on faces without), and the solver coarsens
+                            beta through its levels (the reference's BETA_COEFF / MASS_DENSITY channels
+                            and galerkinCoarsen, MSBG.cuh:21-34,401-498).  Not differentiated: the
+                            adjoint entry points and rollouts reject it (AW_ERR_UNSUPPORTED) */
 } aw_step_params;
 
 typedef struct {
@@ -219,6 +228,11 @@ int aw_grid_build_topology(aw_grid* grid, const float* d_pos, int64_t n, int* ch
  * aw_step_params.d_radius to re-mark it from the radii every step. */
 int aw_grid_build_topology_radius(aw_grid* grid, const float* d_pos, const float* d_radius, int64_t n,
                                   int* changed);
+/* Marks the grid's face coefficients (aw_layout.d_beta) as written by the
+ * caller: the next solver setup coarsens them through the multigrid levels
+ * (uniform-coefficient fast paths off) until P2G rewrites them.  No reference
+ * counterpart beyond runGalerkin (MSBG.cuh:661-676). */
+int aw_grid_beta_changed(aw_grid* grid);
 /* Explicit native levels (host array, num_blocks) instead of the band rule. */
 int aw_grid_set_levels(aw_grid* grid, const uint8_t* h_levels);
 int aw_grid_layout(aw_grid* grid, aw_layout* out);

@@ -43,8 +43,8 @@ def test_struct_sizes_match_c_layout():
     assert ctypes.sizeof(aw.GridDesc) == 4 * 3 + 4 + 4 + 4 + 4 * 3 + 4 * 4
     assert ctypes.sizeof(aw.SolverParams) == 8 * 4
     assert ctypes.sizeof(aw.SolveStats) == 32
-    # dt, gravity, alpha, pad, rel_tol, max_iters, rebuild, warm_start, pad, d_radius
-    assert ctypes.sizeof(aw.StepParams) == 4 + 12 + 4 + 4 + 8 + 4 + 4 + 4 + 4 + 8
+    # dt, gravity, alpha, pad, rel_tol, max_iters, rebuild, warm_start, pad, d_radius, d_density
+    assert ctypes.sizeof(aw.StepParams) == 4 + 12 + 4 + 4 + 8 + 4 + 4 + 4 + 4 + 8 + 8
 
 
 def test_struct_layouts_match_the_c_compiler(tmp_path):
