@@ -38,7 +38,8 @@ class Layout(ctypes.Structure):
 class SolverParams(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("pre_sweeps", ctypes.c_int), ("post_sweeps", ctypes.c_int),
                 ("coarse_sweeps", ctypes.c_int), ("local_sweeps", ctypes.c_int), ("omega", ctypes.c_float),
-                ("prolong_scale", ctypes.c_float), ("precision", ctypes.c_int)]
+                ("prolong_scale", ctypes.c_float), ("precision", ctypes.c_int), ("adaptive", ctypes.c_int),
+                ("theta", ctypes.c_float), ("max_block_sweeps", ctypes.c_int), ("block_reduction", ctypes.c_float)]
 
 
 class SolveStats(ctypes.Structure):
@@ -61,7 +62,8 @@ class StepStats(ctypes.Structure):
 class LevelInfo(ctypes.Structure):
     _fields_ = [("dims", ctypes.c_int * 3), ("num_blocks", ctypes.c_uint32), ("total_cells", ctypes.c_uint32),
                 ("total_faces", ctypes.c_uint32 * 3), ("aggregated", ctypes.c_int), ("d_e", _V),
-                ("d_cell_off", _V), ("d_x", _V), ("d_b", _V), ("d_r", _V), ("d_liquid", _V), ("d_x64", _V)]
+                ("d_cell_off", _V), ("d_x", _V), ("d_b", _V), ("d_r", _V), ("d_liquid", _V), ("d_x64", _V),
+                ("d_block_sweeps", _V), ("d_active", _V)]
 
 
 class AdjointLayout(ctypes.Structure):
@@ -483,10 +485,14 @@ class G2PManager:
         check(lib().aw_advect(grid.handle, d_pos, int(n), float(dt)), "aw_advect")
 
 
-def solver_params(method=0, pre=2, post=2, coarse=16, local=1, omega=0.85714, prolong_scale=1.0, precision=0):
+def solver_params(method=0, pre=2, post=2, coarse=16, local=1, omega=0.85714, prolong_scale=1.0, precision=0,
+                  adaptive=0, theta=1e-4, max_block_sweeps=10, block_reduction=0.33):
+    """adaptive=1: the reference's adaptive smoothing (method 1 only): early exit
+    per block at block_reduction, activation above theta * max|b| (Smoother.cuh)."""
     p = SolverParams()
     p.method, p.pre_sweeps, p.post_sweeps, p.coarse_sweeps, p.local_sweeps = method, pre, post, coarse, local
     p.omega, p.prolong_scale, p.precision = omega, prolong_scale, precision
+    p.adaptive, p.theta, p.max_block_sweeps, p.block_reduction = adaptive, theta, max_block_sweeps, block_reduction
     return p
 
 
@@ -532,6 +538,11 @@ class VCycleSolver:
             return download(self.grid.ctx, L.d_liquid, np.uint8, L.total_cells)
         if name == "e":
             return download(self.grid.ctx, L.d_e, np.uint8, L.num_blocks)
+        if name in ("block_sweeps", "active"):
+            ptr = L.d_block_sweeps if name == "block_sweeps" else L.d_active
+            if not ptr:
+                raise ValueError("no adaptive smoothing call on this level yet")
+            return download(self.grid.ctx, ptr, np.uint8, L.num_blocks)
         if name == "coff":
             return download(self.grid.ctx, L.d_cell_off, np.uint32, L.num_blocks + 1)
         if name == "x64":

@@ -205,6 +205,15 @@ void k_ids_range(hipStream_t s, const int32_t* ids, int64_t n, int* out);  // ou
 // solver
 void k_smooth(hipStream_t s, const LevelDev& L, float* x, const float* b, const WGMap& wg,
               const uint32_t* blist, float omega, int local_sweeps, int reverse);
+// the reference's adaptive smoothing of one block colour: blocks with act set,
+// up to max_sweeps in-block sweeps with the early exit at `reduction`, then
+// act_next set on blocks above theta * (*bmax) and their face neighbours;
+// sweeps[block] = sweeps run (0 = inactive)
+void k_smooth_adaptive(hipStream_t s, const LevelDev& L, float* x, const float* b, const WGMap& wg, float omega,
+                       int reverse, const uint8_t* act, uint8_t* act_next, uint8_t* sweeps, const float* bmax,
+                       float theta, int max_sweeps, float reduction);
+void k_absmax(hipStream_t s, const float* v, const uint8_t* liquid, int n, float* out);  // *out = max |v| on liquid
+void k_fill_u8(hipStream_t s, uint8_t* p, size_t n, uint8_t v);
 // the same sweep on a uniform coarse level with power-of-two block dims sh
 // (log2), block resolution 1 << lr and cell edge 1 << e, colour col
 void k_smooth_uniform(hipStream_t s, const int* sh, int lr, int e, const uint8_t* liquid, float* x, const float* b,

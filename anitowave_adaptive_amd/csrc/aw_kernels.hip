@@ -2565,6 +2565,112 @@ __global__ __launch_bounds__(ST_T) AW_STENCIL_OCCUPANCY void smooth_kernel(
 }
 
 
+// The reference's adaptive smoothing of one block colour (smoothActiveBlocks,
+// Smoother.cuh:105-202, with its defects A12 fixed: the exit is decided per
+// block in LDS, so every thread of the work-group leaves the loop together).
+// A block runs only if act[B] is set.  It runs up to max_sweeps in-block
+// sweeps (both cell colours each), tracking the largest |r| met by its
+// updates in each sweep; it stops after the first sweep whose maximum is
+// below `reduction` times the first sweep's (when that is above 1e-9).  A
+// block whose last sweep's maximum exceeds theta (absolute; the caller scales
+// the reference's 1e-4 by max |b|, A22) sets act_next on itself and its six
+// face neighbours (activateBlock, Smoother.cuh:97-102,188-201; plain stores of
+// 1, so the order does not matter).  sweeps[B] receives the sweeps run (0:
+// inactive).  The loads and the update are smooth_kernel's.
+__global__ __launch_bounds__(ST_T) AW_STENCIL_OCCUPANCY void adaptive_smooth_kernel(
+    LevelDev L, float* __restrict__ x, const float* __restrict__ b, WGMap wgm, float omega, int reverse,
+    const uint8_t* __restrict__ act, uint8_t* __restrict__ act_next, uint8_t* __restrict__ sweeps,
+    const float* __restrict__ theta_p, float theta_rel, int max_sweeps, float reduction) {
+  __shared__ float sx[512];
+  __shared__ uint32_t smax[512];   // per block of the group: this sweep's max |r| (float bits)
+  __shared__ float sfirst[512];
+  __shared__ uint8_t sdone[512];
+  const WG w = wg_at(wgm, blockIdx.x);
+  CellRef cr;
+  const uint32_t* bd;
+  const int i = threadIdx.x;
+  const bool ok = decode_bd(L, w, i, cr, bd);
+  bool on = false;
+  float bb = 0.f, dg = 0.f, ex = 0.f, cw[6];
+  if (ok) {
+    sx[i] = x[cr.c];
+    on = act[cr.B] != 0;
+    if (on && L.liquid[cr.c]) {
+      bb = b[cr.c];
+      stencil_fast(L, cr, bd, [&](uint32_t cc) { return x[cc]; }, dg, ex, cw);
+    }
+  }
+  const bool live = on && ok && L.liquid[cr.c];
+  if (!__syncthreads_or(on ? 1 : 0)) {   // no active block in the group
+    if (ok && cr.li == 0) sweeps[cr.B] = 0;
+    return;
+  }
+  // block q of the group is active: one thread per block slot
+  const int nblk = w.count;
+  if (i < nblk) {
+    const uint32_t* bq = L.bd + (size_t)(w.start + i) * BD_WORDS;
+    sdone[i] = act[bq[0]] ? 0 : 1;
+    sfirst[i] = 0.f;
+  }
+  const float theta = theta_rel * *theta_p;
+  int ran = 0;
+  for (int ls = 0; ls < max_sweeps; ++ls) {
+    if (i < nblk) smax[i] = 0u;
+    __syncthreads();
+    const bool go = ok && !sdone[cr.q];
+    for (int ph = 0; ph < 2; ++ph) {
+      const int col = reverse ? 1 - ph : ph;
+      if (go && live && ((cr.l[0] + cr.l[1] + cr.l[2]) & 1) == col) {
+        const int base = i - cr.li;
+        const float off = ex + inblock(cw, cr.li, cr.r, [&](int li) { return sx[base + li]; });
+        const float res = bb - (dg * sx[i] - off);
+        if (dg > 0.f) {
+          sx[i] += omega * res / dg;
+          atomicMax(&smax[cr.q], __float_as_uint(fabsf(res)));
+        }
+      }
+      __syncthreads();
+    }
+    if (go) ran = ls + 1;
+    int more = 0;
+    if (i < nblk && !sdone[i]) {
+      const float m = __uint_as_float(smax[i]);
+      if (ls == 0) sfirst[i] = m;
+      if ((sfirst[i] > 1e-9f && m < sfirst[i] * reduction) || ls + 1 == max_sweeps) {
+        sdone[i] = 1;
+        // activation from the last sweep's maximum
+        if (m > theta) {
+          const uint32_t* bq = L.bd + (size_t)(w.start + i) * BD_WORDS;
+          act_next[bq[0]] = 1;
+#pragma unroll
+          for (int d = 0; d < 6; ++d) {
+            const int N = (int)bq[3 + d];
+            if (N >= 0) act_next[N] = 1;
+          }
+        }
+      } else {
+        more = 1;
+      }
+    }
+    if (!__syncthreads_or(more)) break;
+  }
+  if (ok) {
+    if (on) x[cr.c] = sx[i];
+    if (cr.li == 0) sweeps[cr.B] = (uint8_t)(on ? ran : 0);
+  }
+}
+
+// max |v| over liquid cells into *out (float bits; *out cleared by the launcher)
+__global__ __launch_bounds__(TPB) void absmax_kernel(const float* __restrict__ v, const uint8_t* __restrict__ liq, int n,
+                                                     uint32_t* __restrict__ out) {
+  float m = 0.f;
+  for (int i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB)
+    if (liq[i]) m = fmaxf(m, fabsf(v[i]));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
+  if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax(out, __float_as_uint(m));
+}
+
 // smooth_kernel for a uniform coarse level (every block of resolution
 // 1 << lr, beta 1 inside the tank and 0 on its walls, power-of-two block
 // dims): the blocks of one colour are enumerated and every neighbour address
@@ -3622,6 +3728,23 @@ void k_masked_copy(hipStream_t s, float* dst, const float* src, const uint8_t* l
 void k_smooth(hipStream_t s, const LevelDev& L, float* x, const float* b, const WGMap& wg,
               const uint32_t* blist, float omega, int local_sweeps, int reverse) {
   if (wg.n > 0) smooth_kernel<<<wg.n, ST_T, 0, s>>>(L, x, b, wg, blist, omega, local_sweeps, reverse);
+}
+
+void k_smooth_adaptive(hipStream_t s, const LevelDev& L, float* x, const float* b, const WGMap& wg, float omega,
+                       int reverse, const uint8_t* act, uint8_t* act_next, uint8_t* sweeps, const float* bmax,
+                       float theta, int max_sweeps, float reduction) {
+  if (wg.n > 0)
+    adaptive_smooth_kernel<<<wg.n, ST_T, 0, s>>>(L, x, b, wg, omega, reverse, act, act_next, sweeps, bmax, theta,
+                                                 max_sweeps, reduction);
+}
+
+void k_absmax(hipStream_t s, const float* v, const uint8_t* liquid, int n, float* out) {
+  k_fill_u32(s, reinterpret_cast<uint32_t*>(out), 1, 0u);
+  if (n > 0) absmax_kernel<<<grid_for(n) < 1024 ? grid_for(n) : 1024, TPB, 0, s>>>(v, liquid, n, reinterpret_cast<uint32_t*>(out));
+}
+
+void k_fill_u8(hipStream_t s, uint8_t* p, size_t n, uint8_t v) {
+  if (n) fill_kernel<uint8_t><<<grid_for(n), TPB, 0, s>>>(p, n, v);
 }
 
 void k_smooth_uniform(hipStream_t s, const int* sh, int lr, int e, const uint8_t* liquid, float* x, const float* b,

@@ -264,6 +264,9 @@ struct MGLevel {
   int ush[3] = {-1, -1, -1};
   int ulr = 0, ue = 0;
   DevBuf e, coff, foff, nbr, liquid, beta[3], x, b, r, blist, cblk, bdesc;
+  DevBuf act[2], sweeps;           // adaptive smoothing: active flags (current, next), sweeps run per block
+  int act_cur = 0;
+  bool act_valid = false;
   WGMap wgm[3]{};  // stencil launches over block colour 0, colour 1, both
   // algorithmic bytes of one stencil pass over colour 0, colour 1 (x read +
   // x write + b read = 12 B per cell, beta 3 x 4 B per face slot)
@@ -313,6 +316,7 @@ struct aw_solver {
   bool direct = false;
   DevBuf x64, e32, r32;  // FP64 iterative refinement: solution, FP32 correction, FP32 residual
   DevBuf r0, z, pv, q, sc;
+  DevBuf bmax;           // adaptive smoothing: max |b| of the level being smoothed
   double* h_sc = nullptr;       // host scalars (slots added), points at h_sc_sum
   double h_sc_sum[SC_N] = {};
   double* h_slots = nullptr;    // pinned host copy of the slotted device scalars
@@ -903,7 +907,40 @@ int ensure_setup(aw_solver* S) {
   return launch_status("solver setup");
 }
 
+// AdaptiveSmoother::solveLevel (Smoother.cuh:249-291) with device-side flags:
+// every block active at the start (initActiveBlocks), `iters` iterations of
+// both block colours over the active blocks, the flags swapped in between.
+int smooth_level_adaptive(aw_solver* S, int k, float* x, const float* b, int iters, bool reverse) {
+  MGLevel& L = *S->lv[k];
+  hipStream_t s = S->g->ctx->stream;
+  const LevelDev V = L.view();
+  for (DevBuf* f : {&L.act[0], &L.act[1], &L.sweeps}) AW_TRY(f->ensure((size_t)L.nb));
+  AW_TRY(S->bmax.ensure(sizeof(float)));
+  int cur = 0;
+  k_fill_u8(s, L.act[cur].as<uint8_t>(), L.nb, 1);
+  k_fill_u8(s, L.sweeps.as<uint8_t>(), L.nb, 0);
+  k_absmax(s, b, L.d_liq, (int)L.ncells, S->bmax.as<float>());
+  for (int it = 0; it < iters; ++it) {
+    k_fill_u8(s, L.act[1 - cur].as<uint8_t>(), L.nb, 0);
+    for (int ph = 0; ph < 2; ++ph) {
+      const int col = reverse ? 1 - ph : ph;
+      Prof pf(S->g->ctx, k == 0 ? "smooth_adaptive_l0" : "smooth_adaptive_coarse", L.sweep_bytes[col]);
+      k_smooth_adaptive(s, V, x, b, L.wgm[col], S->prm.omega, reverse ? 1 : 0, L.act[cur].as<uint8_t>(),
+                        L.act[1 - cur].as<uint8_t>(), L.sweeps.as<uint8_t>(), S->bmax.as<float>(), S->prm.theta,
+                        S->prm.max_block_sweeps, S->prm.block_reduction);
+    }
+    cur = 1 - cur;
+  }
+  L.act_cur = cur;
+  L.act_valid = true;
+  return launch_status("adaptive smoothing");
+}
+
 void smooth_level(aw_solver* S, int k, float* x, const float* b, int iters, bool reverse) {
+  if (S->prm.adaptive) {
+    (void)smooth_level_adaptive(S, k, x, b, iters, reverse);
+    return;
+  }
   const MGLevel& L = *S->lv[k];
   hipStream_t s = S->g->ctx->stream;
   const LevelDev V = L.view();
@@ -1578,9 +1615,14 @@ void solver_release(aw_solver* S) {
 }
 
 bool valid_params(const aw_solver_params* p) {
-  return p && (p->method == 0 || p->method == 1) && p->pre_sweeps >= 0 && p->post_sweeps >= 0 &&
-         p->coarse_sweeps >= 0 && p->local_sweeps >= 1 && p->omega > 0.f && p->omega < 2.f &&
-         (p->precision == 0 || p->precision == 1);
+  if (!p) return false;
+  const bool base = (p->method == 0 || p->method == 1) && p->pre_sweeps >= 0 && p->post_sweeps >= 0 &&
+                    p->coarse_sweeps >= 0 && p->local_sweeps >= 1 && p->omega > 0.f && p->omega < 2.f &&
+                    (p->precision == 0 || p->precision == 1) && (p->adaptive == 0 || p->adaptive == 1);
+  if (!base || !p->adaptive) return base;
+  // adaptive smoothing is nonlinear: the stationary method only
+  return p->method == 1 && p->theta >= 0.f && p->max_block_sweeps >= 1 && p->max_block_sweeps <= 255 &&
+         p->block_reduction > 0.f && p->block_reduction < 1.f;
 }
 
 }  // namespace
@@ -2001,6 +2043,8 @@ int aw_solver_level_info(aw_solver* S, int k, aw_level_info* o) {
   o->d_r = L.d_r;
   o->d_liquid = L.d_liq;
   o->d_x64 = (k == 0 && S->prm.precision == 1 && S->x64.p) ? S->x64.as<double>() : nullptr;
+  o->d_block_sweeps = L.act_valid ? L.sweeps.as<uint8_t>() : nullptr;
+  o->d_active = L.act_valid ? L.act[L.act_cur].as<uint8_t>() : nullptr;
   return AW_OK;
 }
 

@@ -111,6 +111,16 @@ typedef struct {
   float omega;         /* Gauss-Seidel weight (reference 0.85714) */
   float prolong_scale; /* coarse-correction scale (1 for PCG, 0.75 recommended for stationary) */
   int precision;       /* 0 = FP32; 1 = FP32 V-cycles inside FP64 iterative refinement */
+  /* The reference's adaptive smoothing (AdaptiveSmoother, Smoother.cuh:105-291), method 1 only (it
+   * makes the V-cycle nonlinear, which conjugate gradients cannot take): every smoothing call starts
+   * with all blocks active; an active block runs up to max_block_sweeps in-block sweeps and stops
+   * after the first whose largest |r| is below block_reduction times the first sweep's; a block still
+   * above theta * max|b| (the reference's absolute 1e-4, made relative) activates itself and its six
+   * face neighbours for the next iteration.  The flags stay on the device (no host readback). */
+  int adaptive;        /* 0 = plain block red-black sweeps (local_sweeps each); 1 = adaptive */
+  float theta;         /* activation threshold relative to max |b| of the level (reference 1e-4) */
+  int max_block_sweeps; /* reference MAX_BLOCK_ITER 10 */
+  float block_reduction; /* reference TARGET_RESIDUAL_REDUCTION 0.33 */
 } aw_solver_params;
 
 /* Convergence contract of every solve: AW_OK only when the true relative
@@ -176,6 +186,9 @@ typedef struct {
   float* d_r;
   uint8_t* d_liquid;
   const double* d_x64;     /* level 0 with precision 1: the FP64 solution of the last solve */
+  const uint8_t* d_block_sweeps; /* adaptive smoothing: in-block sweeps each block ran in the last
+                                    iteration of the last smoothing call (0 = inactive), or NULL */
+  const uint8_t* d_active;       /* adaptive smoothing: the blocks that call left active, or NULL */
 } aw_level_info;
 
 const char* aw_status_str(int status);

@@ -209,6 +209,10 @@ class VCycleSolver {
     p.omega = 0.85714f;
     p.prolong_scale = 1.0f;
     p.precision = 0;
+    p.adaptive = 0;
+    p.theta = 1e-4f;
+    p.max_block_sweeps = 10;
+    p.block_reduction = 0.33f;
     return p;
   }
   // builds the hierarchy for the grid's current topology and liquid mask

@@ -75,6 +75,7 @@ struct Level {              // one multigrid level
   std::vector<int> parent;
   std::vector<double> x, b, r;
   std::vector<int> bcol[2];
+  std::vector<uint8_t> act, sweeps;   // adaptive smoothing: flags left active, sweeps run per block
 };
 
 struct OGrid {
@@ -106,6 +107,10 @@ struct OGrid {
   double omega = 0.85714f;
   int local_sweeps = 1, pre = 2, post = 2, coarse_sweeps = 16;
   double prolong_scale = 1.0;
+  // adaptive smoothing (AdaptiveSmoother, Smoother.cuh:105-291)
+  int adaptive = 0;
+  double theta = 1e-4, block_reduction = 0.33;
+  int max_block_sweeps = 10;
 };
 
 inline int block_res(const OGrid& g, int b) { return B >> g.level[b]; }
@@ -494,7 +499,15 @@ void mg_refresh(OGrid& g) {
 // order (block colour 1 then 0, cell colour 1 then 0) makes the post-smoother
 // the adjoint of the pre-smoother, so the V-cycle is a symmetric
 // preconditioner.
-void smooth(OGrid& g, Level& L, int iters, bool reverse) {
+void smooth_adaptive(OGrid& g, Level& L, int iters, bool reverse);
+
+// plain = true: the plain sweeps even with adaptive smoothing on (the
+// coarsest level's symmetric sweeps)
+void smooth(OGrid& g, Level& L, int iters, bool reverse, bool plain = false) {
+  if (g.adaptive && !plain) {
+    smooth_adaptive(g, L, iters, reverse);
+    return;
+  }
   for (int it = 0; it < iters; ++it)
     for (int ph = 0; ph < 2; ++ph) {
       int bc = reverse ? 1 - ph : ph;
@@ -518,6 +531,79 @@ void smooth(OGrid& g, Level& L, int iters, bool reverse) {
           }
       }
     }
+}
+
+// The reference's adaptive smoothing (smoothActiveBlocks + solveLevel,
+// Smoother.cuh:105-291), its divergent early exit (A12) decided per block:
+// every call starts with all blocks active; an active block of the phase's
+// colour runs up to max_block_sweeps in-block sweeps (both cell colours, in
+// the phase order) and stops after the first sweep whose largest |r| over
+// its updates is below block_reduction times the first sweep's (when that is
+// above 1e-9); if the last sweep's largest |r| exceeds theta max|b| (the
+// reference's absolute 1e-4 made relative, A22) the block and its six face
+// neighbours are active in the next iteration.
+void smooth_adaptive(OGrid& g, Level& L, int iters, bool reverse) {
+  std::vector<uint8_t> act(L.nb, 1), nxt(L.nb, 0);
+  L.sweeps.assign(L.nb, 0);
+  double bmax = 0.0;
+  for (int c = 0; c < L.ncells; ++c)
+    if (L.liquid[c]) bmax = std::max(bmax, std::fabs(L.b[c]));
+  const double theta = g.theta * bmax;
+  for (int it = 0; it < iters; ++it) {
+    std::fill(nxt.begin(), nxt.end(), 0);
+    for (int ph = 0; ph < 2; ++ph) {
+      int bc = reverse ? 1 - ph : ph;
+      const std::vector<int>& blocks = L.bcol[bc];
+#pragma omp parallel for schedule(dynamic, 16)
+      for (size_t bi = 0; bi < blocks.size(); ++bi) {
+        int b = blocks[bi];
+        if (!act[b]) {
+          L.sweeps[b] = 0;
+          continue;
+        }
+        int r = L.bs >> L.e[b];
+        double first = 0.0, last = 0.0;
+        int ran = 0;
+        for (int ls = 0; ls < g.max_block_sweeps; ++ls) {
+          double m = 0.0;
+          for (int cp = 0; cp < 2; ++cp) {
+            int cc = reverse ? 1 - cp : cp;
+            for (uint32_t c = L.coff[b]; c < L.coff[b + 1]; ++c) {
+              int li = c - L.coff[b];
+              int lx = li % r, ly = (li / r) % r, lz = li / (r * r);
+              if (((lx + ly + lz) & 1) != cc || !L.liquid[c] || !(L.diag[c] > 0)) continue;
+              double ax = L.diag[c] * L.x[c];
+              for (int k = L.rowptr[c]; k < L.rowptr[c + 1]; ++k) ax += L.val[k] * L.x[L.col[k]];
+              double res = L.b[c] - ax;
+              L.x[c] += g.omega * res / L.diag[c];
+              m = std::max(m, std::fabs(res));
+            }
+          }
+          ran = ls + 1;
+          if (ls == 0) first = m;
+          last = m;
+          if (first > 1e-9 && m < first * g.block_reduction) break;
+        }
+        L.sweeps[b] = (uint8_t)ran;
+        if (last > theta) {
+          int c[3];
+          block_coords(L.D, b, c);
+#pragma omp atomic write
+          nxt[b] = 1;
+          for (int a = 0; a < 3; ++a)
+            for (int sd = -1; sd <= 1; sd += 2) {
+              int q[3] = {c[0], c[1], c[2]};
+              q[a] += sd;
+              if (q[a] < 0 || q[a] >= L.D[a]) continue;
+#pragma omp atomic write
+              nxt[block_id(L.D, q[0], q[1], q[2])] = 1;
+            }
+        }
+      }
+    }
+    act.swap(nxt);
+  }
+  L.act = act;
 }
 
 void apply(const Level& L, const double* x, double* y) {
@@ -616,8 +702,8 @@ void vcycle(OGrid& g, int k) {
       return;
     }
     for (int i = 0; i < g.coarse_sweeps; ++i) {
-      smooth(g, L, 1, false);
-      smooth(g, L, 1, true);
+      smooth(g, L, 1, false, true);
+      smooth(g, L, 1, true, true);
     }
     return;
   }
@@ -1721,6 +1807,27 @@ int og_set_params(void* p, double omega, int local_sweeps, int pre, int post, in
   g.pre = pre;
   g.post = post;
   g.coarse_sweeps = coarse_sweeps;
+  return 0;
+}
+
+int og_set_adaptive(void* p, int on, double theta, int max_block_sweeps, double block_reduction) {
+  OGrid& g = *static_cast<OGrid*>(p);
+  if (max_block_sweeps < 1 || !(block_reduction > 0.0 && block_reduction < 1.0) || theta < 0.0) return 1;
+  g.adaptive = on;
+  g.theta = theta;
+  g.max_block_sweeps = max_block_sweeps;
+  g.block_reduction = block_reduction;
+  return 0;
+}
+
+// per-block sweeps run and flags left active by the last adaptive smoothing of level k
+int og_mg_adaptive(void* p, int k, uint8_t* sweeps, uint8_t* act) {
+  OGrid& g = *static_cast<OGrid*>(p);
+  if (k < 0 || k >= (int)g.mg.size()) return 1;
+  const Level& L = g.mg[k];
+  if ((int)L.sweeps.size() != L.nb) return 1;
+  std::copy(L.sweeps.begin(), L.sweeps.end(), sweeps);
+  std::copy(L.act.begin(), L.act.end(), act);
   return 0;
 }
 

@@ -51,6 +51,8 @@ def lib():
             "og_divergence": (ctypes.c_int, [_V]),
             "og_set_params": (ctypes.c_int, [_V, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
             "og_mg_build": (ctypes.c_int, [_V]),
+            "og_set_adaptive": (ctypes.c_int, [_V, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double]),
+            "og_mg_adaptive": (ctypes.c_int, [_V, ctypes.c_int, _U8, _U8]),
             "og_num_mg_levels": (ctypes.c_int, [_V]),
             "og_mg_level_info": (ctypes.c_int, [_V, ctypes.c_int, _I]),
             "og_mg_level_e": (ctypes.c_int, [_V, ctypes.c_int, _U8, _U32]),
@@ -258,6 +260,21 @@ class OracleGrid:
     def set_params(self, omega=float(np.float32(0.85714)), local_sweeps=1, pre=2, post=2, coarse_sweeps=16,
                    prolong_scale=1.0):
         lib().og_set_params(self._h, float(omega), local_sweeps, pre, post, coarse_sweeps, float(prolong_scale))
+
+    def set_adaptive(self, on=True, theta=1e-4, max_block_sweeps=10, block_reduction=0.33):
+        """The reference's adaptive smoothing on every smoothing call except the coarsest level's sweeps."""
+        if lib().og_set_adaptive(self._h, int(bool(on)), float(theta), int(max_block_sweeps),
+                                 float(block_reduction)) != 0:
+            raise ValueError("bad adaptive smoothing parameters")
+
+    def mg_adaptive(self, k):
+        """(sweeps run per block, flags left active) of the last adaptive smoothing of level k."""
+        nb = self.mg_info(k)["nb"]
+        sw = np.zeros(nb, np.uint8)
+        act = np.zeros(nb, np.uint8)
+        if lib().og_mg_adaptive(self._h, int(k), _p(sw, _U8), _p(act, _U8)) != 0:
+            raise ValueError("no adaptive smoothing on this level")
+        return sw, act
 
     def mg_build(self):
         lib().og_mg_build(self._h)

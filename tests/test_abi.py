@@ -41,7 +41,7 @@ def test_status_strings_and_errors_without_device():
 def test_struct_sizes_match_c_layout():
     # sizes fixed by the C declarations (x86-64 SysV)
     assert ctypes.sizeof(aw.GridDesc) == 4 * 3 + 4 + 4 + 4 + 4 * 3 + 4 * 4
-    assert ctypes.sizeof(aw.SolverParams) == 8 * 4
+    assert ctypes.sizeof(aw.SolverParams) == 12 * 4
     assert ctypes.sizeof(aw.SolveStats) == 32
     # dt, gravity, alpha, pad, rel_tol, max_iters, rebuild, warm_start, pad, d_radius, d_density
     assert ctypes.sizeof(aw.StepParams) == 4 + 12 + 4 + 4 + 8 + 4 + 4 + 4 + 4 + 8 + 8
