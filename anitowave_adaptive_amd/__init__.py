@@ -139,6 +139,7 @@ SIGNATURES = {
     "aw_tape_masks": (_I, [_V, _I, _V, _V, ctypes.c_uint32, _P(ctypes.c_uint32)]),
     "aw_resample_cells": (_I, [_V, _V, _P(_I), _V]),
     "aw_remap_cells": (_I, [_V, _V, _V, _V]),
+    "aw_remap_cells_noalloc": (_I, [_V, _V, _V, _V]),
     "aw_write_vtk_structured": (_I, [ctypes.c_char_p, ctypes.c_char_p, _V, _P(_I), _P(_F), _P(_F)]),
     "aw_write_vtk_blocks": (_I, [ctypes.c_char_p, _P(_I), _F, _V]),
     "aw_write_vtk_particles": (_I, [ctypes.c_char_p, _V, _V, _V, _I64]),

@@ -63,6 +63,36 @@ void mem_sub(uint64_t b) { g_mem_cur.fetch_sub(b); }
 std::mutex g_user_mu;
 std::unordered_map<void*, uint64_t> g_user_alloc;  // aw_malloc'd buffers and their sizes
 
+// Scratch of the entry points that allocate nothing (aw_remap_cells_noalloc):
+// the previous layout's cell offsets (num_blocks + 1 words) and the chunk
+// sums of their scan, for grids of up to NOALLOC_MAX_BLOCKS blocks.  Each GPU
+// holds its own copy until the process ends; host code uses only the device
+// address resolved per device (scratch_dev), never the array's name.
+constexpr int NOALLOC_MAX_BLOCKS = 1 << 18;
+constexpr size_t NOALLOC_SCRATCH_WORDS = (size_t)NOALLOC_MAX_BLOCKS + 1024;
+static __device__ __align__(256) uint32_t scratch[NOALLOC_SCRATCH_WORDS];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_dev[NOALLOC_MAX_DEVICES] = {};
+
+// device address of `scratch` on the current device `device` (looked up once per device)
+int scratch_address(int device, uint32_t** out) {
+  *out = nullptr;
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return AW_OK;  // that device has no no-allocation entry points
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      (void)hipGetLastError();
+      return fail(AW_ERR_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    }
+    g_scratch_dev[device] = static_cast<uint32_t*>(ptr);
+  }
+  *out = g_scratch_dev[device];
+  return AW_OK;
+}
+
 // grow-only device allocation
 struct DevBuf {
   void* p = nullptr;
@@ -144,6 +174,7 @@ struct aw_ctx {
   int refs = 0;
   bool closed = false;
   int device = 0;
+  uint32_t* scratch_dev = nullptr;  // this device's copy of the no-allocation scratch
   hipStream_t stream = nullptr;
   bool own_stream = false;
   bool prof = false;
@@ -1656,8 +1687,11 @@ int aw_ctx_create(int device, void* stream, aw_ctx** out) {
   }
   if (device < 0 || device >= ndev) return fail(AW_ERR_INVALID, "device %d out of range [0,%d)", device, ndev);
   AW_HIP(hipSetDevice(device));
+  uint32_t* scr = nullptr;
+  AW_TRY(scratch_address(device, &scr));
   auto* c = new aw_ctx();
   c->device = device;
+  c->scratch_dev = scr;
   if (stream) {
     c->stream = static_cast<hipStream_t>(stream);
   } else {
@@ -2401,6 +2435,29 @@ int aw_remap_cells(aw_grid* g, const uint8_t* d_level_prev, const float* d_src, 
   Prof pf(g->ctx, "pressure_remap", 8.0 * g->ncells + 9.0 * g->nb);
   k_pressure_remap(s, g->view(), d_level_prev, g->remap_coff.as<uint32_t>(), d_src, d_dst);
   return launch_status("remap_cells");
+}
+
+int aw_remap_cells_noalloc(aw_grid* g, const uint8_t* d_level_prev, const float* d_src, float* d_dst) {
+  if (!g || !d_level_prev || !d_src || !d_dst) return fail(AW_ERR_INVALID, "NULL argument");
+  if (!g->have_topology) return fail(AW_ERR_INVALID, "no topology");
+  if (d_src == d_dst) return fail(AW_ERR_INVALID, "d_src and d_dst must not alias");
+  if (g->nb > NOALLOC_MAX_BLOCKS)
+    return fail(AW_ERR_INVALID, "grid of %d blocks: aw_remap_cells_noalloc handles at most %d", g->nb,
+                NOALLOC_MAX_BLOCKS);
+  uint32_t* scr = g->ctx->scratch_dev;
+  if (!scr) return fail(AW_ERR_UNSUPPORTED, "no scratch on device %d", g->ctx->device);
+  static_assert(NOALLOC_MAX_BLOCKS + 1 + 2 * (NOALLOC_MAX_BLOCKS / 4096 + 2) <= (int)NOALLOC_SCRATCH_WORDS,
+                "scratch holds the offsets and the scan's chunk sums");
+  if ((size_t)g->nb + 1 + scan_scratch_words(g->nb) > NOALLOC_SCRATCH_WORDS)
+    return fail(AW_ERR_INVALID, "scratch too small for %d blocks", g->nb);
+  // the scan writes every word it reads (no clearing needed); no profiler
+  // events (they could allocate)
+  hipStream_t s = g->ctx->stream;
+  uint32_t* coff_prev = scr;
+  uint32_t* scan_tmp = scr + g->nb + 1;
+  k_cell_offsets(s, d_level_prev, g->nb, coff_prev, scan_tmp);
+  k_pressure_remap(s, g->view(), d_level_prev, coff_prev, d_src, d_dst);
+  return launch_status("remap_cells_noalloc");
 }
 
 int aw_resample_cells(aw_grid* g, const float* d_field, const int out_dims[3], float* d_out) {

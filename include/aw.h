@@ -34,6 +34,7 @@
  *   aw_step                              the main.cu step sequence, completed with the projection
  *   aw_resample_cells                    sampleMSBG                          include/vtx_visualization.cuh:8-62
  *   aw_remap_cells                       (no counterpart: warm start across topology changes)
+ *   aw_remap_cells_noalloc               (no counterpart: aw_remap_cells for callers that cannot allocate)
  *   aw_write_vtk_*, aw_export_*_vtk      exportUniformPressure_VTK, exportMultigrid_VTK, exportParticles_VTK
  *                                                                            include/vtx_visualization.cuh:64-216
  *   aw_*_adjoint, aw_tape_*, aw_rollout_* (no counterpart: the reference has no gradient code; these
@@ -385,6 +386,15 @@ int aw_tape_masks(aw_tape* tape, int step, uint8_t* h_levels, uint8_t* h_liquid,
  * cell above).  The warm start of aw_step remaps the pressure this way.
  * Asynchronous; d_src and d_dst must not alias. */
 int aw_remap_cells(aw_grid* grid, const uint8_t* d_level_prev, const float* d_src, float* d_dst);
+/* aw_remap_cells without any device allocation (no reference counterpart):
+ * for callers that cannot allocate, such as code under HIP graph capture or
+ * tight serving loops.  Its scratch (the previous layout's cell offsets and
+ * their scan's chunk sums) is a fixed array in device memory, one per GPU,
+ * held until the process ends (about 1 MB); grids of more than 262,144
+ * blocks (2^18, the 512^3 grid's count) are rejected with AW_ERR_INVALID.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (other streams or threads) race.  Same result as aw_remap_cells. */
+int aw_remap_cells_noalloc(aw_grid* grid, const uint8_t* d_level_prev, const float* d_src, float* d_dst);
 
 /* ---- Uniform resample and VTK export (include/vtx_visualization.cuh) ----
  * aw_resample_cells: nearest-cell resample of a cell field (float[total_cells]
