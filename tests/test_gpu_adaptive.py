@@ -102,7 +102,7 @@ def test_one_call_against_the_oracle(ctx, k):
 def test_theta_zero_one_sweep_is_the_plain_smoother(ctx):
     outs = []
     for adaptive in (0, 1):
-        og, gg, gp, sv = _setup(ctx, adaptive=adaptive, theta=0.0, max_block_sweeps=1)
+        og, gg, gp, sv = _setup(ctx, adaptive=adaptive, theta=0.0, max_block_sweeps=1, local=1)
         n = og.mg_info(0)["ncells"]
         rng = np.random.default_rng(5)
         liq = og.mg_get(0, "liquid") > 0

@@ -103,6 +103,7 @@ def test_theta_zero_one_sweep_is_the_plain_smoother(k):
     rng = np.random.default_rng(1)
     n = g.mg_info(k)["ncells"]
     x0, b = rng.standard_normal(n), rng.standard_normal(n)
+    g.set_params(local_sweeps=1)
     out = []
     for adaptive in (False, True):
         g.set_adaptive(adaptive, theta=0.0, max_block_sweeps=1)

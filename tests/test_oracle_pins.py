@@ -196,15 +196,15 @@ def numpy_vcycle(g, k, b, pre=2, post=2, local_sweeps=1, scale=1.0):
     return smooth_by_classes(A, liq, bc, cc, x, b, post, True, local_sweeps)
 
 
-@pytest.mark.parametrize("scale", [1.0, 0.75])
-def test_vcycle_matches_dense_restatement(scale):
+@pytest.mark.parametrize("scale,local_sweeps", [(1.0, 1), (0.75, 1), (1.0, 2)])
+def test_vcycle_matches_dense_restatement(scale, local_sweeps):
     # a small pool, 16^3 cells in 2^3 blocks of native levels 0 and 1: a
     # hierarchy of four levels (cell edges up to 8) down to 8 cells
     pos, vel = scenes.pool_sphere(n=16, radius=0.2, depth=0.4)
     g = O.OracleGrid((2, 2, 2), num_levels=2, h=1.0 / 16)
     g.set_levels(np.array([0, 0, 1, 1, 0, 1, 1, 1], np.uint8))
     g.p2g(pos, vel)
-    g.set_params(prolong_scale=scale)
+    g.set_params(prolong_scale=scale, local_sweeps=local_sweeps)
     g.mg_build()
     assert g.num_mg_levels() == 4
     assert g.mg_info(g.num_mg_levels() - 1)["ncells"] <= 64
@@ -212,7 +212,7 @@ def test_vcycle_matches_dense_restatement(scale):
     liq = g.mg_get(0, "liquid") > 0
     b = np.where(liq, rng.standard_normal(len(liq)), 0.0)
     got = vcycle(g, b)
-    want = numpy_vcycle(g, 0, b, scale=scale)
+    want = numpy_vcycle(g, 0, b, local_sweeps=local_sweeps, scale=scale)
     np.testing.assert_allclose(got, want, rtol=0, atol=1e-11 * np.abs(want).max())
 
 
