@@ -63,7 +63,7 @@ class LevelInfo(ctypes.Structure):
     _fields_ = [("dims", ctypes.c_int * 3), ("num_blocks", ctypes.c_uint32), ("total_cells", ctypes.c_uint32),
                 ("total_faces", ctypes.c_uint32 * 3), ("aggregated", ctypes.c_int), ("d_e", _V),
                 ("d_cell_off", _V), ("d_x", _V), ("d_b", _V), ("d_r", _V), ("d_liquid", _V), ("d_x64", _V),
-                ("d_block_sweeps", _V), ("d_active", _V)]
+                ("d_block_sweeps", _V), ("d_active", _V), ("smoothed_blocks", ctypes.c_uint32)]
 
 
 class AdjointLayout(ctypes.Structure):

@@ -214,6 +214,9 @@ void k_smooth_adaptive(hipStream_t s, const LevelDev& L, float* x, const float* 
                        float theta, int max_sweeps, float reduction);
 void k_absmax(hipStream_t s, const float* v, const uint8_t* liquid, int n, float* out);  // *out = max |v| on liquid
 void k_fill_u8(hipStream_t s, uint8_t* p, size_t n, uint8_t v);
+// live[B] = block B holds a liquid cell; out[j] = descriptor idx[j] of bd
+void k_block_live(hipStream_t s, const LevelDev& L, uint8_t* live);
+void k_gather_bd(hipStream_t s, const uint32_t* bd, const uint32_t* idx, int n, uint32_t* out);
 // the same sweep on a uniform coarse level with power-of-two block dims sh
 // (log2), block resolution 1 << lr and cell edge 1 << e, colour col
 void k_smooth_uniform(hipStream_t s, const int* sh, int lr, int e, const uint8_t* liquid, float* x, const float* b,

@@ -2564,6 +2564,29 @@ __global__ __launch_bounds__(ST_T) AW_STENCIL_OCCUPANCY void smooth_kernel(
     if (act[k]) x[cr[k].c] = sx[threadIdx.x + k * ST_T];
 }
 
+// live[B] = 1 when block B of the level holds a liquid cell: one wave per
+// block, its lanes strided over the block's cells.
+constexpr int LIVE_T = 256;
+__global__ __launch_bounds__(LIVE_T) void block_live_kernel(LevelDev L, uint8_t* __restrict__ live) {
+  const int B = blockIdx.x * (LIVE_T / 64) + (int)(threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63;
+  if (B >= L.nb) return;  // the whole wave leaves together
+  const uint32_t c0 = L.coff[B], c1 = L.coff[B + 1];
+  bool any = false;
+  for (uint32_t c = c0 + lane; c < c1; c += 64) any = any || L.liquid[c] != 0;
+  const bool wave_any = __ballot(any) != 0;
+  if (lane == 0) live[B] = wave_any ? 1 : 0;
+}
+
+// out[j] = descriptor idx[j] of bd, one word per thread
+__global__ __launch_bounds__(TPB) void gather_bd_kernel(const uint32_t* __restrict__ bd,
+                                                        const uint32_t* __restrict__ idx, int n,
+                                                        uint32_t* __restrict__ out) {
+  const int t = blockIdx.x * TPB + threadIdx.x;
+  if (t >= n * BD_WORDS) return;
+  const int j = t / BD_WORDS, w = t - j * BD_WORDS;
+  out[t] = bd[(size_t)idx[j] * BD_WORDS + w];
+}
 
 // The reference's adaptive smoothing of one block colour (smoothActiveBlocks,
 // Smoother.cuh:105-202, with its defects A12 fixed: the exit is decided per
@@ -3728,6 +3751,16 @@ void k_masked_copy(hipStream_t s, float* dst, const float* src, const uint8_t* l
 void k_smooth(hipStream_t s, const LevelDev& L, float* x, const float* b, const WGMap& wg,
               const uint32_t* blist, float omega, int local_sweeps, int reverse) {
   if (wg.n > 0) smooth_kernel<<<wg.n, ST_T, 0, s>>>(L, x, b, wg, blist, omega, local_sweeps, reverse);
+}
+
+void k_block_live(hipStream_t s, const LevelDev& L, uint8_t* live) {
+  const int per = LIVE_T / 64;
+  if (L.nb > 0) block_live_kernel<<<(L.nb + per - 1) / per, LIVE_T, 0, s>>>(L, live);
+}
+
+void k_gather_bd(hipStream_t s, const uint32_t* bd, const uint32_t* idx, int n, uint32_t* out) {
+  const int words = n * BD_WORDS;
+  if (n > 0) gather_bd_kernel<<<(words + TPB - 1) / TPB, TPB, 0, s>>>(bd, idx, n, out);
 }
 
 void k_smooth_adaptive(hipStream_t s, const LevelDev& L, float* x, const float* b, const WGMap& wg, float omega,

@@ -302,6 +302,19 @@ struct MGLevel {
   // algorithmic bytes of one stencil pass over colour 0, colour 1 (x read +
   // x write + b read = 12 B per cell, beta 3 x 4 B per face slot)
   double sweep_bytes[2] = {0.0, 0.0};
+  int tables_version = 0;  // bumped with every upload of the work list and descriptors
+  // The smoother's launches without the all-air blocks (it writes nothing on
+  // air cells, so leaving them out changes no value): per block colour the
+  // runs of wgm over the work-list entries whose block holds a liquid cell,
+  // in work-list order, their descriptors gathered into bdesc_live.  Set by
+  // refresh_live from the liquid masks; live_on false: the full lists.
+  bool live_on = false;
+  int live_tables_version = -1;
+  WGMap wgm_live[2]{};
+  DevBuf bdesc_live, live_idx;
+  std::vector<uint32_t> h_live_idx;  // work-list entries of bdesc_live (host source of live_idx)
+  uint32_t live_blocks = 0;
+  double live_bytes[2] = {0.0, 0.0};
   // device views (level 0 aliases the grid)
   const uint8_t* d_e = nullptr;
   const uint32_t* d_coff = nullptr;
@@ -329,6 +342,11 @@ struct MGLevel {
     L.bd = bdesc.as<uint32_t>();
     return L;
   }
+  LevelDev live_view() const {
+    LevelDev L = view();
+    L.bd = bdesc_live.as<uint32_t>();
+    return L;
+  }
   const uint32_t* bl() const { return blist.as<uint32_t>(); }
 };
 
@@ -348,6 +366,9 @@ struct aw_solver {
   DevBuf x64, e32, r32;  // FP64 iterative refinement: solution, FP32 correction, FP32 residual
   DevBuf r0, z, pv, q, sc;
   DevBuf bmax;           // adaptive smoothing: max |b| of the level being smoothed
+  DevBuf live_flags;     // per block of the list-smoothed levels: holds a liquid cell
+  uint8_t* h_live = nullptr;  // pinned host copy of live_flags
+  size_t h_live_cap = 0;
   double* h_sc = nullptr;       // host scalars (slots added), points at h_sc_sum
   double h_sc_sum[SC_N] = {};
   double* h_slots = nullptr;    // pinned host copy of the slotted device scalars
@@ -729,6 +750,8 @@ int upload_level(hipStream_t s, MGLevel& L, bool own_topology) {
     }
     AW_TRY(L.bdesc.ensure(bd.size() * sizeof(uint32_t)));
     AW_HIP(hipMemcpyAsync(L.bdesc.p, bd.data(), bd.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    L.tables_version++;
+    L.live_on = false;
   }
   AW_TRY(L.blist.ensure(blist.size() * sizeof(uint32_t)));
   AW_HIP(hipMemcpyAsync(L.blist.p, blist.data(), blist.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
@@ -925,11 +948,108 @@ void coarsen_betas(aw_solver* S) {
   S->coarse_beta_variable = g->beta_variable;
 }
 
+// Level k's plain smoother runs the arithmetic stencil of a uniform level
+// (beta = 1 inside the tank) rather than the work-list kernel.
+bool uniform_smoother(const aw_solver* S, int k) {
+  return k > 0 && S->lv[k]->ush[0] >= 0 && !S->g->beta_variable;
+}
+
+// The smoother's work lists without the all-air blocks (MGLevel::live_on),
+// from the liquid masks as they are now: every level below the tail that the
+// plain work-list smoother runs.  One flag per block is read back (one
+// synchronisation); a list whose entries and tables are unchanged is kept,
+// any other is uploaded, its descriptors gathered on the device, and the
+// captured PCG iteration (whose launches hold the lists) captured again.
+int refresh_live(aw_solver* S) {
+  hipStream_t s = S->g->ctx->stream;
+  const int nl = S->prm.adaptive ? 0 : S->tail;
+  std::vector<size_t> off((size_t)nl + 1, 0);
+  size_t tot = 0;
+  for (int k = 0; k < (int)S->lv.size(); ++k) S->lv[k]->live_on = false;
+  for (int k = 0; k < nl; ++k) {
+    off[k] = tot;
+    if (!uniform_smoother(S, k)) tot += (size_t)S->lv[k]->nb;
+  }
+  if (tot == 0) return AW_OK;
+  AW_TRY(S->live_flags.ensure(tot));
+  uint8_t* flags = S->live_flags.as<uint8_t>();
+  for (int k = 0; k < nl; ++k)
+    if (!uniform_smoother(S, k)) k_block_live(s, S->lv[k]->view(), flags + off[k]);
+  if (S->h_live_cap < tot) {
+    if (S->h_live) (void)hipHostFree(S->h_live);
+    S->h_live = nullptr;
+    S->h_live_cap = 0;
+    AW_HIP(hipHostMalloc((void**)&S->h_live, tot, hipHostMallocDefault));
+    S->h_live_cap = tot;
+  }
+  AW_HIP(hipMemcpyAsync(S->h_live, flags, tot, hipMemcpyDeviceToHost, s));
+  AW_HIP(hipStreamSynchronize(s));
+  AW_TRY(launch_status("liquid blocks"));
+  // every earlier copy from a level's h_live_idx has completed (synchronised above)
+  std::vector<uint32_t> idx;
+  for (int k = 0; k < nl; ++k) {
+    if (uniform_smoother(S, k)) continue;
+    MGLevel& L = *S->lv[k];
+    const uint8_t* f = S->h_live + off[k];
+    idx.clear();
+    WGMap m[2]{};
+    double by[2] = {0.0, 0.0};
+    for (int col = 0; col < 2; ++col) {
+      for (int j = 0; j < WGMAP_RUNS; ++j) {
+        m[col].g0[j] = INT_MAX;
+        m[col].s0[j] = m[col].cnt[j] = m[col].lr[j] = 0;
+      }
+      const WGMap& full = L.wgm[col];
+      int nrun = 0;
+      for (int j = 0; j < WGMAP_RUNS; ++j) {
+        if (full.cnt[j] == 0) continue;
+        const int start = (int)idx.size();
+        for (int q = full.s0[j]; q < full.s0[j] + full.cnt[j]; ++q) {
+          const uint32_t B = L.h_blist[q];
+          if (!f[B]) continue;
+          idx.push_back((uint32_t)q);
+          const double r = (double)(L.bs >> L.h_e[B]);
+          by[col] += 12.0 * r * r * r + 12.0 * (r + 1.0) * r * r;
+        }
+        const int cnt = (int)idx.size() - start;
+        if (cnt == 0) continue;
+        const int per = 512 >> (3 * full.lr[j]);
+        const int t = nrun++;
+        m[col].g0[t] = m[col].n;
+        m[col].s0[t] = start;
+        m[col].cnt[t] = cnt;
+        m[col].lr[t] = full.lr[j];
+        m[col].n += (cnt + per - 1) / per;
+      }
+    }
+    if (L.live_tables_version != L.tables_version || idx != L.h_live_idx) {
+      L.h_live_idx.swap(idx);
+      const size_t n = L.h_live_idx.size();
+      AW_TRY(L.live_idx.ensure(std::max<size_t>(n, 1) * sizeof(uint32_t)));
+      AW_TRY(L.bdesc_live.ensure(std::max<size_t>(n, 1) * BD_WORDS * sizeof(uint32_t)));
+      if (n > 0) {
+        AW_HIP(hipMemcpyAsync(L.live_idx.p, L.h_live_idx.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        k_gather_bd(s, L.bdesc.as<uint32_t>(), L.live_idx.as<uint32_t>(), (int)n, L.bdesc_live.as<uint32_t>());
+      }
+      L.live_tables_version = L.tables_version;
+      S->graph_epoch++;
+    }
+    L.wgm_live[0] = m[0];
+    L.wgm_live[1] = m[1];
+    L.live_bytes[0] = by[0];
+    L.live_bytes[1] = by[1];
+    L.live_blocks = (uint32_t)L.h_live_idx.size();
+    L.live_on = true;
+  }
+  return launch_status("smoother lists");
+}
+
 int ensure_setup(aw_solver* S) {
   if (!S->g->have_topology) return fail(AW_ERR_INVALID, "grid has no topology");
   if (S->built_version != S->g->topo_version) AW_TRY(build_hierarchy(S));
   coarsen_betas(S);
   coarsen_masks(S);
+  AW_TRY(refresh_live(S));
   if (S->direct) {
     const MGLevel& C = *S->lv.back();
     Prof pf(S->g->ctx, "coarse_factor", 8.0 * C.ncells * C.ncells);
@@ -974,18 +1094,21 @@ void smooth_level(aw_solver* S, int k, float* x, const float* b, int iters, bool
   }
   const MGLevel& L = *S->lv[k];
   hipStream_t s = S->g->ctx->stream;
-  const LevelDev V = L.view();
+  // the arithmetic stencil of uniform levels assumes beta = 1 inside the tank
+  const bool uni = uniform_smoother(S, k);
+  const bool live = !uni && L.live_on;
+  const LevelDev V = live ? L.live_view() : L.view();
   for (int it = 0; it < iters; ++it)
     for (int ph = 0; ph < 2; ++ph) {
       const int col = reverse ? 1 - ph : ph;
-      // the arithmetic stencil of uniform levels assumes beta = 1 inside the tank
-      const bool uni = k > 0 && L.ush[0] >= 0 && !S->g->beta_variable;
-      Prof pf(S->g->ctx, k == 0 ? "smooth_l0" : (uni ? "smooth_uniform" : "smooth_coarse"), L.sweep_bytes[col]);
+      Prof pf(S->g->ctx, k == 0 ? "smooth_l0" : (uni ? "smooth_uniform" : "smooth_coarse"),
+              live ? L.live_bytes[col] : L.sweep_bytes[col]);
       if (uni)
         k_smooth_uniform(s, L.ush, L.ulr, L.ue, L.d_liq, x, b, col, S->prm.omega, S->prm.local_sweeps,
                          reverse ? 1 : 0);
       else
-        k_smooth(s, V, x, b, L.wgm[col], L.bl(), S->prm.omega, S->prm.local_sweeps, reverse ? 1 : 0);
+        k_smooth(s, V, x, b, live ? L.wgm_live[col] : L.wgm[col], L.bl(), S->prm.omega, S->prm.local_sweeps,
+                 reverse ? 1 : 0);
     }
 }
 
@@ -1638,6 +1761,7 @@ void solver_free(aw_solver* S) {
     if (c.exec) (void)hipGraphExecDestroy(c.exec);
   if (S->upload_event) (void)hipEventDestroy(S->upload_event);
   if (S->h_slots) (void)hipHostFree(S->h_slots);
+  if (S->h_live) (void)hipHostFree(S->h_live);
   delete S;
   grid_release(g);
 }
@@ -2079,6 +2203,7 @@ int aw_solver_level_info(aw_solver* S, int k, aw_level_info* o) {
   o->d_x64 = (k == 0 && S->prm.precision == 1 && S->x64.p) ? S->x64.as<double>() : nullptr;
   o->d_block_sweeps = L.act_valid ? L.sweeps.as<uint8_t>() : nullptr;
   o->d_active = L.act_valid ? L.act[L.act_cur].as<uint8_t>() : nullptr;
+  o->smoothed_blocks = L.live_on ? L.live_blocks : (uint32_t)L.nb;
   return AW_OK;
 }
 

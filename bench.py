@@ -93,6 +93,12 @@ def run_gpu(args, w, pos, vel):
         iso = ctx.profile_report()
         ctx.profile(False)
     lv = list(st.num_blocks_by_level)
+    # per multigrid level: blocks, and those the smoother's launches cover
+    # (the blocks holding a liquid cell)
+    mg_blocks = []
+    for k in range(solver.num_levels()):
+        L = solver.level_info(k)
+        mg_blocks.append([int(L.num_blocks), int(L.smoothed_blocks)])
     # the step's own high-water mark, before the legs below allocate more
     step_peak = aw.memory_stats()[1]
     # nearest-cell resample of the last pressure onto a uniform grid
@@ -139,7 +145,7 @@ def run_gpu(args, w, pos, vel):
                resample=resample, steps_above_tol=floors, worst_true_rel_residual=worst_true,
                worst_recurrence_rel_residual=worst_rec, liquid_cells=int(st.liquid_cells),
                total_cells=int(st.total_cells), rel_residual=st.solve.rel_residual, profile=prof, isolated=iso,
-               peak_bytes=step_peak)
+               peak_bytes=step_peak, mg_blocks=mg_blocks)
     return res
 
 
@@ -555,6 +561,7 @@ def main():
             "liquid_cells": gpu["liquid_cells"],
             "blocks_by_level": gpu["blocks_by_level"][: w["levels"]],
             "mg0_cells": gpu["total_cells"],
+            "mg_blocks_and_smoothed": gpu["mg_blocks"],
             "pcg_iters_mean": round(float(np.mean(gpu["iters"])), 2),
             "topology_changes": gpu["topology_changes"],
             "peak_hbm_gib": round(gpu["peak_bytes"] / 2**30, 3),

@@ -192,6 +192,10 @@ typedef struct {
   const uint8_t* d_block_sweeps; /* adaptive smoothing: in-block sweeps each block ran in the last
                                     iteration of the last smoothing call (0 = inactive), or NULL */
   const uint8_t* d_active;       /* adaptive smoothing: the blocks that call left active, or NULL */
+  uint32_t smoothed_blocks;      /* blocks the plain smoother's launches cover: those holding a
+                                    liquid cell at the last setup or solve (all-air blocks are left
+                                    out), num_blocks where the level keeps the full lists (tail and
+                                    uniform levels, adaptive smoothing) */
 } aw_level_info;
 
 const char* aw_status_str(int status);
