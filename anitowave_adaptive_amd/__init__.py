@@ -76,7 +76,8 @@ class RolloutStats(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int), ("forward_ms", ctypes.c_double), ("recompute_ms", ctypes.c_double),
                 ("backward_ms", ctypes.c_double), ("forward_iterations", ctypes.c_int),
                 ("adjoint_iterations", ctypes.c_int), ("forward_rel_residual", ctypes.c_double),
-                ("adjoint_rel_residual", ctypes.c_double)]
+                ("adjoint_rel_residual", ctypes.c_double), ("replay_mismatched_steps", ctypes.c_int),
+                ("replay_max_state_diff", ctypes.c_double)]
 
 
 _I = ctypes.c_int
@@ -97,6 +98,7 @@ SIGNATURES = {
     "aw_copy": (_I, [_V, _V, _V, ctypes.c_uint64]),
     "aw_fill_f32": (_I, [_V, _V, ctypes.c_uint64, _F]),
     "aw_memory_stats": (_I, [_P(ctypes.c_uint64), _P(ctypes.c_uint64), _I]),
+    "aw_set_memory_limit": (_I, [ctypes.c_uint64]),
     "aw_profile_enable": (_I, [_V, _I]),
     "aw_profile_report": (_I, [_V, ctypes.c_char_p, ctypes.c_uint64]),
     "aw_grid_create": (_I, [_V, _P(GridDesc), _P(_V)]),
@@ -255,6 +257,11 @@ def memory_stats(reset_peak=False):
     cur, peak = ctypes.c_uint64(), ctypes.c_uint64()
     check(lib().aw_memory_stats(ctypes.byref(cur), ctypes.byref(peak), int(bool(reset_peak))), "aw_memory_stats")
     return cur.value, peak.value
+
+
+def set_memory_limit(nbytes):
+    """Cap on the device memory the library holds (0: none); past it, AW_ERR_OOM."""
+    check(lib().aw_set_memory_limit(int(nbytes)), "aw_set_memory_limit")
 
 
 def download(ctx, ptr, dtype, count):

@@ -216,6 +216,12 @@ void k_absmax(hipStream_t s, const float* v, const uint8_t* liquid, int n, float
 void k_fill_u8(hipStream_t s, uint8_t* p, size_t n, uint8_t v);
 // live[B] = block B holds a liquid cell; out[j] = descriptor idx[j] of bd
 void k_block_live(hipStream_t s, const LevelDev& L, uint8_t* live);
+// order-independent 64-bit fingerprint of a liquid mask
+void k_mask_hash(hipStream_t s, const uint8_t* m, uint32_t n, unsigned long long* out);
+// out[0] = max |x - x_ref|, |v - v_ref| (float bits) over particles matched by
+// id, out[1] = 1 if an id of rids lies outside [0, n); inv: n ints of scratch
+void k_state_diff(hipStream_t s, const float* pos, const float* vel, const int32_t* ids, const float* rpos,
+                  const float* rvel, const int32_t* rids, int64_t n, int32_t* inv, uint32_t* out);
 void k_gather_bd(hipStream_t s, const uint32_t* bd, const uint32_t* idx, int n, uint32_t* out);
 // the same sweep on a uniform coarse level with power-of-two block dims sh
 // (log2), block resolution 1 << lr and cell edge 1 << e, colour col

@@ -3506,6 +3506,75 @@ __global__ __launch_bounds__(TPB) void count_nonzero_kernel(const uint8_t* __res
   if (threadIdx.x == 0 && s) atomicAdd(out, s);
 }
 
+// Order-independent fingerprint of a liquid mask: the 64-bit wrapping sum of
+// a mixed hash of every liquid cell's index (splitmix64's finaliser).
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+__global__ __launch_bounds__(TPB) void mask_hash_kernel(const uint8_t* __restrict__ m, uint32_t n,
+                                                        unsigned long long* __restrict__ out) {
+  unsigned long long h = 0;
+  for (uint32_t i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB)
+    if (m[i]) h += mix64((unsigned long long)i + 0x9e3779b97f4a7c15ull);
+  const unsigned long long s = block_reduce_sum<unsigned long long>(h);
+  if (threadIdx.x == 0 && s) atomicAdd(out, s);
+}
+
+// inv[ids[i]] = i for a permutation ids of [0, n); an id outside marks bad
+__global__ void inverse_ids_kernel(const int32_t* __restrict__ ids, int64_t n, int32_t* __restrict__ inv,
+                                   uint32_t* __restrict__ bad) {
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) {
+    const int32_t id = ids[i];
+    if (id < 0 || id >= n)
+      atomicOr(bad, 1u);
+    else
+      inv[id] = (int32_t)i;
+  }
+}
+
+// Largest |difference| of positions and velocities between two particle sets
+// matched by id (ref slot inv[id]); NaN counts as infinite.  out holds the
+// float's bits (non-negative floats order as unsigned integers).
+__global__ __launch_bounds__(TPB) void state_diff_kernel(const float* __restrict__ pos, const float* __restrict__ vel,
+                                                         const int32_t* __restrict__ ids,
+                                                         const float* __restrict__ rpos,
+                                                         const float* __restrict__ rvel,
+                                                         const int32_t* __restrict__ inv, int64_t n,
+                                                         uint32_t* __restrict__ out) {
+  float d = 0.f;
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) {
+    const int32_t id = ids[i];
+    if (id < 0 || id >= n) {
+      d = INFINITY;
+      continue;
+    }
+    const int64_t j = inv[id];
+    if (j < 0) {  // an id the reference set does not hold
+      d = INFINITY;
+      continue;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float dp = fabsf(pos[3 * i + a] - rpos[3 * j + a]);
+      const float dv = fabsf(vel[3 * i + a] - rvel[3 * j + a]);
+      d = fmaxf(d, dp != dp ? INFINITY : dp);
+      d = fmaxf(d, dv != dv ? INFINITY : dv);
+    }
+  }
+  __shared__ float red[TPB / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) d = fmaxf(d, __shfl_down(d, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = d;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = 0.f;
+    for (int w = 0; w < TPB / 64; ++w) m = fmaxf(m, red[w]);
+    if (m > 0.f) atomicMax(out, __float_as_uint(m));
+  }
+}
+
 __global__ void mask_kernel(float* __restrict__ x, const uint8_t* __restrict__ liq, int n) {
   for (int i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB)
     if (!liq[i]) x[i] = 0.f;
@@ -3859,6 +3928,22 @@ void k_pcg_p(hipStream_t s, const float* z, float* p, int n, const double* sc, i
 void k_pcg_xr(hipStream_t s, float* x, float* r, const float* p, const float* q, const uint8_t* liquid, int n,
               double* sc) {
   pcg_xr_kernel<<<red_grid(n), TPB, 0, s>>>(x, r, p, q, liquid, n, sc);
+}
+
+void k_mask_hash(hipStream_t s, const uint8_t* m, uint32_t n, unsigned long long* out) {
+  k_fill_u32(s, reinterpret_cast<uint32_t*>(out), 2, 0u);
+  if (n > 0) mask_hash_kernel<<<grid_for(n) < 1024 ? grid_for(n) : 1024, TPB, 0, s>>>(m, n, out);
+}
+
+void k_state_diff(hipStream_t s, const float* pos, const float* vel, const int32_t* ids, const float* rpos,
+                  const float* rvel, const int32_t* rids, int64_t n, int32_t* inv, uint32_t* out) {
+  // out[0]: max difference (float bits), out[1]: an id outside [0, n)
+  k_fill_u32(s, out, 2, 0u);
+  if (n <= 0) return;
+  const unsigned grid = grid_for((size_t)n) < 2048 ? grid_for((size_t)n) : 2048;
+  k_fill_u32(s, reinterpret_cast<uint32_t*>(inv), (size_t)n, 0xFFFFFFFFu);
+  inverse_ids_kernel<<<grid, TPB, 0, s>>>(rids, n, inv, out + 1);
+  state_diff_kernel<<<grid, TPB, 0, s>>>(pos, vel, ids, rpos, rvel, inv, n, out);
 }
 
 void k_count_nonzero(hipStream_t s, const uint8_t* m, uint32_t n, uint32_t* out) {

@@ -60,6 +60,14 @@ void mem_add(uint64_t b) {
   }
 }
 void mem_sub(uint64_t b) { g_mem_cur.fetch_sub(b); }
+// cap on the device memory the library holds (aw_set_memory_limit), 0: none
+std::atomic<uint64_t> g_mem_limit{0};
+int over_limit(uint64_t want) {
+  const uint64_t lim = g_mem_limit.load(), cur = g_mem_cur.load();
+  if (lim == 0 || cur + want <= lim) return AW_OK;
+  return fail(AW_ERR_OOM, "allocation of %llu bytes would pass the memory limit (%llu of %llu bytes held)",
+              (unsigned long long)want, (unsigned long long)cur, (unsigned long long)lim);
+}
 std::mutex g_user_mu;
 std::unordered_map<void*, uint64_t> g_user_alloc;  // aw_malloc'd buffers and their sizes
 
@@ -118,6 +126,7 @@ struct DevBuf {
     const size_t want = cap ? bytes + bytes / 8 : bytes;
     p = nullptr;
     cap = 0;
+    AW_TRY(over_limit(want));
     if (hipMalloc(&p, want) != hipSuccess) {
       (void)hipGetLastError();
       return fail(AW_ERR_OOM, "hipMalloc of %zu bytes failed", want);
@@ -1355,7 +1364,9 @@ int solve_fp32(aw_solver* S, float* x, const float* b, double tol, int max_iters
     st->outer_iterations = 1;
     st->recurrence_rel_residual = 0.0;
   }
-  if (!(bn > 0.0)) {
+  // finiteness first: a NaN norm fails every comparison, the zero test too
+  if (!std::isfinite(bn) || !std::isfinite(rn)) return fail(AW_ERR_NAN, "non-finite right-hand side");
+  if (bn == 0.0) {
     k_fill_f32(s, x, n, 0.f);
     if (st) {
       st->iterations = 0;
@@ -1363,7 +1374,6 @@ int solve_fp32(aw_solver* S, float* x, const float* b, double tol, int max_iters
     }
     return launch_status("solve");
   }
-  if (!std::isfinite(bn) || !std::isfinite(rn)) return fail(AW_ERR_NAN, "non-finite right-hand side");
   if (S->prm.method == 1) {
     while (rn > tol * bn && it < max_iters) {
       k_fill_f32(s, z, n, 0.f);
@@ -1458,12 +1468,13 @@ int solve_mixed(aw_solver* S, double tol, int max_iters, aw_solve_stats* st) {
     st->rel_residual = 0.0;
     st->recurrence_rel_residual = 0.0;
   }
-  if (!(bn > 0.0)) {
+  // finiteness first: a NaN norm fails every comparison, the zero test too
+  if (!std::isfinite(bn)) return fail(AW_ERR_NAN, "non-finite right-hand side");
+  if (bn == 0.0) {
     k_fill_f32(s, g->p.as<float>(), n, 0.f);
     k_fill_f64(s, x64, n, 0.0);
     return launch_status("solve");
   }
-  if (!std::isfinite(bn)) return fail(AW_ERR_NAN, "non-finite right-hand side");
   int inner_total = 0, outer = 0;
   double rn = 0.0;
   for (;;) {
@@ -1670,6 +1681,11 @@ struct aw_tape {
   DevBuf wpos, wvel, wids;
   DevBuf radius;  // rebuild_topology 2: the tape's own copy of the radii (the recompute re-reads them)
   std::vector<std::vector<uint8_t>> mask_level, mask_liquid;
+  // replay check: every forward step's native levels and liquid-mask
+  // fingerprint; scratch of the fingerprint and of the state comparison
+  std::vector<std::vector<uint8_t>> fwd_level;
+  std::vector<unsigned long long> fwd_hash;
+  DevBuf hash, diff, inv;
 };
 
 namespace {
@@ -1699,6 +1715,17 @@ int tape_record(aw_tape* t, aw_tape::Step& r) {
   AW_TRY(copy_d2d(s, r.count, g->count.p, (size_t)g->nb * sizeof(uint32_t)));
   AW_TRY(copy_d2d(s, r.poff, g->poff.p, (size_t)(g->nb + 1) * sizeof(uint32_t)));
   return AW_OK;
+}
+
+// fingerprint of the grid's liquid mask after a step (synchronises)
+int liquid_fingerprint(aw_tape* t, unsigned long long* out) {
+  aw_grid* g = t->g;
+  hipStream_t s = g->ctx->stream;
+  AW_TRY(t->hash.ensure(sizeof(unsigned long long)));
+  k_mask_hash(s, g->liquid.as<uint8_t>(), g->ncells, t->hash.as<unsigned long long>());
+  AW_HIP(hipMemcpyAsync(out, t->hash.p, sizeof *out, hipMemcpyDeviceToHost, s));
+  AW_HIP(hipStreamSynchronize(s));
+  return launch_status("liquid fingerprint");
 }
 
 int tape_restore(aw_tape* t, const aw_tape::Step& r) {
@@ -1898,9 +1925,15 @@ int aw_memory_stats(uint64_t* current_bytes, uint64_t* peak_bytes, int reset_pea
   return AW_OK;
 }
 
+int aw_set_memory_limit(uint64_t bytes) {
+  g_mem_limit.store(bytes);
+  return AW_OK;
+}
+
 int aw_malloc(aw_ctx* c, uint64_t bytes, void** out) {
   if (!c || !out) return fail(AW_ERR_INVALID, "NULL argument");
   *out = nullptr;
+  AW_TRY(over_limit(bytes ? bytes : 1));
   AW_HIP(hipSetDevice(c->device));
   if (hipMalloc(out, bytes ? bytes : 1) != hipSuccess) {
     (void)hipGetLastError();
@@ -2447,6 +2480,8 @@ int aw_rollout_forward(aw_tape* t, float* d_pos, float* d_vel, int32_t* d_ids, c
   t->have_forward = false;
   t->mask_level.clear();
   t->mask_liquid.clear();
+  t->fwd_level.clear();
+  t->fwd_hash.clear();
   AW_TRY(validate_ids(g, d_ids, n));
   aw_rollout_stats rs{};
   AW_HIP(hipStreamSynchronize(s));
@@ -2465,6 +2500,10 @@ int aw_rollout_forward(aw_tape* t, float* d_pos, float* d_vel, int32_t* d_ids, c
     if (rc != AW_OK) worst = rc;
     rs.forward_iterations += ss.solve.iterations;
     rs.forward_rel_residual = std::max(rs.forward_rel_residual, ss.solve.rel_residual);
+    unsigned long long h = 0;
+    AW_TRY(liquid_fingerprint(t, &h));
+    t->fwd_level.push_back(g->h_level);
+    t->fwd_hash.push_back(h);
     if (t->keep_masks) {
       t->mask_level.push_back(g->h_level);
       std::vector<uint8_t> lq(g->ncells);
@@ -2494,6 +2533,10 @@ int aw_rollout_backward(aw_tape* t, float* d_xbar, float* d_vbar, aw_rollout_sta
   rs.recompute_ms = rs.backward_ms = 0.0;
   rs.adjoint_iterations = 0;
   rs.adjoint_rel_residual = 0.0;
+  rs.replay_mismatched_steps = 0;
+  rs.replay_max_state_diff = 0.0;
+  AW_TRY(t->diff.ensure(2 * sizeof(uint32_t)));
+  AW_TRY(t->inv.ensure((size_t)n * sizeof(int32_t)));
   int worst = AW_OK;
   const int nseg = (int)t->ck.size();
   for (int seg = nseg - 1; seg >= 0; --seg) {
@@ -2508,7 +2551,24 @@ int aw_rollout_backward(aw_tape* t, float* d_xbar, float* d_vbar, aw_rollout_sta
       aw_step_stats ss{};
       int rc = aw_step(g, t->S, t->wpos.as<float>(), t->wvel.as<float>(), t->wids.as<int32_t>(), n, &t->prm, &ss);
       if (rc != AW_OK && rc != AW_ERR_NOT_CONVERGED) return rc;
+      unsigned long long h = 0;
+      AW_TRY(liquid_fingerprint(t, &h));
+      if (g->h_level != t->fwd_level[k] || h != t->fwd_hash[k]) rs.replay_mismatched_steps++;
       if (k < k1 - 1) AW_TRY(tape_record(t, *t->rec[k - k0]));
+    }
+    if (seg + 1 < nseg) {
+      // the segment's end is the next checkpoint's state
+      const aw_tape::Ckpt& nx = *t->ck[seg + 1];
+      uint32_t hd[2] = {0u, 0u};
+      k_state_diff(s, t->wpos.as<float>(), t->wvel.as<float>(), t->wids.as<int32_t>(), nx.pos.as<float>(),
+                   nx.vel.as<float>(), nx.ids.as<int32_t>(), n, t->inv.as<int32_t>(), t->diff.as<uint32_t>());
+      AW_HIP(hipMemcpyAsync(hd, t->diff.p, sizeof hd, hipMemcpyDeviceToHost, s));
+      AW_HIP(hipStreamSynchronize(s));
+      AW_TRY(launch_status("replay check"));
+      float d;
+      std::memcpy(&d, &hd[0], sizeof d);
+      if (hd[1]) d = INFINITY;
+      rs.replay_max_state_diff = std::max(rs.replay_max_state_diff, (double)d);
     }
     AW_HIP(hipStreamSynchronize(s));
     double tb = now_ms();

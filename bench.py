@@ -332,6 +332,8 @@ def run_adjoint(args, name=None, steps=None, scene=None):
                "ratio": round((st.recompute_ms + st.backward_ms) / fwd, 3),
                "forward_iterations": st.forward_iterations, "adjoint_iterations": st.adjoint_iterations,
                "worst_adjoint_rel_residual": st.adjoint_rel_residual,
+               "replay_mismatched_steps": st.replay_mismatched_steps,
+               "replay_max_state_diff": st.replay_max_state_diff,
                "grad_norm": float(np.linalg.norm(g)), "loss": float(c @ c),
                "peak_hbm_gib": round(aw.memory_stats()[1] / 2**30, 3)}
         log(f"[bench] adjoint pass {rep}: {res}")

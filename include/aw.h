@@ -218,6 +218,11 @@ int aw_fill_f32(aw_ctx* ctx, float* d, uint64_t n, float value);
  * mark since load or the last reset (no reference counterpart).  reset_peak
  * sets the mark to the current figure after reading it. */
 int aw_memory_stats(uint64_t* current_bytes, uint64_t* peak_bytes, int reset_peak);
+/* Cap on that memory (no reference counterpart; 0 = none, the default): an
+ * allocation that would take the library past it fails with AW_ERR_OOM, as
+ * a failed hipMalloc does, and the object it was for stays destroyable.  For
+ * a caller that shares the GPU under a budget. */
+int aw_set_memory_limit(uint64_t bytes);
 /* Kernel timing with HIP events on the context's stream (no reference
  * counterpart).  on = 1: an event pair around every launch; on = 2: one pair
  * around each run of consecutive launches of one kernel group (no events
@@ -366,6 +371,16 @@ typedef struct {
   int adjoint_iterations;   /* adjoint solver iterations summed over the steps */
   double forward_rel_residual;  /* worst forward solve */
   double adjoint_rel_residual;  /* worst adjoint solve */
+  /* Replay check of aw_rollout_backward's recomputation against the forward
+   * pass (the gradient follows the recomputed trajectory):
+   * replay_mismatched_steps: recomputed steps whose native levels or liquid
+   * mask differ from the forward step's; replay_max_state_diff: largest
+   * |position| or |velocity| difference, particle by particle id, between a
+   * recomputed segment's end and the checkpoint the forward pass stored
+   * there.  0 and 0.0 when the replay is bit-identical; the particle sort and
+   * the P2G deposit add floats in arrival order, so round-off can remain. */
+  int replay_mismatched_steps;
+  double replay_max_state_diff;
 } aw_rollout_stats;
 
 int aw_tape_create(aw_grid* grid, aw_solver* s, int64_t n, int num_steps, int checkpoint_every, int keep_masks,

@@ -162,6 +162,13 @@ def test_rollout_gradient_matches_oracle_and_central_differences(ctx):
     ratio = (st.recompute_ms + st.backward_ms) / st.forward_ms
     print(f"[adjoint] forward {st.forward_ms:.1f} ms, recompute {st.recompute_ms:.1f} ms, backward "
           f"{st.backward_ms:.1f} ms, adjoint/forward {ratio:.2f}", flush=True)
+    # the recomputation replays the forward pass: same levels and liquid masks
+    # every step, and each segment ends on the checkpoint stored there up to
+    # the round-off of arrival-order float sums (positions and velocities)
+    print(f"[adjoint] replay: {st.replay_mismatched_steps} mismatched steps, max state difference "
+          f"{st.replay_max_state_diff:.1e}", flush=True)
+    assert st.replay_mismatched_steps == 0
+    assert st.replay_max_state_diff < 1e-4 * max(np.abs(vel).max(), 1.0)
     masks = [ro.masks(k) for k in range(steps)]
 
     og = O.OracleGrid((n_cells // 8,) * 3, num_levels=levels, h=1.0 / n_cells, band=band)

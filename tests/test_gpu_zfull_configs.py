@@ -205,7 +205,8 @@ def test_slosh128_rollout_gradient(ctx):
     g_gpu = dvb.download(np.float32, 3 * n).reshape(-1, 3).astype(np.float64)
     ratio = (st.recompute_ms + st.backward_ms) / st.forward_ms
     log(f"[configs] slosh128 GPU: forward {st.forward_ms:.1f} ms, recompute {st.recompute_ms:.1f} ms, backward "
-        f"{st.backward_ms:.1f} ms, adjoint/forward {ratio:.2f}")
+        f"{st.backward_ms:.1f} ms, adjoint/forward {ratio:.2f}; replay: {st.replay_mismatched_steps} mismatched "
+        f"steps, max state difference {st.replay_max_state_diff:.1e}")
     masks = [ro.masks(k) for k in range(steps)]
     og = O.OracleGrid(w["dims"], num_levels=w["levels"], h=1.0 / w["n"], band=w["band"])
     og.set_params()
