@@ -8,6 +8,15 @@
 
 namespace aw {
 
+// Events for the kernel launches that follow (the per-launch profiler):
+// start recorded by the next launch's dispatch, stop by every launch's until
+// taken back.
+struct LaunchEvents {
+  hipEvent_t start, stop;
+};
+void set_launch_events(hipEvent_t start, hipEvent_t stop);
+LaunchEvents take_launch_events();
+
 constexpr int BASE = 8;        // cells per block edge at native level 0
 constexpr int LOGB = 3;
 constexpr int PAD = 900;       // padded fine nodes per axis per block (9 x 10 x 10)

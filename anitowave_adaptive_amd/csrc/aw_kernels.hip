@@ -8,10 +8,40 @@
 //
 // All kernels are HBM-bound gathers/scatters and stencils; none uses MFMA.
 
+#include <hip/hip_ext.h>
+
 #include "aw_internal.h"
 
 namespace aw {
+
+// Timing events for the launches that follow (the per-launch profiler): the
+// dispatch packets themselves carry the timestamps (hipExtLaunchKernel), so
+// timing puts no marker packet, and no wait, between two kernels.
 namespace {
+thread_local LaunchEvents g_launch_events;
+}
+void set_launch_events(hipEvent_t start, hipEvent_t stop) { g_launch_events = {start, stop}; }
+LaunchEvents take_launch_events() {
+  const LaunchEvents e = g_launch_events;
+  g_launch_events = {nullptr, nullptr};
+  return e;
+}
+
+namespace {
+
+// Every kernel launch: plain, or with the profiler's events (the start event
+// on the first launch after they are set, the stop event on every launch, so
+// a run of launches spans the first start to the last end).
+template <typename K, typename... A>
+void launch(K kernel, dim3 grid, dim3 block, size_t shmem, hipStream_t s, A... args) {
+  if (g_launch_events.stop) {
+    hipExtLaunchKernelGGL(kernel, grid, block, (uint32_t)shmem, s, g_launch_events.start, g_launch_events.stop, 0,
+                          args...);
+    g_launch_events.start = nullptr;
+  } else {
+    kernel<<<grid, block, shmem, s>>>(args...);
+  }
+}
 
 constexpr int TPB = 256;
 
@@ -3594,18 +3624,18 @@ inline int red_grid(int n) {
 
 // ------------------------------------------------------------- launchers
 void k_fill_f32(hipStream_t s, float* p, size_t n, float v) {
-  if (n) fill_kernel<float><<<grid_for(n), TPB, 0, s>>>(p, n, v);
+  if (n) launch(fill_kernel<float>, grid_for(n), TPB, 0, s, p, n, v);
 }
 void k_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v) {
-  if (n) fill_kernel<uint32_t><<<grid_for(n), TPB, 0, s>>>(p, n, v);
+  if (n) launch(fill_kernel<uint32_t>, grid_for(n), TPB, 0, s, p, n, v);
 }
 void k_fill_f64(hipStream_t s, double* p, size_t n, double v) {
-  if (n) fill_kernel<double><<<grid_for(n), TPB, 0, s>>>(p, n, v);
+  if (n) launch(fill_kernel<double>, grid_for(n), TPB, 0, s, p, n, v);
 }
 
 void k_bin(hipStream_t s, const GridDev& g, const float* pos, int64_t n, int32_t* key, uint32_t* count) {
   if (n > 0)
-    bin_kernel<<<(unsigned)((n + TPB * AGG_PPT - 1) / (TPB * AGG_PPT)), TPB, 0, s>>>(g, pos, n, key, count);
+    launch(bin_kernel, (unsigned)((n + TPB * AGG_PPT - 1) / (TPB * AGG_PPT)), TPB, 0, s, g, pos, n, key, count);
 }
 
 // 2:1 grading to its fixed point: level l_b ends as min over blocks q of
@@ -3615,7 +3645,7 @@ static void grade_passes(hipStream_t s, const int* D, int L, uint8_t* level, uin
   const int nb = D[0] * D[1] * D[2];
   const int gr = (nb + TPB - 1) / TPB;
   for (int it = 0; it < L - 1; ++it) {
-    grade_kernel<<<gr, TPB, 0, s>>>(D[0], D[1], D[2], level, tmp);
+    launch(grade_kernel, gr, TPB, 0, s, D[0], D[1], D[2], level, tmp);
     hipMemcpyAsync(level, tmp, nb, hipMemcpyDeviceToDevice, s);
   }
 }
@@ -3624,22 +3654,21 @@ void k_levels(hipStream_t s, const int* D, int L, const int* band, const uint32_
               uint8_t* level, uint8_t* tmp) {
   const int nb = D[0] * D[1] * D[2];
   const int gr = (nb + TPB - 1) / TPB;
-  surface_kernel<<<gr, TPB, 0, s>>>(D[0], D[1], D[2], count, surf);
+  launch(surface_kernel, gr, TPB, 0, s, D[0], D[1], D[2], count, surf);
   Bands bd;
   int bmax = 0;
   for (int i = 0; i < 4; ++i) bd.v[i] = band[i];
   for (int i = 0; i < L - 1; ++i) bmax = band[i] > bmax ? band[i] : bmax;
   // z, y, x passes: surf -> tmp -> level -> tmp (levels), then back into level
-  cheb_pass_kernel<<<gr, TPB, 0, s>>>(D[0], D[1], D[2], 2, bmax, 1, 0, bd, surf, tmp);
-  cheb_pass_kernel<<<gr, TPB, 0, s>>>(D[0], D[1], D[2], 1, bmax, 0, 0, bd, tmp, level);
-  cheb_pass_kernel<<<gr, TPB, 0, s>>>(D[0], D[1], D[2], 0, bmax, 0, L, bd, level, tmp);
+  launch(cheb_pass_kernel, gr, TPB, 0, s, D[0], D[1], D[2], 2, bmax, 1, 0, bd, surf, tmp);
+  launch(cheb_pass_kernel, gr, TPB, 0, s, D[0], D[1], D[2], 1, bmax, 0, 0, bd, tmp, level);
+  launch(cheb_pass_kernel, gr, TPB, 0, s, D[0], D[1], D[2], 0, bmax, 0, L, bd, level, tmp);
   hipMemcpyAsync(level, tmp, nb, hipMemcpyDeviceToDevice, s);
   grade_passes(s, D, L, level, tmp);
 }
 
 void k_resample(hipStream_t s, const GridDev& g, const float* field, const int* o, float* out) {
-  resample_kernel<<<dim3((o[0] + RS_X * RS_V - 1) / (RS_X * RS_V), (o[1] + RS_Y - 1) / RS_Y, o[2]), RS_X * RS_Y, 0,
-                    s>>>(g, field, o[0], o[1], o[2], out);
+  launch(resample_kernel, dim3((o[0] + RS_X * RS_V - 1) / (RS_X * RS_V), (o[1] + RS_Y - 1) / RS_Y, o[2]), RS_X * RS_Y, 0, s, g, field, o[0], o[1], o[2], out);
 }
 
 void k_radius_levels(hipStream_t s, const GridDev& g, int logh, int L, const float* pos, const float* rad,
@@ -3650,9 +3679,9 @@ void k_radius_levels(hipStream_t s, const GridDev& g, int logh, int L, const flo
   k_fill_u32(s, bad, 1, 0u);
   const int smax = max(g.D[0], max(g.D[1], g.D[2]));
   if (n > 0)
-    radius_mark_kernel<<<(unsigned)((n + TPB * RM_PPT - 1) / (TPB * RM_PPT)), TPB, 0, s>>>(g, logh, L, smax, pos, rad,
+    launch(radius_mark_kernel, (unsigned)((n + TPB * RM_PPT - 1) / (TPB * RM_PPT)), TPB, 0, s, g, logh, L, smax, pos, rad,
                                                                                          ids, n, direct, own, bad);
-  radius_dilate_kernel<<<(nb + TPB - 1) / TPB, TPB, 0, s>>>(g.D[0], g.D[1], g.D[2], direct, own, level);
+  launch(radius_dilate_kernel, (nb + TPB - 1) / TPB, TPB, 0, s, g.D[0], g.D[1], g.D[2], direct, own, level);
   grade_passes(s, g.D, L, level, tmp);
 }
 
@@ -3664,15 +3693,15 @@ size_t scan_scratch_words(int n) { return 2 * (size_t)(n / SCAN_CH + 2); }
 static void scan_any(hipStream_t s, const uint32_t* in, const uint8_t* lv, int mode, uint32_t* out, int n,
                      uint32_t* scratch) {
   if (n <= SCAN_CH) {
-    scan_kernel<<<1, SCAN_T, 0, s>>>(in, lv, mode, out, n);
+    launch(scan_kernel, 1, SCAN_T, 0, s, in, lv, mode, out, n);
     return;
   }
   const int nch = (n + SCAN_CH - 1) / SCAN_CH;
   uint32_t* sums = scratch;
   uint32_t* cbase = scratch + nch;
-  scan_reduce_kernel<<<nch, TPB, 0, s>>>(in, lv, mode, n, sums);
-  scan_kernel<<<1, SCAN_T, 0, s>>>(sums, nullptr, 0, cbase, nch);
-  scan_final_kernel<<<nch, TPB, 0, s>>>(in, lv, mode, n, cbase, nch, out);
+  launch(scan_reduce_kernel, nch, TPB, 0, s, in, lv, mode, n, sums);
+  launch(scan_kernel, 1, SCAN_T, 0, s, sums, nullptr, 0, cbase, nch);
+  launch(scan_final_kernel, nch, TPB, 0, s, in, lv, mode, n, cbase, nch, out);
 }
 
 void k_scan_u32(hipStream_t s, const uint32_t* in, uint32_t* out, int n, uint32_t* scratch) {
@@ -3694,67 +3723,67 @@ void k_layout_offsets(hipStream_t s, const uint8_t* level, int nb, uint32_t* cof
 void k_scatter(hipStream_t s, const float* pos, const float* vel, const int32_t* ids, const int32_t* key,
                uint32_t* cursor, int64_t n, float* spos, float* svel, int32_t* sids, const float* dens, float* sdens) {
   if (n > 0)
-    scatter_kernel<<<(unsigned)((n + TPB * AGG_PPT - 1) / (TPB * AGG_PPT)), TPB, 0, s>>>(
+    launch(scatter_kernel, (unsigned)((n + TPB * AGG_PPT - 1) / (TPB * AGG_PPT)), TPB, 0, s, 
         pos, vel, ids, key, cursor, n, spos, svel, sids, dens, sdens);
 }
 
 void k_init_beta(hipStream_t s, const LevelDev& L, float* bx, float* by, float* bz) {
-  init_beta_kernel<<<(L.nb + TPB / 64 - 1) / (TPB / 64), TPB, 0, s>>>(L, bx, by, bz);
+  launch(init_beta_kernel, (L.nb + TPB / 64 - 1) / (TPB / 64), TPB, 0, s, L, bx, by, bz);
 }
 
 void k_p2g_deposit(hipStream_t s, const GridDev& g, const float* spos, const float* svel, const uint32_t* pcount,
                    const uint32_t* poff, float* slab, uint8_t* liquid, const float* sdens) {
   if (sdens)
-    p2g_deposit_kernel<true><<<g.nb, DEP_T, 0, s>>>(g, spos, sdens, pcount, poff, slab, liquid);
+    launch(p2g_deposit_kernel<true>, g.nb, DEP_T, 0, s, g, spos, sdens, pcount, poff, slab, liquid);
   else
-    p2g_deposit_kernel<false><<<g.nb, DEP_T, 0, s>>>(g, spos, svel, pcount, poff, slab, liquid);
+    launch(p2g_deposit_kernel<false>, g.nb, DEP_T, 0, s, g, spos, svel, pcount, poff, slab, liquid);
 }
 
 void k_p2g_gather(hipStream_t s, const GridDev& g, const uint32_t* pcount, const float* slab, float* const* Macc,
                   float* const* macc) {
-  p2g_gather_kernel<<<g.nb, TPB, 0, s>>>(g, pcount, slab, Macc[0], Macc[1], Macc[2], macc[0], macc[1], macc[2]);
+  launch(p2g_gather_kernel, g.nb, TPB, 0, s, g, pcount, slab, Macc[0], Macc[1], Macc[2], macc[0], macc[1], macc[2]);
 }
 
 static FacePtrs fp(float* const* p) { return FacePtrs{{p[0], p[1], p[2]}}; }
 
 void k_fold(hipStream_t s, const GridDev& g, float* const* Macc, float* const* macc, float* const* uold,
             float* const* ustar, float* const* mass, float dt, const float* grav) {
-  fold_kernel<<<g.nb, TPB, 0, s>>>(g, fp(Macc), fp(macc), fp(uold), fp(ustar), fp(mass), dt, grav[0], grav[1],
+  launch(fold_kernel, g.nb, TPB, 0, s, g, fp(Macc), fp(macc), fp(uold), fp(ustar), fp(mass), dt, grav[0], grav[1],
                                     grav[2]);
 }
 
 void k_beta_fold(hipStream_t s, const GridDev& g, float* const* macc, float* const* Dacc, float* const* beta) {
-  beta_fold_kernel<<<g.nb, TPB, 0, s>>>(g, fp(macc), fp(Dacc), fp(beta));
+  launch(beta_fold_kernel, g.nb, TPB, 0, s, g, fp(macc), fp(Dacc), fp(beta));
 }
 
 void k_divergence(hipStream_t s, const GridDev& g, const uint8_t* liquid, float* const* ustar, float* rhs) {
-  divergence_kernel<<<g.nb, TPB, 0, s>>>(g, liquid, fp(ustar), rhs);
+  launch(divergence_kernel, g.nb, TPB, 0, s, g, liquid, fp(ustar), rhs);
 }
 
 void k_project(hipStream_t s, const GridDev& g, const float* p, float* const* ustar, float* const* beta,
                float* const* unew) {
-  project_kernel<<<g.nb, TPB, 0, s>>>(g, p, fp(ustar), fp(beta), fp(unew));
+  launch(project_kernel, g.nb, TPB, 0, s, g, p, fp(ustar), fp(beta), fp(unew));
 }
 
 void k_pressure_remap(hipStream_t s, const GridDev& g, const uint8_t* level_prev, const uint32_t* coff_prev,
                       const float* p_prev, float* p) {
-  pressure_remap_kernel<<<g.nb, TPB, 0, s>>>(g, level_prev, coff_prev, p_prev, p);
+  launch(pressure_remap_kernel, g.nb, TPB, 0, s, g, level_prev, coff_prev, p_prev, p);
 }
 
 void k_to_fine(hipStream_t s, const GridDev& g, float* fine, const uint32_t* count, uint8_t* halo) {
-  halo_flags_kernel<<<(g.nb + TPB - 1) / TPB, TPB, 0, s>>>(g, count, halo);
-  to_fine_kernel<<<g.nb, TPB, 0, s>>>(g, fine);
+  launch(halo_flags_kernel, (g.nb + TPB - 1) / TPB, TPB, 0, s, g, count, halo);
+  launch(to_fine_kernel, g.nb, TPB, 0, s, g, fine);
 }
 
 template <int MODE>
 static void g2p_launch(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const float* svel,
                        int64_t n, float alpha, float dt, float* pos_out, float* vel_out, uint32_t* list) {
-  fill_kernel<uint32_t><<<1, 64, 0, s>>>(list, 1, 0u);
-  g2p_advect_kernel<false, MODE><<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, s>>>(g, fine, spos, svel, n, alpha, dt,
+  launch(fill_kernel<uint32_t>, 1, 64, 0, s, list, 1, 0u);
+  launch(g2p_advect_kernel<false, MODE>, (unsigned)((n + TPB - 1) / TPB), TPB, 0, s, g, fine, spos, svel, n, alpha, dt,
                                                                                   pos_out, vel_out, list);
   const int64_t nwg = (n + TPB - 1) / TPB;
   const unsigned fix = (unsigned)(nwg < 1024 ? nwg : 1024);
-  g2p_advect_kernel<true, MODE><<<fix, TPB, 0, s>>>(g, fine, spos, svel, n, alpha, dt, pos_out, vel_out, list);
+  launch(g2p_advect_kernel<true, MODE>, fix, TPB, 0, s, g, fine, spos, svel, n, alpha, dt, pos_out, vel_out, list);
 }
 
 void k_g2p_advect(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const float* svel,
@@ -3774,79 +3803,79 @@ void k_g2p_adjoint_stages(hipStream_t s, const GridDev& g, const float* fine, co
                           int64_t n, float alpha, float dt, float* xbar, float* vbar, void* rec, uint32_t* list) {
   if (n <= 0) return;
   AdjRec* r = static_cast<AdjRec*>(rec);
-  fill_kernel<uint32_t><<<1, 64, 0, s>>>(list, 1, 0u);
+  launch(fill_kernel<uint32_t>, 1, 64, 0, s, list, 1, 0u);
   const int64_t nwg = (n + TPB - 1) / TPB;
-  g2p_adjoint_stage_kernel<false><<<(unsigned)nwg, TPB, 0, s>>>(g, fine, spos, sids, n, alpha, dt, xbar, vbar, r, list);
+  launch(g2p_adjoint_stage_kernel<false>, (unsigned)nwg, TPB, 0, s, g, fine, spos, sids, n, alpha, dt, xbar, vbar, r, list);
   const unsigned fix = (unsigned)(nwg < 1024 ? nwg : 1024);
-  g2p_adjoint_stage_kernel<true><<<fix, TPB, 0, s>>>(g, fine, spos, sids, n, alpha, dt, xbar, vbar, r, list);
+  launch(g2p_adjoint_stage_kernel<true>, fix, TPB, 0, s, g, fine, spos, sids, n, alpha, dt, xbar, vbar, r, list);
 }
 
 void k_g2p_adjoint_scatter(hipStream_t s, const GridDev& g, const float* spos, const uint32_t* pcount,
                            const uint32_t* poff, const void* rec, float* fbar) {
-  g2p_adjoint_scatter_kernel<<<g.nb, ADJ_T, 0, s>>>(g, spos, pcount, poff, static_cast<const AdjRec*>(rec), fbar);
+  launch(g2p_adjoint_scatter_kernel, g.nb, ADJ_T, 0, s, g, spos, pcount, poff, static_cast<const AdjRec*>(rec), fbar);
 }
 void k_pi_transpose(hipStream_t s, const GridDev& g, const float* fbar, float* const* unb, float* const* uob) {
-  pi_transpose_kernel<<<g.nb, TPB, 0, s>>>(g, fbar, fp(unb), fp(uob));
+  launch(pi_transpose_kernel, g.nb, TPB, 0, s, g, fbar, fp(unb), fp(uob));
 }
 void k_project_adjoint(hipStream_t s, const GridDev& g, float* const* beta, float* const* unb, float* const* usb,
                        float* pbar) {
-  project_adjoint_kernel<<<g.nb, TPB, 0, s>>>(g, fp(beta), fp(unb), fp(usb), pbar);
+  launch(project_adjoint_kernel, g.nb, TPB, 0, s, g, fp(beta), fp(unb), fp(usb), pbar);
 }
 void k_divergence_adjoint(hipStream_t s, const GridDev& g, const uint8_t* liquid, const float* rhsbar,
                           float* const* usb) {
-  divergence_adjoint_kernel<<<g.nb, TPB, 0, s>>>(g, liquid, rhsbar, fp(usb));
+  launch(divergence_adjoint_kernel, g.nb, TPB, 0, s, g, liquid, rhsbar, fp(usb));
 }
 void k_fold_adjoint(hipStream_t s, const GridDev& g, float* const* uold, float* const* mass, float* const* uob,
                     float* const* usb, float* const* Mb, float* const* mb) {
-  fold_adjoint_kernel<<<g.nb, TPB, 0, s>>>(g, fp(uold), fp(mass), fp(uob), fp(usb), fp(Mb), fp(mb));
+  launch(fold_adjoint_kernel, g.nb, TPB, 0, s, g, fp(uold), fp(mass), fp(uob), fp(usb), fp(Mb), fp(mb));
 }
 void k_pi_forward2(hipStream_t s, const GridDev& g, float* const* A, float* const* Bf, float* out) {
-  pi_forward2_kernel<<<g.nb, TPB, 0, s>>>(g, fp(A), fp(Bf), out);
+  launch(pi_forward2_kernel, g.nb, TPB, 0, s, g, fp(A), fp(Bf), out);
 }
 void k_p2g_adjoint(hipStream_t s, const GridDev& g, const float* fm, const float* spos, const float* svel,
                    const int32_t* sids, int64_t n, float* xbar, float* vbar) {
   if (n > 0)
-    p2g_adjoint_kernel<<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, s>>>(g, fm, spos, svel, sids, n, xbar, vbar);
+    launch(p2g_adjoint_kernel, (unsigned)((n + TPB - 1) / TPB), TPB, 0, s, g, fm, spos, svel, sids, n, xbar, vbar);
 }
 void k_ids_range(hipStream_t s, const int32_t* ids, int64_t n, int* out) {
   int64_t blocks = (n + TPB - 1) / TPB;
   if (blocks > 2048) blocks = 2048;
-  if (n > 0) ids_range_kernel<<<(unsigned)blocks, TPB, 0, s>>>(ids, n, out);
+  if (n > 0) launch(ids_range_kernel, (unsigned)blocks, TPB, 0, s, ids, n, out);
 }
 void k_masked_copy(hipStream_t s, float* dst, const float* src, const uint8_t* liquid, int n) {
-  if (n > 0) masked_copy_kernel<<<(n + TPB - 1) / TPB, TPB, 0, s>>>(dst, src, liquid, n);
+  if (n > 0) launch(masked_copy_kernel, (n + TPB - 1) / TPB, TPB, 0, s, dst, src, liquid, n);
 }
 
 void k_smooth(hipStream_t s, const LevelDev& L, float* x, const float* b, const WGMap& wg,
               const uint32_t* blist, float omega, int local_sweeps, int reverse) {
-  if (wg.n > 0) smooth_kernel<<<wg.n, ST_T, 0, s>>>(L, x, b, wg, blist, omega, local_sweeps, reverse);
+  if (wg.n > 0) launch(smooth_kernel, wg.n, ST_T, 0, s, L, x, b, wg, blist, omega, local_sweeps, reverse);
 }
 
 void k_block_live(hipStream_t s, const LevelDev& L, uint8_t* live) {
   const int per = LIVE_T / 64;
-  if (L.nb > 0) block_live_kernel<<<(L.nb + per - 1) / per, LIVE_T, 0, s>>>(L, live);
+  if (L.nb > 0) launch(block_live_kernel, (L.nb + per - 1) / per, LIVE_T, 0, s, L, live);
 }
 
 void k_gather_bd(hipStream_t s, const uint32_t* bd, const uint32_t* idx, int n, uint32_t* out) {
   const int words = n * BD_WORDS;
-  if (n > 0) gather_bd_kernel<<<(words + TPB - 1) / TPB, TPB, 0, s>>>(bd, idx, n, out);
+  if (n > 0) launch(gather_bd_kernel, (words + TPB - 1) / TPB, TPB, 0, s, bd, idx, n, out);
 }
 
 void k_smooth_adaptive(hipStream_t s, const LevelDev& L, float* x, const float* b, const WGMap& wg, float omega,
                        int reverse, const uint8_t* act, uint8_t* act_next, uint8_t* sweeps, const float* bmax,
                        float theta, int max_sweeps, float reduction) {
   if (wg.n > 0)
-    adaptive_smooth_kernel<<<wg.n, ST_T, 0, s>>>(L, x, b, wg, omega, reverse, act, act_next, sweeps, bmax, theta,
+    launch(adaptive_smooth_kernel, wg.n, ST_T, 0, s, L, x, b, wg, omega, reverse, act, act_next, sweeps, bmax, theta,
                                                  max_sweeps, reduction);
 }
 
 void k_absmax(hipStream_t s, const float* v, const uint8_t* liquid, int n, float* out) {
   k_fill_u32(s, reinterpret_cast<uint32_t*>(out), 1, 0u);
-  if (n > 0) absmax_kernel<<<grid_for(n) < 1024 ? grid_for(n) : 1024, TPB, 0, s>>>(v, liquid, n, reinterpret_cast<uint32_t*>(out));
+  if (n > 0) launch(absmax_kernel, grid_for(n) < 1024 ? grid_for(n) : 1024, TPB, 0, s, v, liquid, n, reinterpret_cast<uint32_t*>(out));
 }
 
 void k_fill_u8(hipStream_t s, uint8_t* p, size_t n, uint8_t v) {
-  if (n) fill_kernel<uint8_t><<<grid_for(n), TPB, 0, s>>>(p, n, v);
+  if (n) launch(fill_kernel<uint8_t>, grid_for(n), TPB, 0, s, p, n, v);
 }
 
 void k_smooth_uniform(hipStream_t s, const int* sh, int lr, int e, const uint8_t* liquid, float* x, const float* b,
@@ -3859,80 +3888,80 @@ void k_smooth_uniform(hipStream_t s, const int* sh, int lr, int e, const uint8_t
   const int per = 512 >> (3 * lr);
   const int nwg = (ncol + per - 1) / per;
   if (nwg > 0)
-    uniform_smooth_kernel<<<nwg, ST_T, 0, s>>>(U, liquid, x, b, col, omega, local_sweeps, reverse);
+    launch(uniform_smooth_kernel, nwg, ST_T, 0, s, U, liquid, x, b, col, omega, local_sweeps, reverse);
 }
 
 void k_residual(hipStream_t s, const LevelDev& L, const float* x, const float* b, float* r, const WGMap& wg,
                 const uint32_t* blist, const Restr& rs) {
-  if (wg.n > 0) residual_kernel<<<wg.n, ST_T, 0, s>>>(L, x, b, r, wg, blist, rs);
+  if (wg.n > 0) launch(residual_kernel, wg.n, ST_T, 0, s, L, x, b, r, wg, blist, rs);
 }
 
 void k_apply_dot(hipStream_t s, const LevelDev& L, const float* x, float* y, const WGMap& wg,
                  const uint32_t* blist, double* acc) {
-  if (wg.n > 0) apply_dot_kernel<<<wg.n, ST_T, 0, s>>>(L, x, y, wg, blist, acc);
+  if (wg.n > 0) launch(apply_dot_kernel, wg.n, ST_T, 0, s, L, x, y, wg, blist, acc);
 }
 
 void k_restrict(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, const float* rf, float* bc,
                 float* xc, const WGMap& wgc, const uint32_t* blistc) {
-  if (wgc.n > 0) restrict_kernel<<<wgc.n, TPB, 0, s>>>(F, C, aggregated, rf, bc, xc, wgc, blistc);
+  if (wgc.n > 0) launch(restrict_kernel, wgc.n, TPB, 0, s, F, C, aggregated, rf, bc, xc, wgc, blistc);
 }
 
 void k_prolong(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, float* xf, const float* xc,
                float scale, const WGMap& wgf, const uint32_t* blistf) {
-  if (wgf.n > 0) prolong_kernel<<<wgf.n, ST_T, 0, s>>>(F, C, aggregated, xf, xc, scale, wgf, blistf);
+  if (wgf.n > 0) launch(prolong_kernel, wgf.n, ST_T, 0, s, F, C, aggregated, xf, xc, scale, wgf, blistf);
 }
 
 void k_coarsen_liquid(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, uint8_t* liquid_c,
                       const WGMap& wgc, const uint32_t* blistc) {
-  if (wgc.n > 0) coarsen_liquid_kernel<<<wgc.n, TPB, 0, s>>>(F, C, aggregated, liquid_c, wgc, blistc);
+  if (wgc.n > 0) launch(coarsen_liquid_kernel, wgc.n, TPB, 0, s, F, C, aggregated, liquid_c, wgc, blistc);
 }
 
 void k_coarsen_beta(hipStream_t s, const LevelDev& F, const LevelDev& C, int aggregated, float* bx, float* by,
                     float* bz) {
-  coarsen_beta_kernel<<<C.nb, TPB, 0, s>>>(F, C, aggregated, bx, by, bz);
+  launch(coarsen_beta_kernel, C.nb, TPB, 0, s, F, C, aggregated, bx, by, bz);
 }
 
 void k_tail(hipStream_t s, const TailArgs& args, size_t lds_bytes) {
-  tail_kernel<<<1, TAIL_T, lds_bytes, s>>>(args);
+  launch(tail_kernel, 1, TAIL_T, lds_bytes, s, args);
 }
 
 void k_coarse_factor(hipStream_t s, const LevelDev& L, int n, double* inv) {
-  coarse_factor_kernel<<<1, TPB, 0, s>>>(L, n, inv);
+  launch(coarse_factor_kernel, 1, TPB, 0, s, L, n, inv);
 }
 
 void k_residual64(hipStream_t s, const LevelDev& L, const double* x, const float* b, float* r32, const WGMap& wg,
                   const uint32_t* blist, double* rr) {
-  if (wg.n > 0) residual64_kernel<<<wg.n, TPB, 0, s>>>(L, x, b, r32, wg, blist, rr);
+  if (wg.n > 0) launch(residual64_kernel, wg.n, TPB, 0, s, L, x, b, r32, wg, blist, rr);
 }
 
 void k_axpy64(hipStream_t s, double* x, const float* e, const uint8_t* liquid, int n) {
-  axpy64_kernel<<<red_grid(n), TPB, 0, s>>>(x, e, liquid, n);
+  launch(axpy64_kernel, red_grid(n), TPB, 0, s, x, e, liquid, n);
 }
 
 void k_to64(hipStream_t s, double* x, const float* p, const uint8_t* liquid, int n) {
-  to64_kernel<<<red_grid(n), TPB, 0, s>>>(x, p, liquid, n);
+  launch(to64_kernel, red_grid(n), TPB, 0, s, x, p, liquid, n);
 }
 
-void k_to32(hipStream_t s, float* p, const double* x, int n) { to32_kernel<<<red_grid(n), TPB, 0, s>>>(p, x, n); }
+void k_to32(hipStream_t s, float* p, const double* x, int n) { launch(to32_kernel, red_grid(n), TPB, 0, s, p, x, n); }
 
 void k_dot(hipStream_t s, const float* a, const float* b, const uint8_t* liquid, int n, double* acc) {
-  dot_kernel<<<red_grid(n), TPB, 0, s>>>(a, b, liquid, n, acc);
+  launch(dot_kernel, red_grid(n), TPB, 0, s, a, b, liquid, n, acc);
 }
 
-void k_scalar_shift(hipStream_t s, double* sc) { scalar_shift_kernel<<<1, 64, 0, s>>>(sc); }
+void k_scalar_shift(hipStream_t s, double* sc) { launch(scalar_shift_kernel, 1, 64, 0, s, sc); }
 
 void k_pcg_p(hipStream_t s, const float* z, float* p, int n, const double* sc, int first) {
-  pcg_p_kernel<<<red_grid(n), TPB, 0, s>>>(z, p, n, sc, first);
+  launch(pcg_p_kernel, red_grid(n), TPB, 0, s, z, p, n, sc, first);
 }
 
 void k_pcg_xr(hipStream_t s, float* x, float* r, const float* p, const float* q, const uint8_t* liquid, int n,
               double* sc) {
-  pcg_xr_kernel<<<red_grid(n), TPB, 0, s>>>(x, r, p, q, liquid, n, sc);
+  launch(pcg_xr_kernel, red_grid(n), TPB, 0, s, x, r, p, q, liquid, n, sc);
 }
 
 void k_mask_hash(hipStream_t s, const uint8_t* m, uint32_t n, unsigned long long* out) {
   k_fill_u32(s, reinterpret_cast<uint32_t*>(out), 2, 0u);
-  if (n > 0) mask_hash_kernel<<<grid_for(n) < 1024 ? grid_for(n) : 1024, TPB, 0, s>>>(m, n, out);
+  if (n > 0) launch(mask_hash_kernel, grid_for(n) < 1024 ? grid_for(n) : 1024, TPB, 0, s, m, n, out);
 }
 
 void k_state_diff(hipStream_t s, const float* pos, const float* vel, const int32_t* ids, const float* rpos,
@@ -3942,21 +3971,21 @@ void k_state_diff(hipStream_t s, const float* pos, const float* vel, const int32
   if (n <= 0) return;
   const unsigned grid = grid_for((size_t)n) < 2048 ? grid_for((size_t)n) : 2048;
   k_fill_u32(s, reinterpret_cast<uint32_t*>(inv), (size_t)n, 0xFFFFFFFFu);
-  inverse_ids_kernel<<<grid, TPB, 0, s>>>(rids, n, inv, out + 1);
-  state_diff_kernel<<<grid, TPB, 0, s>>>(pos, vel, ids, rpos, rvel, inv, n, out);
+  launch(inverse_ids_kernel, grid, TPB, 0, s, rids, n, inv, out + 1);
+  launch(state_diff_kernel, grid, TPB, 0, s, pos, vel, ids, rpos, rvel, inv, n, out);
 }
 
 void k_count_nonzero(hipStream_t s, const uint8_t* m, uint32_t n, uint32_t* out) {
   k_fill_u32(s, out, 1, 0u);
-  if (n > 0) count_nonzero_kernel<<<grid_for(n) < 1024 ? grid_for(n) : 1024, TPB, 0, s>>>(m, n, out);
+  if (n > 0) launch(count_nonzero_kernel, grid_for(n) < 1024 ? grid_for(n) : 1024, TPB, 0, s, m, n, out);
 }
 
 void k_mask(hipStream_t s, float* x, const uint8_t* liquid, int n) {
-  mask_kernel<<<red_grid(n), TPB, 0, s>>>(x, liquid, n);
+  launch(mask_kernel, red_grid(n), TPB, 0, s, x, liquid, n);
 }
 
 void k_add(hipStream_t s, float* x, const float* z, const uint8_t* liquid, int n) {
-  add_kernel<<<red_grid(n), TPB, 0, s>>>(x, z, liquid, n);
+  launch(add_kernel, red_grid(n), TPB, 0, s, x, z, liquid, n);
 }
 
 }  // namespace aw

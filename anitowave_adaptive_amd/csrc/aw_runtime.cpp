@@ -202,27 +202,35 @@ struct aw_ctx {
 };
 
 namespace {
-// RAII: records a start event at construction and an end event at
-// destruction around the launches in between, with their algorithmic bytes.
+// RAII: times the kernel launches in its scope with an event pair that the
+// launches' own dispatches record (start by the first, stop by each), with
+// their algorithmic bytes; no marker packets go between kernels.
 struct Prof {
   aw_ctx* c;
   long idx = -1;
   Prof(aw_ctx* ctx, const char* name, double bytes) : c(ctx) {
     if (!c->prof) return;
     if (c->prof_runs && !c->recs.empty() && std::strcmp(c->recs.back().name, name) == 0) {
-      // extends the run: its end event is recorded again after this launch
+      // extends the run: this scope's launches move its stop event on
       idx = (long)c->recs.size() - 1;
       c->recs[idx].bytes += bytes;
       c->recs[idx].launches++;
+      set_launch_events(nullptr, c->recs[idx].b);
       return;
     }
     ProfRec r{name, c->event(), c->event(), bytes, 1};
-    (void)hipEventRecord(r.a, c->stream);
     c->recs.push_back(r);
     idx = (long)c->recs.size() - 1;
+    set_launch_events(r.a, r.b);
   }
   ~Prof() {
-    if (idx >= 0) (void)hipEventRecord(c->recs[idx].b, c->stream);
+    if (idx < 0) return;
+    const LaunchEvents e = take_launch_events();
+    if (e.start) {
+      // nothing was launched in a new record's scope: an empty interval here
+      (void)hipEventRecord(c->recs[idx].a, c->stream);
+      (void)hipEventRecord(c->recs[idx].b, c->stream);
+    }
   }
 };
 }  // namespace

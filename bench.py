@@ -3,6 +3,12 @@
 dominant kernel and the FP64 oracle timed beside it on the host cores.
 
     python bench.py [--steps K] [--warmup W] [--workload pool256]
+    python bench.py --traffic-from FETCH.csv WRITE.csv --copy-bytes B
+
+The second form turns two rocprofv3 --pmc runs (FETCH_SIZE, then
+WRITE_SIZE, counter-collection CSVs) into HBM bytes per dispatch per kernel,
+scaled by what each counter reads for the largest device-to-device copy in
+the run (B bytes), and writes profiles/traffic.json for this kernel source.
 
 Default workload: 256^3-effective domain, 3 native levels (cell sizes h, 2h,
 4h, h = 1/256, 8^3-cell blocks), level 0 within one block (8 cells) of the
@@ -473,6 +479,63 @@ def run_cpu_baseline(args, w, pos, vel, firsts):
             "errors_vs_gpu_mixed": errors}
 
 
+def _kernel_key(name):
+    """Short kernel name from a rocprofv3 kernel name; the two G2P passes apart."""
+    base = name.replace("(anonymous namespace)::", "").split("(")[0]
+    tmpl = ""
+    if "<" in base:
+        base, tmpl = base.split("<", 1)
+    base = base.replace("void ", "").split("::")[-1]
+    if base == "g2p_advect_kernel":
+        return base + (" (slow pass)" if tmpl.startswith("true") else "")
+    return base
+
+
+def traffic_from(fetch_csv, write_csv, copy_bytes):
+    """bytes_per_launch per kernel from two --pmc runs, each counter scaled so
+    that the largest __amd_rocclr_copyBuffer dispatch reads copy_bytes."""
+    import csv
+    out, scale = {}, {}
+    for path, counter in ((fetch_csv, "FETCH_SIZE"), (write_csv, "WRITE_SIZE")):
+        rows = [r for r in csv.DictReader(open(path)) if r["Counter_Name"] == counter]
+        copies = [r for r in rows if "copyBuffer" in r["Kernel_Name"]]
+        big = max(int(r["Grid_Size"]) for r in copies)
+        cal = [float(r["Counter_Value"]) for r in copies if int(r["Grid_Size"]) == big]
+        kib = sum(cal) / len(cal)
+        scale[counter] = copy_bytes / (kib * 1024.0)
+        per = {}
+        for r in rows:
+            if "copyBuffer" in r["Kernel_Name"]:
+                continue
+            k = _kernel_key(r["Kernel_Name"])
+            a = per.setdefault(k, [0.0, 0])
+            a[0] += float(r["Counter_Value"]) * 1024.0 * scale[counter]
+            a[1] += 1
+        for k, (b, n) in per.items():
+            o = out.setdefault(k, {"dispatches": n, "FETCH_SIZE": 0.0, "WRITE_SIZE": 0.0})
+            o[counter] = b / n
+    for o in out.values():
+        o["bytes"] = o["FETCH_SIZE"] + o["WRITE_SIZE"]
+    return out, scale
+
+
+def write_traffic(args):
+    out, scale = traffic_from(args.traffic_from[0], args.traffic_from[1], args.copy_bytes)
+    doc = {"kernel_source_sha": kernel_source_sha(),
+           "measured_by": (f"rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE (profiles/pool256_traffic.md); "
+                           f"scaled {scale['FETCH_SIZE']:.3f} / {scale['WRITE_SIZE']:.3f} by a "
+                           f"{args.copy_bytes:.0f}-byte device copy"),
+           "bytes_per_launch": {k: round(v["bytes"]) for k, v in sorted(out.items())}}
+    with open(os.path.join(ROOT, "profiles", "traffic.json"), "w") as f:
+        json.dump(doc, f, indent=1)
+        f.write("\n")
+    print("| Kernel | Dispatches | FETCH (scaled, MB) | WRITE (scaled, MB) | Traffic (MB) |")
+    print("|---|---:|---:|---:|---:|")
+    for k, v in sorted(out.items(), key=lambda kv: -kv[1]["bytes"] * kv[1]["dispatches"]):
+        print(f"| {k} | {v['dispatches']} | {v['FETCH_SIZE'] / 1e6:.1f} | {v['WRITE_SIZE'] / 1e6:.1f} | "
+              f"{v['bytes'] / 1e6:.1f} |")
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -498,7 +561,14 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=0, help="rollout length (0: the workload's own)")
     ap.add_argument("--no-large", dest="large", action="store_false",
                     help="skip the 512^3 double-dam leg (plain, mixed 1e-8, 8-step adjoint)")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write profiles/traffic.json from two rocprofv3 --pmc counter-collection CSVs")
+    ap.add_argument("--copy-bytes", type=float, default=0.0,
+                    help="bytes of the largest device copy in the profiled run (the counters' scale)")
     args = ap.parse_args()
+    if args.traffic_from:
+        write_traffic(args)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only",
                           "n_gpus": args.gpus}))
