@@ -2414,38 +2414,95 @@ __device__ __forceinline__ WG wg_at(const WGMap& m, int g) {
   return w;
 }
 
-// Work-list cell i of group w with its block descriptor (one dependent load
-// for every offset the stencil needs).
-__device__ __forceinline__ bool decode_bd(const LevelDev& L, const WG& w, int i, CellRef& o, const uint32_t*& bd) {
+// Descriptor of one work-list entry, held in registers: read with six
+// 16-byte loads at the start of a stencil kernel, so that every data load of
+// the cell (its value, mask, right-hand side, face coefficients and the
+// neighbour values) depends on that one load and all of them are in flight
+// together.  Read lazily instead (a load per field, behind the branches that
+// use it), the smoother's level-0 phase waited on about a dozen dependent
+// loads per thread.
+struct BDR {
+  uint32_t B, coff, foff, m0, m1;
+  int nbr[6];
+  uint32_t ncoff[6], nfoff[6];
+};
+
+typedef const __attribute__((address_space(4))) uint32_t* ConstU32;
+
+// UNIFORM: p is the same for every thread; the table is read through the
+// constant address space (it does not change during a launch), so the
+// descriptor is loaded with scalar loads into scalar registers.
+template <bool UNIFORM>
+__device__ __forceinline__ void bd_read(const uint32_t* p, BDR& d) {
+  uint32_t v[BD_WORDS];
+#pragma unroll
+  for (int k = 0; k < BD_WORDS / 4; ++k) {
+    if (UNIFORM) {
+      const ConstU32 c = (ConstU32)p + 4 * k;
+      v[4 * k] = c[0];
+      v[4 * k + 1] = c[1];
+      v[4 * k + 2] = c[2];
+      v[4 * k + 3] = c[3];
+    } else {
+      const uint4 t = reinterpret_cast<const uint4*>(p)[k];
+      v[4 * k] = t.x;
+      v[4 * k + 1] = t.y;
+      v[4 * k + 2] = t.z;
+      v[4 * k + 3] = t.w;
+    }
+  }
+  d.B = v[0];
+  d.coff = v[1];
+  d.foff = v[2];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    d.nbr[j] = (int)v[3 + j];
+    d.ncoff[j] = v[9 + j];
+    d.nfoff[j] = v[15 + j];
+  }
+  d.m0 = v[21];
+  d.m1 = v[22];
+}
+
+// Work-list cell i of group w and its block's descriptor.  ONE: the group is
+// a single block (resolution 8), so the descriptor address is the same for
+// every thread and the descriptor lands in scalar registers.  A thread past
+// the group's cells gets false and the cell of the same local index in the
+// group's first block (in bounds), so that its loads need no branch.
+template <bool ONE>
+__device__ __forceinline__ bool decode_r(const LevelDev& L, const WG& w, int i, CellRef& o, BDR& d) {
   const int sh = 3 * w.lr;
-  if (i >= (w.count << sh)) return false;
-  o.q = i >> sh;
+  const bool ok = i < (w.count << sh);
+  o.q = ONE ? 0 : (ok ? i >> sh : 0);
   o.li = i & ((1 << sh) - 1);
-  bd = L.bd + (size_t)(w.start + o.q) * BD_WORDS;
-  o.B = (int)bd[0];
+  bd_read<ONE>(L.bd + (size_t)(w.start + o.q) * BD_WORDS, d);
+  o.B = (int)d.B;
   o.lr = w.lr;
   o.r = 1 << w.lr;
   o.l[0] = o.li & (o.r - 1);
   o.l[1] = (o.li >> w.lr) & (o.r - 1);
   o.l[2] = o.li >> (2 * w.lr);
-  o.c = bd[1] + o.li;
-  return true;
+  o.c = d.coff + o.li;
+  return ok;
 }
 
-// The same stencil with every face's loads issued before any is used: the
-// coefficient from the block's own face slot, and one neighbour value per
-// boundary face from an address chosen with selects (the cell itself where
-// the face has no single neighbour).  Only a face with four finer neighbours
-// branches into loads of its own.  rd(cell) reads the value of a cell.
-// Sums run in face order as in stencil_split.
+// log2 cell edge of face neighbour d from the packed descriptor words
+__device__ __forceinline__ int bd_ne(const BDR& bd, int d) {
+  return (int)((d < 3 ? bd.m0 >> (8 * (d + 1)) : bd.m1 >> (8 * (d - 3))) & 255u);
+}
+
+// The stencil of stencil_split from a register descriptor, with every face's
+// loads issued before any is used: the coefficient from the block's own face
+// slot, and the neighbour value of each boundary face with a single
+// neighbour (a load under the lane's own predicate, no branch on loaded
+// data).  Only a face with four finer neighbours loads again.  rd(cell) reads
+// the value of a cell.  Sums run in face order as in stencil_split.
 template <typename T, class Rd>
-__device__ __forceinline__ void stencil_fast(const LevelDev& L, const CellRef& cr, const uint32_t* bd, Rd rd,
-                                             T& diag, T& ext, T* w) {
+__device__ __forceinline__ void stencil_r(const LevelDev& L, const CellRef& cr, const BDR& bd, Rd rd, T& diag,
+                                          T& ext, T* w) {
   const int r = cr.r;
-  const uint32_t m0 = bd[21], m1 = bd[22];
-  const int e = (int)(m0 & 255u);
+  const int e = (int)(bd.m0 & 255u);
   const T s = (T)(1 << e);
-  const uint32_t fo = bd[2];
   T bt[6], xn[6];
   int kind[6];  // 0 in-block, 1 wall, 2 equal neighbour, 3 coarser neighbour, 4 finer neighbours
 #pragma unroll
@@ -2456,9 +2513,9 @@ __device__ __forceinline__ void stencil_fast(const LevelDev& L, const CellRef& c
     const bool inb = nl >= 0 && nl < r;
     int sl[3] = {cr.l[0], cr.l[1], cr.l[2]};
     sl[a] += side;
-    bt[d] = (T)L.beta[a][fo + face_idx(a, sl, r)];
-    const int N = inb ? -1 : (int)bd[3 + d];
-    const int en = (int)((d < 3 ? m0 >> (8 * (d + 1)) : m1 >> (8 * (d - 3))) & 255u);
+    bt[d] = (T)L.beta[a][bd.foff + face_idx(a, sl, r)];
+    const int N = bd.nbr[d];
+    const int en = bd_ne(bd, d);
     const int rn = L.bs >> en;
     const bool coarser = en > e;
     int nc[3] = {cr.l[0], cr.l[1], cr.l[2]};
@@ -2466,8 +2523,8 @@ __device__ __forceinline__ void stencil_fast(const LevelDev& L, const CellRef& c
     nc[t1] = coarser ? cr.l[t1] >> 1 : cr.l[t1];
     nc[t2] = coarser ? cr.l[t2] >> 1 : cr.l[t2];
     const bool single = !inb && N >= 0 && en >= e;
-    const uint32_t cc = single ? bd[9 + d] + cell_idx(nc, rn) : cr.c;
-    xn[d] = rd(cc);
+    xn[d] = (T)0;
+    if (single) xn[d] = rd(bd.ncoff[d] + cell_idx(nc, rn));
     kind[d] = inb ? 0 : (N < 0 ? 1 : (en == e ? 2 : (coarser ? 3 : 4)));
   }
   diag = 0;
@@ -2490,8 +2547,8 @@ __device__ __forceinline__ void stencil_fast(const LevelDev& L, const CellRef& c
     } else if (kind[d] == 4) {
       const int a = d >> 1, side = d & 1;
       const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
-      const int en = (int)((d < 3 ? m0 >> (8 * (d + 1)) : m1 >> (8 * (d - 3))) & 255u), rn = L.bs >> en;
-      const uint32_t con = bd[9 + d], fon = bd[15 + d];
+      const int rn = L.bs >> bd_ne(bd, d);
+      const uint32_t con = bd.ncoff[d], fon = bd.nfoff[d];
       int nc[3], ps[3];
       nc[a] = side ? 0 : rn - 1;
       ps[a] = side ? 0 : rn;
@@ -2551,47 +2608,45 @@ __device__ __forceinline__ uint32_t parent_cell(const LevelDev& F, const LevelDe
 // with 256 threads x 2 cells, level 0 even; 4 cells per thread: 33 us).
 constexpr int ST_T = 512, ST_CPT = 512 / ST_T;
 
+static_assert(ST_CPT == 1, "the stencil kernels below give each thread one cell");
+
+template <bool ONE>
+__device__ __forceinline__ void smooth_group(const LevelDev& L, float* __restrict__ x, const float* __restrict__ b,
+                                             const WG& w, float omega, int local_sweeps, int reverse, float* sx) {
+  const int i = threadIdx.x;
+  CellRef cr;
+  BDR bd;
+  const bool ok = decode_r<ONE>(L, w, i, cr, bd);
+  float dg, ex, cw[6];
+  const float xv = x[cr.c];
+  const bool act = ok & (L.liquid[cr.c] != 0);
+  const float bb = b[cr.c];
+  stencil_r(L, cr, bd, [&](uint32_t cc) { return x[cc]; }, dg, ex, cw);
+  sx[i] = xv;
+  __syncthreads();
+  const int base = i - cr.li;
+  for (int ls = 0; ls < local_sweeps; ++ls)
+    for (int ph = 0; ph < 2; ++ph) {
+      const int col = reverse ? 1 - ph : ph;
+      if (act && ((cr.l[0] + cr.l[1] + cr.l[2]) & 1) == col) {
+        const float off = ex + inblock(cw, cr.li, cr.r, [&](int li) { return sx[base + li]; });
+        const float res = bb - (dg * sx[i] - off);
+        if (dg > 0.f) sx[i] += omega * res / dg;
+      }
+      __syncthreads();
+    }
+  if (act) x[cr.c] = sx[i];
+}
+
 __global__ __launch_bounds__(ST_T) AW_STENCIL_OCCUPANCY void smooth_kernel(
     LevelDev L, float* __restrict__ x, const float* __restrict__ b, WGMap wgm,
     const uint32_t* __restrict__ blist, float omega, int local_sweeps, int reverse) {
   __shared__ float sx[512];
   const WG w = wg_at(wgm, blockIdx.x);
-  CellRef cr[ST_CPT];
-  bool act[ST_CPT];
-  float bb[ST_CPT], dg[ST_CPT], ex[ST_CPT], cw[ST_CPT][6];
-#pragma unroll
-  for (int k = 0; k < ST_CPT; ++k) {
-    const int i = threadIdx.x + k * ST_T;
-    act[k] = false;
-    const uint32_t* bd;
-    if (!decode_bd(L, w, i, cr[k], bd)) continue;
-    const uint32_t c = cr[k].c;
-    sx[i] = x[c];
-    if (!L.liquid[c]) continue;
-    act[k] = true;
-    bb[k] = b[c];
-    stencil_fast(L, cr[k], bd, [&](uint32_t cc) { return x[cc]; }, dg[k], ex[k], cw[k]);
-  }
-  __syncthreads();
-  for (int ls = 0; ls < local_sweeps; ++ls)
-    for (int ph = 0; ph < 2; ++ph) {
-      const int col = reverse ? 1 - ph : ph;
-#pragma unroll
-      for (int k = 0; k < ST_CPT; ++k) {
-        if (!act[k]) continue;
-        const CellRef& c = cr[k];
-        if (((c.l[0] + c.l[1] + c.l[2]) & 1) != col) continue;
-        const int i = threadIdx.x + k * ST_T;
-        const int base = i - c.li;
-        const float off = ex[k] + inblock(cw[k], c.li, c.r, [&](int li) { return sx[base + li]; });
-        const float res = bb[k] - (dg[k] * sx[i] - off);
-        if (dg[k] > 0.f) sx[i] += omega * res / dg[k];
-      }
-      __syncthreads();
-    }
-#pragma unroll
-  for (int k = 0; k < ST_CPT; ++k)
-    if (act[k]) x[cr[k].c] = sx[threadIdx.x + k * ST_T];
+  if (w.lr == 3)
+    smooth_group<true>(L, x, b, w, omega, local_sweeps, reverse, sx);
+  else
+    smooth_group<false>(L, x, b, w, omega, local_sweeps, reverse, sx);
 }
 
 // live[B] = 1 when block B of the level holds a liquid cell: one wave per
@@ -2640,20 +2695,19 @@ __global__ __launch_bounds__(ST_T) AW_STENCIL_OCCUPANCY void adaptive_smooth_ker
   __shared__ uint8_t sdone[512];
   const WG w = wg_at(wgm, blockIdx.x);
   CellRef cr;
-  const uint32_t* bd;
+  BDR bd;
   const int i = threadIdx.x;
-  const bool ok = decode_bd(L, w, i, cr, bd);
-  bool on = false;
+  const bool ok = decode_r<false>(L, w, i, cr, bd);
+  bool on = false, liq = false;
   float bb = 0.f, dg = 0.f, ex = 0.f, cw[6];
   if (ok) {
     sx[i] = x[cr.c];
     on = act[cr.B] != 0;
-    if (on && L.liquid[cr.c]) {
-      bb = b[cr.c];
-      stencil_fast(L, cr, bd, [&](uint32_t cc) { return x[cc]; }, dg, ex, cw);
-    }
+    liq = L.liquid[cr.c] != 0;
+    bb = b[cr.c];
+    stencil_r(L, cr, bd, [&](uint32_t cc) { return x[cc]; }, dg, ex, cw);
   }
-  const bool live = on && ok && L.liquid[cr.c];
+  const bool live = on && ok && liq;
   if (!__syncthreads_or(on ? 1 : 0)) {   // no active block in the group
     if (ok && cr.li == 0) sweeps[cr.B] = 0;
     return;
@@ -2820,49 +2874,46 @@ __global__ __launch_bounds__(ST_T) AW_STENCIL_OCCUPANCY void uniform_smooth_kern
 // r = b - A x (computeExplicitResidual, VCycle.cuh:81-131), 0 on air cells;
 // optionally fused with the restriction b_c = sum of children
 // (restrictResidualsKernel, MultigridTransfer.cuh:9-59) and x_c = 0.
-__global__ __launch_bounds__(ST_T) AW_STENCIL_OCCUPANCY void residual_kernel(
-    LevelDev L, const float* __restrict__ x, const float* __restrict__ b, float* __restrict__ r,
-    WGMap wgm, const uint32_t* __restrict__ blist, Restr rs) {
-  __shared__ float sx[512];
-  __shared__ float sr[512];
-  const WG w = wg_at(wgm, blockIdx.x);
-  CellRef cr[ST_CPT];
-  bool ok[ST_CPT];
-  const uint32_t* bd[ST_CPT];
-#pragma unroll
-  for (int k = 0; k < ST_CPT; ++k) {
-    const int i = threadIdx.x + k * ST_T;
-    ok[k] = decode_bd(L, w, i, cr[k], bd[k]);
-    if (ok[k]) sx[i] = x[cr[k].c];
+template <bool ONE>
+__device__ __forceinline__ void residual_group(const LevelDev& L, const float* __restrict__ x,
+                                               const float* __restrict__ b, float* __restrict__ r, const WG& w,
+                                               const Restr& rs, float* sx, float* sr, uint32_t* sco, int* shalf) {
+  const int i = threadIdx.x;
+  CellRef cr;
+  BDR bd;
+  const bool ok = decode_r<ONE>(L, w, i, cr, bd);
+  float dg, ex, cw[6];
+  const float xv = x[cr.c];
+  const bool liq = L.liquid[cr.c] != 0;
+  const float bb = b[cr.c];
+  stencil_r(L, cr, bd, [&](uint32_t cc) { return x[cc]; }, dg, ex, cw);
+  // the fused restriction's coarse cell offset of each block of the group
+  // (and whether the coarse level halves the group's blocks), in LDS
+  if (rs.on && ok && cr.li == 0) {
+    sco[cr.q] = rs.C.coff[cr.B];
+    if (cr.q == 0) *shalf = (int)rs.C.e[cr.B] != (int)(bd.m0 & 255u);
   }
+  sx[i] = xv;
   __syncthreads();
-#pragma unroll
-  for (int k = 0; k < ST_CPT; ++k) {
-    if (!ok[k]) continue;
-    const int i = threadIdx.x + k * ST_T;
-    const uint32_t c = cr[k].c;
+  if (ok) {
     float rv = 0.f;
-    if (L.liquid[c]) {
-      float dg, ex, cw[6];
-      stencil_fast(L, cr[k], bd[k], [&](uint32_t cc) { return x[cc]; }, dg, ex, cw);
-      const int base = i - cr[k].li;
-      const float off = ex + inblock(cw, cr[k].li, cr[k].r, [&](int li) { return sx[base + li]; });
-      rv = b[c] - (dg * sx[i] - off);
+    if (liq) {
+      const int base = i - cr.li;
+      const float off = ex + inblock(cw, cr.li, cr.r, [&](int li) { return sx[base + li]; });
+      rv = bb - (dg * sx[i] - off);
     }
     sr[i] = rv;
-    if (r) r[c] = rv;
+    if (r) r[cr.c] = rv;
   }
   if (!rs.on) return;
   __syncthreads();
   // coarse cells of this group's blocks (all of one resolution)
   const int rF = 1 << w.lr;
-  const int B0 = blist[w.start];
-  const bool halves = rs.C.e[B0] != L.e[B0];
+  const bool halves = *shalf != 0;
   const int rC = halves ? rF >> 1 : rF;
   const int cpbC = rC * rC * rC, cpbF = rF * rF * rF;
   for (int j = threadIdx.x; j < w.count * cpbC; j += ST_T) {
     const int q = j / cpbC, lj = j - q * cpbC;
-    const int B = blist[w.start + q];
     const int l[3] = {lj % rC, (lj / rC) % rC, lj / (rC * rC)};
     float s = 0.f;
     if (halves) {
@@ -2873,46 +2924,58 @@ __global__ __launch_bounds__(ST_T) AW_STENCIL_OCCUPANCY void residual_kernel(
     } else {
       s = sr[q * cpbF + lj];
     }
-    const uint32_t cc = rs.C.coff[B] + lj;
+    const uint32_t cc = sco[q] + lj;
     rs.bc[cc] = s;
     rs.xc[cc] = 0.f;
   }
 }
 
+__global__ __launch_bounds__(ST_T) AW_STENCIL_OCCUPANCY void residual_kernel(
+    LevelDev L, const float* __restrict__ x, const float* __restrict__ b, float* __restrict__ r,
+    WGMap wgm, const uint32_t* __restrict__ blist, Restr rs) {
+  __shared__ float sx[512];
+  __shared__ float sr[512];
+  __shared__ uint32_t sco[512];
+  __shared__ int shalf;
+  const WG w = wg_at(wgm, blockIdx.x);
+  if (w.lr == 3)
+    residual_group<true>(L, x, b, r, w, rs, sx, sr, sco, &shalf);
+  else
+    residual_group<false>(L, x, b, r, w, rs, sx, sr, sco, &shalf);
+}
+
 // y = A x with sum.acc += x . y
+template <bool ONE>
+__device__ __forceinline__ double apply_group(const LevelDev& L, const float* __restrict__ x, float* __restrict__ y,
+                                              const WG& w, float* sx) {
+  const int i = threadIdx.x;
+  CellRef cr;
+  BDR bd;
+  const bool ok = decode_r<ONE>(L, w, i, cr, bd);
+  float dg, ex, cw[6];
+  const float xv = x[cr.c];
+  const bool liq = L.liquid[cr.c] != 0;
+  stencil_r(L, cr, bd, [&](uint32_t cc) { return x[cc]; }, dg, ex, cw);
+  sx[i] = xv;
+  __syncthreads();
+  if (!ok) return 0.0;
+  if (!liq) {
+    y[cr.c] = 0.f;
+    return 0.0;
+  }
+  const int base = i - cr.li;
+  const float off = ex + inblock(cw, cr.li, cr.r, [&](int li) { return sx[base + li]; });
+  const float v = dg * sx[i] - off;
+  y[cr.c] = v;
+  return (double)sx[i] * (double)v;
+}
+
 __global__ __launch_bounds__(ST_T) AW_STENCIL_OCCUPANCY void apply_dot_kernel(
     LevelDev L, const float* __restrict__ x, float* __restrict__ y, WGMap wgm,
     const uint32_t* __restrict__ blist, double* acc) {
   __shared__ float sx[512];
   const WG w = wg_at(wgm, blockIdx.x);
-  CellRef cr[ST_CPT];
-  bool ok[ST_CPT];
-  const uint32_t* bd[ST_CPT];
-#pragma unroll
-  for (int k = 0; k < ST_CPT; ++k) {
-    const int i = threadIdx.x + k * ST_T;
-    ok[k] = decode_bd(L, w, i, cr[k], bd[k]);
-    if (ok[k]) sx[i] = x[cr[k].c];
-  }
-  __syncthreads();
-  double part = 0.0;
-#pragma unroll
-  for (int k = 0; k < ST_CPT; ++k) {
-    if (!ok[k]) continue;
-    const int i = threadIdx.x + k * ST_T;
-    const uint32_t c = cr[k].c;
-    if (!L.liquid[c]) {
-      y[c] = 0.f;
-      continue;
-    }
-    float dg, ex, cw[6];
-    stencil_fast(L, cr[k], bd[k], [&](uint32_t cc) { return x[cc]; }, dg, ex, cw);
-    const int base = i - cr[k].li;
-    const float off = ex + inblock(cw, cr[k].li, cr[k].r, [&](int li) { return sx[base + li]; });
-    const float v = dg * sx[i] - off;
-    y[c] = v;
-    part += (double)sx[i] * (double)v;
-  }
+  const double part = w.lr == 3 ? apply_group<true>(L, x, y, w, sx) : apply_group<false>(L, x, y, w, sx);
   if (acc) grid_sum<ST_T>(part, acc);
 }
 
@@ -3443,14 +3506,14 @@ __global__ __launch_bounds__(TPB) void residual64_kernel(LevelDev L, const doubl
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     CellRef c;
-    const uint32_t* bd;
-    if (!decode_bd(L, w, threadIdx.x + k * TPB, c, bd)) continue;
+    BDR bd;
+    if (!decode_r<false>(L, w, threadIdx.x + k * TPB, c, bd)) continue;
     if (!L.liquid[c.c]) {
       r32[c.c] = 0.f;
       continue;
     }
     double dg, ex, cw[6];
-    stencil_fast(L, c, bd, [&](uint32_t cc) { return x[cc]; }, dg, ex, cw);
+    stencil_r(L, c, bd, [&](uint32_t cc) { return x[cc]; }, dg, ex, cw);
     const uint32_t base = c.c - c.li;
     double off = ex;
     const int st[3] = {1, c.r, c.r * c.r};
@@ -3519,11 +3582,13 @@ __global__ __launch_bounds__(TPB) void pcg_xr_kernel(float* __restrict__ x, floa
   const double pq = sc_get(sc, SC_PQ);
   const float alpha = pq > 0.0 ? (float)(sc_get(sc, SC_RZ) / pq) : 0.f;
   double s = 0.0;
+  // liquid cells only: q = A p may be left unwritten elsewhere (p is 0 there)
   for (int i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) {
+    if (!liq[i]) continue;
     x[i] += alpha * p[i];
     const float ri = r[i] - alpha * q[i];
     r[i] = ri;
-    if (liq[i]) s += (double)ri * (double)ri;
+    s += (double)ri * (double)ri;
   }
   grid_sum(s, sc + SC_RR * SC_SLOTS);
 }

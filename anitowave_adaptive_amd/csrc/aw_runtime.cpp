@@ -327,11 +327,12 @@ struct MGLevel {
   // refresh_live from the liquid masks; live_on false: the full lists.
   bool live_on = false;
   int live_tables_version = -1;
-  WGMap wgm_live[2]{};
+  WGMap wgm_live[3]{};  // colour 0, colour 1, both (the runs of colour 0, then those of colour 1)
   DevBuf bdesc_live, live_idx;
   std::vector<uint32_t> h_live_idx;  // work-list entries of bdesc_live (host source of live_idx)
   uint32_t live_blocks = 0;
   double live_bytes[2] = {0.0, 0.0};
+  double live_cells = 0.0, live_ccells = 0.0;  // cells of those blocks, and their cells on the next level
   // device views (level 0 aliases the grid)
   const uint8_t* d_e = nullptr;
   const uint32_t* d_coff = nullptr;
@@ -1010,7 +1011,8 @@ int refresh_live(aw_solver* S) {
     const uint8_t* f = S->h_live + off[k];
     idx.clear();
     WGMap m[2]{};
-    double by[2] = {0.0, 0.0};
+    double by[2] = {0.0, 0.0}, cells = 0.0, ccells = 0.0;
+    const MGLevel* C = k + 1 < (int)S->lv.size() ? S->lv[k + 1].get() : nullptr;
     for (int col = 0; col < 2; ++col) {
       for (int j = 0; j < WGMAP_RUNS; ++j) {
         m[col].g0[j] = INT_MAX;
@@ -1027,6 +1029,11 @@ int refresh_live(aw_solver* S) {
           idx.push_back((uint32_t)q);
           const double r = (double)(L.bs >> L.h_e[B]);
           by[col] += 12.0 * r * r * r + 12.0 * (r + 1.0) * r * r;
+          cells += r * r * r;
+          if (C && !C->aggregated) {
+            const double rc = (double)(C->bs >> C->h_e[B]);
+            ccells += rc * rc * rc;
+          }
         }
         const int cnt = (int)idx.size() - start;
         if (cnt == 0) continue;
@@ -1053,10 +1060,33 @@ int refresh_live(aw_solver* S) {
     }
     L.wgm_live[0] = m[0];
     L.wgm_live[1] = m[1];
+    {
+      // both colours in one launch: colour 1's entries follow colour 0's in
+      // bdesc_live, so its runs keep their starts and shift their groups
+      WGMap& u = L.wgm_live[2];
+      u = m[0];
+      int t = 0;
+      while (t < WGMAP_RUNS && u.cnt[t] > 0) ++t;
+      for (int j = 0; j < WGMAP_RUNS && m[1].cnt[j] > 0; ++j, ++t) {
+        if (t >= WGMAP_RUNS) return fail(AW_ERR_UNSUPPORTED, "smoother lists: more than WGMAP_RUNS runs");
+        u.g0[t] = m[1].g0[j] + m[0].n;
+        u.s0[t] = m[1].s0[j];
+        u.cnt[t] = m[1].cnt[j];
+        u.lr[t] = m[1].lr[j];
+      }
+      u.n = m[0].n + m[1].n;
+    }
     L.live_bytes[0] = by[0];
     L.live_bytes[1] = by[1];
+    L.live_cells = cells;
+    L.live_ccells = ccells;
     L.live_blocks = (uint32_t)L.h_live_idx.size();
     L.live_on = true;
+    // The V-cycle's fused residual and restriction then run over these lists
+    // too, and the restriction zeroes the coarse guess only under the
+    // blocks holding liquid: the rest of the next level's guess (air cells,
+    // which neighbours read as 0 and nothing else writes) is cleared here.
+    if (k + 1 < (int)S->lv.size()) k_fill_f32(s, S->lv[k + 1]->d_x, S->lv[k + 1]->ncells, 0.f);
   }
   return launch_status("smoother lists");
 }
@@ -1207,11 +1237,17 @@ void residual_level(aw_solver* S, int k, const float* x, const float* b, float* 
     rs.on = 1;
     bytes += 8.0 * C.ncells - 4.0 * L.ncells;  // r is not stored; b_c and x_c are
   }
+  // fused with the restriction, only the blocks holding liquid matter: the
+  // fine residual is not stored, and the coarse right-hand side of air cells
+  // is not read (their guess stays 0)
+  const bool live = fuse_restrict && L.live_on;
+  if (live) bytes = L.live_bytes[0] + L.live_bytes[1] + 8.0 * L.live_ccells - 4.0 * L.live_cells;
   Prof pf(S->g->ctx,
           k == 0 ? (fuse_restrict ? "residual_restrict_l0" : "residual_l0")
                  : (fuse_restrict ? "residual_restrict_coarse" : "residual_coarse"),
           bytes);
-  k_residual(S->g->ctx->stream, L.view(), x, b, fuse_restrict ? nullptr : r, L.wgm[2], L.bl(), rs);
+  k_residual(S->g->ctx->stream, live ? L.live_view() : L.view(), x, b, fuse_restrict ? nullptr : r,
+             live ? L.wgm_live[2] : L.wgm[2], L.bl(), rs);
 }
 
 void restrict_level(aw_solver* S, int k, const float* r, float* bc, float* xc = nullptr) {
@@ -1292,8 +1328,14 @@ void pcg_body(aw_solver* S, float* x, int first) {
     k_pcg_p(s, z, pv, n, sc, first);
   }
   {
-    Prof pf(g->ctx, "apply_l0", L0.sweep_bytes[0] + L0.sweep_bytes[1] - 4.0 * n);
-    k_apply_dot(s, L0.view(), pv, q, L0.wgm[2], L0.bl(), scp(S, SC_PQ));
+    // q = A p is read on liquid cells only (p.q here, r -= alpha q below):
+    // the blocks holding liquid are enough
+    const bool live = L0.live_on;
+    Prof pf(g->ctx, "apply_l0",
+            live ? L0.live_bytes[0] + L0.live_bytes[1] - 4.0 * L0.live_cells
+                 : L0.sweep_bytes[0] + L0.sweep_bytes[1] - 4.0 * n);
+    k_apply_dot(s, live ? L0.live_view() : L0.view(), pv, q, live ? L0.wgm_live[2] : L0.wgm[2], L0.bl(),
+                scp(S, SC_PQ));
   }
   {
     Prof pf(g->ctx, "pcg_update", 28.0 * n);
