@@ -188,6 +188,11 @@ void k_pressure_remap(hipStream_t s, const GridDev& g, const uint8_t* level_prev
 // and RK3 advection; 1: the blend only (vel_out); 2: the advection only (pos_out)
 void k_g2p_advect(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const float* svel,
                   int64_t n, float alpha, float dt, float* pos_out, float* vel_out, uint32_t* list, int mode = 0);
+// the step's G2P + advection of block-sorted particles (block B's particles
+// at poff[B] .. poff[B] + pcount[B]), each block's lattice read into LDS once
+void k_g2p_advect_sorted(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const float* svel,
+                         const uint32_t* pcount, const uint32_t* poff, int64_t n, float alpha, float dt,
+                         float* pos_out, float* vel_out, uint32_t* list);
 
 // adjoint (fine-lattice adjoint buffers: [block][field][axis][512], stride ADJ)
 // G2P adjoint: stage pass over the n sorted particles (partials, and one

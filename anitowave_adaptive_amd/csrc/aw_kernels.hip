@@ -1706,6 +1706,142 @@ __global__ __launch_bounds__(TPB) void g2p_advect_kernel(GridDev g, const float*
   g2p_advect_one<false, MODE>(g, F, spos, svel, i, alpha, dt, pos_out, vel_out, list);
 }
 
+// ---- G2P of the step's block-sorted particles with the block's lattice in LDS ----
+// Trilinear sample of component A of `field` from one block's padded lattice
+// staged in LDS (lat: the block's SLAB floats; o: the block's first fine
+// cell per axis), with sample_comp's node rule and sums.  Returns false, and
+// reads nothing, when the sample's 2x2x2 nodes are not all in that lattice
+// (a sample point outside the block).
+template <int A>
+__device__ __forceinline__ bool sample_lds(const GridDev& g, const float* lat, int field, const int* o, const float* u,
+                                           float& out) {
+  constexpr int n0 = A == 0 ? 9 : 10, n1 = A == 1 ? 9 : 10, n2 = A == 2 ? 9 : 10;
+  int lp[3];
+  float t[3];
+  bool zero = false, in = true;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    int i;
+    if (d == A) {
+      zero = zero || u[d] < 0.f || u[d] > (float)g.N[d];
+      const float c = fminf(fmaxf(u[d], 0.f), (float)g.N[d]);
+      i = min((int)floorf(c), g.N[d] - 1);
+      t[d] = c - (float)i;
+    } else {
+      const float c = fminf(fmaxf(u[d] - 0.5f, 0.f), (float)(g.N[d] - 1));
+      i = (int)floorf(c);
+      t[d] = c - (float)i;
+    }
+    lp[d] = i - o[d] + (d == A ? 0 : 1);
+    const int nd = d == 0 ? n0 : (d == 1 ? n1 : n2);
+    in = in && lp[d] >= 0 && lp[d] <= nd - 2;
+  }
+  if (!in) return false;
+  const float* f = lat + field * 3 * PAD + A * PAD + lp[0] + n0 * (lp[1] + n1 * lp[2]);
+  float s = 0.f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int dd[3] = {k & 1, (k >> 1) & 1, k >> 2};
+    const float w = (dd[0] ? t[0] : 1.f - t[0]) * (dd[1] ? t[1] : 1.f - t[1]) * (dd[2] ? t[2] : 1.f - t[2]);
+    s += w * f[dd[0] + n0 * (dd[1] + n1 * dd[2])];
+  }
+  out = zero ? 0.f : s;
+  return true;
+}
+
+// component a of field at u: from the LDS lattice, else through the global
+// lattice's halo path (sample_comp; a sample it cannot serve clears ok)
+template <int A>
+__device__ __forceinline__ float sample_blk(const GridDev& g, const float* lat, const float* __restrict__ F, int field,
+                                            const int* o, const float* u, const Near& nr, bool& ok) {
+  float v;
+  if (!sample_lds<A>(g, lat, field, o, u, v)) v = sample_comp<A, false>(g, F, field, u, nr, ok);
+  return v;
+}
+
+// One work-group per block: the block's padded lattice (both fields, 21.6 KB)
+// is read into LDS once, and the block's particles (consecutive after the
+// binning sort, poff[B] .. poff[B] + pcount[B]) sample it there; a stage
+// point outside the block samples the global lattice, and a particle none of
+// whose paths can serve goes to the list that g2p_advect_kernel<true>
+// finishes.  Same arithmetic as g2p_advect_one<false, G2P_BOTH>, so the
+// same results.  (One thread per particle over the global lattice, the 96
+// gathers of a particle go through the vector L1/L2 and the kernel ran at
+// 1.64 ms for 40.9 M particles.)
+constexpr int G2B_T = 256;
+__global__ __launch_bounds__(G2B_T) void g2p_block_kernel(GridDev g, const float* __restrict__ F,
+                                                          const float* __restrict__ spos,
+                                                          const float* __restrict__ svel,
+                                                          const uint32_t* __restrict__ pcount,
+                                                          const uint32_t* __restrict__ poff, float alpha, float dt,
+                                                          float* __restrict__ pos_out, float* __restrict__ vel_out,
+                                                          uint32_t* __restrict__ list) {
+  __shared__ float lat[SLAB];
+  const int B = blockIdx.x;
+  const uint32_t cnt = pcount[B];
+  if (cnt == 0) return;
+  {
+    // SLAB floats per block: a multiple of 4, and every block's lattice 16-byte aligned
+    const float4* src = reinterpret_cast<const float4*>(F + (size_t)B * SLAB);
+    float4* dst = reinterpret_cast<float4*>(lat);
+    for (int k = threadIdx.x; k < SLAB / 4; k += G2B_T) dst[k] = src[k];
+  }
+  Near nr;
+  block_coords(g.D, B, nr.bc);
+  nr.own = true;
+  const int o[3] = {BASE * nr.bc[0], BASE * nr.bc[1], BASE * nr.bc[2]};
+  __syncthreads();
+  const uint32_t p0 = poff[B];
+  for (uint32_t j = threadIdx.x; j < cnt; j += G2B_T) {
+    const int64_t i = (int64_t)p0 + j;
+    bool ok = true;
+    float x[3], u[3], v1[3], vd[3], vp[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      x[a] = spos[3 * i + a];
+      vp[a] = svel[3 * i + a];
+      u[a] = fminf(fmaxf(x[a] * g.inv_h, 0.f), (float)g.N[a] - 0.0009765625f);
+    }
+    v1[0] = sample_blk<0>(g, lat, F, 0, o, u, nr, ok);
+    vd[0] = sample_blk<0>(g, lat, F, 1, o, u, nr, ok);
+    v1[1] = sample_blk<1>(g, lat, F, 0, o, u, nr, ok);
+    vd[1] = sample_blk<1>(g, lat, F, 1, o, u, nr, ok);
+    v1[2] = sample_blk<2>(g, lat, F, 0, o, u, nr, ok);
+    vd[2] = sample_blk<2>(g, lat, F, 1, o, u, nr, ok);
+    float vn[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) vn[a] = alpha * (vp[a] + vd[a]) + (1.f - alpha) * v1[a];
+    float xx[3], v2[3], v3[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      xx[a] = x[a] + 0.5f * dt * v1[a];
+      u[a] = fminf(fmaxf(xx[a] * g.inv_h, -1.f), (float)g.N[a] + 1.f);
+    }
+    v2[0] = sample_blk<0>(g, lat, F, 0, o, u, nr, ok);
+    v2[1] = sample_blk<1>(g, lat, F, 0, o, u, nr, ok);
+    v2[2] = sample_blk<2>(g, lat, F, 0, o, u, nr, ok);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      xx[a] = x[a] + 0.75f * dt * v2[a];
+      u[a] = fminf(fmaxf(xx[a] * g.inv_h, -1.f), (float)g.N[a] + 1.f);
+    }
+    v3[0] = sample_blk<0>(g, lat, F, 0, o, u, nr, ok);
+    v3[1] = sample_blk<1>(g, lat, F, 0, o, u, nr, ok);
+    v3[2] = sample_blk<2>(g, lat, F, 0, o, u, nr, ok);
+    if (!ok) {
+      list[1 + atomicAdd(&list[0], 1u)] = (uint32_t)i;
+      continue;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float xn = x[a] + dt * ((2.f / 9.f) * v1[a] + (3.f / 9.f) * v2[a] + (4.f / 9.f) * v3[a]);
+      const float lo = 0.25f * g.h, hi = (float)g.N[a] * g.h - 0.25f * g.h;
+      vel_out[3 * i + a] = vn[a];
+      pos_out[3 * i + a] = fminf(fmaxf(xn, lo), hi);
+    }
+  }
+}
+
 // ------------------------------------------------------------- adjoint
 // Reverse-mode derivative of the step with the topology, the liquid mask and
 // the mass thresholds held at their forward values.  Fine-lattice adjoint
@@ -3860,6 +3996,18 @@ void k_g2p_advect(hipStream_t s, const GridDev& g, const float* fine, const floa
     g2p_launch<G2P_POS>(s, g, fine, spos, svel, n, alpha, dt, pos_out, vel_out, list);
   else
     g2p_launch<G2P_BOTH>(s, g, fine, spos, svel, n, alpha, dt, pos_out, vel_out, list);
+}
+
+void k_g2p_advect_sorted(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const float* svel,
+                         const uint32_t* pcount, const uint32_t* poff, int64_t n, float alpha, float dt,
+                         float* pos_out, float* vel_out, uint32_t* list) {
+  if (n <= 0) return;
+  launch(fill_kernel<uint32_t>, 1, 64, 0, s, list, 1, 0u);
+  launch(g2p_block_kernel, g.nb, G2B_T, 0, s, g, fine, spos, svel, pcount, poff, alpha, dt, pos_out, vel_out, list);
+  const int64_t nwg = (n + TPB - 1) / TPB;
+  const unsigned fix = (unsigned)(nwg < 1024 ? nwg : 1024);
+  launch(g2p_advect_kernel<true, G2P_BOTH>, fix, TPB, 0, s, g, fine, spos, svel, n, alpha, dt, pos_out, vel_out,
+         list);
 }
 
 size_t adj_record_bytes() { return sizeof(AdjRec); }

@@ -2403,8 +2403,9 @@ int aw_step(aw_grid* g, aw_solver* S, float* d_pos, float* d_vel, int32_t* d_ids
   {
     Prof pf(g->ctx, "g2p_advect", 48.0 * n + 2.0 * 4.0 * 3 * FINE * g->occupied);
     AW_TRY(g->g2p_list.ensure((size_t)(n + 1) * sizeof(uint32_t)));
-    k_g2p_advect(s, g->view(), g->slab.as<float>(), g->spos.as<float>(), g->svel.as<float>(), n, prm->flip_alpha,
-                 prm->dt, d_pos, d_vel, g->g2p_list.as<uint32_t>());
+    k_g2p_advect_sorted(s, g->view(), g->slab.as<float>(), g->spos.as<float>(), g->svel.as<float>(),
+                        g->count.as<uint32_t>(), g->poff.as<uint32_t>(), n, prm->flip_alpha, prm->dt, d_pos, d_vel,
+                        g->g2p_list.as<uint32_t>());
   }
   if (d_ids && n > 0) AW_HIP(hipMemcpyAsync(d_ids, g->sids.p, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
   if (stats) {

@@ -159,7 +159,7 @@ def run_gpu(args, w, pos, vel):
 KERNEL_OF = {"smooth_l0": "smooth_kernel", "smooth_coarse": "smooth_kernel", "smooth_uniform": "uniform_smooth_kernel",
              "residual_l0": "residual_kernel",
              "residual_coarse": "residual_kernel", "residual_restrict_l0": "residual_kernel",
-             "residual_restrict_coarse": "residual_kernel", "g2p_advect": "g2p_advect_kernel",
+             "residual_restrict_coarse": "residual_kernel", "g2p_advect": "g2p_block_kernel",
              "p2g_deposit": "p2g_deposit_kernel", "p2g_gather": "p2g_gather_kernel", "to_fine": "to_fine_kernel",
              "apply_l0": "apply_dot_kernel", "sort_particles": "scatter_kernel", "vcycle_tail": "tail_kernel"}
 
