@@ -2953,9 +2953,11 @@ __global__ __launch_bounds__(ST_T) AW_STENCIL_OCCUPANCY void uniform_smooth_kern
     const int l[3] = {li & (r - 1), (li >> lr) & (r - 1), li >> (2 * lr)};
     const int Bc[3] = {bx, by, bz};
     c[k] = (B << sh3) + li;
+    // every load of the cell issued before any is used (the neighbour
+    // addresses come from coordinates): value, mask, right-hand side and
+    // the out-of-block neighbours
     sx[i] = x[c[k]];
-    if (!liquid[c[k]]) continue;
-    act[k] = true;
+    act[k] = liquid[c[k]] != 0;
     bb[k] = b[c[k]];
     float dgv = 0.f, exv = 0.f;
     uint8_t m = 0;
@@ -3240,19 +3242,31 @@ __global__ __launch_bounds__(TPB) void coarsen_liquid_kernel(LevelDev F, LevelDe
 }
 
 // x_f += scale * x_c[parent] on liquid cells (prolongateCorrection,
-// MultigridTransfer.cuh:63-102)
+// MultigridTransfer.cuh:63-102).  The fine block's descriptor in registers
+// (scalar when the group is one block), then the cell's mask, value and
+// parent offset together, then the parent's value.
+template <bool ONE>
+__device__ __forceinline__ void prolong_group(const LevelDev& F, const LevelDev& C, int aggregated,
+                                              float* __restrict__ xf, const float* __restrict__ xc, float scale,
+                                              const WG& w) {
+  CellRef cr;
+  BDR bd;
+  if (!decode_r<ONE>(F, w, threadIdx.x, cr, bd)) return;
+  const bool liq = F.liquid[cr.c] != 0;
+  const float xv = xf[cr.c];
+  const uint32_t pc = parent_cell(F, C, aggregated, cr.B, cr.l, cr.r, cr.li);
+  if (liq) xf[cr.c] = xv + scale * xc[pc];
+}
+
 __global__ __launch_bounds__(ST_T) void prolong_kernel(LevelDev F, LevelDev C, int aggregated, float* __restrict__ xf,
                                                        const float* __restrict__ xc, float scale,
                                                        WGMap wgm,
                                                        const uint32_t* __restrict__ blist) {
   const WG w = wg_at(wgm, blockIdx.x);
-#pragma unroll
-  for (int k = 0; k < ST_CPT; ++k) {
-    CellRef c;
-    if (!decode(F, w, blist, threadIdx.x + k * ST_T, c)) continue;
-    if (!F.liquid[c.c]) continue;
-    xf[c.c] += scale * xc[parent_cell(F, C, aggregated, c.B, c.l, c.r, c.li)];
-  }
+  if (w.lr == 3)
+    prolong_group<true>(F, C, aggregated, xf, xc, scale, w);
+  else
+    prolong_group<false>(F, C, aggregated, xf, xc, scale, w);
 }
 
 // ---- single-work-group V-cycle over the small trailing levels, in LDS ----
@@ -3419,16 +3433,27 @@ __device__ __forceinline__ void tail_smooth(const TailLevel& T, float* sm, const
                                             bool reverse, float omega, int local_sweeps) {
   float* xs = sm + T.off;
   const float* bs = sm + T.off + T.ncells;
-  // blocks of one cell have a single cell colour: its second sub-phase is empty
+  // Blocks of one cell have a single cell colour, and their in-block sweeps
+  // relax the same cell against frozen neighbours: k updates with weight w
+  // leave (1 - w)^k of its residual, so they are one update with weight
+  // 1 - (1 - w)^k (equal up to rounding), one barrier instead of k.
   const bool one_cell = T.ulr == 0;
+  float w = omega;
+  int ls_n = local_sweeps;
+  if (one_cell) {
+    float keep = 1.f;
+    for (int ls = 0; ls < local_sweeps; ++ls) keep *= 1.f - omega;
+    w = 1.f - keep;
+    ls_n = 1;
+  }
   for (int it = 0; it < iters; ++it)
     for (int ph = 0; ph < 2; ++ph) {
       const int bc = reverse ? 1 - ph : ph;
-      for (int ls = 0; ls < local_sweeps; ++ls)
+      for (int ls = 0; ls < ls_n; ++ls)
         for (int cp = 0; cp < 2; ++cp) {
           const int cc = reverse ? 1 - cp : cp;
           if (one_cell && cc == 1) continue;
-          tail_update(T, xs, bs, liq, (bc << 1) | cc, omega);
+          tail_update(T, xs, bs, liq, (bc << 1) | cc, w);
           __syncthreads();
         }
     }

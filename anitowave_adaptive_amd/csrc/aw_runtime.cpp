@@ -1260,9 +1260,11 @@ void restrict_level(aw_solver* S, int k, const float* r, float* bc, float* xc = 
 void prolong_level(aw_solver* S, int k, float* x, const float* xc) {
   const MGLevel& F = *S->lv[k];
   const MGLevel& C = *S->lv[k + 1];
-  Prof pf(S->g->ctx, "prolong", 8.0 * F.ncells + 4.0 * C.ncells);
-  k_prolong(S->g->ctx->stream, F.view(), C.view(), C.aggregated, x, xc, S->prm.prolong_scale, F.wgm[2],
-            F.bl());
+  // air cells take no correction: the blocks holding liquid are enough
+  const bool live = F.live_on;
+  Prof pf(S->g->ctx, "prolong", live ? 8.0 * F.live_cells + 4.0 * F.live_ccells : 8.0 * F.ncells + 4.0 * C.ncells);
+  k_prolong(S->g->ctx->stream, live ? F.live_view() : F.view(), C.view(), C.aggregated, x, xc,
+            S->prm.prolong_scale, live ? F.wgm_live[2] : F.wgm[2], F.bl());
 }
 
 // recursive V-cycle (VCycleSolver::runCycle, VCycle.cuh:251-304): the
