@@ -1295,43 +1295,103 @@ __device__ __forceinline__ float node_from_faces(const GridDev& g, int field, in
 }
 
 // Every padded node of a flagged block, own or halo, from the faces of the
-// block that owns it (the same sums for a node in every lattice that holds it)
+// block that owns it (the same sums for a node in every lattice that holds
+// it).  The owners' levels and face offsets (the block and its 26
+// neighbours) are read into LDS first, so a node's face loads depend on
+// nothing else; a node takes its 2x2x2 slot products in pi_weights's order,
+// loading only the slots of nonzero weight (the same sums), and a thread
+// works on two nodes at a time.
+__device__ __forceinline__ void fine_node_sums(const GridDev& g, int a, int e, uint32_t fo, const int* l, float& vn,
+                                               float& vd) {
+  const int s = 1 << e, r = BASE >> e;
+  int i0[3];
+  float t[3];
+  const float inv_s = 1.0f / (float)s;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (d == a) {
+      i0[d] = l[d] >> e;
+      t[d] = (float)(l[d] & (s - 1)) * inv_s;
+    } else {
+      float eta = ((float)l[d] + 0.5f) * inv_s - 0.5f;
+      eta = fminf(fmaxf(eta, 0.f), (float)(r - 1));
+      int j = (int)floorf(eta);
+      j = r >= 2 ? min(j, r - 2) : 0;
+      i0[d] = j;
+      t[d] = eta - (float)j;
+    }
+  }
+  float un[8], uo[8], ww[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
+    ww[k] = (dx ? t[0] : 1.f - t[0]) * (dy ? t[1] : 1.f - t[1]) * (dz ? t[2] : 1.f - t[2]);
+    un[k] = uo[k] = 0.f;
+    if (ww[k] != 0.f) {
+      const size_t id = fo + face_idx(a, i0[0] + dx, i0[1] + dy, i0[2] + dz, r);
+      un[k] = g.un[a][id];
+      uo[k] = g.uo[a][id];
+    }
+  }
+  vn = 0.f;
+  vd = 0.f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (ww[k] == 0.f) continue;
+    vn += ww[k] * un[k];
+    vd += ww[k] * (un[k] - uo[k]);
+  }
+}
+
 __global__ __launch_bounds__(TPB) void to_fine_kernel(GridDev g, float* __restrict__ fine) {
+  __shared__ int nlev[27];
+  __shared__ uint32_t nfo[27];
   const int B = blockIdx.x;
   if (!g.halo[B]) return;
   int Bc[3];
   block_coords(g.D, B, Bc);
+  if (threadIdx.x < 27) {
+    const int t = threadIdx.x;
+    const int q[3] = {Bc[0] + t % 3 - 1, Bc[1] + (t / 3) % 3 - 1, Bc[2] + t / 9 - 1};
+    const bool in = q[0] >= 0 && q[1] >= 0 && q[2] >= 0 && q[0] < g.D[0] && q[1] < g.D[1] && q[2] < g.D[2];
+    const int Bo = in ? block_id(g.D, q) : B;
+    nlev[t] = g.level[Bo];
+    nfo[t] = g.foff[0][Bo];   // the three axes' face offsets coincide
+  }
+  __syncthreads();
   float* dst = fine + (size_t)B * SLAB;
-  for (int k = threadIdx.x; k < 3 * PAD; k += TPB) {
-    const int a = k / PAD, pi = k - a * PAD;
-    const int n0 = a == 0 ? 9 : 10, n1 = a == 1 ? 9 : 10;
-    const int lp[3] = {pi % n0, (pi / n0) % n1, pi / (n0 * n1)};
-    int q[3];
-    bool outside = false;
+  for (int k0 = threadIdx.x; k0 < 3 * PAD; k0 += 2 * TPB) {
+    float vn[2], vd[2];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      q[d] = BASE * Bc[d] + lp[d] - (d == a ? 0 : 1);
-      if (d == a)
-        outside = outside || q[d] >= g.N[d];
-      else
-        q[d] = iclamp(q[d], 0, g.N[d] - 1);
-    }
-    float vn = 0.f, vd = 0.f;
-    if (!outside) {
-      const int Bo = (q[0] >> LOGB) + g.D[0] * ((q[1] >> LOGB) + g.D[1] * (q[2] >> LOGB));
-      const int l[3] = {q[0] & 7, q[1] & 7, q[2] & 7};
-      int slot[8];
-      float w[8];
-      const int n = pi_weights(a, g.level[Bo], l, slot, w);
-      for (int j = 0; j < n; ++j) {
-        const size_t id = g.foff[a][Bo] + slot[j];
-        const float un = g.un[a][id];
-        vn += w[j] * un;
-        vd += w[j] * (un - g.uo[a][id]);
+    for (int h = 0; h < 2; ++h) {
+      const int k = k0 + h * TPB;
+      vn[h] = vd[h] = 0.f;
+      if (k >= 3 * PAD) continue;
+      const int a = k / PAD, pi = k - a * PAD;
+      const int n0 = a == 0 ? 9 : 10, n1 = a == 1 ? 9 : 10;
+      const int lp[3] = {pi % n0, (pi / n0) % n1, pi / (n0 * n1)};
+      int q[3];
+      bool outside = false;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        q[d] = BASE * Bc[d] + lp[d] - (d == a ? 0 : 1);
+        if (d == a)
+          outside = outside || q[d] >= g.N[d];
+        else
+          q[d] = iclamp(q[d], 0, g.N[d] - 1);
       }
+      if (outside) continue;
+      const int nb = ((q[0] >> LOGB) - Bc[0] + 1) + 3 * ((q[1] >> LOGB) - Bc[1] + 1) + 9 * ((q[2] >> LOGB) - Bc[2] + 1);
+      const int l[3] = {q[0] & 7, q[1] & 7, q[2] & 7};
+      fine_node_sums(g, a, nlev[nb], nfo[nb], l, vn[h], vd[h]);
     }
-    dst[k] = vn;
-    dst[3 * PAD + k] = vd;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int k = k0 + h * TPB;
+      if (k >= 3 * PAD) continue;
+      dst[k] = vn[h];
+      dst[3 * PAD + k] = vd[h];
+    }
   }
 }
 
