@@ -813,16 +813,26 @@ __global__ __launch_bounds__(TPB) void init_beta_kernel(LevelDev L, float* __res
 // ------------------------------------------------------------- P2G
 // One work-group per block: the block's particles (contiguous after the
 // binning sort) scatter their trilinear weights into the block's padded fine
-// nodes, held in LDS as (M, m) pairs, and the node sums then go to the
-// block's slab with plain stores, so P2G issues no global float atomics
-// (P2G.cuh:16-135 scatters with global float atomics instead).  The LDS adds
-// are 64-bit compare-and-swaps of the pair (lds_add2_spread), with the lanes
-// of a wave spread over the block's particles so that they rarely hit the
-// same node.  (Measured on MI355X at 256^3 / 512^3: 1.23 / 5.31 ms; a gather
-// form that sorts the particles by half-cell in LDS and lets each node sum
-// its support took 2.17 / 10.6 ms, two 32-bit compare-and-swaps per node
-// 1.60 / 6.22 ms with one contiguous run of particles per thread, which
-// itself took 1.59 / 5.12 ms with the pair.)
+// nodes held in LDS, and the node sums then go to the block's slab with
+// plain stores, so P2G issues no global float atomics (P2G.cuh:16-135
+// scatters with global float atomics instead).
+//
+// The sums are fixed point: each weight w and each w v is rounded to a
+// 64-bit integer at a per-block scale 2^S and added with LDS integer adds,
+// which commute, so the deposit does not depend on the order in which
+// threads add (the same bits on every run).  The scale comes from a first,
+// coalesced pass over the block's particles: the largest |v| (< 2^E) and the
+// largest particle count of a fine cell, c_max.  A node collects particles
+// of at most 18 fine cells (two along its face normal, three along each
+// transverse axis), so |sum| < 18 c_max 2^E and S = 62 - ceil(log2(18
+// c_max)) - E keeps every sum, rounding included, inside int64: the
+// resolution 2^-S is about 2^-62 of that bound, far below the rounding of
+// the FP32 product w v that each term starts from.  (A 32-bit sum at that
+// bound, 2^-30 of it, lost up to 3e-4 of max |u| on nodes of small mass.)  A
+// non-finite velocity makes the block's sums NaN, as float sums would.  On
+// gfx950 a no-return LDS u64 add retires about as many pair-adds as a float
+// compare-and-swap pair (1.05 against 0.96 T pair-adds/s measured), and LDS
+// float atomics (ds_add_f32) a tenth.
 constexpr int DEP_T = 256;            // threads per work-group
 constexpr int DEP_NODES = 3 * PAD;    // padded nodes of all three axes
 
@@ -835,8 +845,10 @@ __global__ __launch_bounds__(DEP_T) void p2g_deposit_kernel(GridDev g, const flo
                                                           const uint32_t* __restrict__ pcount,
                                                           const uint32_t* __restrict__ poff, float* __restrict__ slab,
                                                           uint8_t* __restrict__ liquid) {
-  __shared__ float2 acc[DEP_NODES];      // (M, m) per padded node
+  __shared__ unsigned long long accM[DEP_NODES];   // sum of w v per padded node (int64 bits, scale 2^sM)
+  __shared__ unsigned long long accm[DEP_NODES];   // sum of w (scale 2^sm)
   __shared__ uint32_t ccnt[512];         // particles per fine cell
+  __shared__ uint32_t svmax, scmax;      // largest |v| (float bits), largest cell count
   const int B = blockIdx.x, tid = threadIdx.x;
   const uint32_t cnt = pcount[B];
   const int e = g.level[B], r = BASE >> e, nc = r * r * r;
@@ -847,18 +859,59 @@ __global__ __launch_bounds__(DEP_T) void p2g_deposit_kernel(GridDev g, const flo
   }
   int Bc[3];
   block_coords(g.D, B, Bc);
-  for (int k = tid; k < DEP_NODES; k += DEP_T) acc[k] = make_float2(0.f, 0.f);
+  for (int k = tid; k < DEP_NODES; k += DEP_T) accM[k] = accm[k] = 0ull;
   for (int k = tid; k < 512; k += DEP_T) ccnt[k] = 0;
+  if (tid == 0) svmax = scmax = 0u;
   __syncthreads();
   const uint32_t p0 = poff[B];
-  // Particle order: windows of DEP_WIN consecutive particles; in a window
-  // lane l of wave v takes particles DEP_S l + DEP_SPW v + (0 .. DEP_SPW-1).
-  // Lanes sit DEP_S particles apart (particles of one cell are often
-  // neighbours in the sorted order, and lanes of one wave on one node retry
-  // their compare-and-swap), and a work-group's cache lines are those of one
-  // window, which L2 holds until the window is done (one contiguous run per
-  // thread over the whole block re-read the particle arrays from HBM about
-  // nine times over: 11 GB for 1.2 GB at 256^3).
+  // pass 1 (coalesced): fine-cell counts and the largest |v|
+  float vmax = 0.f;
+  bool bad = false;
+  for (uint32_t j = tid; j < cnt; j += DEP_T) {
+    const size_t p = p0 + j;
+    int cell = 0;
+    bool inside = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float ua = fminf(fmaxf(spos[3 * p + a] * g.inv_h, 0.f), (float)g.N[a] - 0.0009765625f);
+      const int ia = (int)ua;
+      inside = inside && (ia >> LOGB) == Bc[a];
+      cell += (ia & 7) << (3 * a);
+    }
+    // a particle outside this block (unsorted input) is skipped
+    if (!inside) continue;
+    atomicAdd(&ccnt[cell], 1u);
+#pragma unroll
+    for (int a = 0; a < (DENS ? 1 : 3); ++a) {
+      const float v = fabsf(DENS ? svel[p] : svel[3 * p + a]);
+      bad = bad || !(v <= 3.402823466e38f);
+      vmax = fmaxf(vmax, v);
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_down(vmax, off, 64));
+  if ((tid & 63) == 0) atomicMax(&svmax, __float_as_uint(vmax));
+  const bool any_bad = __syncthreads_or(bad ? 1 : 0) != 0;
+  {
+    uint32_t cm = 0;
+    for (int k = tid; k < 512; k += DEP_T) cm = max(cm, ccnt[k]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) cm = max(cm, (uint32_t)__shfl_down((int)cm, off, 64));
+    if ((tid & 63) == 0) atomicMax(&scmax, cm);
+  }
+  __syncthreads();
+  int ev, lc = 0;
+  (void)frexpf(__uint_as_float(svmax), &ev);   // |v| < 2^ev
+  while ((1u << lc) < 18u * scmax) ++lc;       // 18 c_max <= 2^lc
+  const int sM = min(100, 62 - lc - ev), sm = 62 - lc;
+  const float scM = ldexpf(1.f, sM), scm = ldexpf(1.f, sm);
+  // pass 2: windows of DEP_WIN consecutive particles; in a window lane l of
+  // wave v takes particles DEP_S l + DEP_SPW v + (0 .. DEP_SPW-1).  Lanes sit
+  // DEP_S particles apart (particles of one cell are often neighbours in the
+  // sorted order, and lanes of one wave on one node serialise), and a
+  // work-group's cache lines are those of one window, which L2 holds until
+  // the window is done (one contiguous run per thread over the whole block
+  // re-read the particle arrays from HBM about nine times over).
   constexpr uint32_t DEP_S = 16, DEP_WIN = 64 * DEP_S, DEP_SPW = DEP_WIN / DEP_T;
   const uint32_t lane = tid & 63, wv = tid >> 6;
   const uint32_t nit = (cnt + DEP_WIN - 1) / DEP_WIN * DEP_SPW;
@@ -875,9 +928,7 @@ __global__ __launch_bounds__(DEP_T) void p2g_deposit_kernel(GridDev g, const flo
       v[a] = DENS ? svel[p] : svel[3 * p + a];
       inside = inside && ((int)ua >> LOGB) == Bc[a];
     }
-    // a particle outside this block (unsorted input) is skipped
     if (!inside) continue;
-    atomicAdd(&ccnt[(int)u[0] + 8 * ((int)u[1] + 8 * (int)u[2])], 1u);
     // per axis: lower node and the two hat weights on the face-normal lattice
     // (nodes at integer u, padded index = node) and on the transverse one
     // (nodes at u - 1/2, padded index = node + 1).  A transverse node outside
@@ -907,20 +958,23 @@ __global__ __launch_bounds__(DEP_T) void p2g_deposit_kernel(GridDev g, const flo
       const float* wy = a == 1 ? wn[1] : wt[1];
       const float* wz = a == 2 ? wn[2] : wt[2];
       const int base = a * PAD + lx + nx * (ly + ny * lz);
+      const float vs = v[a] * scM;
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
         const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
         const float w = wx[dx] * wy[dy] * wz[dz];
-        lds_add2_spread(&acc[base + dx + nx * (dy + ny * dz)], w * v[a], w);
+        const int k = base + dx + nx * (dy + ny * dz);
+        atomicAdd(&accM[k], (unsigned long long)__float2ll_rn(w * vs));
+        atomicAdd(&accm[k], (unsigned long long)__float2ll_rn(w * scm));
       }
     }
   }
   __syncthreads();
   float* dst = slab + (size_t)B * SLAB;
+  const float iM = any_bad ? __uint_as_float(0x7fc00000u) : ldexpf(1.f, -sM), im = ldexpf(1.f, -sm);
   for (int k = tid; k < DEP_NODES; k += DEP_T) {
-    const float2 s = acc[k];
-    dst[k] = s.x;
-    dst[DEP_NODES + k] = s.y;
+    dst[k] = (float)(long long)accM[k] * iM;
+    dst[DEP_NODES + k] = (float)(long long)accm[k] * im;
   }
   // native-cell liquid mask from the fine-cell counts
   for (int k = tid; k < nc; k += DEP_T) {
