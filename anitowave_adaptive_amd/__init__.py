@@ -493,7 +493,7 @@ class G2PManager:
         check(lib().aw_advect(grid.handle, d_pos, int(n), float(dt)), "aw_advect")
 
 
-def solver_params(method=0, pre=2, post=2, coarse=16, local=2, omega=0.85714, prolong_scale=1.0, precision=0,
+def solver_params(method=0, pre=2, post=2, coarse=16, local=3, omega=0.85714, prolong_scale=1.0, precision=0,
                   adaptive=0, theta=1e-4, max_block_sweeps=10, block_reduction=0.33):
     """adaptive=1: the reference's adaptive smoothing (method 1 only): early exit
     per block at block_reduction, activation above theta * max|b| (Smoother.cuh)."""

@@ -2828,6 +2828,20 @@ __device__ __forceinline__ float inblock(const float* w, int li, int r, In inb) 
   return s;
 }
 
+// inblock without branches: every face reads a value (the cell's own where
+// the face has no in-block neighbour, weighed by 0), the same sum
+template <class In>
+__device__ __forceinline__ float inblock_all(const float* w, int li, int r, In inb) {
+  float s = 0.f;
+  const int st[3] = {1, r, r * r};
+#pragma unroll
+  for (int d = 0; d < 6; ++d) {
+    const int nl = li + ((d & 1) ? st[d >> 1] : -st[d >> 1]);
+    s += w[d] * inb(w[d] != 0.f ? nl : li);
+  }
+  return s;
+}
+
 // parent cell on the next coarser level: same block halving (local / 2),
 // same block keeping its resolution, or the aggregated block (coords / 2)
 __device__ __forceinline__ uint32_t parent_cell(const LevelDev& F, const LevelDev& C, int aggregated, int B,
@@ -2873,15 +2887,19 @@ __device__ __forceinline__ void smooth_group(const LevelDev& L, float* __restric
   const float bb = b[cr.c];
   stencil_r(L, cr, bd, [&](uint32_t cc) { return x[cc]; }, dg, ex, cw);
   sx[i] = xv;
+  // the relaxation weight omega / diag once per cell (not a division per update)
+  const float wod = dg > 0.f ? omega / dg : 0.f;
+  const bool mine = act && dg > 0.f;
+  const int cc = (cr.l[0] + cr.l[1] + cr.l[2]) & 1;
   __syncthreads();
   const int base = i - cr.li;
   for (int ls = 0; ls < local_sweeps; ++ls)
     for (int ph = 0; ph < 2; ++ph) {
       const int col = reverse ? 1 - ph : ph;
-      if (act && ((cr.l[0] + cr.l[1] + cr.l[2]) & 1) == col) {
-        const float off = ex + inblock(cw, cr.li, cr.r, [&](int li) { return sx[base + li]; });
+      if (mine && cc == col) {
+        const float off = ex + inblock_all(cw, cr.li, cr.r, [&](int li) { return sx[base + li]; });
         const float res = bb - (dg * sx[i] - off);
-        if (dg > 0.f) sx[i] += omega * res / dg;
+        sx[i] += wod * res;
       }
       __syncthreads();
     }
@@ -3098,6 +3116,14 @@ __global__ __launch_bounds__(ST_T) AW_STENCIL_OCCUPANCY void uniform_smooth_kern
     lo[k][1] = l[1];
     lo[k][2] = l[2];
   }
+  // the relaxation weight omega / diag once per cell; every face reads a
+  // value (the cell's own, weighed by 0, where it has no in-block neighbour)
+  float wod[ST_CPT];
+#pragma unroll
+  for (int k = 0; k < ST_CPT; ++k) {
+    wod[k] = act[k] && dg[k] > 0.f ? omega / dg[k] : 0.f;
+    act[k] = act[k] && dg[k] > 0.f;
+  }
   __syncthreads();
   const int st[3] = {1, r, r * r};
   for (int ls = 0; ls < local_sweeps; ++ls)
@@ -3110,11 +3136,13 @@ __global__ __launch_bounds__(ST_T) AW_STENCIL_OCCUPANCY void uniform_smooth_kern
         const int i = threadIdx.x + k * ST_T;
         float in = 0.f;
 #pragma unroll
-        for (int d = 0; d < 6; ++d)
-          if (inb[k] & (1 << d)) in += s * sx[i + ((d & 1) ? st[d >> 1] : -st[d >> 1])];
+        for (int d = 0; d < 6; ++d) {
+          const bool has = inb[k] & (1 << d);
+          in += (has ? s : 0.f) * sx[has ? i + ((d & 1) ? st[d >> 1] : -st[d >> 1]) : i];
+        }
         const float off = ex[k] + in;
         const float res = bb[k] - (dg[k] * sx[i] - off);
-        if (dg[k] > 0.f) sx[i] += omega * res / dg[k];
+        sx[i] += wod[k] * res;
       }
       __syncthreads();
     }

@@ -108,9 +108,10 @@ typedef struct {
   int pre_sweeps;      /* V(pre, post) */
   int post_sweeps;
   int coarse_sweeps;   /* symmetric sweeps on the coarsest level */
-  int local_sweeps;    /* in-block red-black sweeps per block phase (2 in the
-                          Python and C++ wrappers: fewer cycles for one more
-                          sweep held in LDS) */
+  int local_sweeps;    /* in-block red-black sweeps per block phase (3 in the
+                          Python and C++ wrappers: the sweeps run in LDS, and
+                          each one more saves V-cycles; the reference sweeps
+                          a block until its residual falls by 0.33, at most 10) */
   float omega;         /* Gauss-Seidel weight (reference 0.85714) */
   float prolong_scale; /* coarse-correction scale (1 for PCG, 0.75 recommended for stationary) */
   int precision;       /* 0 = FP32; 1 = FP32 V-cycles inside FP64 iterative refinement */

@@ -205,7 +205,7 @@ class VCycleSolver {
     p.pre_sweeps = 2;
     p.post_sweeps = 2;
     p.coarse_sweeps = 16;
-    p.local_sweeps = 2;
+    p.local_sweeps = 3;
     p.omega = 0.85714f;
     p.prolong_scale = 1.0f;
     p.precision = 0;

@@ -105,7 +105,7 @@ struct OGrid {
   // multigrid
   std::vector<Level> mg;
   double omega = 0.85714f;
-  int local_sweeps = 2, pre = 2, post = 2, coarse_sweeps = 16;
+  int local_sweeps = 3, pre = 2, post = 2, coarse_sweeps = 16;
   double prolong_scale = 1.0;
   // adaptive smoothing (AdaptiveSmoother, Smoother.cuh:105-291)
   int adaptive = 0;

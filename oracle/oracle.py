@@ -257,7 +257,7 @@ class OracleGrid:
         lib().og_set_cell(self._h, CELL_FIELDS[name], _p(vals, _D))
 
     # solver
-    def set_params(self, omega=float(np.float32(0.85714)), local_sweeps=2, pre=2, post=2, coarse_sweeps=16,
+    def set_params(self, omega=float(np.float32(0.85714)), local_sweeps=3, pre=2, post=2, coarse_sweeps=16,
                    prolong_scale=1.0):
         lib().og_set_params(self._h, float(omega), local_sweeps, pre, post, coarse_sweeps, float(prolong_scale))
 
