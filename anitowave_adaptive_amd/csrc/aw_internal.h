@@ -157,16 +157,27 @@ void k_cell_offsets(hipStream_t s, const uint8_t* level, int nb, uint32_t* coff,
 void k_layout_offsets(hipStream_t s, const uint8_t* level, int nb, uint32_t* coff, uint32_t* fx, uint32_t* fy,
                       uint32_t* fz, uint32_t* scratch);
 // dens (needs ids): per-particle density by particle id into sdens (sorted order), or null
+// Largest-|value| bounds (the P2G deposit's fixed-point scale) are VB_SLOTS
+// words of float bits each, the largest slot being the bound; NaN bits mark a
+// value that is not finite.
+constexpr int VB_SLOTS = 64;
+// the binning sort's copy; also the bounds of |vel| (vbound[0 .. VB_SLOTS))
+// and |dens| (vbound[VB_SLOTS .. 2 VB_SLOTS), when dens)
 void k_scatter(hipStream_t s, const float* pos, const float* vel, const int32_t* ids, const int32_t* key,
-               uint32_t* cursor, int64_t n, float* spos, float* svel, int32_t* sids, const float* dens = nullptr,
-               float* sdens = nullptr);
+               uint32_t* cursor, int64_t n, float* spos, float* svel, int32_t* sids, const float* dens,
+               float* sdens, uint32_t* vbound);
+// the bound of |v[i]|, i < n, into out[0 .. VB_SLOTS)
+void k_vbound(hipStream_t s, const float* v, int64_t n, uint32_t* out);
 void k_init_beta(hipStream_t s, const LevelDev& L, float* bx, float* by, float* bz);
 
 // transfers
 // sdens (sorted): deposit the particles' density instead of their velocity (the
 // slab's first field then holds the density-weighted weights)
+// vbound: the bounds of the deposited particles' |velocity| (and, with
+// sdens, |density|), which set the fixed-point scale
 void k_p2g_deposit(hipStream_t s, const GridDev& g, const float* spos, const float* svel, const uint32_t* pcount,
-                   const uint32_t* poff, float* slab, uint8_t* liquid, const float* sdens = nullptr);
+                   const uint32_t* poff, float* slab, uint8_t* liquid, const uint32_t* vbound,
+                   const float* sdens = nullptr);
 // beta = m / D folded (two-phase coefficients), 0 on walls
 void k_beta_fold(hipStream_t s, const GridDev& g, float* const* macc, float* const* Dacc, float* const* beta);
 void k_p2g_gather(hipStream_t s, const GridDev& g, const uint32_t* pcount, const float* slab, float* const* Macc,

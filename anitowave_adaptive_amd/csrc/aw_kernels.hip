@@ -730,12 +730,27 @@ __global__ __launch_bounds__(TPB) void scan_final_kernel(const uint32_t* __restr
 
 // dens (with ids): per-particle density by particle id, gathered into sdens
 // in the sorted order
+// A largest-|v| bound is kept in VB_SLOTS words (float bits, work-group b
+// into slot b mod VB_SLOTS), so that tens of thousands of work-groups do not
+// serialise on one address; vbound_get takes the largest slot.  Call
+// vbound_add from every thread of the work-group.
+__device__ __forceinline__ void vbound_add(uint32_t* slots, uint32_t bits) {
+  __shared__ uint32_t wmax;
+  if (threadIdx.x == 0) wmax = 0u;
+  __syncthreads();
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) bits = max(bits, (uint32_t)__shfl_down((int)bits, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(&wmax, bits);
+  __syncthreads();
+  if (threadIdx.x == 0 && wmax) atomicMax(&slots[blockIdx.x % VB_SLOTS], wmax);
+}
+
 __global__ __launch_bounds__(TPB) void scatter_kernel(const float* __restrict__ pos, const float* __restrict__ vel,
                                                       const int32_t* __restrict__ ids,
                                                       const int32_t* __restrict__ key, uint32_t* __restrict__ cursor,
                                                       int64_t n, float* __restrict__ spos, float* __restrict__ svel,
                                                       int32_t* __restrict__ sids, const float* __restrict__ dens,
-                                                      float* __restrict__ sdens) {
+                                                      float* __restrict__ sdens, uint32_t* __restrict__ vbound) {
   __shared__ int hk[AGG_SLOTS];
   __shared__ uint32_t hc[AGG_SLOTS];
   for (int s = threadIdx.x; s < AGG_SLOTS; s += TPB) {
@@ -769,6 +784,11 @@ __global__ __launch_bounds__(TPB) void scatter_kernel(const float* __restrict__ 
   for (int s = threadIdx.x; s < AGG_SLOTS; s += TPB)
     if (hk[s] >= 0) hc[s] = atomicAdd(&cursor[hk[s]], hc[s]);
   __syncthreads();
+  // the largest |v| and |density| on the way (the P2G deposit's fixed-point
+  // scale); a non-finite value gives NaN bits, which compare above every
+  // finite value's
+  float vm = 0.f, dm = 0.f;
+  bool bad = false, dbad = false;
 #pragma unroll
   for (int j = 0; j < AGG_PPT; ++j) {
     const int64_t i = base + j * TPB + threadIdx.x;
@@ -777,11 +797,34 @@ __global__ __launch_bounds__(TPB) void scatter_kernel(const float* __restrict__ 
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       spos[3 * s + a] = pos[3 * i + a];
-      svel[3 * s + a] = vel[3 * i + a];
+      const float v = vel[3 * i + a];
+      svel[3 * s + a] = v;
+      bad = bad || !(fabsf(v) <= 3.402823466e38f);
+      vm = fmaxf(vm, fabsf(v));
     }
     if (ids) sids[s] = ids[i];
-    if (dens) sdens[s] = dens[ids[i]];
+    if (dens) {
+      const float d = dens[ids[i]];
+      sdens[s] = d;
+      dbad = dbad || !(fabsf(d) <= 3.402823466e38f);
+      dm = fmaxf(dm, fabsf(d));
+    }
   }
+  vbound_add(vbound, bad ? 0x7fc00000u : __float_as_uint(vm));
+  if (dens) vbound_add(vbound + VB_SLOTS, dbad ? 0x7fc00000u : __float_as_uint(dm));
+}
+
+// the largest |value| of v[0..n) into *out (float bits, NaN bits for a
+// non-finite value; *out cleared by the launcher)
+__global__ __launch_bounds__(TPB) void vbound_kernel(const float* __restrict__ v, int64_t n, uint32_t* __restrict__ out) {
+  float m = 0.f;
+  bool bad = false;
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) {
+    const float a = fabsf(v[i]);
+    bad = bad || !(a <= 3.402823466e38f);
+    m = fmaxf(m, a);
+  }
+  vbound_add(out, bad ? 0x7fc00000u : __float_as_uint(m));
 }
 
 // beta = 1 on interior slots, 0 on tank-wall slots
@@ -820,19 +863,17 @@ __global__ __launch_bounds__(TPB) void init_beta_kernel(LevelDev L, float* __res
 // The sums are fixed point: each weight w and each w v is rounded to a
 // 64-bit integer at a per-block scale 2^S and added with LDS integer adds,
 // which commute, so the deposit does not depend on the order in which
-// threads add (the same bits on every run).  The scale comes from a first,
-// coalesced pass over the block's particles: the largest |v| (< 2^E) and the
-// largest particle count of a fine cell, c_max.  A node collects particles
-// of at most 18 fine cells (two along its face normal, three along each
-// transverse axis), so |sum| < 18 c_max 2^E and S = 62 - ceil(log2(18
-// c_max)) - E keeps every sum, rounding included, inside int64: the
-// resolution 2^-S is about 2^-62 of that bound, far below the rounding of
-// the FP32 product w v that each term starts from.  (A 32-bit sum at that
-// bound, 2^-30 of it, lost up to 3e-4 of max |u| on nodes of small mass.)  A
-// non-finite velocity makes the block's sums NaN, as float sums would.  On
-// gfx950 a no-return LDS u64 add retires about as many pair-adds as a float
-// compare-and-swap pair (1.05 against 0.96 T pair-adds/s measured), and LDS
-// float atomics (ds_add_f32) a tenth.
+// threads add (the same bits on every run).  With n the block's particles
+// and |v| < 2^E (E from vbound, the largest |v| of the launch, which the
+// binning sort records as it copies the particles), every node sum is below
+// n 2^E, so S = 62 - ceil(log2 n) - E keeps it, rounding included, inside
+// int64: the resolution 2^-S, about 1e-13 of max |v| at 8^3 blocks, is far
+// below the rounding of the FP32 product w v each term starts from.  (A
+// 32-bit sum, 2^-30 of even a tighter bound, lost up to 3e-4 of max |u| on
+// nodes of small mass.)  A non-finite velocity makes the sums NaN, as float
+// sums would.  On gfx950 a no-return LDS u64 add retires about as many
+// pair-adds as a float compare-and-swap pair (1.05 against 0.96 T
+// pair-adds/s measured), and LDS float atomics (ds_add_f32) a tenth.
 constexpr int DEP_T = 256;            // threads per work-group
 constexpr int DEP_NODES = 3 * PAD;    // padded nodes of all three axes
 
@@ -844,11 +885,11 @@ __global__ __launch_bounds__(DEP_T) void p2g_deposit_kernel(GridDev g, const flo
                                                           const float* __restrict__ svel,
                                                           const uint32_t* __restrict__ pcount,
                                                           const uint32_t* __restrict__ poff, float* __restrict__ slab,
-                                                          uint8_t* __restrict__ liquid) {
+                                                          uint8_t* __restrict__ liquid,
+                                                          const uint32_t* __restrict__ vbound) {
   __shared__ unsigned long long accM[DEP_NODES];   // sum of w v per padded node (int64 bits, scale 2^sM)
   __shared__ unsigned long long accm[DEP_NODES];   // sum of w (scale 2^sm)
-  __shared__ uint32_t ccnt[512];         // particles per fine cell
-  __shared__ uint32_t svmax, scmax;      // largest |v| (float bits), largest cell count
+  __shared__ uint32_t ccnt[512];                   // particles per fine cell
   const int B = blockIdx.x, tid = threadIdx.x;
   const uint32_t cnt = pcount[B];
   const int e = g.level[B], r = BASE >> e, nc = r * r * r;
@@ -861,51 +902,24 @@ __global__ __launch_bounds__(DEP_T) void p2g_deposit_kernel(GridDev g, const flo
   block_coords(g.D, B, Bc);
   for (int k = tid; k < DEP_NODES; k += DEP_T) accM[k] = accm[k] = 0ull;
   for (int k = tid; k < 512; k += DEP_T) ccnt[k] = 0;
-  if (tid == 0) svmax = scmax = 0u;
-  __syncthreads();
-  const uint32_t p0 = poff[B];
-  // pass 1 (coalesced): fine-cell counts and the largest |v|
-  float vmax = 0.f;
-  bool bad = false;
-  for (uint32_t j = tid; j < cnt; j += DEP_T) {
-    const size_t p = p0 + j;
-    int cell = 0;
-    bool inside = true;
+  __shared__ uint32_t svb;
+  if (tid < 64) {
+    uint32_t m = tid < VB_SLOTS ? vbound[(DENS ? VB_SLOTS : 0) + tid] : 0u;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const float ua = fminf(fmaxf(spos[3 * p + a] * g.inv_h, 0.f), (float)g.N[a] - 0.0009765625f);
-      const int ia = (int)ua;
-      inside = inside && (ia >> LOGB) == Bc[a];
-      cell += (ia & 7) << (3 * a);
-    }
-    // a particle outside this block (unsorted input) is skipped
-    if (!inside) continue;
-    atomicAdd(&ccnt[cell], 1u);
-#pragma unroll
-    for (int a = 0; a < (DENS ? 1 : 3); ++a) {
-      const float v = fabsf(DENS ? svel[p] : svel[3 * p + a]);
-      bad = bad || !(v <= 3.402823466e38f);
-      vmax = fmaxf(vmax, v);
-    }
-  }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_down(vmax, off, 64));
-  if ((tid & 63) == 0) atomicMax(&svmax, __float_as_uint(vmax));
-  const bool any_bad = __syncthreads_or(bad ? 1 : 0) != 0;
-  {
-    uint32_t cm = 0;
-    for (int k = tid; k < 512; k += DEP_T) cm = max(cm, ccnt[k]);
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) cm = max(cm, (uint32_t)__shfl_down((int)cm, off, 64));
-    if ((tid & 63) == 0) atomicMax(&scmax, cm);
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_down((int)m, o, 64));
+    if (tid == 0) svb = m;
   }
   __syncthreads();
+  const uint32_t vb = svb;
+  const bool bad = vb >= 0x7f800000u;              // a non-finite |v|
   int ev, lc = 0;
-  (void)frexpf(__uint_as_float(svmax), &ev);   // |v| < 2^ev
-  while ((1u << lc) < 18u * scmax) ++lc;       // 18 c_max <= 2^lc
+  (void)frexpf(bad ? 1.f : __uint_as_float(vb), &ev);   // |v| < 2^ev
+  while (lc < 32 && (1u << lc) < cnt) ++lc;              // cnt <= 2^lc
   const int sM = min(100, 62 - lc - ev), sm = 62 - lc;
   const float scM = ldexpf(1.f, sM), scm = ldexpf(1.f, sm);
-  // pass 2: windows of DEP_WIN consecutive particles; in a window lane l of
+  __syncthreads();
+  const uint32_t p0 = poff[B];
+  // windows of DEP_WIN consecutive particles; in a window lane l of
   // wave v takes particles DEP_S l + DEP_SPW v + (0 .. DEP_SPW-1).  Lanes sit
   // DEP_S particles apart (particles of one cell are often neighbours in the
   // sorted order, and lanes of one wave on one node serialise), and a
@@ -928,7 +942,9 @@ __global__ __launch_bounds__(DEP_T) void p2g_deposit_kernel(GridDev g, const flo
       v[a] = DENS ? svel[p] : svel[3 * p + a];
       inside = inside && ((int)ua >> LOGB) == Bc[a];
     }
+    // a particle outside this block (unsorted input) is skipped
     if (!inside) continue;
+    atomicAdd(&ccnt[(int)u[0] + 8 * ((int)u[1] + 8 * (int)u[2])], 1u);
     // per axis: lower node and the two hat weights on the face-normal lattice
     // (nodes at integer u, padded index = node) and on the transverse one
     // (nodes at u - 1/2, padded index = node + 1).  A transverse node outside
@@ -971,7 +987,7 @@ __global__ __launch_bounds__(DEP_T) void p2g_deposit_kernel(GridDev g, const flo
   }
   __syncthreads();
   float* dst = slab + (size_t)B * SLAB;
-  const float iM = any_bad ? __uint_as_float(0x7fc00000u) : ldexpf(1.f, -sM), im = ldexpf(1.f, -sm);
+  const float iM = bad ? __uint_as_float(0x7fc00000u) : ldexpf(1.f, -sM), im = ldexpf(1.f, -sm);
   for (int k = tid; k < DEP_NODES; k += DEP_T) {
     dst[k] = (float)(long long)accM[k] * iM;
     dst[DEP_NODES + k] = (float)(long long)accm[k] * im;
@@ -3179,7 +3195,7 @@ __device__ __forceinline__ void residual_group(const LevelDev& L, const float* _
     float rv = 0.f;
     if (liq) {
       const int base = i - cr.li;
-      const float off = ex + inblock(cw, cr.li, cr.r, [&](int li) { return sx[base + li]; });
+      const float off = ex + inblock_all(cw, cr.li, cr.r, [&](int li) { return sx[base + li]; });
       rv = bb - (dg * sx[i] - off);
     }
     sr[i] = rv;
@@ -3244,7 +3260,7 @@ __device__ __forceinline__ double apply_group(const LevelDev& L, const float* __
     return 0.0;
   }
   const int base = i - cr.li;
-  const float off = ex + inblock(cw, cr.li, cr.r, [&](int li) { return sx[base + li]; });
+  const float off = ex + inblock_all(cw, cr.li, cr.r, [&](int li) { return sx[base + li]; });
   const float v = dg * sx[i] - off;
   y[cr.c] = v;
   return (double)sx[i] * (double)v;
@@ -4089,10 +4105,18 @@ void k_layout_offsets(hipStream_t s, const uint8_t* level, int nb, uint32_t* cof
 }
 
 void k_scatter(hipStream_t s, const float* pos, const float* vel, const int32_t* ids, const int32_t* key,
-               uint32_t* cursor, int64_t n, float* spos, float* svel, int32_t* sids, const float* dens, float* sdens) {
+               uint32_t* cursor, int64_t n, float* spos, float* svel, int32_t* sids, const float* dens, float* sdens,
+               uint32_t* vbound) {
+  launch(fill_kernel<uint32_t>, 1, 64, 0, s, vbound, 2 * VB_SLOTS, 0u);
   if (n > 0)
-    launch(scatter_kernel, (unsigned)((n + TPB * AGG_PPT - 1) / (TPB * AGG_PPT)), TPB, 0, s, 
-        pos, vel, ids, key, cursor, n, spos, svel, sids, dens, sdens);
+    launch(scatter_kernel, (unsigned)((n + TPB * AGG_PPT - 1) / (TPB * AGG_PPT)), TPB, 0, s, pos, vel, ids, key,
+           cursor, n, spos, svel, sids, dens, sdens, vbound);
+}
+
+void k_vbound(hipStream_t s, const float* v, int64_t n, uint32_t* out) {
+  launch(fill_kernel<uint32_t>, 1, 64, 0, s, out, VB_SLOTS, 0u);
+  const int64_t wg = (n + TPB - 1) / TPB;
+  if (n > 0) launch(vbound_kernel, (unsigned)(wg < 2048 ? wg : 2048), TPB, 0, s, v, n, out);
 }
 
 void k_init_beta(hipStream_t s, const LevelDev& L, float* bx, float* by, float* bz) {
@@ -4100,11 +4124,11 @@ void k_init_beta(hipStream_t s, const LevelDev& L, float* bx, float* by, float* 
 }
 
 void k_p2g_deposit(hipStream_t s, const GridDev& g, const float* spos, const float* svel, const uint32_t* pcount,
-                   const uint32_t* poff, float* slab, uint8_t* liquid, const float* sdens) {
+                   const uint32_t* poff, float* slab, uint8_t* liquid, const uint32_t* vbound, const float* sdens) {
   if (sdens)
-    launch(p2g_deposit_kernel<true>, g.nb, DEP_T, 0, s, g, spos, sdens, pcount, poff, slab, liquid);
+    launch(p2g_deposit_kernel<true>, g.nb, DEP_T, 0, s, g, spos, sdens, pcount, poff, slab, liquid, vbound);
   else
-    launch(p2g_deposit_kernel<false>, g.nb, DEP_T, 0, s, g, spos, svel, pcount, poff, slab, liquid);
+    launch(p2g_deposit_kernel<false>, g.nb, DEP_T, 0, s, g, spos, svel, pcount, poff, slab, liquid, vbound);
 }
 
 void k_p2g_gather(hipStream_t s, const GridDev& g, const uint32_t* pcount, const float* slab, float* const* Macc,

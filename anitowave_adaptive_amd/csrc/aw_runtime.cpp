@@ -245,6 +245,12 @@ struct aw_grid {
   float h = 0.f, inv_h = 0.f;
   DevBuf level, surf, ltmp, count, poff, cursor, coff, foff, nbr, key, halo, g2p_list, scan_tmp, rmap;
   DevBuf spos, svel, sids;
+  // largest |vel| and |density| of the particles in spos / svel, sdens (float
+  // bits), recorded by the binning sort; vbound_of is the velocity array
+  // they describe (another array gets its own pass before a deposit)
+  DevBuf vbound;
+  const void* vbound_of = nullptr;
+  const void* vbound_dens_of = nullptr;
   DevBuf liquid, p, rhs, slab;
   DevBuf beta[3], Macc[3], macc[3], uold[3], ustar[3], unew[3], mass[3];
   // adjoint fields: fine-lattice adjoints (two fields each, stride ADJ), per-slot and per-cell adjoints
@@ -556,9 +562,12 @@ int sort_particles(aw_grid* g, const float* d_pos, const float* d_vel, const int
   AW_TRY(g->scan_tmp.ensure(scan_scratch_words(g->nb) * sizeof(uint32_t)));
   k_scan_u32(s, g->count.as<uint32_t>(), g->poff.as<uint32_t>(), g->nb, g->scan_tmp.as<uint32_t>());
   AW_HIP(hipMemcpyAsync(g->cursor.p, g->poff.p, (size_t)g->nb * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+  AW_TRY(g->vbound.ensure(2 * VB_SLOTS * sizeof(uint32_t)));
   k_scatter(s, d_pos, d_vel, d_ids, g->key.as<int32_t>(), g->cursor.as<uint32_t>(), n, g->spos.as<float>(),
             g->svel.as<float>(), d_ids ? g->sids.as<int32_t>() : nullptr, d_density,
-            d_density ? g->sdens.as<float>() : nullptr);
+            d_density ? g->sdens.as<float>() : nullptr, g->vbound.as<uint32_t>());
+  g->vbound_of = g->svel.p;
+  g->vbound_dens_of = d_density ? g->sdens.p : nullptr;
   g->n_sorted = n;
   g->sorted_ids = d_ids != nullptr;
   return launch_status("sort_particles");
@@ -573,13 +582,19 @@ int p2g_sorted(aw_grid* g, const float* spos, const float* svel, float dt, const
   GridDev gv = g->view();
   const double nf3 = 3.0 * g->nfaces;
   float* mm[3] = {g->macc[0].as<float>(), g->macc[1].as<float>(), g->macc[2].as<float>()};
+  // the deposit's fixed-point scale needs the largest |v| (and |density|):
+  // recorded by the binning sort for its own copy, measured here otherwise
+  AW_TRY(g->vbound.ensure(2 * VB_SLOTS * sizeof(uint32_t)));
+  uint32_t* vb = g->vbound.as<uint32_t>();
+  if (svel != g->vbound_of) k_vbound(s, svel, 3 * g->n_sorted, vb);
+  if (sdens && sdens != g->vbound_dens_of) k_vbound(s, sdens, g->n_sorted, vb + VB_SLOTS);
   if (sdens) {
     for (int a = 0; a < 3; ++a) AW_TRY(g->Dacc[a].ensure((size_t)g->nfaces * sizeof(float)));
     float* D[3] = {g->Dacc[0].as<float>(), g->Dacc[1].as<float>(), g->Dacc[2].as<float>()};
     {
       Prof pf(g->ctx, "p2g_deposit_density", 8.0 * g->n_sorted + 4.0 * SLAB * g->occupied + g->ncells);
       k_p2g_deposit(s, gv, spos, nullptr, g->count.as<uint32_t>(), g->poff.as<uint32_t>(), g->slab.as<float>(),
-                    g->liquid.as<uint8_t>(), sdens);
+                    g->liquid.as<uint8_t>(), vb, sdens);
     }
     {
       // the weights go to macc here and are deposited again with the velocities below
@@ -608,7 +623,7 @@ int p2g_sorted(aw_grid* g, const float* spos, const float* svel, float dt, const
   {
     Prof pf(g->ctx, "p2g_deposit", 24.0 * g->n_sorted + 4.0 * SLAB * g->occupied + g->ncells);
     k_p2g_deposit(s, gv, spos, svel, g->count.as<uint32_t>(), g->poff.as<uint32_t>(), g->slab.as<float>(),
-                  g->liquid.as<uint8_t>());
+                  g->liquid.as<uint8_t>(), vb);
   }
   float* M[3] = {g->Macc[0].as<float>(), g->Macc[1].as<float>(), g->Macc[2].as<float>()};
   float* m[3] = {g->macc[0].as<float>(), g->macc[1].as<float>(), g->macc[2].as<float>()};
@@ -1797,6 +1812,7 @@ int tape_restore(aw_tape* t, const aw_tape::Step& r) {
   }
   AW_HIP(hipMemcpyAsync(g->spos.p, r.spos.p, (size_t)t->n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s));
   AW_HIP(hipMemcpyAsync(g->svel.p, r.svel.p, (size_t)t->n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s));
+  g->vbound_of = g->vbound_dens_of = nullptr;   // the sorted arrays no longer hold what the bound was taken of
   AW_HIP(hipMemcpyAsync(g->sids.p, r.sids.p, (size_t)t->n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
   AW_HIP(hipMemcpyAsync(g->count.p, r.count.p, (size_t)g->nb * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
   AW_HIP(hipMemcpyAsync(g->poff.p, r.poff.p, (size_t)(g->nb + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
