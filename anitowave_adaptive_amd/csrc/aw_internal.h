@@ -210,8 +210,11 @@ void k_g2p_advect_sorted(hipStream_t s, const GridDev& g, const float* fine, con
 // scatter record of adj_record_bytes() per particle), then the scatter pass,
 // one work-group per block over its particles [poff[b], poff[b] + pcount[b])
 size_t adj_record_bytes();
+// pcount, poff: the block-sorted particles' per-block ranges (one work-group
+// per block, the forward lattice in LDS), or null (one thread per particle)
 void k_g2p_adjoint_stages(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const int32_t* sids,
-                          int64_t n, float alpha, float dt, float* xbar, float* vbar, void* rec, uint32_t* list);
+                          int64_t n, float alpha, float dt, float* xbar, float* vbar, void* rec, uint32_t* list,
+                          const uint32_t* pcount = nullptr, const uint32_t* poff = nullptr);
 void k_g2p_adjoint_scatter(hipStream_t s, const GridDev& g, const float* spos, const uint32_t* pcount,
                            const uint32_t* poff, const void* rec, float* fbar);
 void k_pi_transpose(hipStream_t s, const GridDev& g, const float* fbar, float* const* unb, float* const* uob);

@@ -2093,12 +2093,118 @@ struct AdjRec {
 // particle with a stage point in a block without halo goes to `list`
 // (list[0] counts) and nothing is written for it; the fix-up pass (SLOW)
 // redoes the listed particles with the per-node path.
+// Samplers of the stage pass.  GlobalSmp: the padded lattice in global
+// memory (SLOW: any point, through the per-node path where a block has no
+// lattice; otherwise such a point clears ok).  BlockSmp: one block's lattice
+// staged in LDS, points outside it through the global halo path.
 template <bool SLOW>
-__device__ __forceinline__ void g2p_adjoint_one(const GridDev& g, const float* __restrict__ F,
-                                                const float* __restrict__ spos, const int32_t* __restrict__ sids,
-                                                int64_t i, float alpha, float dt, float* __restrict__ xbar,
-                                                float* __restrict__ vbar, AdjRec* __restrict__ rec,
-                                                uint32_t* __restrict__ list) {
+struct GlobalSmp {
+  const GridDev& g;
+  const float* __restrict__ F;
+  Near nr;
+  __device__ float comp(int field, int a, const float* u, bool& ok) const {
+    return sample_comp<SLOW>(g, F, field, a, u, nr, ok);
+  }
+  __device__ void grad(int field, int a, const float* u, float* G, bool& ok) const {
+    (void)sample_grad_sel<SLOW>(g, F, field, a, u, nr, G, ok);
+  }
+  __device__ float grad_v(int field, int a, const float* u, float* G, bool& ok) const {
+    return sample_grad_sel<SLOW>(g, F, field, a, u, nr, G, ok);
+  }
+};
+
+// sample_grad_impl<A, true> on one block's padded lattice staged in LDS
+// (lat; o: the block's first fine cell per axis), the same sums; false, and
+// nothing read, when the 2x2x2 nodes are not all in it
+template <int A>
+__device__ __forceinline__ bool sample_grad_lds(const GridDev& g, const float* lat, int field, const int* o,
+                                                const float* u, SG& r) {
+  constexpr int n0 = A == 0 ? 9 : 10, n1 = A == 1 ? 9 : 10, n2 = A == 2 ? 9 : 10;
+  int lp[3];
+  float t[3], dm[3];
+  bool zero = false, in = true;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    int i;
+    if (d == A) {
+      zero = zero || u[d] < 0.f || u[d] > (float)g.N[d];
+      const float c = fminf(fmaxf(u[d], 0.f), (float)g.N[d]);
+      i = min((int)floorf(c), g.N[d] - 1);
+      t[d] = c - (float)i;
+      dm[d] = 1.f;
+    } else {
+      const float cr = u[d] - 0.5f;
+      const float c = fminf(fmaxf(cr, 0.f), (float)(g.N[d] - 1));
+      i = (int)floorf(c);
+      t[d] = c - (float)i;
+      dm[d] = (cr >= 0.f && cr < (float)(g.N[d] - 1)) ? 1.f : 0.f;
+    }
+    lp[d] = i - o[d] + (d == A ? 0 : 1);
+    const int nd = d == 0 ? n0 : (d == 1 ? n1 : n2);
+    in = in && lp[d] >= 0 && lp[d] <= nd - 2;
+  }
+  if (!in) return false;
+  const float* p = lat + field * 3 * PAD + A * PAD + lp[0] + n0 * (lp[1] + n1 * lp[2]);
+  r = SG{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int dd[3] = {k & 1, (k >> 1) & 1, k >> 2};
+    const float f = p[dd[0] + n0 * (dd[1] + n1 * dd[2])];
+    const float w0 = dd[0] ? t[0] : 1.f - t[0], w1 = dd[1] ? t[1] : 1.f - t[1], w2 = dd[2] ? t[2] : 1.f - t[2];
+    r.s += w0 * w1 * w2 * f;
+    r.g0 += (dd[0] ? f : -f) * w1 * w2;
+    r.g1 += w0 * (dd[1] ? f : -f) * w2;
+    r.g2 += w0 * w1 * (dd[2] ? f : -f);
+  }
+  if (zero) {
+    r = SG{0.f, 0.f, 0.f, 0.f};
+    return true;
+  }
+  r.g0 *= dm[0];
+  r.g1 *= dm[1];
+  r.g2 *= dm[2];
+  return true;
+}
+
+template <int A>
+__device__ __forceinline__ float sample_grad_blk(const GridDev& g, const float* lat, const float* __restrict__ F,
+                                                 int field, const int* o, const float* u, const Near& nr, float* G,
+                                                 bool& ok) {
+  SG r;
+  if (sample_grad_lds<A>(g, lat, field, o, u, r)) {
+    G[0] = r.g0;
+    G[1] = r.g1;
+    G[2] = r.g2;
+    return r.s;
+  }
+  return sample_grad_fast<A>(g, F, field, u, nr, G, ok);
+}
+
+struct BlockSmp {
+  const GridDev& g;
+  const float* __restrict__ F;
+  const float* lat;
+  const int* o;
+  Near nr;
+  __device__ float comp(int field, int a, const float* u, bool& ok) const {
+    return a == 0 ? sample_blk<0>(g, lat, F, field, o, u, nr, ok)
+                  : (a == 1 ? sample_blk<1>(g, lat, F, field, o, u, nr, ok) : sample_blk<2>(g, lat, F, field, o, u, nr, ok));
+  }
+  __device__ float grad_v(int field, int a, const float* u, float* G, bool& ok) const {
+    return a == 0 ? sample_grad_blk<0>(g, lat, F, field, o, u, nr, G, ok)
+                  : (a == 1 ? sample_grad_blk<1>(g, lat, F, field, o, u, nr, G, ok)
+                            : sample_grad_blk<2>(g, lat, F, field, o, u, nr, G, ok));
+  }
+  __device__ void grad(int field, int a, const float* u, float* G, bool& ok) const { (void)grad_v(field, a, u, G, ok); }
+};
+
+// S samples the forward lattice (GlobalSmp or BlockSmp); without SLOW a
+// particle with a point S cannot serve goes to the fix-up list.
+template <bool SLOW, class Smp>
+__device__ __forceinline__ void g2p_adjoint_one(const GridDev& g, const Smp& S, const float* __restrict__ spos,
+                                                const int32_t* __restrict__ sids, int64_t i, float alpha, float dt,
+                                                float* __restrict__ xbar, float* __restrict__ vbar,
+                                                AdjRec* __restrict__ rec, uint32_t* __restrict__ list) {
   const float umax[3] = {(float)g.N[0] - 0.0009765625f, (float)g.N[1] - 0.0009765625f,
                          (float)g.N[2] - 0.0009765625f};
   const int64_t id = sids[i];
@@ -2112,17 +2218,16 @@ __device__ __forceinline__ void g2p_adjoint_one(const GridDev& g, const float* _
     x[a] = spos[3 * i + a];
     u[a] = fminf(fmaxf(x[a] * g.inv_h, 0.f), umax[a]);
   }
-  const Near nr = near_of(g, u);
   // forward stages: x2 = x + 1/2 dt v1, x3 = x + 3/4 dt v2, x' = x + dt (2 v1 + 3 v2 + 4 v3) / 9
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float v1 = sample_comp<SLOW>(g, F, 0, a, u, nr, ok);
+    const float v1 = S.comp(0, a, u, ok);
     u2[a] = fminf(fmaxf((x[a] + 0.5f * dt * v1) * g.inv_h, -1.f), (float)g.N[a] + 1.f);
     xn[a] = (2.f / 9.f) * v1;
   }
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float v2 = sample_comp<SLOW>(g, F, 0, a, u2, nr, ok);
+    const float v2 = S.comp(0, a, u2, ok);
     u3[a] = fminf(fmaxf((x[a] + 0.75f * dt * v2) * g.inv_h, -1.f), (float)g.N[a] + 1.f);
     xn[a] += (3.f / 9.f) * v2;
   }
@@ -2130,7 +2235,7 @@ __device__ __forceinline__ void g2p_adjoint_one(const GridDev& g, const float* _
   float G[3][3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float v3 = sample_grad_sel<SLOW>(g, F, 0, a, u3, nr, G[a], ok);
+    const float v3 = S.grad_v(0, a, u3, G[a], ok);
     xn[a] = x[a] + dt * (xn[a] + (4.f / 9.f) * v3);
   }
   float k1[3], k2[3], k3[3], xacc[3], vo[3];
@@ -2158,7 +2263,7 @@ __device__ __forceinline__ void g2p_adjoint_one(const GridDev& g, const float* _
   rec[i].q2 = make_float4(k3[2], k2[0], k2[1], k2[2]);
   // stage 2
 #pragma unroll
-  for (int a = 0; a < 3; ++a) (void)sample_grad_sel<SLOW>(g, F, 0, a, u2, nr, G[a], ok);
+  for (int a = 0; a < 3; ++a) S.grad(0, a, u2, G[a], ok);
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const float m2 = (u2[d] > -1.f && u2[d] < (float)g.N[d] + 1.f) ? 1.f : 0.f;
@@ -2174,8 +2279,8 @@ __device__ __forceinline__ void g2p_adjoint_one(const GridDev& g, const float* _
     s1[a] = vnb + vdb;
     d1[a] = vdb;
     float Gn[3], Gd[3];
-    (void)sample_grad_sel<SLOW>(g, F, 0, a, u, nr, Gn, ok);
-    (void)sample_grad_sel<SLOW>(g, F, 1, a, u, nr, Gd, ok);
+    S.grad(0, a, u, Gn, ok);
+    S.grad(1, a, u, Gd, ok);
 #pragma unroll
     for (int d = 0; d < 3; ++d) gr[d] += vnb * Gn[d] + vdb * Gd[d];
   }
@@ -2204,12 +2309,53 @@ __global__ __launch_bounds__(TPB) void g2p_adjoint_stage_kernel(GridDev g, const
   const int64_t i = (FIXUP ? (int64_t)blockIdx.x : xcd_group(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
   if (FIXUP) {
     const int64_t m = (int64_t)list[0];
-    for (int64_t j = i; j < m; j += (int64_t)gridDim.x * blockDim.x)
-      g2p_adjoint_one<true>(g, F, spos, sids, (int64_t)list[1 + j], alpha, dt, xbar, vbar, rec, list);
+    for (int64_t j = i; j < m; j += (int64_t)gridDim.x * blockDim.x) {
+      const int64_t q = (int64_t)list[1 + j];
+      float u[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) u[a] = fminf(fmaxf(spos[3 * q + a] * g.inv_h, 0.f), (float)g.N[a] - 0.0009765625f);
+      const GlobalSmp<true> S{g, F, near_of(g, u)};
+      g2p_adjoint_one<true>(g, S, spos, sids, q, alpha, dt, xbar, vbar, rec, list);
+    }
     return;
   }
   if (i >= n) return;
-  g2p_adjoint_one<false>(g, F, spos, sids, i, alpha, dt, xbar, vbar, rec, list);
+  float u[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) u[a] = fminf(fmaxf(spos[3 * i + a] * g.inv_h, 0.f), (float)g.N[a] - 0.0009765625f);
+  const GlobalSmp<false> S{g, F, near_of(g, u)};
+  g2p_adjoint_one<false>(g, S, spos, sids, i, alpha, dt, xbar, vbar, rec, list);
+}
+
+// The stage pass of the step's block-sorted particles, one work-group per
+// block with the block's forward lattice (both fields) in LDS, as the
+// forward's g2p_block_kernel; the same arithmetic as g2p_adjoint_stage_kernel.
+__global__ __launch_bounds__(G2B_T) void g2p_adjoint_block_kernel(GridDev g, const float* __restrict__ F,
+                                                                  const float* __restrict__ spos,
+                                                                  const int32_t* __restrict__ sids,
+                                                                  const uint32_t* __restrict__ pcount,
+                                                                  const uint32_t* __restrict__ poff, float alpha,
+                                                                  float dt, float* __restrict__ xbar,
+                                                                  float* __restrict__ vbar, AdjRec* __restrict__ rec,
+                                                                  uint32_t* __restrict__ list) {
+  __shared__ float lat[SLAB];
+  const int B = blockIdx.x;
+  const uint32_t cnt = pcount[B];
+  if (cnt == 0) return;
+  {
+    const float4* src = reinterpret_cast<const float4*>(F + (size_t)B * SLAB);
+    float4* dst = reinterpret_cast<float4*>(lat);
+    for (int k = threadIdx.x; k < SLAB / 4; k += G2B_T) dst[k] = src[k];
+  }
+  Near nr;
+  block_coords(g.D, B, nr.bc);
+  nr.own = true;
+  const int o[3] = {BASE * nr.bc[0], BASE * nr.bc[1], BASE * nr.bc[2]};
+  __syncthreads();
+  const BlockSmp S{g, F, lat, o, nr};
+  const uint32_t p0 = poff[B];
+  for (uint32_t j = threadIdx.x; j < cnt; j += G2B_T)
+    g2p_adjoint_one<false>(g, S, spos, sids, (int64_t)p0 + j, alpha, dt, xbar, vbar, rec, list);
 }
 
 // 512 threads per work-group (the LDS tile leaves room for three per CU, 24
@@ -4204,12 +4350,18 @@ void k_g2p_advect_sorted(hipStream_t s, const GridDev& g, const float* fine, con
 size_t adj_record_bytes() { return sizeof(AdjRec); }
 
 void k_g2p_adjoint_stages(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const int32_t* sids,
-                          int64_t n, float alpha, float dt, float* xbar, float* vbar, void* rec, uint32_t* list) {
+                          int64_t n, float alpha, float dt, float* xbar, float* vbar, void* rec, uint32_t* list,
+                          const uint32_t* pcount, const uint32_t* poff) {
   if (n <= 0) return;
   AdjRec* r = static_cast<AdjRec*>(rec);
   launch(fill_kernel<uint32_t>, 1, 64, 0, s, list, 1, 0u);
   const int64_t nwg = (n + TPB - 1) / TPB;
-  launch(g2p_adjoint_stage_kernel<false>, (unsigned)nwg, TPB, 0, s, g, fine, spos, sids, n, alpha, dt, xbar, vbar, r, list);
+  if (pcount)
+    launch(g2p_adjoint_block_kernel, g.nb, G2B_T, 0, s, g, fine, spos, sids, pcount, poff, alpha, dt, xbar, vbar, r,
+           list);
+  else
+    launch(g2p_adjoint_stage_kernel<false>, (unsigned)nwg, TPB, 0, s, g, fine, spos, sids, n, alpha, dt, xbar, vbar,
+           r, list);
   const unsigned fix = (unsigned)(nwg < 1024 ? nwg : 1024);
   launch(g2p_adjoint_stage_kernel<true>, fix, TPB, 0, s, g, fine, spos, sids, n, alpha, dt, xbar, vbar, r, list);
 }

@@ -1640,7 +1640,8 @@ int g2p_adjoint(aw_grid* g, float alpha, float dt, float* xbar, float* vbar) {
     // particle position and id 16 B + adjoints read and written 48 B + record; forward fine lattice per block
     Prof pf(g->ctx, "g2p_adjoint_stages", (64.0 + adj_record_bytes()) * n + 2.0 * 4.0 * 3 * FINE * g->nb);
     k_g2p_adjoint_stages(s, gv, g->slab.as<float>(), g->spos.as<float>(), g->sids.as<int32_t>(), n, alpha, dt, xbar,
-                         vbar, g->adjrec.p, g->g2p_list.as<uint32_t>());
+                         vbar, g->adjrec.p, g->g2p_list.as<uint32_t>(), g->count.as<uint32_t>(),
+                         g->poff.as<uint32_t>());
   }
   {
     // position 12 B + record per particle; the fine-lattice adjoint per block
