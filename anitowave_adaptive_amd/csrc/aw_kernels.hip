@@ -2027,16 +2027,20 @@ __device__ __forceinline__ float sample_vjp(const GridDev& g, const float* __res
 }
 
 // LDS tile of the fine-lattice adjoint around one block: nodes at block-local
-// coordinates [-AH, 8 + AH) on every axis, per field and component.
+// coordinates [-AH, 8 + AH) on every axis, per component, the two fields of
+// a node side by side (float2: field 0, field 1).
 constexpr int AH = 2, AT = BASE + 2 * AH, AT3 = AT * AT * AT;
 
-// F[node] += w(u) * val on component a, into the block's LDS tile where the
-// node lies in it and straight into the global buffer (stride ADJ) otherwise.
-// Node indices are resolved as in sampling: normal index outside [0, N) is a
-// dead node, transverse indices clamp to [0, N - 1].
-__device__ __forceinline__ void tile_scatter(const GridDev& g, float* tile, float* fbar, const int* o, int field,
-                                             int a, const float* u, float val) {
-  if (val == 0.f) return;
+// F[node] += w(u) * val (field 0; PAIR: and w(u) * val1 on field 1) on
+// component a, into the block's LDS tile where the node lies in it and
+// straight into the global buffer (stride ADJ) otherwise.  Node indices are
+// resolved as in sampling: normal index outside [0, N) is a dead node,
+// transverse indices clamp to [0, N - 1].  The two fields at one point share
+// the weights, so PAIR adds both with one 64-bit compare-and-swap.
+template <bool PAIR>
+__device__ __forceinline__ void tile_scatter(const GridDev& g, float2* tile, float* fbar, const int* o, int a,
+                                             const float* u, float val, float val1 = 0.f) {
+  if (val == 0.f && (!PAIR || val1 == 0.f)) return;
   int i0[3];
   float t[3];
 #pragma unroll
@@ -2057,11 +2061,16 @@ __device__ __forceinline__ void tile_scatter(const GridDev& g, float* tile, floa
     const float w = (dd[0] ? t[0] : 1.f - t[0]) * (dd[1] ? t[1] : 1.f - t[1]) * (dd[2] ? t[2] : 1.f - t[2]);
     const int t0 = q[0] - o[0], t1 = q[1] - o[1], t2 = q[2] - o[2];
     if ((unsigned)t0 < (unsigned)AT && (unsigned)t1 < (unsigned)AT && (unsigned)t2 < (unsigned)AT) {
-      lds_add_spread(&tile[(field * 3 + a) * AT3 + t0 + AT * (t1 + AT * t2)], w * val);
+      float2* cell = &tile[a * AT3 + t0 + AT * (t1 + AT * t2)];
+      if (PAIR)
+        lds_add2_spread(cell, w * val, w * val1);
+      else
+        lds_add_spread(&cell->x, w * val);
     } else {
       const int B = (q[0] >> LOGB) + g.D[0] * ((q[1] >> LOGB) + g.D[1] * (q[2] >> LOGB));
       const int l = (q[0] & 7) + 8 * ((q[1] & 7) + 8 * (q[2] & 7));
-      unsafeAtomicAdd(fbar + (int64_t)B * ADJ + field * 3 * FINE + a * FINE + l, w * val);
+      if (val != 0.f) unsafeAtomicAdd(fbar + (int64_t)B * ADJ + a * FINE + l, w * val);
+      if (PAIR && val1 != 0.f) unsafeAtomicAdd(fbar + (int64_t)B * ADJ + 3 * FINE + a * FINE + l, w * val1);
     }
   }
 }
@@ -2330,7 +2339,7 @@ __global__ __launch_bounds__(TPB) void g2p_adjoint_stage_kernel(GridDev g, const
 // The stage pass of the step's block-sorted particles, one work-group per
 // block with the block's forward lattice (both fields) in LDS, as the
 // forward's g2p_block_kernel; the same arithmetic as g2p_adjoint_stage_kernel.
-__global__ __launch_bounds__(G2B_T) void g2p_adjoint_block_kernel(GridDev g, const float* __restrict__ F,
+__global__ __launch_bounds__(G2B_T) __attribute__((amdgpu_waves_per_eu(4))) void g2p_adjoint_block_kernel(GridDev g, const float* __restrict__ F,
                                                                   const float* __restrict__ spos,
                                                                   const int32_t* __restrict__ sids,
                                                                   const uint32_t* __restrict__ pcount,
@@ -2361,19 +2370,20 @@ __global__ __launch_bounds__(G2B_T) void g2p_adjoint_block_kernel(GridDev g, con
 // 512 threads per work-group (the LDS tile leaves room for three per CU, 24
 // waves to cover the latency of the compare-and-swap adds): measured at
 // 512^3, 16.8 ms against 24.5 ms with 256 threads and 20.4 ms with 1024.
-// (Holding both fields of a node as one pair for 64-bit compare-and-swap
-// measured 17.7 ms: two of the three stages scatter into field 0 only.)
+// (Both fields of a node are held side by side: the samples at the particle
+// position add both with one 64-bit compare-and-swap, the two other stages,
+// field 0 only, with a 32-bit one on the pair's first word.)
 constexpr int ADJ_T = 512;
 
 __global__ __launch_bounds__(ADJ_T) void g2p_adjoint_scatter_kernel(GridDev g, const float* __restrict__ spos,
                                                                   const uint32_t* __restrict__ pcount,
                                                                   const uint32_t* __restrict__ poff,
                                                                   const AdjRec* __restrict__ rec, float* fbar) {
-  __shared__ float tile[2 * 3 * AT3];
+  __shared__ float2 tile[3 * AT3];
   const int B = blockIdx.x;
   const uint32_t cnt = pcount[B];
   if (cnt == 0) return;
-  for (int k = threadIdx.x; k < 2 * 3 * AT3; k += ADJ_T) tile[k] = 0.f;
+  for (int k = threadIdx.x; k < 3 * AT3; k += ADJ_T) tile[k] = make_float2(0.f, 0.f);
   int o[3];
   block_coords(g.D, B, o);
 #pragma unroll
@@ -2402,27 +2412,25 @@ __global__ __launch_bounds__(ADJ_T) void g2p_adjoint_scatter_kernel(GridDev g, c
     const float k3[3] = {r.q1.z, r.q1.w, r.q2.x}, k2[3] = {r.q2.y, r.q2.z, r.q2.w};
     const float s1[3] = {r.q3.x, r.q3.y, r.q3.z}, d1[3] = {r.q3.w, r.q4.x, r.q4.y};
 #pragma unroll
-    for (int a = 0; a < 3; ++a) tile_scatter(g, tile, fbar, o, 0, a, u3, k3[a]);
+    for (int a = 0; a < 3; ++a) tile_scatter<false>(g, tile, fbar, o, a, u3, k3[a]);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) tile_scatter(g, tile, fbar, o, 0, a, u2, k2[a]);
+    for (int a = 0; a < 3; ++a) tile_scatter<false>(g, tile, fbar, o, a, u2, k2[a]);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      tile_scatter(g, tile, fbar, o, 0, a, u, s1[a]);
-      tile_scatter(g, tile, fbar, o, 1, a, u, d1[a]);
-    }
+    for (int a = 0; a < 3; ++a) tile_scatter<true>(g, tile, fbar, o, a, u, s1[a], d1[a]);
   }
   __syncthreads();
   // flush: tile nodes are in-domain (clamped) by construction except the
   // normal-axis ones past the +wall, which were never written
-  for (int k = threadIdx.x; k < 2 * 3 * AT3; k += ADJ_T) {
-    const float v = tile[k];
-    if (v == 0.f) continue;
-    const int fa = k / AT3, r = k - fa * AT3;
+  for (int k = threadIdx.x; k < 3 * AT3; k += ADJ_T) {
+    const float2 v = tile[k];
+    if (v.x == 0.f && v.y == 0.f) continue;
+    const int a = k / AT3, r = k - a * AT3;
     const int q[3] = {o[0] + r % AT, o[1] + (r / AT) % AT, o[2] + r / (AT * AT)};
     const int Bq = (q[0] >> LOGB) + g.D[0] * ((q[1] >> LOGB) + g.D[1] * (q[2] >> LOGB));
     const int l = (q[0] & 7) + 8 * ((q[1] & 7) + 8 * (q[2] & 7));
-    const int field = fa / 3, a = fa - 3 * field;
-    unsafeAtomicAdd(fbar + (int64_t)Bq * ADJ + field * 3 * FINE + a * FINE + l, v);
+    float* dst = fbar + (int64_t)Bq * ADJ + a * FINE + l;
+    if (v.x != 0.f) unsafeAtomicAdd(dst, v.x);
+    if (v.y != 0.f) unsafeAtomicAdd(dst + 3 * FINE, v.y);
   }
 }
 
