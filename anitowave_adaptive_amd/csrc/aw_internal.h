@@ -225,8 +225,11 @@ void k_divergence_adjoint(hipStream_t s, const GridDev& g, const uint8_t* liquid
 void k_fold_adjoint(hipStream_t s, const GridDev& g, float* const* uold, float* const* mass, float* const* uob,
                     float* const* usb, float* const* Mb, float* const* mb);
 void k_pi_forward2(hipStream_t s, const GridDev& g, float* const* A, float* const* Bf, float* out);
+// pcount, poff: the block-sorted particles' per-block ranges (one work-group
+// per block, the lattice's padded copy in LDS), or null (one thread per particle)
 void k_p2g_adjoint(hipStream_t s, const GridDev& g, const float* fm, const float* spos, const float* svel,
-                   const int32_t* sids, int64_t n, float* xbar, float* vbar);
+                   const int32_t* sids, int64_t n, float* xbar, float* vbar, const uint32_t* pcount = nullptr,
+                   const uint32_t* poff = nullptr);
 void k_masked_copy(hipStream_t s, float* dst, const float* src, const uint8_t* liquid, int n);
 void k_ids_range(hipStream_t s, const int32_t* ids, int64_t n, int* out);  // out: {min, max}, preset by the caller
 

@@ -2694,6 +2694,108 @@ __global__ __launch_bounds__(TPB) void p2g_adjoint_kernel(GridDev g, const float
   for (int d = 0; d < 3; ++d) xbar[3 * id + d] += gr[d] * g.inv_h * m1[d];
 }
 
+// sample_vjp on one block's padded copy of the two-field fine lattice in LDS
+// (lat: field f, component a at f 3 PAD + a PAD + pad_idx, every node already
+// resolved by fine_node's rule; o: the block's first fine cell per axis), the
+// same sums; false, and nothing read, when a node is not in it.
+template <int A>
+__device__ __forceinline__ bool sample_vjp_lds(const float* lat, const int* o, int field, const float* u, float wb,
+                                               float* grad, float& out) {
+  constexpr int n0 = A == 0 ? 9 : 10, n1 = A == 1 ? 9 : 10, n2 = A == 2 ? 9 : 10;
+  int lp[3];
+  float t[3];
+  bool in = true;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float c = d == A ? u[d] : u[d] - 0.5f;
+    const float fl = floorf(c);
+    lp[d] = (int)fl - o[d] + (d == A ? 0 : 1);
+    t[d] = c - fl;
+    const int nd = d == 0 ? n0 : (d == 1 ? n1 : n2);
+    in = in && lp[d] >= 0 && lp[d] <= nd - 2;
+  }
+  if (!in) return false;
+  const float* p = lat + field * 3 * PAD + A * PAD + lp[0] + n0 * (lp[1] + n1 * lp[2]);
+  float s = 0.f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int dd[3] = {k & 1, (k >> 1) & 1, k >> 2};
+    const float f = p[dd[0] + n0 * (dd[1] + n1 * dd[2])];
+    const float f0 = dd[0] ? t[0] : 1.f - t[0], f1 = dd[1] ? t[1] : 1.f - t[1], f2 = dd[2] ? t[2] : 1.f - t[2];
+    s += f0 * f1 * f2 * f;
+    const float wf = wb * f;
+    grad[0] += wf * (dd[0] ? 1.f : -1.f) * f1 * f2;
+    grad[1] += wf * f0 * (dd[1] ? 1.f : -1.f) * f2;
+    grad[2] += wf * f0 * f1 * (dd[2] ? 1.f : -1.f);
+  }
+  out = s;
+  return true;
+}
+
+template <int A>
+__device__ __forceinline__ float sample_vjp_blk(const GridDev& g, const float* lat, const float* __restrict__ fm,
+                                                const int* o, int field, const float* u, float wb, float* grad) {
+  float s;
+  if (sample_vjp_lds<A>(lat, o, field, u, wb, grad, s)) return s;
+  return sample_vjp(g, fm, ADJ, field, A, u, wb, grad);
+}
+
+// p2g_adjoint_kernel for the block-sorted particles, one work-group per block:
+// the block's padded copy of the two fields (its own nodes and a one-node
+// halo from the neighbours, 21.6 KB) is gathered into LDS once, and the
+// block's particles sample it there (the same sums; a particle outside the
+// block samples the global lattice).
+__global__ __launch_bounds__(G2B_T) void p2g_adjoint_block_kernel(GridDev g, const float* __restrict__ fm,
+                                                                  const float* __restrict__ spos,
+                                                                  const float* __restrict__ svel,
+                                                                  const int32_t* __restrict__ sids,
+                                                                  const uint32_t* __restrict__ pcount,
+                                                                  const uint32_t* __restrict__ poff,
+                                                                  float* __restrict__ xbar, float* __restrict__ vbar) {
+  __shared__ float lat[SLAB];
+  const int B = blockIdx.x;
+  const uint32_t cnt = pcount[B];
+  if (cnt == 0) return;
+  int o[3];
+  block_coords(g.D, B, o);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) o[d] *= BASE;
+  for (int k = threadIdx.x; k < SLAB; k += G2B_T) {
+    const int f = k / (3 * PAD), r = k - f * 3 * PAD;
+    const int a = r / PAD, pi = r - a * PAD;
+    const int n0 = a == 0 ? 9 : 10, n1 = a == 1 ? 9 : 10;
+    int q[3] = {o[0] + pi % n0 - (a == 0 ? 0 : 1), o[1] + (pi / n0) % n1 - (a == 1 ? 0 : 1),
+                o[2] + pi / (n0 * n1) - (a == 2 ? 0 : 1)};
+    const int64_t ad = fine_node(g, ADJ, f, a, q);
+    lat[k] = ad >= 0 ? fm[ad] : 0.f;
+  }
+  __syncthreads();
+  const uint32_t p0 = poff[B];
+  for (uint32_t j = threadIdx.x; j < cnt; j += G2B_T) {
+    const int64_t i = (int64_t)p0 + j;
+    const int64_t id = sids[i];
+    float u[3], m1[3], gr[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float umax = (float)g.N[a] - 0.0009765625f;
+      const float raw = spos[3 * i + a] * g.inv_h;
+      u[a] = fminf(fmaxf(raw, 0.f), umax);
+      m1[a] = (raw > 0.f && raw < umax) ? 1.f : 0.f;
+    }
+    float sv[3];
+    sv[0] = sample_vjp_blk<0>(g, lat, fm, o, 0, u, svel[3 * i + 0], gr);
+    sample_vjp_blk<0>(g, lat, fm, o, 1, u, 1.f, gr);
+    sv[1] = sample_vjp_blk<1>(g, lat, fm, o, 0, u, svel[3 * i + 1], gr);
+    sample_vjp_blk<1>(g, lat, fm, o, 1, u, 1.f, gr);
+    sv[2] = sample_vjp_blk<2>(g, lat, fm, o, 0, u, svel[3 * i + 2], gr);
+    sample_vjp_blk<2>(g, lat, fm, o, 1, u, 1.f, gr);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) vbar[3 * id + a] += sv[a];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) xbar[3 * id + d] += gr[d] * g.inv_h * m1[d];
+  }
+}
+
 // out[0] = min id, out[1] = max id (out preset to INT_MAX, INT_MIN)
 __global__ __launch_bounds__(TPB) void ids_range_kernel(const int32_t* __restrict__ ids, int64_t n, int* out) {
   int lo = 0x7FFFFFFF, hi = -0x7FFFFFFF - 1;
@@ -4397,8 +4499,12 @@ void k_pi_forward2(hipStream_t s, const GridDev& g, float* const* A, float* cons
   launch(pi_forward2_kernel, g.nb, TPB, 0, s, g, fp(A), fp(Bf), out);
 }
 void k_p2g_adjoint(hipStream_t s, const GridDev& g, const float* fm, const float* spos, const float* svel,
-                   const int32_t* sids, int64_t n, float* xbar, float* vbar) {
-  if (n > 0)
+                   const int32_t* sids, int64_t n, float* xbar, float* vbar, const uint32_t* pcount,
+                   const uint32_t* poff) {
+  if (n <= 0) return;
+  if (pcount)
+    launch(p2g_adjoint_block_kernel, g.nb, G2B_T, 0, s, g, fm, spos, svel, sids, pcount, poff, xbar, vbar);
+  else
     launch(p2g_adjoint_kernel, (unsigned)((n + TPB - 1) / TPB), TPB, 0, s, g, fm, spos, svel, sids, n, xbar, vbar);
 }
 void k_ids_range(hipStream_t s, const int32_t* ids, int64_t n, int* out) {

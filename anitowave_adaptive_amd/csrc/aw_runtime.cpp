@@ -1702,7 +1702,7 @@ int p2g_adjoint(aw_grid* g, float* xbar, float* vbar) {
   {
     Prof pf(g->ctx, "p2g_adjoint", 76.0 * g->n_sorted + 4.0 * ADJ * g->nb);
     k_p2g_adjoint(s, gv, g->fmbar.as<float>(), g->spos.as<float>(), g->svel.as<float>(), g->sids.as<int32_t>(),
-                  g->n_sorted, xbar, vbar);
+                  g->n_sorted, xbar, vbar, g->count.as<uint32_t>(), g->poff.as<uint32_t>());
   }
   return launch_status("p2g_adjoint");
 }
