@@ -1364,15 +1364,13 @@ __device__ __forceinline__ float node_from_faces(const GridDev& g, int field, in
   return field ? vd : vn;
 }
 
-// Every padded node of a flagged block, own or halo, from the faces of the
-// block that owns it (the same sums for a node in every lattice that holds
-// it).  The owners' levels and face offsets (the block and its 26
-// neighbours) are read into LDS first, so a node's face loads depend on
-// nothing else; a node takes its 2x2x2 slot products in pi_weights's order,
-// loading only the slots of nonzero weight (the same sums), and a thread
-// works on two nodes at a time.
-__device__ __forceinline__ void fine_node_sums(const GridDev& g, int a, int e, uint32_t fo, const int* l, float& vn,
-                                               float& vd) {
+// Fine node l (block-local fine coordinates) on axis a of a block of level
+// e, from that block's face slots on axis a (un_a, uo_a, in face_idx order):
+// Pi(u_new) and Pi(u_new - u_old).  The node takes its 2x2x2 slot products
+// in pi_weights's order, loading only the slots of nonzero weight, so a
+// node has the same sums in every lattice that holds it.
+__device__ __forceinline__ void fine_node_from(int a, int e, const float* un_a, const float* uo_a, const int* l,
+                                               float& vn, float& vd) {
   const int s = 1 << e, r = BASE >> e;
   int i0[3];
   float t[3];
@@ -1398,9 +1396,9 @@ __device__ __forceinline__ void fine_node_sums(const GridDev& g, int a, int e, u
     ww[k] = (dx ? t[0] : 1.f - t[0]) * (dy ? t[1] : 1.f - t[1]) * (dz ? t[2] : 1.f - t[2]);
     un[k] = uo[k] = 0.f;
     if (ww[k] != 0.f) {
-      const size_t id = fo + face_idx(a, i0[0] + dx, i0[1] + dy, i0[2] + dz, r);
-      un[k] = g.un[a][id];
-      uo[k] = g.uo[a][id];
+      const int id = face_idx(a, i0[0] + dx, i0[1] + dy, i0[2] + dz, r);
+      un[k] = un_a[id];
+      uo[k] = uo_a[id];
     }
   }
   vn = 0.f;
@@ -1413,9 +1411,18 @@ __device__ __forceinline__ void fine_node_sums(const GridDev& g, int a, int e, u
   }
 }
 
+// The padded lattice of every flagged block, each node from the faces of
+// the block that owns it.  Its own nodes (block-local fine coordinates in
+// [0, 8)^3, 512 per axis) come from the block's faces, staged in LDS with
+// coalesced loads; the 388 halo nodes per axis (the normal layer at 8 and
+// the transverse rings at -1 and 8) from their owners' faces in global
+// memory, two per thread, the owners' levels and face offsets (the block
+// and its 26 neighbours) read into LDS first.
+constexpr int TF_HALO = PAD - FINE;
 __global__ __launch_bounds__(TPB) void to_fine_kernel(GridDev g, float* __restrict__ fine) {
   __shared__ int nlev[27];
   __shared__ uint32_t nfo[27];
+  __shared__ float sun[3 * 576], suo[3 * 576];   // the block's face slots, (r + 1) r^2 <= 576 per axis
   const int B = blockIdx.x;
   if (!g.halo[B]) return;
   int Bc[3];
@@ -1428,18 +1435,65 @@ __global__ __launch_bounds__(TPB) void to_fine_kernel(GridDev g, float* __restri
     nlev[t] = g.level[Bo];
     nfo[t] = g.foff[0][Bo];   // the three axes' face offsets coincide
   }
+  const int e = g.level[B], r = BASE >> e, nf = (r + 1) * r * r;
+  const uint32_t fo = g.foff[0][B];
+  for (int k = threadIdx.x; k < 3 * nf; k += TPB) {
+    const int a = k / nf, i = k - a * nf;
+    sun[a * 576 + i] = g.un[a][fo + i];
+    suo[a * 576 + i] = g.uo[a][fo + i];
+  }
   __syncthreads();
   float* dst = fine + (size_t)B * SLAB;
-  for (int k0 = threadIdx.x; k0 < 3 * PAD; k0 += 2 * TPB) {
+  for (int k = threadIdx.x; k < 3 * FINE; k += TPB) {
+    const int a = k >> 9, li = k & (FINE - 1);
+    const int l[3] = {li & 7, (li >> 3) & 7, li >> 6};
+    float vn, vd;
+    fine_node_from(a, e, sun + a * 576, suo + a * 576, l, vn, vd);
+    const int pi = a * PAD + pad_idx(a, l);
+    dst[pi] = vn;
+    dst[3 * PAD + pi] = vd;
+  }
+  for (int k0 = threadIdx.x; k0 < 3 * TF_HALO; k0 += 2 * TPB) {
     float vn[2], vd[2];
+    int pi[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int k = k0 + h * TPB;
       vn[h] = vd[h] = 0.f;
-      if (k >= 3 * PAD) continue;
-      const int a = k / PAD, pi = k - a * PAD;
+      pi[h] = -1;
+      if (k >= 3 * TF_HALO) continue;
+      const int a = k / TF_HALO, hh = k - a * TF_HALO;
+      // padded coordinates: normal tn in [0, 9), transverse t1, t2 in [0, 10)
+      // (the lower axis first); the core tn < 8, 1 <= t1, t2 <= 8 is own
+      int tn, t1, t2;
+      if (hh < 8 * 36) {
+        tn = hh / 36;
+        const int ri = hh - tn * 36;
+        if (ri < 10) {
+          t1 = ri;
+          t2 = 0;
+        } else if (ri < 20) {
+          t1 = ri - 10;
+          t2 = 9;
+        } else if (ri < 28) {
+          t1 = 0;
+          t2 = ri - 19;
+        } else {
+          t1 = 9;
+          t2 = ri - 27;
+        }
+      } else {
+        tn = 8;
+        t1 = (hh - 8 * 36) % 10;
+        t2 = (hh - 8 * 36) / 10;
+      }
+      const int d1 = a == 0 ? 1 : 0, d2 = a == 2 ? 1 : 2;
+      int lp[3];
+      lp[a] = tn;
+      lp[d1] = t1;
+      lp[d2] = t2;
       const int n0 = a == 0 ? 9 : 10, n1 = a == 1 ? 9 : 10;
-      const int lp[3] = {pi % n0, (pi / n0) % n1, pi / (n0 * n1)};
+      pi[h] = a * PAD + lp[0] + n0 * (lp[1] + n1 * lp[2]);
       int q[3];
       bool outside = false;
 #pragma unroll
@@ -1453,14 +1507,13 @@ __global__ __launch_bounds__(TPB) void to_fine_kernel(GridDev g, float* __restri
       if (outside) continue;
       const int nb = ((q[0] >> LOGB) - Bc[0] + 1) + 3 * ((q[1] >> LOGB) - Bc[1] + 1) + 9 * ((q[2] >> LOGB) - Bc[2] + 1);
       const int l[3] = {q[0] & 7, q[1] & 7, q[2] & 7};
-      fine_node_sums(g, a, nlev[nb], nfo[nb], l, vn[h], vd[h]);
+      fine_node_from(a, nlev[nb], g.un[a] + nfo[nb], g.uo[a] + nfo[nb], l, vn[h], vd[h]);
     }
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const int k = k0 + h * TPB;
-      if (k >= 3 * PAD) continue;
-      dst[k] = vn[h];
-      dst[3 * PAD + k] = vd[h];
+      if (pi[h] < 0) continue;
+      dst[pi[h]] = vn[h];
+      dst[3 * PAD + pi[h]] = vd[h];
     }
   }
 }
