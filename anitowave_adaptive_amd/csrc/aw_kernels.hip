@@ -1550,6 +1550,26 @@ __device__ __forceinline__ Near near_of(const GridDev& g, const float* u) {
 
 // SLOW: a sample whose block has no halo reads each node from its owner
 // block; otherwise such a sample clears ok (and its value is not used).
+// Trilinear interpolation of the 2x2x2 values v[k] (k = dx + 2 dy + 4 dz)
+// at fractions t: seven blends (1 - t) a + t b, each one multiply and one
+// fused multiply-add, exact at t = 0 and t = 1.
+__device__ __forceinline__ float trilerp(const float* v, const float* t) {
+  const float s0 = 1.f - t[0], s1 = 1.f - t[1], s2 = 1.f - t[2];
+  const float c00 = fmaf(t[0], v[1], s0 * v[0]), c10 = fmaf(t[0], v[3], s0 * v[2]);
+  const float c01 = fmaf(t[0], v[5], s0 * v[4]), c11 = fmaf(t[0], v[7], s0 * v[6]);
+  const float c0 = fmaf(t[1], c10, s1 * c00), c1 = fmaf(t[1], c11, s1 * c01);
+  return fmaf(t[2], c1, s2 * c0);
+}
+
+// trilerp of the nodes at fixed offsets from f in an n0 x n1 x . array
+template <int n0, int n1>
+__device__ __forceinline__ float trilerp_at(const float* f, const float* t) {
+  float v[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) v[k] = f[(k & 1) + n0 * (((k >> 1) & 1) + n1 * (k >> 2))];
+  return trilerp(v, t);
+}
+
 template <int A, bool SLOW>
 __device__ __forceinline__ float sample_comp(const GridDev& g, const float* __restrict__ F, int field,
                                              const float* u, const Near& nr, bool& ok) {
@@ -1582,18 +1602,13 @@ __device__ __forceinline__ float sample_comp(const GridDev& g, const float* __re
   if (!SLOW && !fast) ok = false;
   if (!SLOW || fast) {
     const float* f = F + (size_t)B * SLAB + field * 3 * PAD + A * PAD + lp[0] + n0 * (lp[1] + n1 * lp[2]);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int dd[3] = {k & 1, (k >> 1) & 1, k >> 2};
-      const float w = (dd[0] ? t[0] : 1.f - t[0]) * (dd[1] ? t[1] : 1.f - t[1]) * (dd[2] ? t[2] : 1.f - t[2]);
-      s += w * f[dd[0] + n0 * (dd[1] + n1 * dd[2])];
-    }
+    s = trilerp_at<n0, n1>(f, t);
   } else if (SLOW) {
     // no halo: each node from its owner block's own nodes
+    float vk[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const int dd[3] = {k & 1, (k >> 1) & 1, k >> 2};
-      const float w = (dd[0] ? t[0] : 1.f - t[0]) * (dd[1] ? t[1] : 1.f - t[1]) * (dd[2] ? t[2] : 1.f - t[2]);
       int q[3];
 #pragma unroll
       for (int d = 0; d < 3; ++d) q[d] = BASE * bc[d] + lp[d] - (d == A ? 0 : 1) + dd[d];
@@ -1606,8 +1621,9 @@ __device__ __forceinline__ float sample_comp(const GridDev& g, const float* __re
         const int l[3] = {q[0] & 7, q[1] & 7, q[2] & 7};
         v = own_node(g, F, field, A, Bo, l);
       }
-      s += w * v;
+      vk[k] = v;
     }
+    s = trilerp(vk, t);
   }
   return zero ? 0.f : s;
 }
@@ -1920,14 +1936,7 @@ __device__ __forceinline__ bool sample_lds(const GridDev& g, const float* lat, i
     in = in && lp[d] >= 0 && lp[d] <= nd - 2;
   }
   if (!in) return false;
-  const float* f = lat + field * 3 * PAD + A * PAD + lp[0] + n0 * (lp[1] + n1 * lp[2]);
-  float s = 0.f;
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int dd[3] = {k & 1, (k >> 1) & 1, k >> 2};
-    const float w = (dd[0] ? t[0] : 1.f - t[0]) * (dd[1] ? t[1] : 1.f - t[1]) * (dd[2] ? t[2] : 1.f - t[2]);
-    s += w * f[dd[0] + n0 * (dd[1] + n1 * dd[2])];
-  }
+  const float s = trilerp_at<n0, n1>(lat + field * 3 * PAD + A * PAD + lp[0] + n0 * (lp[1] + n1 * lp[2]), t);
   out = zero ? 0.f : s;
   return true;
 }
