@@ -43,6 +43,13 @@ void launch(K kernel, dim3 grid, dim3 block, size_t shmem, hipStream_t s, A... a
   }
 }
 
+// Device-to-device copy in stream order.  Like the kernel launches above it
+// reports nothing itself: the runtime's launch check (hipGetLastError, which
+// holds the last error of any runtime call) sees a failure.
+inline void copy_async(hipStream_t s, void* dst, const void* src, size_t bytes) {
+  (void)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s);
+}
+
 constexpr int TPB = 256;
 
 __device__ __forceinline__ int face_idx(int a, int x, int y, int z, int r) {
@@ -1652,13 +1659,6 @@ __device__ __forceinline__ float sample_comp_any(const GridDev& g, const float* 
   bool ok = true;
   const float v = sample_comp<A, false>(g, F, field, u, nr, ok);
   return ok ? v : sample_comp_slow<A>(g, F, field, u[0], u[1], u[2]);
-}
-
-// any sample point: the halo path, or the per-node path out of line
-__device__ __forceinline__ float sample_comp(const GridDev& g, const float* __restrict__ F, int field, int a,
-                                             const float* u, const Near& nr) {
-  return a == 0 ? sample_comp_any<0>(g, F, field, u, nr)
-                : (a == 1 ? sample_comp_any<1>(g, F, field, u, nr) : sample_comp_any<2>(g, F, field, u, nr));
 }
 
 // sample_comp<A, true> with the gradient of the interpolant with respect to
@@ -4348,7 +4348,7 @@ static void grade_passes(hipStream_t s, const int* D, int L, uint8_t* level, uin
   const int gr = (nb + TPB - 1) / TPB;
   for (int it = 0; it < L - 1; ++it) {
     launch(grade_kernel, gr, TPB, 0, s, D[0], D[1], D[2], level, tmp);
-    hipMemcpyAsync(level, tmp, nb, hipMemcpyDeviceToDevice, s);
+    copy_async(s, level, tmp, nb);
   }
 }
 
@@ -4365,7 +4365,7 @@ void k_levels(hipStream_t s, const int* D, int L, const int* band, const uint32_
   launch(cheb_pass_kernel, gr, TPB, 0, s, D[0], D[1], D[2], 2, bmax, 1, 0, bd, surf, tmp);
   launch(cheb_pass_kernel, gr, TPB, 0, s, D[0], D[1], D[2], 1, bmax, 0, 0, bd, tmp, level);
   launch(cheb_pass_kernel, gr, TPB, 0, s, D[0], D[1], D[2], 0, bmax, 0, L, bd, level, tmp);
-  hipMemcpyAsync(level, tmp, nb, hipMemcpyDeviceToDevice, s);
+  copy_async(s, level, tmp, nb);
   grade_passes(s, D, L, level, tmp);
 }
 
@@ -4418,8 +4418,8 @@ void k_layout_offsets(hipStream_t s, const uint8_t* level, int nb, uint32_t* cof
                       uint32_t* fz, uint32_t* scratch) {
   scan_any(s, nullptr, level, 1, coff, nb, scratch);
   scan_any(s, nullptr, level, 2, fx, nb, scratch);
-  if (fy != fx) hipMemcpyAsync(fy, fx, (nb + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
-  if (fz != fx) hipMemcpyAsync(fz, fx, (nb + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+  if (fy != fx) copy_async(s, fy, fx, (nb + 1) * sizeof(uint32_t));
+  if (fz != fx) copy_async(s, fz, fx, (nb + 1) * sizeof(uint32_t));
 }
 
 void k_scatter(hipStream_t s, const float* pos, const float* vel, const int32_t* ids, const int32_t* key,

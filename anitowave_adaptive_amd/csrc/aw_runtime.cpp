@@ -654,7 +654,6 @@ int project(aw_grid* g) {
   float* us[3] = {g->ustar[0].as<float>(), g->ustar[1].as<float>(), g->ustar[2].as<float>()};
   float* bt[3] = {g->beta[0].as<float>(), g->beta[1].as<float>(), g->beta[2].as<float>()};
   float* un[3] = {g->unew[0].as<float>(), g->unew[1].as<float>(), g->unew[2].as<float>()};
-  float* uo[3] = {g->uold[0].as<float>(), g->uold[1].as<float>(), g->uold[2].as<float>()};
   {
     Prof pf(g->ctx, "project", 36.0 * g->nfaces + 4.0 * g->ncells);
     k_project(s, gv, g->p.as<float>(), us, bt, un);
@@ -1413,10 +1412,7 @@ int solve_fp32(aw_solver* S, float* x, const float* b, double tol, int max_iters
   const uint8_t* liq = g->liquid.as<uint8_t>();
   float* r = S->r0.as<float>();
   float* z = S->z.as<float>();
-  float* pv = S->pv.as<float>();
-  float* q = S->q.as<float>();
   double* sc = S->sc.as<double>();
-  const LevelDev V0 = L0.view();
   k_mask(s, x, liq, n);
   k_fill_f64(s, sc, SC_N * SC_SLOTS, 0.0);
   residual_level(S, 0, x, b, r);
