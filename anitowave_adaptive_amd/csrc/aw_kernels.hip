@@ -1669,6 +1669,23 @@ struct SG {
   float s, g0, g1, g2;
 };
 
+// trilerp of v at t (the same value, bit for bit) with its derivatives in
+// t0, t1, t2: the x blends and x differences of the four (y, z) node pairs,
+// then blends of those in y and z
+__device__ __forceinline__ SG trilerp_grad(const float* v, const float* t) {
+  const float s0 = 1.f - t[0], s1 = 1.f - t[1], s2 = 1.f - t[2];
+  const float c00 = fmaf(t[0], v[1], s0 * v[0]), c10 = fmaf(t[0], v[3], s0 * v[2]);
+  const float c01 = fmaf(t[0], v[5], s0 * v[4]), c11 = fmaf(t[0], v[7], s0 * v[6]);
+  const float d00 = v[1] - v[0], d10 = v[3] - v[2], d01 = v[5] - v[4], d11 = v[7] - v[6];
+  const float c0 = fmaf(t[1], c10, s1 * c00), c1 = fmaf(t[1], c11, s1 * c01);
+  SG r;
+  r.s = fmaf(t[2], c1, s2 * c0);
+  r.g0 = fmaf(t[2], fmaf(t[1], d11, s1 * d01), s2 * fmaf(t[1], d10, s1 * d00));
+  r.g1 = fmaf(t[2], c11 - c01, s2 * (c10 - c00));
+  r.g2 = c1 - c0;
+  return r;
+}
+
 template <int A, bool FAST>
 __device__ __forceinline__ SG sample_grad_impl(const GridDev& g, const float* __restrict__ F, int field,
                                                const float* u) {
@@ -1697,7 +1714,7 @@ __device__ __forceinline__ SG sample_grad_impl(const GridDev& g, const float* __
   }
   const int B = bc[0] + g.D[0] * (bc[1] + g.D[1] * bc[2]);
   const float* p = F + (size_t)B * SLAB + field * 3 * PAD + A * PAD + lp[0] + n0 * (lp[1] + n1 * lp[2]);
-  SG r = {0.f, 0.f, 0.f, 0.f};
+  float v[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int dd[3] = {k & 1, (k >> 1) & 1, k >> 2};
@@ -1718,12 +1735,9 @@ __device__ __forceinline__ SG sample_grad_impl(const GridDev& g, const float* __
         f = own_node(g, F, field, A, Bo, l);
       }
     }
-    const float w0 = dd[0] ? t[0] : 1.f - t[0], w1 = dd[1] ? t[1] : 1.f - t[1], w2 = dd[2] ? t[2] : 1.f - t[2];
-    r.s += w0 * w1 * w2 * f;
-    r.g0 += (dd[0] ? f : -f) * w1 * w2;
-    r.g1 += w0 * (dd[1] ? f : -f) * w2;
-    r.g2 += w0 * w1 * (dd[2] ? f : -f);
+    v[k] = f;
   }
+  SG r = trilerp_grad(v, t);
   if (zero) return SG{0.f, 0.f, 0.f, 0.f};
   r.g0 *= dm[0];
   r.g1 *= dm[1];
@@ -2071,21 +2085,19 @@ __device__ __forceinline__ float sample_vjp(const GridDev& g, const float* __res
     i0[d] = (int)fl;
     t[d] = c - fl;
   }
-  float s = 0.f;
+  float v[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int dd[3] = {k & 1, (k >> 1) & 1, k >> 2};
     int q[3] = {i0[0] + dd[0], i0[1] + dd[1], i0[2] + dd[2]};
     const int64_t ad = fine_node(g, stride, field, a, q);
-    const float f = ad >= 0 ? F[ad] : 0.f;
-    const float f0 = dd[0] ? t[0] : 1.f - t[0], f1 = dd[1] ? t[1] : 1.f - t[1], f2 = dd[2] ? t[2] : 1.f - t[2];
-    s += f0 * f1 * f2 * f;
-    const float wf = wb * f;
-    grad[0] += wf * (dd[0] ? 1.f : -1.f) * f1 * f2;
-    grad[1] += wf * f0 * (dd[1] ? 1.f : -1.f) * f2;
-    grad[2] += wf * f0 * f1 * (dd[2] ? 1.f : -1.f);
+    v[k] = ad >= 0 ? F[ad] : 0.f;
   }
-  return s;
+  const SG r = trilerp_grad(v, t);
+  grad[0] = fmaf(wb, r.g0, grad[0]);
+  grad[1] = fmaf(wb, r.g1, grad[1]);
+  grad[2] = fmaf(wb, r.g2, grad[2]);
+  return r.s;
 }
 
 // LDS tile of the fine-lattice adjoint around one block: nodes at block-local
@@ -2216,17 +2228,10 @@ __device__ __forceinline__ bool sample_grad_lds(const GridDev& g, const float* l
   }
   if (!in) return false;
   const float* p = lat + field * 3 * PAD + A * PAD + lp[0] + n0 * (lp[1] + n1 * lp[2]);
-  r = SG{0.f, 0.f, 0.f, 0.f};
+  float v[8];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int dd[3] = {k & 1, (k >> 1) & 1, k >> 2};
-    const float f = p[dd[0] + n0 * (dd[1] + n1 * dd[2])];
-    const float w0 = dd[0] ? t[0] : 1.f - t[0], w1 = dd[1] ? t[1] : 1.f - t[1], w2 = dd[2] ? t[2] : 1.f - t[2];
-    r.s += w0 * w1 * w2 * f;
-    r.g0 += (dd[0] ? f : -f) * w1 * w2;
-    r.g1 += w0 * (dd[1] ? f : -f) * w2;
-    r.g2 += w0 * w1 * (dd[2] ? f : -f);
-  }
+  for (int k = 0; k < 8; ++k) v[k] = p[(k & 1) + n0 * (((k >> 1) & 1) + n1 * (k >> 2))];
+  r = trilerp_grad(v, t);
   if (zero) {
     r = SG{0.f, 0.f, 0.f, 0.f};
     return true;
@@ -2778,19 +2783,14 @@ __device__ __forceinline__ bool sample_vjp_lds(const float* lat, const int* o, i
   }
   if (!in) return false;
   const float* p = lat + field * 3 * PAD + A * PAD + lp[0] + n0 * (lp[1] + n1 * lp[2]);
-  float s = 0.f;
+  float v[8];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int dd[3] = {k & 1, (k >> 1) & 1, k >> 2};
-    const float f = p[dd[0] + n0 * (dd[1] + n1 * dd[2])];
-    const float f0 = dd[0] ? t[0] : 1.f - t[0], f1 = dd[1] ? t[1] : 1.f - t[1], f2 = dd[2] ? t[2] : 1.f - t[2];
-    s += f0 * f1 * f2 * f;
-    const float wf = wb * f;
-    grad[0] += wf * (dd[0] ? 1.f : -1.f) * f1 * f2;
-    grad[1] += wf * f0 * (dd[1] ? 1.f : -1.f) * f2;
-    grad[2] += wf * f0 * f1 * (dd[2] ? 1.f : -1.f);
-  }
-  out = s;
+  for (int k = 0; k < 8; ++k) v[k] = p[(k & 1) + n0 * (((k >> 1) & 1) + n1 * (k >> 2))];
+  const SG r = trilerp_grad(v, t);
+  grad[0] = fmaf(wb, r.g0, grad[0]);
+  grad[1] = fmaf(wb, r.g1, grad[1]);
+  grad[2] = fmaf(wb, r.g2, grad[2]);
+  out = r.s;
   return true;
 }
 
