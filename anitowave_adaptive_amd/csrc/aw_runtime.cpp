@@ -139,6 +139,12 @@ struct DevBuf {
   T* as() const {
     return static_cast<T*>(p);
   }
+  // exchange the allocations (no copy; kernels already enqueued keep the
+  // pointers they were launched with)
+  void swap(DevBuf& o) noexcept {
+    std::swap(p, o.p);
+    std::swap(cap, o.cap);
+  }
 };
 
 int launch_status(const char* what) {
@@ -1773,9 +1779,19 @@ int tape_record(aw_tape* t, aw_tape::Step& r) {
     AW_TRY(copy_d2d(s, r.mass[a], g->mass[a].p, nf));
     AW_TRY(copy_d2d(s, r.unew[a], g->unew[a].p, nf));
   }
-  AW_TRY(copy_d2d(s, r.spos, g->spos.p, (size_t)t->n * 3 * sizeof(float)));
-  AW_TRY(copy_d2d(s, r.svel, g->svel.p, (size_t)t->n * 3 * sizeof(float)));
-  AW_TRY(copy_d2d(s, r.sids, g->sids.p, (size_t)t->n * sizeof(int32_t)));
+  // The sorted particle arrays move into the record instead of being
+  // copied: the grid takes the record's previous buffers, whose contents the
+  // next step's sort rewrites entry by entry (and which are grown here if the
+  // record's were smaller).  The bound the sort took of the velocities is
+  // of a buffer the grid no longer holds.
+  r.spos.swap(g->spos);
+  r.svel.swap(g->svel);
+  r.sids.swap(g->sids);
+  const size_t np = (size_t)std::max<int64_t>(t->n, 1);
+  AW_TRY(g->spos.ensure(np * 3 * sizeof(float)));
+  AW_TRY(g->svel.ensure(np * 3 * sizeof(float)));
+  AW_TRY(g->sids.ensure(np * sizeof(int32_t)));
+  g->vbound_of = g->vbound_dens_of = nullptr;
   AW_TRY(copy_d2d(s, r.count, g->count.p, (size_t)g->nb * sizeof(uint32_t)));
   AW_TRY(copy_d2d(s, r.poff, g->poff.p, (size_t)(g->nb + 1) * sizeof(uint32_t)));
   return AW_OK;
@@ -1792,7 +1808,7 @@ int liquid_fingerprint(aw_tape* t, unsigned long long* out) {
   return launch_status("liquid fingerprint");
 }
 
-int tape_restore(aw_tape* t, const aw_tape::Step& r) {
+int tape_restore(aw_tape* t, aw_tape::Step& r) {
   aw_grid* g = t->g;
   hipStream_t s = g->ctx->stream;
   if (g->h_level != r.level) {
@@ -1807,10 +1823,12 @@ int tape_restore(aw_tape* t, const aw_tape::Step& r) {
     AW_HIP(hipMemcpyAsync(g->mass[a].p, r.mass[a].p, nf, hipMemcpyDeviceToDevice, s));
     AW_HIP(hipMemcpyAsync(g->unew[a].p, r.unew[a].p, nf, hipMemcpyDeviceToDevice, s));
   }
-  AW_HIP(hipMemcpyAsync(g->spos.p, r.spos.p, (size_t)t->n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s));
-  AW_HIP(hipMemcpyAsync(g->svel.p, r.svel.p, (size_t)t->n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s));
+  // the record's sorted particle arrays move back into the grid (the record
+  // keeps the grid's buffers, rewritten when it next records)
+  g->spos.swap(r.spos);
+  g->svel.swap(r.svel);
+  g->sids.swap(r.sids);
   g->vbound_of = g->vbound_dens_of = nullptr;   // the sorted arrays no longer hold what the bound was taken of
-  AW_HIP(hipMemcpyAsync(g->sids.p, r.sids.p, (size_t)t->n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
   AW_HIP(hipMemcpyAsync(g->count.p, r.count.p, (size_t)g->nb * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
   AW_HIP(hipMemcpyAsync(g->poff.p, r.poff.p, (size_t)(g->nb + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
   g->n_sorted = t->n;
