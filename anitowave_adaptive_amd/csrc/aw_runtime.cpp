@@ -1023,13 +1023,20 @@ int refresh_live(aw_solver* S) {
   AW_HIP(hipMemcpyAsync(S->h_live, flags, tot, hipMemcpyDeviceToHost, s));
   AW_HIP(hipStreamSynchronize(s));
   AW_TRY(launch_status("liquid blocks"));
-  // every earlier copy from a level's h_live_idx has completed (synchronised above)
+  // every earlier copy from a level's h_live_idx has completed (synchronised
+  // above).  This host pass runs with the GPU idle, so its inner loop only
+  // filters the work list: a run's blocks share one resolution, so its bytes
+  // follow from its count; the next level's cells under the live blocks
+  // (byte accounting of the fused restriction and the prolongation) are
+  // counted only while the per-launch profiler is on.
+  const bool prof = S->g->ctx->prof;
   std::vector<uint32_t> idx;
   for (int k = 0; k < nl; ++k) {
     if (uniform_smoother(S, k)) continue;
     MGLevel& L = *S->lv[k];
     const uint8_t* f = S->h_live + off[k];
     idx.clear();
+    idx.reserve(L.h_blist.size());
     WGMap m[2]{};
     double by[2] = {0.0, 0.0}, cells = 0.0, ccells = 0.0;
     const MGLevel* C = k + 1 < (int)S->lv.size() ? S->lv[k + 1].get() : nullptr;
@@ -1043,20 +1050,19 @@ int refresh_live(aw_solver* S) {
       for (int j = 0; j < WGMAP_RUNS; ++j) {
         if (full.cnt[j] == 0) continue;
         const int start = (int)idx.size();
-        for (int q = full.s0[j]; q < full.s0[j] + full.cnt[j]; ++q) {
-          const uint32_t B = L.h_blist[q];
-          if (!f[B]) continue;
-          idx.push_back((uint32_t)q);
-          const double r = (double)(L.bs >> L.h_e[B]);
-          by[col] += 12.0 * r * r * r + 12.0 * (r + 1.0) * r * r;
-          cells += r * r * r;
-          if (C && !C->aggregated) {
-            const double rc = (double)(C->bs >> C->h_e[B]);
-            ccells += rc * rc * rc;
-          }
-        }
+        const uint32_t* bl = L.h_blist.data();
+        for (int q = full.s0[j]; q < full.s0[j] + full.cnt[j]; ++q)
+          if (f[bl[q]]) idx.push_back((uint32_t)q);
         const int cnt = (int)idx.size() - start;
         if (cnt == 0) continue;
+        const double r = (double)(1 << full.lr[j]);
+        by[col] += cnt * (12.0 * r * r * r + 12.0 * (r + 1.0) * r * r);
+        cells += cnt * r * r * r;
+        if (prof && C && !C->aggregated)
+          for (int q = start; q < start + cnt; ++q) {
+            const double rc = (double)(C->bs >> C->h_e[bl[idx[q]]]);
+            ccells += rc * rc * rc;
+          }
         const int per = 512 >> (3 * full.lr[j]);
         const int t = nrun++;
         m[col].g0[t] = m[col].n;
