@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <atomic>
 #include <climits>
 #include <chrono>
@@ -267,6 +268,8 @@ struct aw_grid {
   bool beta_variable = false;
   DevBuf Dacc[3], sdens;
   uint32_t* h_liq_count = nullptr;   // pinned host copy
+  uint8_t* h_level_read = nullptr;   // pinned host copy of the level map (nb bytes), read back per topology
+  hipEvent_t level_read_done = nullptr;
   DevBuf ubn[3], ubo[3], ubs[3], Mb[3], mb[3];
   bool sorted_ids = false;  // the last binning carried particle ids
   std::vector<uint8_t> h_level;
@@ -477,9 +480,13 @@ int apply_levels(aw_grid* g) {
 }
 
 // radius_rule: levels from the particle radii d_radius (by particle id
-// through d_rids when given, else by slot) instead of the distance bands
+// through d_rids when given, else by slot) instead of the distance bands.
+// meanwhile (distance bands only): GPU work that needs the binning but not
+// the levels, enqueued before the host waits for the level map, so that the
+// host's compare and layout work runs while the GPU does it.
 int bin_and_levels(aw_grid* g, const float* d_pos, int64_t n, bool mark_levels, int* changed,
-                   bool radius_rule = false, const float* d_radius = nullptr, const int32_t* d_rids = nullptr) {
+                   bool radius_rule = false, const float* d_radius = nullptr, const int32_t* d_rids = nullptr,
+                   const std::function<int()>& meanwhile = nullptr) {
   hipStream_t s = g->ctx->stream;
   // the counts below are rewritten for these positions: the sorted particles
   // of an earlier binning no longer match them, whatever happens next (a
@@ -542,9 +549,12 @@ int bin_and_levels(aw_grid* g, const float* d_pos, int64_t n, bool mark_levels, 
       k_levels(s, g->D, g->desc.num_levels, g->desc.band, g->count.as<uint32_t>(), g->surf.as<uint8_t>(),
                g->level.as<uint8_t>(), g->ltmp.as<uint8_t>());
     }
-    AW_HIP(hipMemcpyAsync(nl.data(), g->level.p, g->nb, hipMemcpyDeviceToHost, s));
-    AW_HIP(hipStreamSynchronize(s));
+    AW_HIP(hipMemcpyAsync(g->h_level_read, g->level.p, g->nb, hipMemcpyDeviceToHost, s));
+    AW_HIP(hipEventRecord(g->level_read_done, s));
+    if (meanwhile) AW_TRY(meanwhile());
+    AW_HIP(hipEventSynchronize(g->level_read_done));
     AW_TRY(launch_status("levels"));
+    nl.assign(g->h_level_read, g->h_level_read + g->nb);
   }
   bool ch = !g->have_topology || nl != g->h_level;
   if (ch) {
@@ -1864,6 +1874,8 @@ void grid_free(aw_grid* g) {
   aw_ctx* c = g->ctx;
   (void)hipStreamSynchronize(c->stream);
   if (g->h_liq_count) (void)hipHostFree(g->h_liq_count);
+  if (g->h_level_read) (void)hipHostFree(g->h_level_read);
+  if (g->level_read_done) (void)hipEventDestroy(g->level_read_done);
   delete g;
   ctx_release(c);
 }
@@ -2109,6 +2121,8 @@ int aw_grid_create(aw_ctx* ctx, const aw_grid_desc* d, aw_grid** out) {
   AW_TRY(g->liq_count.ensure(sizeof(uint32_t)));
   AW_HIP(hipHostMalloc((void**)&g->h_liq_count, sizeof(uint32_t), hipHostMallocDefault));
   *g->h_liq_count = 0;
+  AW_HIP(hipHostMalloc((void**)&g->h_level_read, nb, hipHostMallocDefault));
+  AW_HIP(hipEventCreateWithFlags(&g->level_read_done, hipEventDisableTiming));
   std::vector<int32_t> nbr;
   neighbour_table(g->D, nbr);
   AW_TRY(g->nbr.ensure(nbr.size() * sizeof(int32_t)));
@@ -2412,15 +2426,26 @@ int aw_step(aw_grid* g, aw_solver* S, float* d_pos, float* d_vel, int32_t* d_ids
                           s));
   }
   g->p_valid = false;
+  const float* dens = n > 0 ? prm->d_density : nullptr;
+  bool sorted = false;
   if (prm->rebuild_topology == 2) {
     AW_TRY(bin_and_levels(g, d_pos, n, true, &changed, true, prm->d_radius, d_ids));
   } else if (prm->rebuild_topology || !g->have_topology) {
-    AW_TRY(bin_and_levels(g, d_pos, n, true, &changed));
+    // the sort needs the binning only: it runs while the host handles the
+    // level map (a topology change resets the sorted state, restored below)
+    AW_TRY(bin_and_levels(g, d_pos, n, true, &changed, false, nullptr, nullptr, [&]() {
+      return sort_particles(g, d_pos, d_vel, d_ids, n, dens);
+    }));
+    sorted = true;
   } else {
     AW_TRY(bin_and_levels(g, d_pos, n, false, nullptr));
   }
-  const float* dens = n > 0 ? prm->d_density : nullptr;
-  AW_TRY(sort_particles(g, d_pos, d_vel, d_ids, n, dens));
+  if (sorted) {
+    g->n_sorted = n;
+    g->sorted_ids = d_ids != nullptr;
+  } else {
+    AW_TRY(sort_particles(g, d_pos, d_vel, d_ids, n, dens));
+  }
   AW_TRY(p2g_sorted(g, g->spos.as<float>(), g->svel.as<float>(), prm->dt, prm->gravity,
                     dens ? g->sdens.as<float>() : nullptr));
   AW_TRY(divergence(g));
