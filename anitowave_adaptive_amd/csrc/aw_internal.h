@@ -215,9 +215,14 @@ size_t adj_record_bytes();
 void k_g2p_adjoint_stages(hipStream_t s, const GridDev& g, const float* fine, const float* spos, const int32_t* sids,
                           int64_t n, float alpha, float dt, float* xbar, float* vbar, void* rec, uint32_t* list,
                           const uint32_t* pcount = nullptr, const uint32_t* poff = nullptr);
+// touched (nb bytes): set for every block whose fine-lattice adjoint the
+// scatter writes; the transpose writes 0 for the others without reading
+// them, and k_fbar_clear zeroes the touched blocks' adjoint and flags
 void k_g2p_adjoint_scatter(hipStream_t s, const GridDev& g, const float* spos, const uint32_t* pcount,
-                           const uint32_t* poff, const void* rec, float* fbar);
-void k_pi_transpose(hipStream_t s, const GridDev& g, const float* fbar, float* const* unb, float* const* uob);
+                           const uint32_t* poff, const void* rec, float* fbar, uint8_t* touched);
+void k_fbar_clear(hipStream_t s, int nb, float* fbar, uint8_t* touched);
+void k_pi_transpose(hipStream_t s, const GridDev& g, const float* fbar, const uint8_t* touched, float* const* unb,
+                    float* const* uob);
 void k_project_adjoint(hipStream_t s, const GridDev& g, float* const* beta, float* const* unb, float* const* usb,
                        float* pbar);
 void k_divergence_adjoint(hipStream_t s, const GridDev& g, const uint8_t* liquid, const float* rhsbar,

@@ -2112,8 +2112,8 @@ constexpr int AH = 2, AT = BASE + 2 * AH, AT3 = AT * AT * AT;
 // transverse indices clamp to [0, N - 1].  The two fields at one point share
 // the weights, so PAIR adds both with one 64-bit compare-and-swap.
 template <bool PAIR>
-__device__ __forceinline__ void tile_scatter(const GridDev& g, float2* tile, float* fbar, const int* o, int a,
-                                             const float* u, float val, float val1 = 0.f) {
+__device__ __forceinline__ void tile_scatter(const GridDev& g, float2* tile, float* fbar, uint8_t* touched,
+                                             const int* o, int a, const float* u, float val, float val1 = 0.f) {
   if (val == 0.f && (!PAIR || val1 == 0.f)) return;
   int i0[3];
   float t[3];
@@ -2145,6 +2145,7 @@ __device__ __forceinline__ void tile_scatter(const GridDev& g, float2* tile, flo
       const int l = (q[0] & 7) + 8 * ((q[1] & 7) + 8 * (q[2] & 7));
       if (val != 0.f) unsafeAtomicAdd(fbar + (int64_t)B * ADJ + a * FINE + l, w * val);
       if (PAIR && val1 != 0.f) unsafeAtomicAdd(fbar + (int64_t)B * ADJ + 3 * FINE + a * FINE + l, w * val1);
+      touched[B] = 1;
     }
   }
 }
@@ -2445,7 +2446,8 @@ constexpr int ADJ_T = 512;
 __global__ __launch_bounds__(ADJ_T) void g2p_adjoint_scatter_kernel(GridDev g, const float* __restrict__ spos,
                                                                   const uint32_t* __restrict__ pcount,
                                                                   const uint32_t* __restrict__ poff,
-                                                                  const AdjRec* __restrict__ rec, float* fbar) {
+                                                                  const AdjRec* __restrict__ rec, float* fbar,
+                                                                  uint8_t* touched) {
   __shared__ float2 tile[3 * AT3];
   const int B = blockIdx.x;
   const uint32_t cnt = pcount[B];
@@ -2479,11 +2481,11 @@ __global__ __launch_bounds__(ADJ_T) void g2p_adjoint_scatter_kernel(GridDev g, c
     const float k3[3] = {r.q1.z, r.q1.w, r.q2.x}, k2[3] = {r.q2.y, r.q2.z, r.q2.w};
     const float s1[3] = {r.q3.x, r.q3.y, r.q3.z}, d1[3] = {r.q3.w, r.q4.x, r.q4.y};
 #pragma unroll
-    for (int a = 0; a < 3; ++a) tile_scatter<false>(g, tile, fbar, o, a, u3, k3[a]);
+    for (int a = 0; a < 3; ++a) tile_scatter<false>(g, tile, fbar, touched, o, a, u3, k3[a]);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) tile_scatter<false>(g, tile, fbar, o, a, u2, k2[a]);
+    for (int a = 0; a < 3; ++a) tile_scatter<false>(g, tile, fbar, touched, o, a, u2, k2[a]);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) tile_scatter<true>(g, tile, fbar, o, a, u, s1[a], d1[a]);
+    for (int a = 0; a < 3; ++a) tile_scatter<true>(g, tile, fbar, touched, o, a, u, s1[a], d1[a]);
   }
   __syncthreads();
   // flush: tile nodes are in-domain (clamped) by construction except the
@@ -2496,19 +2498,42 @@ __global__ __launch_bounds__(ADJ_T) void g2p_adjoint_scatter_kernel(GridDev g, c
     const int Bq = (q[0] >> LOGB) + g.D[0] * ((q[1] >> LOGB) + g.D[1] * (q[2] >> LOGB));
     const int l = (q[0] & 7) + 8 * ((q[1] & 7) + 8 * (q[2] & 7));
     float* dst = fbar + (int64_t)Bq * ADJ + a * FINE + l;
+    touched[Bq] = 1;
     if (v.x != 0.f) unsafeAtomicAdd(dst, v.x);
     if (v.y != 0.f) unsafeAtomicAdd(dst + 3 * FINE, v.y);
   }
 }
 
+// Zero the fine-lattice adjoint of the blocks the last scatter touched and
+// clear their flags (one wave per block; every other block's adjoint is
+// already 0), so that the buffer starts each G2P adjoint at zero without a
+// pass over every block's nodes.
+__global__ __launch_bounds__(TPB) void fbar_clear_kernel(int nb, float* __restrict__ fbar, uint8_t* touched) {
+  const int B = blockIdx.x * (TPB / 64) + (int)(threadIdx.x >> 6);
+  if (B >= nb || !touched[B]) return;
+  float4* p = reinterpret_cast<float4*>(fbar + (size_t)B * ADJ);   // ADJ floats, a multiple of 4
+  for (int k = threadIdx.x & 63; k < ADJ / 4; k += 64) p[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+  if ((threadIdx.x & 63) == 0) touched[B] = 0;
+}
+
 // Pi^T of the two fine-lattice adjoint fields onto the block's own slots:
 // unb = Pi^T field 0, uob = -Pi^T field 1.
-__global__ __launch_bounds__(TPB) void pi_transpose_kernel(GridDev g, const float* __restrict__ fbar, FacePtrs unb,
+__global__ __launch_bounds__(TPB) void pi_transpose_kernel(GridDev g, const float* __restrict__ fbar,
+                                                           const uint8_t* __restrict__ touched, FacePtrs unb,
                                                            FacePtrs uob) {
   __shared__ float nA[3 * 576], nB[3 * 576];
   __shared__ float tA[3 * 512], tB[3 * 512];
   const int B = blockIdx.x, tid = threadIdx.x;
   const int e = g.level[B], r = BASE >> e, nf = (r + 1) * r * r;
+  if (!touched[B]) {
+    // nothing was scattered into this block: its fine-lattice adjoint is 0
+    for (int k = tid; k < 3 * nf; k += TPB) {
+      const int a = k / nf, f = k - a * nf;
+      unb.p[a][g.foff[a][B] + f] = 0.f;
+      uob.p[a][g.foff[a][B] + f] = 0.f;
+    }
+    return;
+  }
   const float* src = fbar + (size_t)B * ADJ;
   if (e > 0) {
     pi_transpose_sep(
@@ -4539,11 +4564,17 @@ void k_g2p_adjoint_stages(hipStream_t s, const GridDev& g, const float* fine, co
 }
 
 void k_g2p_adjoint_scatter(hipStream_t s, const GridDev& g, const float* spos, const uint32_t* pcount,
-                           const uint32_t* poff, const void* rec, float* fbar) {
-  launch(g2p_adjoint_scatter_kernel, g.nb, ADJ_T, 0, s, g, spos, pcount, poff, static_cast<const AdjRec*>(rec), fbar);
+                           const uint32_t* poff, const void* rec, float* fbar, uint8_t* touched) {
+  launch(g2p_adjoint_scatter_kernel, g.nb, ADJ_T, 0, s, g, spos, pcount, poff, static_cast<const AdjRec*>(rec), fbar,
+         touched);
 }
-void k_pi_transpose(hipStream_t s, const GridDev& g, const float* fbar, float* const* unb, float* const* uob) {
-  launch(pi_transpose_kernel, g.nb, TPB, 0, s, g, fbar, fp(unb), fp(uob));
+void k_fbar_clear(hipStream_t s, int nb, float* fbar, uint8_t* touched) {
+  const int per = TPB / 64;
+  if (nb > 0) launch(fbar_clear_kernel, (nb + per - 1) / per, TPB, 0, s, nb, fbar, touched);
+}
+void k_pi_transpose(hipStream_t s, const GridDev& g, const float* fbar, const uint8_t* touched, float* const* unb,
+                    float* const* uob) {
+  launch(pi_transpose_kernel, g.nb, TPB, 0, s, g, fbar, touched, fp(unb), fp(uob));
 }
 void k_project_adjoint(hipStream_t s, const GridDev& g, float* const* beta, float* const* unb, float* const* usb,
                        float* pbar) {

@@ -262,6 +262,10 @@ struct aw_grid {
   DevBuf beta[3], Macc[3], macc[3], uold[3], ustar[3], unew[3], mass[3];
   // adjoint fields: fine-lattice adjoints (two fields each, stride ADJ), per-slot and per-cell adjoints
   DevBuf fbar, fmbar, pbar, rhsbar, idrange, adjrec;
+  // blocks whose fbar the last G2P adjoint scattered into (the rest of fbar
+  // is 0), and the fbar allocation those flags describe
+  DevBuf fbar_touched;
+  const void* fbar_zeroed_for = nullptr;
   DevBuf liq_count;                  // liquid cells of the last step, read back with the step's final sync
   // two-phase coefficients: beta = 1 / rho from a per-particle density (the
   // last step deposited one), its density-weighted deposit and sorted densities
@@ -1653,7 +1657,16 @@ int g2p_adjoint(aw_grid* g, float alpha, float dt, float* xbar, float* vbar) {
   const int64_t n = g->n_sorted;
   AW_TRY(g->adjrec.ensure((size_t)std::max<int64_t>(n, 1) * adj_record_bytes()));
   AW_TRY(g->g2p_list.ensure((size_t)(n + 1) * sizeof(uint32_t)));
-  k_fill_f32(s, g->fbar.as<float>(), (size_t)g->nb * ADJ, 0.f);
+  const size_t flag_words = ((size_t)g->nb + 3) / 4;
+  AW_TRY(g->fbar_touched.ensure(flag_words * sizeof(uint32_t)));
+  if (g->fbar_zeroed_for != g->fbar.p) {
+    // a new allocation: zero it all once
+    k_fill_f32(s, g->fbar.as<float>(), (size_t)g->nb * ADJ, 0.f);
+    k_fill_u32(s, g->fbar_touched.as<uint32_t>(), flag_words, 0u);
+    g->fbar_zeroed_for = g->fbar.p;
+  } else {
+    k_fbar_clear(s, g->nb, g->fbar.as<float>(), g->fbar_touched.as<uint8_t>());
+  }
   {
     // particle position and id 16 B + adjoints read and written 48 B + record; forward fine lattice per block
     Prof pf(g->ctx, "g2p_adjoint_stages", (64.0 + adj_record_bytes()) * n + 2.0 * 4.0 * 3 * FINE * g->nb);
@@ -1665,12 +1678,12 @@ int g2p_adjoint(aw_grid* g, float alpha, float dt, float* xbar, float* vbar) {
     // position 12 B + record per particle; the fine-lattice adjoint per block
     Prof pf(g->ctx, "g2p_adjoint_scatter", (12.0 + adj_record_bytes()) * n + 4.0 * ADJ * g->nb);
     k_g2p_adjoint_scatter(s, gv, g->spos.as<float>(), g->count.as<uint32_t>(), g->poff.as<uint32_t>(), g->adjrec.p,
-                          g->fbar.as<float>());
+                          g->fbar.as<float>(), g->fbar_touched.as<uint8_t>());
   }
   {
     Prof pf(g->ctx, "pi_transpose", 4.0 * ADJ * g->nb + 8.0 * 3 * g->nfaces);
     Faces un(g->ubn), uo(g->ubo);
-    k_pi_transpose(s, gv, g->fbar.as<float>(), un.p, uo.p);
+    k_pi_transpose(s, gv, g->fbar.as<float>(), g->fbar_touched.as<uint8_t>(), un.p, uo.p);
   }
   return launch_status("g2p_advect_adjoint");
 }
