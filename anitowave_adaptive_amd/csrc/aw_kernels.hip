@@ -2737,9 +2737,18 @@ __global__ __launch_bounds__(TPB) void fold_adjoint_kernel(GridDev g, FacePtrs u
 // samples that read the result (the deposit adjoint) take their nodes from
 // the particle's block and its neighbours only
 __global__ __launch_bounds__(TPB) void pi_forward2_kernel(GridDev g, FacePtrs A, FacePtrs Bf, float* __restrict__ out) {
+  // the block's slots of both fields, read once with coalesced loads (the
+  // nodes' weighted sums then read LDS, in pi_weights's order as before)
+  __shared__ float sA[3 * 576], sB[3 * 576];   // (r + 1) r^2 <= 576 slots per axis
   const int B = blockIdx.x;
   if (!g.halo[B]) return;
-  const int e = g.level[B];
+  const int e = g.level[B], r = BASE >> e, nf = (r + 1) * r * r;
+  for (int k = threadIdx.x; k < 3 * nf; k += blockDim.x) {
+    const int a = k / nf, i = k - a * nf;
+    sA[a * 576 + i] = A.p[a][g.foff[a][B] + i];
+    sB[a * 576 + i] = Bf.p[a][g.foff[a][B] + i];
+  }
+  __syncthreads();
   float* dst = out + (size_t)B * ADJ;
   for (int k = threadIdx.x; k < 3 * FINE; k += blockDim.x) {
     const int a = k >> 9, li = k & 511;
@@ -2749,9 +2758,8 @@ __global__ __launch_bounds__(TPB) void pi_forward2_kernel(GridDev g, FacePtrs A,
     const int nw = pi_weights(a, e, l, slot, w);
     float va = 0.f, vb = 0.f;
     for (int j = 0; j < nw; ++j) {
-      const size_t id = g.foff[a][B] + slot[j];
-      va += w[j] * A.p[a][id];
-      vb += w[j] * Bf.p[a][id];
+      va += w[j] * sA[a * 576 + slot[j]];
+      vb += w[j] * sB[a * 576 + slot[j]];
     }
     dst[k] = va;
     dst[3 * FINE + k] = vb;
