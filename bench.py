@@ -170,10 +170,10 @@ def kernel_source_sha():
     return hashlib.sha256(open(src, "rb").read()).hexdigest()[:16]
 
 
-def traffic_for(kernel):
+def traffic_for(kernel, workload):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    summary (profiles/traffic.json), only while the kernel source is the one
-    it was measured on (its sha is recorded there); otherwise null."""
+    summary (profiles/traffic.json), only for the workload and the kernel
+    source it was measured on (both recorded there); otherwise null."""
     tf = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         t = json.load(open(tf))
@@ -181,11 +181,13 @@ def traffic_for(kernel):
         return None, "no traffic measurement"
     if t.get("kernel_source_sha") != kernel_source_sha():
         return None, "traffic measurement is for another kernel source: not used"
+    if t.get("workload") != workload:
+        return None, f"traffic measured on workload {t.get('workload')}, not {workload}: not used"
     v = t.get("bytes_per_launch", {}).get(kernel)
     return v, (t.get("measured_by", "profiles/traffic.json") if v is not None else "kernel not measured")
 
 
-def roofline(prof):
+def roofline(prof, workload):
     if not prof:
         return None, {}
     per = {}
@@ -210,7 +212,7 @@ def roofline(prof):
     dom = max(kern, key=lambda k: kern[k]["ms"])
     d = kern[dom]
     gbs = d["bytes"] / (d["ms"] * 1e-3) / 1e9
-    traffic, traffic_note = traffic_for(dom)
+    traffic, traffic_note = traffic_for(dom, workload)
     rl = {"kernel": dom, "groups": d["groups"], "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
           "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_note,
           "bytes_per_launch": round(d["bytes"] / d["launches"]), "us_per_launch": round(1e3 * d["ms"] / d["launches"], 3),
@@ -360,7 +362,7 @@ def run_large(args):
     a.steps, a.warmup, a.profile_steps, a.mixed_steps, a.radius_calls, a.resample = 5, 2, 2, 3, 0, 0
     a.cpu = False
     gpu = run_gpu(a, w, pos, vel)
-    rl, per = roofline(gpu["profile"])
+    rl, per = roofline(gpu["profile"], "double512")
     log(f"[bench] large: {a.steps / gpu['seconds']:.3f} steps/s")
     mixed, _ = run_mixed(a, w, pos, vel)
     log(f"[bench] large mixed precision 1e-8: {mixed}")
@@ -521,7 +523,7 @@ def traffic_from(fetch_csv, write_csv, copy_bytes):
 
 def write_traffic(args):
     out, scale = traffic_from(args.traffic_from[0], args.traffic_from[1], args.copy_bytes)
-    doc = {"kernel_source_sha": kernel_source_sha(),
+    doc = {"kernel_source_sha": kernel_source_sha(), "workload": args.workload,
            "measured_by": (f"rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE (profiles/pool256_traffic.md); "
                            f"scaled {scale['FETCH_SIZE']:.3f} / {scale['WRITE_SIZE']:.3f} by a "
                            f"{args.copy_bytes:.0f}-byte device copy"),
@@ -579,7 +581,7 @@ def main():
     log(f"[bench] seeded {len(pos)} particles in {time.perf_counter() - t:.1f}s")
     gpu = run_gpu(args, w, pos, vel)
     log(f"[bench] gpu {args.steps} steps in {gpu['seconds']:.3f}s, pcg iters {gpu['iters']}")
-    rl, per = roofline(gpu["profile"])
+    rl, per = roofline(gpu["profile"], args.workload)
     mixed, mixed10, firsts = None, None, {}
     if args.mixed_steps > 0:
         mixed, firsts[1e-8] = run_mixed(args, w, pos, vel, 1e-8)
@@ -605,7 +607,7 @@ def main():
     sps = args.steps / gpu["seconds"]
     smoother = per.get("smooth_l0")
     resid = per.get("residual_restrict_l0")
-    _, iso = roofline(gpu["isolated"])
+    _, iso = roofline(gpu["isolated"], args.workload)
     out = {
         "metric": METRIC,
         "value": round(sps, 3),
