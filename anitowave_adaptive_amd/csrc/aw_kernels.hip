@@ -860,6 +860,21 @@ __global__ __launch_bounds__(TPB) void init_beta_kernel(LevelDev L, float* __res
   }
 }
 
+// Work-groups are dealt to the 8 XCDs round-robin (group w on XCD w mod 8),
+// and each XCD has its own L2.  xcd_group gives XCD x the consecutive groups
+// [x q + min(x, r), ...) of a launch of q * 8 + r groups.  Particle kernels
+// sample the padded lattice of the particle's block, and a block's particles
+// are consecutive after the binning sort: with the plain mapping a block's
+// groups run on all 8 XCDs and each XCD's L2 fetches the block's lattice.
+// Kernels of one work-group per block that read their neighbours' data take
+// their block the same way, so that neighbours in x and y (block ids 1 and
+// D_x apart) mostly run on the XCD whose L2 already holds those lines.
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int64_t xcd_group(int64_t w, int64_t nwg) {
+  const int64_t q = nwg / NUM_XCD, r = nwg % NUM_XCD, x = w % NUM_XCD, j = w / NUM_XCD;
+  return x * q + (x < r ? x : r) + j;
+}
+
 // ------------------------------------------------------------- P2G
 // One work-group per block: the block's particles (contiguous after the
 // binning sort) scatter their trilinear weights into the block's padded fine
@@ -1021,7 +1036,7 @@ __global__ __launch_bounds__(TPB) void p2g_gather_kernel(GridDev g, const uint32
   __shared__ float nM[3 * 576], nm[3 * 576];
   __shared__ int occ[27];   // occupied 3x3x3 neighbours, (dx + 1) + 3 (dy + 1) + 9 (dz + 1)
   __shared__ int any_occ;
-  const int B = blockIdx.x, tid = threadIdx.x;
+  const int B = (int)xcd_group(blockIdx.x, gridDim.x), tid = threadIdx.x;
   const int e = g.level[B], r = BASE >> e, nf = (r + 1) * r * r;
   int Bc[3];
   block_coords(g.D, B, Bc);
@@ -1113,7 +1128,7 @@ struct FacePtrs {
 __global__ __launch_bounds__(TPB) void fold_kernel(GridDev g, FacePtrs Macc, FacePtrs macc, FacePtrs uold,
                                                    FacePtrs ustar, FacePtrs mass, float dt, float g0, float g1,
                                                    float g2) {
-  const int B = blockIdx.x;
+  const int B = (int)xcd_group(blockIdx.x, gridDim.x);
   const int e = g.level[B], r = BASE >> e, nf = (r + 1) * r * r;
   const float grav[3] = {g0, g1, g2};
   for (int k = threadIdx.x; k < 3 * nf; k += blockDim.x) {
@@ -1228,7 +1243,7 @@ __global__ __launch_bounds__(TPB) void beta_fold_kernel(GridDev g, FacePtrs macc
 // rhs = -s^2 sum_a (u[+] - u[-]) on liquid cells
 __global__ __launch_bounds__(TPB) void divergence_kernel(GridDev g, const uint8_t* __restrict__ liquid, FacePtrs u,
                                                          float* __restrict__ rhs) {
-  const int B = blockIdx.x;
+  const int B = (int)xcd_group(blockIdx.x, gridDim.x);
   const int e = g.level[B], r = BASE >> e, nc = r * r * r;
   const float s = (float)(1 << e);
   for (int k = threadIdx.x; k < nc; k += blockDim.x) {
@@ -1248,7 +1263,7 @@ __global__ __launch_bounds__(TPB) void divergence_kernel(GridDev g, const uint8_
 // u_new = u* - beta (p+ - p-) / d on every slot of the block
 __global__ __launch_bounds__(TPB) void project_kernel(GridDev g, const float* __restrict__ p, FacePtrs ustar,
                                                       FacePtrs beta, FacePtrs unew) {
-  const int B = blockIdx.x;
+  const int B = (int)xcd_group(blockIdx.x, gridDim.x);
   const int e = g.level[B], r = BASE >> e, nf = (r + 1) * r * r;
   const float s = (float)(1 << e);
   const uint32_t co = g.coff[B];
@@ -1430,7 +1445,7 @@ __global__ __launch_bounds__(TPB) void to_fine_kernel(GridDev g, float* __restri
   __shared__ int nlev[27];
   __shared__ uint32_t nfo[27];
   __shared__ float sun[3 * 576], suo[3 * 576];   // the block's face slots, (r + 1) r^2 <= 576 per axis
-  const int B = blockIdx.x;
+  const int B = (int)xcd_group(blockIdx.x, gridDim.x);
   if (!g.halo[B]) return;
   int Bc[3];
   block_coords(g.D, B, Bc);
@@ -1886,18 +1901,6 @@ __device__ __forceinline__ void g2p_advect_one(const GridDev& g, const float* __
   }
 }
 
-// Work-groups are dealt to the 8 XCDs round-robin (group w on XCD w mod 8),
-// and each XCD has its own L2.  Particle kernels sample the padded lattice of
-// the particle's block, and a block's particles are consecutive after the
-// binning sort: with the plain mapping a block's groups run on all 8 XCDs and
-// each XCD's L2 fetches the block's lattice.  xcd_group gives XCD x the
-// consecutive groups [x q + min(x, r), ...) of a launch of q * 8 + r groups,
-// so that a block's particles stay on one XCD.
-constexpr int NUM_XCD = 8;
-__device__ __forceinline__ int64_t xcd_group(int64_t w, int64_t nwg) {
-  const int64_t q = nwg / NUM_XCD, r = nwg % NUM_XCD, x = w % NUM_XCD, j = w / NUM_XCD;
-  return x * q + (x < r ? x : r) + j;
-}
 
 // FIXUP = false: every particle; FIXUP = true: the particles listed by the
 // first pass (a grid-stride loop over list[0] entries, so the host needs no
@@ -2523,7 +2526,7 @@ __global__ __launch_bounds__(TPB) void pi_transpose_kernel(GridDev g, const floa
                                                            FacePtrs uob) {
   __shared__ float nA[3 * 576], nB[3 * 576];
   __shared__ float tA[3 * 512], tB[3 * 512];
-  const int B = blockIdx.x, tid = threadIdx.x;
+  const int B = (int)xcd_group(blockIdx.x, gridDim.x), tid = threadIdx.x;
   const int e = g.level[B], r = BASE >> e, nf = (r + 1) * r * r;
   if (!touched[B]) {
     // nothing was scattered into this block: its fine-lattice adjoint is 0
@@ -2574,7 +2577,7 @@ __global__ __launch_bounds__(TPB) void pi_transpose_kernel(GridDev g, const floa
 // of a level change) into usb; the pressure terms are scattered into pbar.
 __global__ __launch_bounds__(TPB) void project_adjoint_kernel(GridDev g, FacePtrs beta, FacePtrs unb, FacePtrs usb,
                                                               float* pbar) {
-  const int B = blockIdx.x;
+  const int B = (int)xcd_group(blockIdx.x, gridDim.x);
   const int e = g.level[B], r = BASE >> e, nf = (r + 1) * r * r;
   const float s = (float)(1 << e);
   const uint32_t co = g.coff[B];
@@ -2646,7 +2649,7 @@ __global__ __launch_bounds__(TPB) void project_adjoint_kernel(GridDev g, FacePtr
 // usb += D^T rhsbar: rhs_c = -s^2 sum_a (u[c+] - u[c-]) on liquid cells
 __global__ __launch_bounds__(TPB) void divergence_adjoint_kernel(GridDev g, const uint8_t* __restrict__ liquid,
                                                                  const float* __restrict__ rhsbar, FacePtrs usb) {
-  const int B = blockIdx.x;
+  const int B = (int)xcd_group(blockIdx.x, gridDim.x);
   const int e = g.level[B], r = BASE >> e, nf = (r + 1) * r * r;
   const float s2 = (float)(1 << e) * (float)(1 << e);
   const uint32_t co = g.coff[B];
@@ -2675,7 +2678,7 @@ __global__ __launch_bounds__(TPB) void divergence_adjoint_kernel(GridDev g, cons
 // -c lambda u / m'.  u and m' are the forward's u_old and mass.
 __global__ __launch_bounds__(TPB) void fold_adjoint_kernel(GridDev g, FacePtrs uold, FacePtrs mass, FacePtrs uob,
                                                            FacePtrs usb, FacePtrs Mb, FacePtrs mb) {
-  const int B = blockIdx.x;
+  const int B = (int)xcd_group(blockIdx.x, gridDim.x);
   const int e = g.level[B], r = BASE >> e, nf = (r + 1) * r * r;
   for (int k = threadIdx.x; k < 3 * nf; k += blockDim.x) {
     const int a = k / nf, f = k - a * nf;
