@@ -1060,8 +1060,24 @@ __global__ __launch_bounds__(TPB) void p2g_gather_kernel(GridDev g, const uint32
     }
     return;
   }
-  for (int k = tid; k < 3 * FINE; k += TPB) {
-    const int a = k >> 9, li = k & 511;
+  // the block's own nodes first, every load of the thread in flight together
+  // (a slab is written only for a block holding particles); then the sums
+  // over the owners of each node, in the same order as the node's loop
+  constexpr int NPT = 3 * FINE / TPB;
+  static_assert(NPT * TPB == 3 * FINE, "nodes per thread");
+  const bool own = occ[13] != 0;
+  float oM[NPT], om[NPT];
+#pragma unroll
+  for (int j = 0; j < NPT; ++j) {
+    const int k = tid + j * TPB, a = k >> 9, li = k & 511;
+    const int l[3] = {li & 7, (li >> 3) & 7, li >> 6};
+    const float* src = slab + (size_t)B * SLAB + a * PAD + pad_idx(a, l);
+    oM[j] = own ? src[0] : 0.f;
+    om[j] = own ? src[3 * PAD] : 0.f;
+  }
+#pragma unroll
+  for (int j = 0; j < NPT; ++j) {
+    const int k = tid + j * TPB, a = k >> 9, li = k & 511;
     const int l[3] = {li & 7, (li >> 3) & 7, li >> 6};
     int lo[3], hi[3];
 #pragma unroll
@@ -1074,6 +1090,11 @@ __global__ __launch_bounds__(TPB) void p2g_gather_kernel(GridDev g, const uint32
       for (int dy = lo[1]; dy <= hi[1]; ++dy)
         for (int dx = lo[0]; dx <= hi[0]; ++dx) {
           if (!occ[(dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)]) continue;
+          if ((dx | dy | dz) == 0) {
+            sM += oM[j];
+            sm += om[j];
+            continue;
+          }
           const int q[3] = {Bc[0] + dx, Bc[1] + dy, Bc[2] + dz};
           const int N = block_id(g.D, q);
           const int lp[3] = {l[0] - BASE * dx, l[1] - BASE * dy, l[2] - BASE * dz};
